@@ -1,0 +1,216 @@
+/*
+ * profit_hip.h -- C-ABI of the MI355X (gfx950) implementation of libprofit's
+ * image-generation hot path: radial-profile evaluation with adaptive
+ * sub-pixel refinement, sky/psf/null profiles, and PSF convolution
+ * (BRUTE_OLD / BRUTE / FFT), behind libprofit's Model / Profile / Convolver
+ * semantics.
+ *
+ * Conventions
+ *  - Every entry point returns a status code (PK_OK == 0) and never throws.
+ *    The message of the last failure on the calling thread is available from
+ *    pk_last_error().  Status codes map 1:1 onto libprofit's exceptions
+ *    (include/profit/exceptions.h:41-105): PK_EINVAL -> invalid_parameter,
+ *    PK_EUNKNOWN -> unknown_parameter, PK_EFFT -> fft_error, PK_EDEVICE takes
+ *    the role of opencl_error.
+ *  - Images are row-major with idx = i + j * width
+ *    (include/profit/image.h:522-530).  Host buffers are caller owned.
+ *  - Masks are one byte per pixel, non-zero meaning "evaluate this pixel"
+ *    (include/profit/image.h Mask = surface<bool>).
+ *  - A context binds one device and one HIP stream; calls on one context are
+ *    serialised internally, separate contexts may run concurrently.
+ */
+#ifndef PROFIT_HIP_H
+#define PROFIT_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PK_VERSION_MAJOR 1
+#define PK_VERSION_MINOR 9
+#define PK_VERSION_PATCH 3
+
+/* ---- status codes ------------------------------------------------------ */
+enum {
+	PK_OK = 0,
+	PK_EINVAL = 1,    /* profit::invalid_parameter */
+	PK_EUNKNOWN = 2,  /* profit::unknown_parameter */
+	PK_EDEVICE = 3,   /* HIP runtime / device failure (role of opencl_error) */
+	PK_EFFT = 4,      /* profit::fft_error */
+	PK_ENOMEM = 5,    /* device or host allocation failed; never truncates */
+	PK_ERANGE = 6     /* output buffer too small / work queue capacity exceeded */
+};
+
+/* ---- precision of the evaluation (context property) -------------------- */
+enum {
+	PK_FP64 = 0,  /* everything in FP64: per-pixel parity with libprofit <= 1e-12 */
+	PK_FP32 = 1   /* FP32 bulk evaluation + convolution, FP64 constants/decisions */
+};
+
+/* ---- profile kinds (src/model.cpp:103-135 names) ------------------------ */
+enum {
+	PK_PROFILE_NULL = 0,
+	PK_PROFILE_SKY = 1,
+	PK_PROFILE_SERSIC = 2,
+	PK_PROFILE_MOFFAT = 3,
+	PK_PROFILE_FERRER = 4,
+	PK_PROFILE_CORESERSIC = 5,
+	PK_PROFILE_KING = 6,
+	PK_PROFILE_BROKENEXP = 7,
+	PK_PROFILE_PSF = 8
+};
+
+/* ---- convolver types (include/profit/convolve.h:44-57) ------------------ */
+enum {
+	PK_CONV_BRUTE_OLD = 0,
+	PK_CONV_BRUTE = 1,
+	PK_CONV_OPENCL = 2,   /* accepted as an alias of BRUTE (device direct convolution) */
+	PK_CONV_FFT = 3
+};
+
+/*
+ * Index of the profile-specific parameters inside pk_component.p[].
+ * Radial parameters shared by all radial profiles are named fields
+ * (src/radial.cpp:724-751).
+ */
+enum {
+	/* sersic (src/sersic.cpp:352-360) */
+	PK_SERSIC_RE = 0, PK_SERSIC_NSER = 1,
+	/* moffat (src/moffat.cpp) */
+	PK_MOFFAT_FWHM = 0, PK_MOFFAT_CON = 1,
+	/* ferrer (src/ferrer.cpp) */
+	PK_FERRER_ROUT = 0, PK_FERRER_A = 1, PK_FERRER_B = 2,
+	/* coresersic (src/coresersic.cpp) */
+	PK_CORESERSIC_RE = 0, PK_CORESERSIC_RB = 1, PK_CORESERSIC_NSER = 2,
+	PK_CORESERSIC_A = 3, PK_CORESERSIC_B = 4,
+	/* king (src/king.cpp) */
+	PK_KING_RC = 0, PK_KING_RT = 1, PK_KING_A = 2,
+	/* brokenexp (src/brokenexponential.cpp) */
+	PK_BROKENEXP_H1 = 0, PK_BROKENEXP_H2 = 1, PK_BROKENEXP_RB = 2,
+	PK_BROKENEXP_A = 3,
+	/* sky (src/sky.cpp): requested bg */
+	PK_SKY_BG = 0
+};
+
+#define PK_MAX_PARAMS 6
+
+/*
+ * One profile of a model: the parameters libprofit registers through
+ * Profile::register_parameter (src/profile.cpp:69-83, src/radial.cpp:738-751).
+ * Defaults are those of the libprofit constructors; pk_component_defaults()
+ * fills them in.
+ */
+typedef struct pk_component {
+	int32_t kind;          /* PK_PROFILE_* */
+	int32_t convolve;      /* Profile "convolve" (src/profile.cpp:44) */
+	/* radial / psf common parameters */
+	double xcen, ycen, mag, ang, axrat, box;
+	int32_t rough, adjust, rescale_flux, force_cpu;
+	double acc, rscale_switch, rscale_max;
+	uint32_t resolution, max_recursions;
+	double p[PK_MAX_PARAMS];
+} pk_component;
+
+/*
+ * A model: the state libprofit keeps in profit::Model
+ * (include/profit/model.h:60-408) plus the selected convolver.
+ */
+typedef struct pk_model {
+	uint32_t width, height;          /* requested dimensions */
+	double scale_x, scale_y;         /* image pixel scale */
+	double magzero;
+	uint32_t finesampling;           /* >= 1 */
+	int32_t crop;                    /* Model::set_crop, default 1 */
+	int32_t return_finesampled;      /* default 1 */
+	int32_t adjust_mask;             /* default 1 */
+	int32_t dry_run;                 /* default 0 */
+	const uint8_t *mask;             /* NULL = no mask */
+	uint32_t mask_width, mask_height;
+	const double *psf;               /* as given to Model::set_psf (normalised inside) */
+	uint32_t psf_width, psf_height;
+	double psf_scale_x, psf_scale_y;
+	int32_t convolver;               /* PK_CONV_* ; default BRUTE (src/model.cpp:144-150) */
+	int32_t reuse_krn_fft;
+	uint32_t n_components;
+	const pk_component *components;
+} pk_model;
+
+/* Result metadata of an evaluation. */
+typedef struct pk_result {
+	uint32_t width, height;    /* dimensions of the returned image */
+	int32_t offset_x, offset_y;/* Model::evaluate(offset_out) (src/model.cpp:259-320) */
+	uint64_t n_refined_pixels; /* pixel/component pairs that entered sub-pixel refinement */
+	uint64_t n_subcells;       /* evaluate_at calls made by the refinement */
+	double ms_setup, ms_profiles, ms_convolution, ms_total; /* device timings */
+} pk_result;
+
+typedef struct pk_ctx pk_ctx;
+
+/* ---- library ----------------------------------------------------------- */
+/* profit::version() (include/profit/library.h:38-60) */
+const char *pk_version(void);
+/* profit::init() / profit::finish() (include/profit/library.h:67-80) */
+int pk_init(void);
+void pk_finish(void);
+/* Message of the last error raised on this thread ("" if none). */
+const char *pk_last_error(void);
+/* Number of visible HIP devices (replaces get_opencl_info, include/profit/opencl.h:124-168). */
+int pk_device_count(int *count);
+
+/* ---- context: replaces OpenCLEnvPtr / get_opencl_environment ------------
+ * (include/profit/opencl.h:79-168, consulted by src/radial.cpp:269-273). */
+int pk_ctx_create(int device, int precision, pk_ctx **ctx_out);
+int pk_ctx_destroy(pk_ctx *ctx);
+int pk_ctx_precision(const pk_ctx *ctx);
+
+/* Fills a component with libprofit's constructor defaults for `kind`
+ * (src/radial.cpp:724-737, src/sersic.cpp:352-360, src/moffat.cpp,
+ * src/ferrer.cpp:116-124, src/king.cpp, src/brokenexponential.cpp,
+ * src/coresersic.cpp, src/sky.cpp, src/psf.cpp). */
+int pk_component_defaults(int kind, pk_component *c);
+/* Fills a model with profit::Model's defaults (src/model.cpp:54-73). */
+int pk_model_defaults(uint32_t width, uint32_t height, pk_model *m);
+
+/* Validates a model exactly as Model::analyze_inputs + Profile::validate do
+ * (src/model.cpp:186-219, src/radial.cpp:204-214 and the per-profile validate
+ * methods) and reports the dimensions and offset evaluate() would return. */
+int pk_model_output_dims(const pk_model *m, uint32_t *width, uint32_t *height);
+
+/* Model::evaluate(Point &offset_out) (src/model.cpp:259-320).
+ * Writes the model image (FP64, row-major) into `out`, which must hold at
+ * least `out_capacity` doubles.  `res` (may be NULL) receives dims, offset
+ * and statistics. */
+int pk_evaluate(pk_ctx *ctx, const pk_model *m, double *out, size_t out_capacity,
+                pk_result *res);
+
+/* Convolver::convolve(src, krn, mask, crop, offset_out)
+ * (include/profit/convolve.h:98-99, src/convolve.cpp:55-79) for a convolver of
+ * type `type`; the kernel is used as given (no normalisation). */
+int pk_convolve(pk_ctx *ctx, int type, const double *src, uint32_t src_w, uint32_t src_h,
+                const double *krn, uint32_t krn_w, uint32_t krn_h,
+                const uint8_t *mask, int crop, double *out, size_t out_capacity,
+                uint32_t *out_w, uint32_t *out_h, int32_t *offset_x, int32_t *offset_y);
+
+/* Convolver::padding (src/convolve.cpp:50-53, 263-268). */
+int pk_convolver_padding(int type, uint32_t src_w, uint32_t src_h, uint32_t krn_w, uint32_t krn_h,
+                         int32_t *lo_x, int32_t *lo_y, int32_t *hi_x, int32_t *hi_y);
+
+/* ---- FP64 special functions of the math back-end ------------------------
+ * include/profit/utils.h:52-113 (GSL/Rmath in libprofit, own implementations
+ * here; the same code runs on the device during per-component set-up). */
+double pk_qgamma(double p, double shape);
+double pk_pgamma(double q, double shape);
+double pk_gammafn(double x);
+double pk_beta(double a, double b);
+typedef double (*pk_integration_func_t)(double x, void *params);
+double pk_integrate_qags(pk_integration_func_t f, double a, double b, void *params);
+double pk_integrate_qagi(pk_integration_func_t f, double a, void *params);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* PROFIT_HIP_H */

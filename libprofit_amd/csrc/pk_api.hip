@@ -1,0 +1,974 @@
+// C-ABI of the gfx950 image-generation path (include/profit_hip.h).
+//
+// Host side of one Model::evaluate (src/model.cpp:259-360): input validation
+// and geometry (analyze_inputs / analyze_expansion_requirements), calculation
+// mask adjustment (Model::adjust, Mask::upsample/extend/expand_by), per
+// component FP64 set-up, then one stream-ordered sequence of kernels:
+//   coarse -> refine -> psf profile -> convolution -> compose.
+// Inputs travel in one pinned upload, the result in one pinned download.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "pk_device.h"
+#include "profit_hip.h"
+
+using namespace pk;
+
+// ---------------------------------------------------------------------------
+// errors
+// ---------------------------------------------------------------------------
+static thread_local std::string g_err;
+
+static int set_err(int code, const char *fmt, ...)
+{
+	char buf[512];
+	va_list ap;
+	va_start(ap, fmt);
+	vsnprintf(buf, sizeof(buf), fmt, ap);
+	va_end(ap);
+	g_err = buf;
+	return code;
+}
+
+#define PK_HIP(expr)                                                                                   \
+	do {                                                                                               \
+		hipError_t e_ = (expr);                                                                        \
+		if (e_ != hipSuccess) return set_err(PK_EDEVICE, "%s failed: %s", #expr, hipGetErrorString(e_)); \
+	} while (0)
+
+#define PK_TRY(expr)              \
+	do {                          \
+		int st_ = (expr);         \
+		if (st_ != PK_OK) return st_; \
+	} while (0)
+
+// ---------------------------------------------------------------------------
+// context
+// ---------------------------------------------------------------------------
+struct pk_ctx {
+	int device = 0;
+	int precision = PK_FP64;
+	hipStream_t stream = nullptr;
+	std::mutex mu;
+	void *dmem = nullptr;
+	size_t dcap = 0;
+	void *hmem = nullptr;  // pinned staging
+	size_t hcap = 0;
+	hipEvent_t ev[4] = {};
+};
+
+static int ensure_dev(pk_ctx *ctx, size_t bytes)
+{
+	if (bytes <= ctx->dcap) return PK_OK;
+	if (ctx->dmem) PK_HIP(hipFree(ctx->dmem));
+	ctx->dmem = nullptr;
+	ctx->dcap = 0;
+	size_t cap = std::max(bytes + bytes / 4, (size_t)1 << 20);
+	if (hipMalloc(&ctx->dmem, cap) != hipSuccess) {
+		(void)hipGetLastError();
+		return set_err(PK_ENOMEM, "device allocation of %zu bytes failed", cap);
+	}
+	ctx->dcap = cap;
+	return PK_OK;
+}
+
+static int ensure_host(pk_ctx *ctx, size_t bytes)
+{
+	if (bytes <= ctx->hcap) return PK_OK;
+	if (ctx->hmem) PK_HIP(hipHostFree(ctx->hmem));
+	ctx->hmem = nullptr;
+	ctx->hcap = 0;
+	size_t cap = std::max(bytes + bytes / 4, (size_t)1 << 20);
+	if (hipHostMalloc(&ctx->hmem, cap, hipHostMallocDefault) != hipSuccess) {
+		(void)hipGetLastError();
+		return set_err(PK_ENOMEM, "pinned host allocation of %zu bytes failed", cap);
+	}
+	ctx->hcap = cap;
+	return PK_OK;
+}
+
+// Bump layout of one call's device memory: the first `in_bytes` mirror the
+// pinned staging buffer and are uploaded in one copy.
+struct Layout {
+	size_t off = 0;
+	size_t take(size_t bytes)
+	{
+		size_t o = off;
+		off += (bytes + 255) & ~(size_t)255;
+		return o;
+	}
+};
+
+extern "C" {
+
+const char *pk_version(void) { return "1.9.3"; }
+
+int pk_init(void)
+{
+	int n = 0;
+	hipError_t e = hipGetDeviceCount(&n);
+	if (e != hipSuccess) return set_err(PK_EDEVICE, "hipGetDeviceCount failed: %s", hipGetErrorString(e));
+	return PK_OK;
+}
+
+void pk_finish(void) {}
+
+const char *pk_last_error(void) { return g_err.c_str(); }
+
+int pk_device_count(int *count)
+{
+	if (!count) return set_err(PK_EINVAL, "count is NULL");
+	int n = 0;
+	hipError_t e = hipGetDeviceCount(&n);
+	if (e != hipSuccess) {
+		*count = 0;
+		return set_err(PK_EDEVICE, "hipGetDeviceCount failed: %s", hipGetErrorString(e));
+	}
+	*count = n;
+	return PK_OK;
+}
+
+int pk_ctx_create(int device, int precision, pk_ctx **ctx_out)
+{
+	if (!ctx_out) return set_err(PK_EINVAL, "ctx_out is NULL");
+	if (precision != PK_FP64 && precision != PK_FP32) return set_err(PK_EINVAL, "unknown precision %d", precision);
+	int n = 0;
+	PK_HIP(hipGetDeviceCount(&n));
+	if (device < 0 || device >= n) return set_err(PK_EINVAL, "device %d out of range (%d devices)", device, n);
+	PK_HIP(hipSetDevice(device));
+	pk_ctx *ctx = new pk_ctx();
+	ctx->device = device;
+	ctx->precision = precision;
+	if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) {
+		delete ctx;
+		return set_err(PK_EDEVICE, "hipStreamCreate failed");
+	}
+	for (auto &e : ctx->ev) PK_HIP(hipEventCreate(&e));
+	*ctx_out = ctx;
+	return PK_OK;
+}
+
+int pk_ctx_destroy(pk_ctx *ctx)
+{
+	if (!ctx) return PK_OK;
+	(void)hipSetDevice(ctx->device);
+	(void)hipStreamSynchronize(ctx->stream);
+	for (auto &e : ctx->ev)
+		if (e) (void)hipEventDestroy(e);
+	if (ctx->dmem) (void)hipFree(ctx->dmem);
+	if (ctx->hmem) (void)hipHostFree(ctx->hmem);
+	(void)hipStreamDestroy(ctx->stream);
+	delete ctx;
+	return PK_OK;
+}
+
+int pk_ctx_precision(const pk_ctx *ctx) { return ctx ? ctx->precision : -1; }
+
+int pk_component_defaults(int kind, pk_component *c)
+{
+	if (!c) return set_err(PK_EINVAL, "component is NULL");
+	if (kind < PK_PROFILE_NULL || kind > PK_PROFILE_PSF) return set_err(PK_EINVAL, "Unknown profile kind %d", kind);
+	std::memset(c, 0, sizeof(*c));
+	c->kind = kind;
+	// RadialProfile constructor (src/radial.cpp:724-737)
+	c->mag = 15;
+	c->axrat = 1;
+	c->acc = 0.1;
+	c->rscale_switch = 1;
+	c->resolution = 9;
+	c->max_recursions = 2;
+	c->adjust = 1;
+	switch (kind) {
+	case PK_PROFILE_SERSIC: c->p[PK_SERSIC_RE] = 1; c->p[PK_SERSIC_NSER] = 1; break;
+	case PK_PROFILE_MOFFAT: c->p[PK_MOFFAT_FWHM] = 3; c->p[PK_MOFFAT_CON] = 2; break;
+	case PK_PROFILE_FERRER:
+		c->p[PK_FERRER_ROUT] = 3; c->p[PK_FERRER_A] = 1; c->p[PK_FERRER_B] = 1;
+		c->acc = 1;  // src/ferrer.cpp:120
+		break;
+	case PK_PROFILE_CORESERSIC:
+		c->p[PK_CORESERSIC_RE] = 1; c->p[PK_CORESERSIC_RB] = 1; c->p[PK_CORESERSIC_NSER] = 4;
+		c->p[PK_CORESERSIC_A] = 1; c->p[PK_CORESERSIC_B] = 1;
+		break;
+	case PK_PROFILE_KING: c->p[PK_KING_RC] = 1; c->p[PK_KING_RT] = 3; c->p[PK_KING_A] = 2; break;
+	case PK_PROFILE_BROKENEXP:
+		c->p[PK_BROKENEXP_H1] = 1; c->p[PK_BROKENEXP_H2] = 1; c->p[PK_BROKENEXP_RB] = 1; c->p[PK_BROKENEXP_A] = 1;
+		break;
+	case PK_PROFILE_PSF: c->mag = 0; break;  // src/psf.cpp:147-156
+	default: break;
+	}
+	return PK_OK;
+}
+
+int pk_model_defaults(uint32_t width, uint32_t height, pk_model *m)
+{
+	if (!m) return set_err(PK_EINVAL, "model is NULL");
+	std::memset(m, 0, sizeof(*m));
+	// Model constructor (src/model.cpp:54-73)
+	m->width = width;
+	m->height = height;
+	m->scale_x = m->scale_y = 1;
+	m->finesampling = 1;
+	m->crop = 1;
+	m->return_finesampled = 1;
+	m->adjust_mask = 1;
+	m->psf_scale_x = m->psf_scale_y = 1;
+	m->convolver = PK_CONV_BRUTE;
+	return PK_OK;
+}
+
+}  // extern "C"
+
+// ---------------------------------------------------------------------------
+// validation and geometry (src/model.cpp:152-219)
+// ---------------------------------------------------------------------------
+namespace {
+
+struct Analysis {
+	uint32_t dw = 0, dh = 0, padx = 0, pady = 0;
+	bool conv_required = false, mask_needs_psf_padding = false, mask_needs_convolution = false,
+	     mask_needs_adjustment = false;
+};
+
+bool has_mask(const pk_model *m) { return m->mask && m->mask_width > 0 && m->mask_height > 0; }
+bool has_psf(const pk_model *m) { return m->psf && m->psf_width > 0 && m->psf_height > 0; }
+
+int validate_component(const pk_component &c, bool psf_present)
+{
+	switch (c.kind) {
+	case PK_PROFILE_NULL:
+	case PK_PROFILE_SKY: return PK_OK;
+	case PK_PROFILE_PSF:
+		if (!psf_present) return set_err(PK_EINVAL, "No psf present in the model, cannot produce a psf profile");
+		return PK_OK;
+	case PK_PROFILE_SERSIC:
+	case PK_PROFILE_MOFFAT:
+	case PK_PROFILE_FERRER:
+	case PK_PROFILE_CORESERSIC:
+	case PK_PROFILE_KING:
+	case PK_PROFILE_BROKENEXP: break;
+	default: return set_err(PK_EINVAL, "Unknown profile kind %d", c.kind);
+	}
+	// RadialProfile::validate (src/radial.cpp:204-214)
+	if (c.axrat <= 0) return set_err(PK_EINVAL, "axrat <= 0, must have axrat > 0");
+	if (c.axrat > 1) return set_err(PK_EINVAL, "axrat > 1, must have axrat <= 1");
+	if (c.box <= -2) return set_err(PK_EINVAL, "box <= -2, must have box > -2");
+	const double *p = c.p;
+	switch (c.kind) {
+	case PK_PROFILE_SERSIC:  // src/sersic.cpp:207-218
+		if (p[PK_SERSIC_RE] <= 0) return set_err(PK_EINVAL, "re <= 0, must have re > 0");
+		if (p[PK_SERSIC_NSER] <= 0) return set_err(PK_EINVAL, "nser <= 0, must have nser > 0");
+		break;
+	case PK_PROFILE_MOFFAT:  // src/moffat.cpp:60-70
+		if (p[PK_MOFFAT_FWHM] <= 0) return set_err(PK_EINVAL, "fwhm <= 0, must have fwhm > 0");
+		if (p[PK_MOFFAT_CON] < 0) return set_err(PK_EINVAL, "con < 0, must have con >= 0");
+		break;
+	case PK_PROFILE_FERRER:  // src/ferrer.cpp:63-78
+		if (p[PK_FERRER_ROUT] <= 0) return set_err(PK_EINVAL, "rout <= 0, must have rout >= 0");
+		if (p[PK_FERRER_A] < 0) return set_err(PK_EINVAL, "a < 0, must have a >= 0");
+		if (p[PK_FERRER_B] > 2) return set_err(PK_EINVAL, "b > 2, must have b <= 2");
+		break;
+	case PK_PROFILE_KING:  // src/king.cpp:61-76
+		if (p[PK_KING_RC] <= 0) return set_err(PK_EINVAL, "rc <= 0, must have rc > 0");
+		if (p[PK_KING_RT] <= 0) return set_err(PK_EINVAL, "rt <= 0, must have rt > 0");
+		if (p[PK_KING_A] < 0) return set_err(PK_EINVAL, "a < 0, must have a >=0");
+		break;
+	case PK_PROFILE_BROKENEXP:  // src/brokenexponential.cpp:87-101
+		if (p[PK_BROKENEXP_H1] <= 0) return set_err(PK_EINVAL, "h1 <= 0, must have h1 > 0");
+		if (p[PK_BROKENEXP_H2] <= 0) return set_err(PK_EINVAL, "h2 <= 0, must have h2 > 0");
+		if (p[PK_BROKENEXP_RB] <= 0) return set_err(PK_EINVAL, "rb <= 0, must have rb > 0");
+		break;
+	case PK_PROFILE_CORESERSIC:  // src/coresersic.cpp:62-80
+		if (p[PK_CORESERSIC_RE] <= 0) return set_err(PK_EINVAL, "re <= 0, must have re > 0");
+		if (p[PK_CORESERSIC_RB] <= 0) return set_err(PK_EINVAL, "rb <= 0, must have rb > 0");
+		if (p[PK_CORESERSIC_NSER] <= 0) return set_err(PK_EINVAL, "nser <= 0, must have nser > 0");
+		if (p[PK_CORESERSIC_A] <= 0) return set_err(PK_EINVAL, "a <= 0, must have a > 0");
+		if (p[PK_CORESERSIC_B] > 1.999) return set_err(PK_EINVAL, "b > 1.999, must have b < 1.999");
+		break;
+	default: break;
+	}
+	return PK_OK;
+}
+
+// Mask::bounding_box (include/profit/image.h:451-483)
+void mask_bbox(const uint8_t *m, uint32_t w, uint32_t h, uint32_t b[4])
+{
+	uint32_t lx = w, ly = h, ux = 0, uy = 0;
+	bool any = false;
+	for (uint32_t j = 0; j < h; j++)
+		for (uint32_t i = 0; i < w; i++) {
+			if (!m[i + (size_t)j * w]) continue;
+			lx = std::min(lx, i);
+			ly = std::min(ly, j);
+			ux = std::max(ux, i + 1);
+			uy = std::max(uy, j + 1);
+			any = true;
+		}
+	if (!any) lx = ly = ux = uy = 0;
+	b[0] = lx;
+	b[1] = ly;
+	b[2] = ux;
+	b[3] = uy;
+}
+
+int analyze(const pk_model *m, Analysis &a)
+{
+	// Model::analyze_inputs (src/model.cpp:186-219)
+	if (!(m->width > 0 && m->height > 0)) return set_err(PK_EINVAL, "Model's requested dimensions are 0");
+	if (m->scale_x <= 0) return set_err(PK_EINVAL, "Model's scale_x cannot be negative or zero");
+	if (m->scale_y <= 0) return set_err(PK_EINVAL, "Model's scale_y cannot be negative or zero");
+	if (m->finesampling == 0) return set_err(PK_EINVAL, "finesampling must be > 0");
+	if (m->n_components && !m->components) return set_err(PK_EINVAL, "components is NULL");
+	if (has_mask(m) && m->adjust_mask && (m->mask_width != m->width || m->mask_height != m->height))
+		return set_err(PK_EINVAL, "Mask dimensions != model dimensions");
+	a.conv_required = false;
+	for (uint32_t c = 0; c < m->n_components; c++)
+		if (m->components[c].convolve) a.conv_required = true;
+	if (a.conv_required && !has_psf(m))
+		return set_err(PK_EINVAL, "No psf provided but profile(s) requested convolution");
+	if (a.conv_required && m->convolver != PK_CONV_BRUTE_OLD && m->convolver != PK_CONV_BRUTE &&
+	    m->convolver != PK_CONV_OPENCL && m->convolver != PK_CONV_FFT)
+		return set_err(PK_EINVAL, "Unsupported convolver type: %d", m->convolver);
+	for (uint32_t c = 0; c < m->n_components; c++) PK_TRY(validate_component(m->components[c], has_psf(m)));
+	// analyze_expansion_requirements (src/model.cpp:152-184)
+	const uint32_t fs = m->finesampling;
+	bool need_pad = a.conv_required;
+	if (has_mask(m) && !m->adjust_mask) {
+		need_pad = m->mask_width > m->width * fs && m->mask_height > m->height * fs;
+	} else if (has_mask(m) && a.conv_required) {
+		uint32_t b[4];
+		mask_bbox(m->mask, m->mask_width, m->mask_height, b);
+		const uint32_t lx = b[0] * fs, ly = b[1] * fs, upx = m->mask_width * fs - b[2] * fs,
+		               upy = m->mask_height * fs - b[3] * fs;
+		const uint32_t nx = m->psf_width / 2, ny = m->psf_height / 2;
+		need_pad = !(lx >= nx && ly >= ny) || !(upx >= nx && upy >= ny);
+		a.mask_needs_convolution = true;
+		a.mask_needs_psf_padding = need_pad;
+	}
+	a.padx = need_pad ? m->psf_width / 2 : 0;
+	a.pady = need_pad ? m->psf_height / 2 : 0;
+	a.mask_needs_adjustment = has_mask(m) && (a.mask_needs_convolution || a.mask_needs_psf_padding || fs > 1);
+	a.dw = m->width * fs + 2 * a.padx;
+	a.dh = m->height * fs + 2 * a.pady;
+	return PK_OK;
+}
+
+// Mask::expand_by (src/image.cpp:112-162) as a separable dilation by a
+// (2px+1) x (2py+1) box.
+std::vector<uint8_t> mask_expand(const std::vector<uint8_t> &m, uint32_t w, uint32_t h, uint32_t px, uint32_t py)
+{
+	std::vector<uint8_t> rows(m.size()), out(m.size());
+	for (uint32_t j = 0; j < h; j++) {
+		const uint8_t *r = &m[(size_t)j * w];
+		int last = -1000000000;  // last set column seen
+		std::vector<int> next(w + 1, 1000000000);
+		for (int i = (int)w - 1; i >= 0; i--) next[i] = r[i] ? i : next[i + 1];
+		for (uint32_t i = 0; i < w; i++) {
+			if (r[i]) last = (int)i;
+			bool hit = ((int)i - last) <= (int)px || (next[i] - (int)i) <= (int)px;
+			rows[(size_t)j * w + i] = hit;
+		}
+	}
+	for (uint32_t i = 0; i < w; i++) {
+		int last = -1000000000;
+		std::vector<int> next(h + 1, 1000000000);
+		for (int j = (int)h - 1; j >= 0; j--) next[j] = rows[(size_t)j * w + i] ? j : next[j + 1];
+		for (uint32_t j = 0; j < h; j++) {
+			if (rows[(size_t)j * w + i]) last = (int)j;
+			bool hit = ((int)j - last) <= (int)py || (next[j] - (int)j) <= (int)py;
+			out[(size_t)j * w + i] = hit || m[(size_t)j * w + i];
+		}
+	}
+	return out;
+}
+
+// FFTConvolver::offset_after_convolution (src/convolve.cpp:270-281)
+void fft_offset(uint32_t sw, uint32_t sh, uint32_t kw, uint32_t kh, int &ox, int &oy)
+{
+	ox = (int)(sw / 2);
+	oy = (int)(sh / 2);
+	if (sw % 2 == 0 || kw % 2 == 0) ox -= 1;
+	if (sh % 2 == 0 || kh % 2 == 0) oy -= 1;
+}
+
+// Convolver::padding (src/convolve.cpp:50-53, 263-268); max() of
+// include/profit/image.h:204-207 takes the x component for both axes.
+void conv_padding(int type, uint32_t sw, uint32_t sh, uint32_t kw, uint32_t kh, int p[4])
+{
+	if (type != PK_CONV_FFT) {
+		p[0] = p[1] = p[2] = p[3] = 0;
+		return;
+	}
+	const uint32_t mx = std::max(sw, kw), ex = 2 * mx, ey = 2 * mx;
+	int ox, oy;
+	fft_offset(sw, sh, kw, kh, ox, oy);
+	p[0] = ox;
+	p[1] = oy;
+	p[2] = (int)(ex - ex / 2) - ox;
+	p[3] = (int)(ey - ey / 2) - oy;
+}
+
+// Geometry of Convolver::convolve on a (sw x sh) source for a given type:
+// dims of the result, the same-size-frame shift of its pixels, the position
+// of the source inside the result, and (FFT) the offset reported back.
+struct ConvGeom {
+	uint32_t cw, ch;
+	int csx, csy;  // result(X,Y) = same(X + csx, Y + csy)
+	int mx, my;    // source pixel (X - mx, Y - my) is the mask pixel of result (X,Y)
+	int offx, offy;
+};
+
+int conv_geometry(int type, uint32_t sw, uint32_t sh, uint32_t kw, uint32_t kh, bool crop, ConvGeom &g)
+{
+	g = ConvGeom{sw, sh, 0, 0, 0, 0, 0, 0};
+	if (!(sw >= kw && sh >= kh)) {
+		// src/convolve.cpp:65-75: extend, convolve, crop the extension back
+		const uint32_t mxd = std::max(sw, kw);
+		if (mxd < sh) return set_err(PK_EINVAL, "new_height should be >= height");
+		if (type != PK_CONV_FFT) return PK_OK;  // same-size result, cropped back to sw x sh
+		const uint32_t nw = mxd, nh = mxd;
+		PK_TRY(conv_geometry(type, nw, nh, kw, kh, crop, g));
+		// the crop keeps the top-left (result - (new - src)) block
+		g.cw -= nw - sw;
+		g.ch -= nh - sh;
+		// same-size frame of the extended source equals the source's own frame
+		return PK_OK;
+	}
+	if (type != PK_CONV_FFT) return PK_OK;
+	int ox, oy;
+	fft_offset(sw, sh, kw, kh, ox, oy);
+	if (crop) return PK_OK;  // crop(src_dims, ext_offset) is the same-size convolution
+	const uint32_t mxd = std::max(sw, kw);
+	g.cw = 2 * mxd;
+	g.ch = 2 * mxd;
+	g.csx = -ox;
+	g.csy = -oy;
+	g.mx = ox;
+	g.my = oy;
+	g.offx = ox;
+	g.offy = oy;
+	return PK_OK;
+}
+
+template <typename T>
+T *dptr(pk_ctx *ctx, size_t off) { return reinterpret_cast<T *>(static_cast<char *>(ctx->dmem) + off); }
+
+template <typename T>
+T *hptr(pk_ctx *ctx, size_t off) { return reinterpret_cast<T *>(static_cast<char *>(ctx->hmem) + off); }
+
+uint32_t cdiv(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
+
+size_t conv_lds_bytes(uint32_t kw, uint32_t kh, size_t elem)
+{
+	return (size_t)(CT_W + kw - 1) * (CT_H + kh - 1) * elem;
+}
+
+constexpr size_t LDS_LIMIT = 64 * 1024;
+
+template <typename T>
+int launch_conv(pk_ctx *ctx, const PkJob *djob, uint32_t ntiles, const T *src, const T *krn, const uint8_t *masks,
+                T *dst, uint32_t kw, uint32_t kh)
+{
+	size_t lds = conv_lds_bytes(kw, kh, sizeof(T));
+	if (lds > LDS_LIMIT)
+		return set_err(PK_EINVAL, "PSF of %ux%u exceeds the direct convolution tile (%zu bytes of LDS)", kw, kh, lds);
+	hipLaunchKernelGGL(conv_kernel<T>, dim3(ntiles), dim3(256), lds, ctx->stream, djob, 1u, src, krn, masks, dst);
+	PK_HIP(hipGetLastError());
+	return PK_OK;
+}
+
+// Core of Model::evaluate on the device for one model.
+template <typename TI>
+int evaluate_impl(pk_ctx *ctx, const pk_model *m, const Analysis &a, double *out, size_t out_capacity,
+                  pk_result *res)
+{
+	const bool fp32 = ctx->precision == PK_FP32;
+	const uint32_t fs = m->finesampling;
+	const double sx = m->scale_x / fs, sy = m->scale_y / fs;
+
+	// normalised PSF (Model::set_psf -> Image::normalize, src/image.cpp:208-221)
+	std::vector<double> psf;
+	if (has_psf(m)) {
+		size_t pn = (size_t)m->psf_width * m->psf_height;
+		psf.assign(m->psf, m->psf + pn);
+		double sum = 0;
+		for (double v : psf) sum += v;
+		if (sum > 0)
+			for (double &v : psf) v /= sum;
+	}
+
+	// calculation mask in the drawing frame (src/model.cpp:230-246, 265-285)
+	std::vector<uint8_t> dmask;
+	bool use_mask = false, adjusted = false;
+	if (has_mask(m)) {
+		if (m->adjust_mask && a.mask_needs_adjustment) {
+			uint32_t w = m->mask_width, h = m->mask_height;
+			std::vector<uint8_t> cur(m->mask, m->mask + (size_t)w * h);
+			for (auto &v : cur) v = v ? 1 : 0;
+			if (fs > 1) {
+				std::vector<uint8_t> up((size_t)w * fs * h * fs);
+				for (uint32_t r = 0; r < h * fs; r++)
+					for (uint32_t c = 0; c < w * fs; c++) up[c + (size_t)r * w * fs] = cur[c / fs + (size_t)(r / fs) * w];
+				cur.swap(up);
+				w *= fs;
+				h *= fs;
+			}
+			if (a.mask_needs_psf_padding) {
+				std::vector<uint8_t> ext((size_t)a.dw * a.dh, 0);
+				for (uint32_t j = 0; j < h; j++)
+					for (uint32_t i = 0; i < w; i++) ext[(i + a.padx) + (size_t)(j + a.pady) * a.dw] = cur[i + (size_t)j * w];
+				cur.swap(ext);
+				w = a.dw;
+				h = a.dh;
+			}
+			if (a.mask_needs_convolution) cur = mask_expand(cur, w, h, m->psf_width / 2, m->psf_height / 2);
+			dmask.swap(cur);
+			use_mask = adjusted = true;
+		} else {
+			if (!m->adjust_mask && (m->mask_width != a.dw || m->mask_height != a.dh))
+				return set_err(PK_EINVAL, "Mask dimensions != drawing dimensions: [%u, %u] != [%u, %u]",
+				               m->mask_width, m->mask_height, a.dw, a.dh);
+			dmask.assign(m->mask, m->mask + (size_t)a.dw * a.dh);
+			for (auto &v : dmask) v = v ? 1 : 0;
+			use_mask = true;
+		}
+	}
+
+	// components: radial + sky go to the coarse/refine kernels, psf profiles
+	// become rebinning items, null profiles do nothing
+	std::vector<PkComp> comps;
+	std::vector<PkPsfItem> items;
+	bool any_nc = false;
+	for (uint32_t c = 0; c < m->n_components; c++) {
+		const pk_component &pc = m->components[c];
+		if (pc.kind == PK_PROFILE_NULL) continue;
+		if (pc.kind == PK_PROFILE_PSF) {
+			// PsfProfile::evaluate geometry (src/psf.cpp:58-103), including
+			// its mixed x/y use of the offsets and dimensions
+			PkPsfItem it{};
+			it.to_conv = pc.convolve ? 1 : 0;
+			const double psx = m->psf_scale_x, psy = m->psf_scale_y;
+			const uint32_t pw = m->psf_width, ph = m->psf_height;
+			const double origin_x = pc.xcen + a.padx * sx - pw * psx / 2;
+			const double end_x = pc.xcen + a.pady * sy + pw * psx / 2;
+			const double origin_y = pc.ycen + a.padx * sx - ph * psy / 2;
+			const double end_y = pc.ycen + a.pady * sy + ph * psy / 2;
+			auto bind = [](double v, uint32_t mx) -> uint32_t {
+				int iv = (int)std::floor(v);
+				if (iv < 0) return 0;
+				return std::min((uint32_t)iv, mx);
+			};
+			it.x0 = bind(origin_x / sx, a.dw - 1);
+			it.y0 = bind(origin_y / sy, a.dh - 1);
+			it.x1 = bind(end_x / sx, a.dw - 1);
+			it.y1 = bind(end_y / sy, a.dh - 1);
+			it.origin_x = origin_x;
+			it.origin_y = origin_y;
+			it.scale = std::pow(10, -0.4 * (pc.mag - m->magzero));
+			it.psx = psx;
+			it.psy = psy;
+			it.pw = pw;
+			it.ph = ph;
+			if (!pc.convolve) any_nc = true;
+			items.push_back(it);
+			continue;
+		}
+		SetupIn in{};
+		in.c = pc;
+		in.magzero = m->magzero;
+		in.sx = sx;
+		in.sy = sy;
+		in.img_sx = m->scale_x;
+		in.img_sy = m->scale_y;
+		in.pad_x = a.padx;
+		in.pad_y = a.pady;
+		in.fp32_mode = fp32 ? 1 : 0;
+		in.fs = fs;
+		PkComp pcmp;
+		setup_component(in, pcmp);
+		if (!pc.convolve) any_nc = true;
+		comps.push_back(pcmp);
+	}
+	if (comps.size() > 32) return set_err(PK_EINVAL, "at most 32 radial/sky profiles per model are supported");
+
+	// convolution geometry
+	ConvGeom g{a.dw, a.dh, 0, 0, 0, 0, 0, 0};
+	if (a.conv_required)
+		PK_TRY(conv_geometry(m->convolver, a.dw, a.dh, m->psf_width, m->psf_height, m->crop != 0, g));
+
+	// composition: crop padding (src/model.cpp:289-306), downsample, mask
+	PkJob jb{};
+	jb.dw = a.dw;
+	jb.dh = a.dh;
+	jb.comp_off = 0;
+	jb.ncomp = (uint32_t)comps.size();
+	jb.img_off = 0;
+	jb.sx = sx;
+	jb.sy = sy;
+	jb.tile_off = 0;
+	jb.tiles_x = cdiv(a.dw, 64);
+	jb.has_conv = a.conv_required ? 1 : 0;
+	jb.has_nc = (any_nc || !a.conv_required) ? 1 : 0;
+	jb.kw = m->psf_width;
+	jb.kh = m->psf_height;
+	jb.ctile_off = 0;
+	jb.cw = a.conv_required ? g.cw : a.dw;
+	jb.ch = a.conv_required ? g.ch : a.dh;
+	jb.ctiles_x = cdiv(jb.cw, CT_W);
+	jb.csx = g.csx;
+	jb.csy = g.csy;
+	jb.cmx = g.mx;
+	jb.cmy = g.my;
+	jb.mox = a.conv_required ? g.offx : 0;
+	jb.moy = a.conv_required ? g.offy : 0;
+	int offx = a.conv_required ? g.offx : 0, offy = a.conv_required ? g.offy : 0;
+	int cox = 0, coy = 0;
+	int crw = (int)jb.cw, crh = (int)jb.ch;
+	if (a.padx > 0 && a.pady > 0) {
+		cox = (int)a.padx;
+		coy = (int)a.pady;
+		crw = (int)a.dw - 2 * (int)a.padx;
+		crh = (int)a.dh - 2 * (int)a.pady;
+		if (!m->crop) {
+			int pa[4], pi[4];
+			const int ct = a.conv_required ? m->convolver : PK_CONV_BRUTE;
+			conv_padding(ct, a.dw, a.dh, m->psf_width, m->psf_height, pa);
+			conv_padding(ct, a.dw - 2 * a.padx, a.dh - 2 * a.pady, m->psf_width, m->psf_height, pi);
+			const int odx = pa[0] - pi[0], ody = pa[1] - pi[1], ddx = pa[2] - pi[2], ddy = pa[3] - pi[3];
+			crw = (int)jb.cw - 2 * (int)a.padx - (odx + ddx);
+			crh = (int)jb.ch - 2 * (int)a.pady - (ody + ddy);
+			cox += odx;
+			coy += ody;
+			offx -= odx;
+			offy -= ody;
+		}
+	}
+	if (crw <= 0 || crh <= 0) return set_err(PK_EINVAL, "empty image after cropping");
+	jb.cox = cox;
+	jb.coy = coy;
+	jb.crw = (uint32_t)crw;
+	jb.crh = (uint32_t)crh;
+	jb.fs = 1;
+	uint32_t ow = (uint32_t)crw, oh = (uint32_t)crh;
+	if (fs > 1 && !m->return_finesampled) {
+		jb.fs = fs;
+		ow = cdiv(ow, fs);
+		oh = cdiv(oh, fs);
+		offx /= (int)fs;
+		offy /= (int)fs;
+	}
+	jb.ow = ow;
+	jb.oh = oh;
+	jb.out_off = 0;
+	jb.otile_off = 0;
+	jb.otiles_x = cdiv(ow, 64);
+	jb.omask_off = NONE;
+	jb.omask_w = m->mask_width;
+	jb.omask_h = m->mask_height;
+	jb.omask_f = 1;
+	if (adjusted) jb.omask_f = (ow == m->mask_width) ? 1 : fs;
+	if ((size_t)ow * oh > out_capacity)
+		return set_err(PK_ERANGE, "output buffer holds %zu doubles, %zu needed", out_capacity, (size_t)ow * oh);
+
+	// ---- memory layout: inputs (uploaded) then workspace ----
+	const size_t npix = (size_t)a.dw * a.dh;
+	const size_t ncpix = (size_t)jb.cw * jb.ch;
+	Layout L;
+	const size_t o_job = L.take(sizeof(PkJob));
+	const size_t o_cnt = L.take(sizeof(PkCounters));
+	const size_t o_comp = L.take(sizeof(PkComp) * std::max<size_t>(1, comps.size()));
+	const size_t o_items = L.take(sizeof(PkPsfItem) * std::max<size_t>(1, items.size()));
+	const size_t o_psf64 = L.take(sizeof(double) * std::max<size_t>(1, psf.size()));
+	const size_t o_psfT = L.take(sizeof(TI) * std::max<size_t>(1, psf.size()));
+	const size_t o_mask = L.take(use_mask ? npix : 1);
+	const size_t o_omask = L.take(adjusted ? (size_t)m->mask_width * m->mask_height : 1);
+	const size_t in_bytes = L.off;
+	const size_t o_nc = L.take(sizeof(TI) * npix);
+	const size_t o_c = L.take(sizeof(TI) * (a.conv_required ? npix : 1));
+	const size_t o_conv = L.take(sizeof(TI) * (a.conv_required ? ncpix : 1));
+	const size_t o_ent = L.take(sizeof(RefineEntry) * npix);
+	const size_t o_out = L.take(sizeof(double) * (size_t)ow * oh);
+	PK_TRY(ensure_dev(ctx, L.off));
+	PK_TRY(ensure_host(ctx, std::max(in_bytes, sizeof(double) * (size_t)ow * oh + 256 + sizeof(PkCounters))));
+
+	jb.mask_off = use_mask ? 0 : NONE;
+	jb.psf_off = 0;
+	jb.conv_off = 0;
+	if (adjusted) jb.omask_off = 0;
+	for (auto &it : items) it.psf_off = 0;
+
+	// stage inputs
+	std::memset(ctx->hmem, 0, in_bytes);
+	*hptr<PkJob>(ctx, o_job) = jb;
+	if (!comps.empty()) std::memcpy(hptr<void>(ctx, o_comp), comps.data(), sizeof(PkComp) * comps.size());
+	if (!items.empty()) std::memcpy(hptr<void>(ctx, o_items), items.data(), sizeof(PkPsfItem) * items.size());
+	if (!psf.empty()) {
+		std::memcpy(hptr<void>(ctx, o_psf64), psf.data(), sizeof(double) * psf.size());
+		TI *pt = hptr<TI>(ctx, o_psfT);
+		for (size_t t = 0; t < psf.size(); t++) pt[t] = (TI)psf[t];
+	}
+	if (use_mask) std::memcpy(hptr<void>(ctx, o_mask), dmask.data(), npix);
+	if (adjusted) {
+		uint8_t *om = hptr<uint8_t>(ctx, o_omask);
+		for (size_t t = 0; t < (size_t)m->mask_width * m->mask_height; t++) om[t] = m->mask[t] ? 1 : 0;
+	}
+
+	hipStream_t s = ctx->stream;
+	PK_HIP(hipMemcpyAsync(ctx->dmem, ctx->hmem, in_bytes, hipMemcpyHostToDevice, s));
+	PK_HIP(hipEventRecord(ctx->ev[0], s));
+
+	const PkJob *djob = dptr<PkJob>(ctx, o_job);
+	PkCounters *dcnt = dptr<PkCounters>(ctx, o_cnt);
+	TI *img_nc = dptr<TI>(ctx, o_nc);
+	TI *img_c = dptr<TI>(ctx, o_c);
+	const uint8_t *dm = dptr<uint8_t>(ctx, o_mask);
+	RefineEntry *dent = dptr<RefineEntry>(ctx, o_ent);
+	const PkComp *dcomp = dptr<PkComp>(ctx, o_comp);
+
+	const uint32_t ntiles = jb.tiles_x * cdiv(a.dh, 4);
+	if (fp32) {
+		hipLaunchKernelGGL((coarse_kernel<TI, true>), dim3(ntiles), dim3(256), 0, s, djob, 1u, dcomp, dm, img_nc, img_c,
+		                   dent, dcnt);
+	} else {
+		hipLaunchKernelGGL((coarse_kernel<TI, false>), dim3(ntiles), dim3(256), 0, s, djob, 1u, dcomp, dm, img_nc,
+		                   img_c, dent, dcnt);
+	}
+	PK_HIP(hipGetLastError());
+	const uint32_t nref_blocks = (uint32_t)std::min<size_t>(npix, 2048);
+	if (!comps.empty()) {
+		if (fp32) {
+			hipLaunchKernelGGL((refine_kernel<TI, true>), dim3(nref_blocks), dim3(64), 0, s, djob, dcomp, dent, dcnt,
+			                   img_nc, img_c);
+		} else {
+			hipLaunchKernelGGL((refine_kernel<TI, false>), dim3(nref_blocks), dim3(64), 0, s, djob, dcomp, dent, dcnt,
+			                   img_nc, img_c);
+		}
+		PK_HIP(hipGetLastError());
+	}
+	for (size_t k = 0; k < items.size(); k++) {
+		const auto &it = items[k];
+		if (it.x1 < it.x0 || it.y1 < it.y0) continue;
+		const uint32_t n = (it.x1 - it.x0 + 1) * (it.y1 - it.y0 + 1);
+		hipLaunchKernelGGL(psf_kernel<TI>, dim3(cdiv(n, 256), 1), dim3(256), 0, s,
+		                   dptr<PkPsfItem>(ctx, o_items + k * sizeof(PkPsfItem)), djob, dptr<double>(ctx, o_psf64),
+		                   img_nc, img_c);
+		PK_HIP(hipGetLastError());
+	}
+	PK_HIP(hipEventRecord(ctx->ev[1], s));
+	if (a.conv_required) {
+		const uint32_t nct = jb.ctiles_x * cdiv(jb.ch, CT_H);
+		PK_TRY(launch_conv<TI>(ctx, djob, nct, img_c, dptr<TI>(ctx, o_psfT), dm, dptr<TI>(ctx, o_conv), jb.kw, jb.kh));
+	}
+	PK_HIP(hipEventRecord(ctx->ev[2], s));
+	const uint32_t notiles = jb.otiles_x * cdiv(oh, 4);
+	hipLaunchKernelGGL((compose_kernel<TI, double>), dim3(notiles), dim3(256), 0, s, djob, 1u, img_nc,
+	                   dptr<TI>(ctx, o_conv), dptr<uint8_t>(ctx, o_omask), dptr<double>(ctx, o_out));
+	PK_HIP(hipGetLastError());
+	PK_HIP(hipEventRecord(ctx->ev[3], s));
+
+	// download result + counters
+	const size_t out_bytes = sizeof(double) * (size_t)ow * oh;
+	const size_t ho_cnt = (out_bytes + 255) & ~(size_t)255;
+	PK_HIP(hipMemcpyAsync(ctx->hmem, dptr<void>(ctx, o_out), out_bytes, hipMemcpyDeviceToHost, s));
+	PK_HIP(hipMemcpyAsync(hptr<void>(ctx, ho_cnt), dcnt, sizeof(PkCounters), hipMemcpyDeviceToHost, s));
+	PK_HIP(hipStreamSynchronize(s));
+	const PkCounters cnt = *hptr<PkCounters>(ctx, ho_cnt);
+	if (cnt.overflow)
+		return set_err(PK_ERANGE, "sub-pixel refinement needed more than %d pending cells for one pixel", RQ_CAP);
+	std::memcpy(out, ctx->hmem, out_bytes);
+	if (res) {
+		res->width = ow;
+		res->height = oh;
+		res->offset_x = offx;
+		res->offset_y = offy;
+		res->n_refined_pixels = cnt.n_entries;
+		res->n_subcells = cnt.n_subcells;
+		float t01 = 0, t12 = 0, t23 = 0, t03 = 0;
+		(void)hipEventElapsedTime(&t01, ctx->ev[0], ctx->ev[1]);
+		(void)hipEventElapsedTime(&t12, ctx->ev[1], ctx->ev[2]);
+		(void)hipEventElapsedTime(&t23, ctx->ev[2], ctx->ev[3]);
+		(void)hipEventElapsedTime(&t03, ctx->ev[0], ctx->ev[3]);
+		res->ms_setup = 0;
+		res->ms_profiles = t01;
+		res->ms_convolution = t12;
+		res->ms_total = t03;
+	}
+	return PK_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pk_model_output_dims(const pk_model *m, uint32_t *width, uint32_t *height)
+{
+	if (!m) return set_err(PK_EINVAL, "model is NULL");
+	Analysis a;
+	PK_TRY(analyze(m, a));
+	if (width) *width = a.dw;
+	if (height) *height = a.dh;
+	return PK_OK;
+}
+
+int pk_evaluate(pk_ctx *ctx, const pk_model *m, double *out, size_t out_capacity, pk_result *res)
+{
+	if (!ctx) return set_err(PK_EINVAL, "context is NULL");
+	if (!m) return set_err(PK_EINVAL, "model is NULL");
+	if (!out && !m->dry_run) return set_err(PK_EINVAL, "output buffer is NULL");
+	std::lock_guard<std::mutex> lock(ctx->mu);
+	PK_HIP(hipSetDevice(ctx->device));
+	Analysis a;
+	PK_TRY(analyze(m, a));
+	if (m->dry_run) {
+		// src/model.cpp:263-267
+		const size_t n = (size_t)a.dw * a.dh;
+		if (n > out_capacity) return set_err(PK_ERANGE, "output buffer holds %zu doubles, %zu needed", out_capacity, n);
+		if (out) std::memset(out, 0, n * sizeof(double));
+		if (res) {
+			std::memset(res, 0, sizeof(*res));
+			res->width = a.dw;
+			res->height = a.dh;
+		}
+		return PK_OK;
+	}
+	if (ctx->precision == PK_FP32) return evaluate_impl<float>(ctx, m, a, out, out_capacity, res);
+	return evaluate_impl<double>(ctx, m, a, out, out_capacity, res);
+}
+
+int pk_convolver_padding(int type, uint32_t src_w, uint32_t src_h, uint32_t krn_w, uint32_t krn_h, int32_t *lo_x,
+                         int32_t *lo_y, int32_t *hi_x, int32_t *hi_y)
+{
+	int p[4];
+	conv_padding(type, src_w, src_h, krn_w, krn_h, p);
+	if (lo_x) *lo_x = p[0];
+	if (lo_y) *lo_y = p[1];
+	if (hi_x) *hi_x = p[2];
+	if (hi_y) *hi_y = p[3];
+	return PK_OK;
+}
+
+}  // extern "C"
+
+namespace {
+
+template <typename T>
+int convolve_impl(pk_ctx *ctx, int type, const double *src, uint32_t sw, uint32_t sh, const double *krn, uint32_t kw,
+                  uint32_t kh, const uint8_t *mask, int crop, double *out, size_t cap, uint32_t *out_w,
+                  uint32_t *out_h, int32_t *offset_x, int32_t *offset_y)
+{
+	ConvGeom g;
+	PK_TRY(conv_geometry(type, sw, sh, kw, kh, crop != 0, g));
+	if ((size_t)g.cw * g.ch > cap)
+		return set_err(PK_ERANGE, "output buffer holds %zu doubles, %zu needed", cap, (size_t)g.cw * g.ch);
+	PkJob jb{};
+	jb.dw = sw;
+	jb.dh = sh;
+	jb.kw = kw;
+	jb.kh = kh;
+	jb.cw = g.cw;
+	jb.ch = g.ch;
+	jb.csx = g.csx;
+	jb.csy = g.csy;
+	jb.cmx = g.mx;
+	jb.cmy = g.my;
+	jb.ctiles_x = cdiv(g.cw, CT_W);
+	jb.mask_off = mask ? 0 : NONE;
+	jb.has_conv = 1;
+	jb.has_nc = 0;
+	jb.cox = jb.coy = 0;
+	jb.crw = g.cw;
+	jb.crh = g.ch;
+	jb.fs = 1;
+	jb.ow = g.cw;
+	jb.oh = g.ch;
+	jb.otiles_x = cdiv(g.cw, 64);
+	jb.omask_off = NONE;
+	jb.omask_f = 1;
+	const size_t ns = (size_t)sw * sh, nk = (size_t)kw * kh, no = (size_t)g.cw * g.ch;
+	Layout L;
+	const size_t o_job = L.take(sizeof(PkJob));
+	const size_t o_src = L.take(sizeof(T) * ns);
+	const size_t o_krn = L.take(sizeof(T) * nk);
+	const size_t o_mask = L.take(mask ? ns : 1);
+	const size_t in_bytes = L.off;
+	const size_t o_conv = L.take(sizeof(T) * no);
+	const size_t o_out = L.take(sizeof(double) * no);
+	PK_TRY(ensure_dev(ctx, L.off));
+	PK_TRY(ensure_host(ctx, std::max(in_bytes, sizeof(double) * no)));
+	*hptr<PkJob>(ctx, o_job) = jb;
+	T *hs = hptr<T>(ctx, o_src);
+	for (size_t t = 0; t < ns; t++) hs[t] = (T)src[t];
+	T *hk = hptr<T>(ctx, o_krn);
+	for (size_t t = 0; t < nk; t++) hk[t] = (T)krn[t];
+	if (mask) {
+		uint8_t *hm = hptr<uint8_t>(ctx, o_mask);
+		for (size_t t = 0; t < ns; t++) hm[t] = mask[t] ? 1 : 0;
+	}
+	hipStream_t s = ctx->stream;
+	PK_HIP(hipMemcpyAsync(ctx->dmem, ctx->hmem, in_bytes, hipMemcpyHostToDevice, s));
+	const PkJob *djob = dptr<PkJob>(ctx, o_job);
+	const uint32_t nct = jb.ctiles_x * cdiv(g.ch, CT_H);
+	PK_TRY(launch_conv<T>(ctx, djob, nct, dptr<T>(ctx, o_src), dptr<T>(ctx, o_krn), dptr<uint8_t>(ctx, o_mask),
+	                      dptr<T>(ctx, o_conv), kw, kh));
+	hipLaunchKernelGGL((compose_kernel<T, double>), dim3(jb.otiles_x * cdiv(g.ch, 4)), dim3(256), 0, s, djob, 1u,
+	                   dptr<T>(ctx, o_conv), dptr<T>(ctx, o_conv), (const uint8_t *)nullptr,
+	                   dptr<double>(ctx, o_out));
+	PK_HIP(hipGetLastError());
+	PK_HIP(hipMemcpyAsync(ctx->hmem, dptr<void>(ctx, o_out), sizeof(double) * no, hipMemcpyDeviceToHost, s));
+	PK_HIP(hipStreamSynchronize(s));
+	std::memcpy(out, ctx->hmem, sizeof(double) * no);
+	if (out_w) *out_w = g.cw;
+	if (out_h) *out_h = g.ch;
+	if (offset_x) *offset_x = g.offx;
+	if (offset_y) *offset_y = g.offy;
+	return PK_OK;
+}
+
+}  // namespace
+
+extern "C" int pk_convolve(pk_ctx *ctx, int type, const double *src, uint32_t src_w, uint32_t src_h,
+                           const double *krn, uint32_t krn_w, uint32_t krn_h, const uint8_t *mask, int crop,
+                           double *out, size_t out_capacity, uint32_t *out_w, uint32_t *out_h, int32_t *offset_x,
+                           int32_t *offset_y)
+{
+	if (!ctx || !src || !krn || !out) return set_err(PK_EINVAL, "NULL argument");
+	if (type != PK_CONV_BRUTE_OLD && type != PK_CONV_BRUTE && type != PK_CONV_OPENCL && type != PK_CONV_FFT)
+		return set_err(PK_EINVAL, "Unsupported convolver type: %d", type);
+	if (src_w == 0 || src_h == 0 || krn_w == 0 || krn_h == 0) return set_err(PK_EINVAL, "empty image or kernel");
+	std::lock_guard<std::mutex> lock(ctx->mu);
+	PK_HIP(hipSetDevice(ctx->device));
+	if (ctx->precision == PK_FP32)
+		return convolve_impl<float>(ctx, type, src, src_w, src_h, krn, krn_w, krn_h, mask, crop, out, out_capacity,
+		                            out_w, out_h, offset_x, offset_y);
+	return convolve_impl<double>(ctx, type, src, src_w, src_h, krn, krn_w, krn_h, mask, crop, out, out_capacity, out_w,
+	                             out_h, offset_x, offset_y);
+}
+
+// ---------------------------------------------------------------------------
+// special functions (host copies of the device set-up code)
+// ---------------------------------------------------------------------------
+extern "C" {
+
+double pk_qgamma(double p, double shape) { return pk::qgamma(p, shape); }
+double pk_pgamma(double q, double shape) { return pk::pgamma(q, shape); }
+double pk_gammafn(double x) { return pk::gammafn(x); }
+double pk_beta(double a, double b) { return pk::beta(a, b); }
+
+double pk_integrate_qags(pk_integration_func_t f, double a, double b, void *params)
+{
+	return pk::integrate_adaptive([&](double x) { return f(x, params); }, a, b);
+}
+
+double pk_integrate_qagi(pk_integration_func_t f, double a, void *params)
+{
+	return pk::integrate_to_inf([&](double x) { return f(x, params); }, a);
+}
+
+}  // extern "C"

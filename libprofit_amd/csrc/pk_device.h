@@ -1,0 +1,494 @@
+// gfx950 kernels of the image-generation path (included once, by pk_api.hip).
+//
+//  coarse_kernel   one lane per drawing pixel: classifies every component as
+//                  zero / direct / needs-refinement (src/radial.cpp:291-344),
+//                  evaluates the direct ones, appends refinement work with one
+//                  atomic per wavefront (ballot + mbcnt compaction).
+//  refine_kernel   one wavefront per refinement entry: the recursive
+//                  subsample_pixel (src/radial.cpp:55-135) as an iterative LIFO
+//                  of cells kept in LDS; sub-cells of one or more cells are
+//                  spread over the 64 lanes, cells that fail the accuracy test
+//                  are compacted back onto the stack with a ballot.
+//  psf_kernel      PsfProfile rebinning (src/psf.cpp:58-145).
+//  conv_kernel     direct "same" convolution (src/convolve.cpp:101-231), one
+//                  64x16 output tile per 256-lane workgroup, source tile in LDS,
+//                  reversed PSF read through scalar loads.
+//  compose_kernel  crop of the PSF padding, SUM downsampling and the final mask
+//                  (src/model.cpp:289-316).
+//
+// All FP64 code is compiled with -ffp-contract=off so that products and sums
+// round as in libprofit's CPU build; FP32 hot loops use explicit fmaf.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "pk_kernels.h"
+
+namespace pk {
+
+// ---------------------------------------------------------------------------
+// image -> profile coordinates (src/radial.cpp:47-53)
+// ---------------------------------------------------------------------------
+__device__ inline void to_prof(const PkComp &c, double x, double y, double &xp, double &yp)
+{
+	x -= c.xcen;
+	y -= c.ycen;
+	xp = x * c.cos_ang + y * c.sin_ang;
+	yp = -x * c.sin_ang + y * c.cos_ang;
+	yp /= c.axrat;
+}
+
+__device__ inline uint32_t find_job(const PkJob *jobs, uint32_t njobs, uint32_t block, int which)
+{
+	int lo = 0, hi = (int)njobs - 1;
+	while (lo < hi) {
+		int mid = (lo + hi + 1) >> 1;
+		uint32_t start = which == 0 ? jobs[mid].tile_off : which == 1 ? jobs[mid].ctile_off : jobs[mid].otile_off;
+		if (start <= block) lo = mid;
+		else hi = mid - 1;
+	}
+	return (uint32_t)lo;
+}
+
+__device__ inline uint32_t lane_id() { return __lane_id(); }
+
+// ---------------------------------------------------------------------------
+// coarse pass
+// ---------------------------------------------------------------------------
+template <typename TI, bool FP32>
+__global__ __launch_bounds__(256) void coarse_kernel(const PkJob *__restrict__ jobs, uint32_t njobs,
+                                                     const PkComp *__restrict__ comps,
+                                                     const uint8_t *__restrict__ masks, TI *__restrict__ img_nc,
+                                                     TI *__restrict__ img_c, RefineEntry *__restrict__ entries,
+                                                     PkCounters *__restrict__ counters)
+{
+	const uint32_t jn = find_job(jobs, njobs, blockIdx.x, 0);
+	const PkJob &jb = jobs[jn];
+	const uint32_t t = blockIdx.x - jb.tile_off;
+	const uint32_t i = (t % jb.tiles_x) * 64 + (threadIdx.x & 63);
+	const uint32_t j = (t / jb.tiles_x) * 4 + (threadIdx.x >> 6);
+	const bool inside = i < jb.dw && j < jb.dh;
+	uint32_t cmask = 0;
+	uint32_t pix = 0;
+	if (inside) {
+		pix = j * jb.dw + i;
+		double acc_nc = 0, acc_c = 0;
+		const bool masked = jb.mask_off != NONE && !masks[jb.mask_off + pix];
+		if (!masked) {
+			const double hx = jb.sx / 2., hy = jb.sy / 2.;
+			const double x = hx + i * jb.sx, y = hy + j * jb.sy;
+			for (uint32_t c = 0; c < jb.ncomp; c++) {
+				const PkComp &cp = comps[jb.comp_off + c];
+				double v;
+				if (cp.kind == PK_PROFILE_SKY) {
+					v = cp.q[0];
+				} else {
+					double xp, yp;
+					to_prof(cp, x, y, xp, yp);
+					double r = sqrt(xp * xp + yp * yp);
+					if (cp.rscale_max > 0 && r / cp.rscale > cp.rscale_max) continue;
+					if (cp.rough || r / cp.rscale > cp.rscale_switch) {
+						if (FP32 && cp.fp32) v = (double)evaluate_at32(cp, (float)xp, (float)yp) * cp.unfold;
+						else v = evaluate_at(cp, xp, yp);
+						v *= cp.flux_scale;
+					} else {
+						cmask |= 1u << c;
+						continue;
+					}
+				}
+				if (cp.convolve) acc_c += v;
+				else acc_nc += v;
+			}
+		}
+		if (jb.has_nc) img_nc[jb.img_off + pix] = (TI)acc_nc;
+		if (jb.has_conv) img_c[jb.img_off + pix] = (TI)acc_c;
+	}
+	// wavefront-aggregated append of the pixels that need refinement
+	const bool need = cmask != 0;
+	const uint64_t ballot = __ballot(need);
+	if (ballot) {
+		const uint32_t n = __popcll(ballot);
+		const uint32_t rank = __popcll(ballot & ((1ull << lane_id()) - 1ull));
+		const uint32_t leader = __ffsll((long long)ballot) - 1;
+		uint32_t base = 0;
+		if (lane_id() == leader) base = atomicAdd(&counters->n_entries, n);
+		base = __shfl(base, (int)leader);
+		if (need) entries[base + rank] = RefineEntry{jn, pix, cmask};
+	}
+}
+
+// ---------------------------------------------------------------------------
+// refinement
+// ---------------------------------------------------------------------------
+constexpr int RQ_CAP = 768;
+
+struct RefStack {
+	double x0[RQ_CAP], x1[RQ_CAP], y0[RQ_CAP], y1[RQ_CAP];
+	uint8_t lvl[RQ_CAP];
+};
+
+__device__ inline double wave_sum(double v)
+{
+	// fixed butterfly order: deterministic across runs
+	for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+	return v;
+}
+
+// Integrates one component over one pixel; returns the pixel value in
+// libprofit units (before flux_scale), identical on every lane.
+template <bool FP32>
+__device__ double refine_pixel(const PkComp &cp, RefStack &st, double x0, double x1, double y0, double y1,
+                               int res, int max_rec, uint32_t &overflow, unsigned long long &nev)
+{
+	const int lane = (int)lane_id();
+	if (res <= 0) return NAN;  // libprofit divides 0 by 0 (src/radial.cpp:134)
+	const int r2 = res * res;
+	const int tasks_per_pop = r2 >= 64 ? 1 : 64 / r2;
+	const int rounds = (r2 + 63) / 64;
+	const double r2d = (double)r2;
+	const double acc_thr = cp.acc;
+	const bool lane_fp32 = FP32 && cp.fp32;
+	double total = 0;
+	if (lane == 0) {
+		st.x0[0] = x0;
+		st.x1[0] = x1;
+		st.y0[0] = y0;
+		st.y1[0] = y1;
+		st.lvl[0] = 0;
+	}
+	__syncthreads();
+	int top = 1;
+	while (top > 0) {
+		const int ntask = top < tasks_per_pop ? top : tasks_per_pop;
+		const int base = top - ntask;
+		// every lane loads the cell of its task before anything is pushed
+		const int my_task = r2 >= 64 ? 0 : lane / r2;
+		const bool task_ok = my_task < ntask;
+		const int slot = base + (task_ok ? my_task : 0);
+		const double cx0 = st.x0[slot], cx1 = st.x1[slot], cy0 = st.y0[slot], cy1 = st.y1[slot];
+		const int level = st.lvl[slot];
+		__syncthreads();
+		top = base;
+		const double xbin = (cx1 - cx0) / res;
+		const double ybin = (cy1 - cy0) / res;
+		const double hxb = xbin / 2.;
+		const double hyb = ybin / 2.;
+		const bool recurse = res > 1 && level < max_rec;
+		double w = 1.0;
+		for (int k = 0; k <= level; k++) w /= r2d;
+		for (int rd = 0; rd < rounds; rd++) {
+			const int s = r2 >= 64 ? lane + 64 * rd : lane % r2;
+			const bool active = task_ok && s < r2;
+			bool fail = false;
+			double x = 0, y = 0;
+			if (active) {
+				const int si = s / res, sj = s % res;
+				// sub-cell centres accumulated exactly as src/radial.cpp:83-104
+				x = cx0;
+				for (int k = 0; k < si; k++) {
+					x += hxb;
+					x += hxb;
+				}
+				x += hxb;
+				y = cy0;
+				for (int k = 0; k < sj; k++) {
+					y += hyb;
+					y += hyb;
+				}
+				y += hyb;
+				double xp, yp;
+				to_prof(cp, x, y, xp, yp);
+				double val;
+				if (recurse) {
+					const double dyp = (-xbin * cp.sin_ang + ybin * cp.cos_ang) / cp.axrat;
+					const double txp = fabs(xp), typ = fabs(yp) + fabs(dyp);
+					bool decided = false;
+					if (lane_fp32) {
+						const float v32 = evaluate_at32(cp, (float)xp, (float)yp);
+						const float t32 = evaluate_at32(cp, (float)txp, (float)typ);
+						const float m = fabsf(t32 / v32 - 1.f);
+						const double guard = 1e-4 * (1.0 + acc_thr);
+						if (v32 > 0.f && isfinite(m) && fabs((double)m - acc_thr) > guard) {
+							fail = (double)m > acc_thr;
+							val = (double)v32 * cp.unfold;
+							decided = true;
+						}
+						nev += 2;
+					}
+					if (!decided) {
+						val = evaluate_at(cp, xp, yp);
+						const double tv = evaluate_at(cp, txp, typ);
+						fail = fabs(tv / val - 1.0) > acc_thr;
+						nev += 2;
+					}
+				} else {
+					if (lane_fp32) val = (double)evaluate_at32(cp, (float)xp, (float)yp) * cp.unfold;
+					else val = evaluate_at(cp, xp, yp);
+					nev += 1;
+				}
+				if (!fail) total += val * w;
+			}
+			// compaction of failing sub-cells onto the LIFO
+			const uint64_t ballot = __ballot(fail);
+			if (ballot) {
+				const int n = __popcll(ballot);
+				if (top + n > RQ_CAP) {
+					overflow = 1;
+				} else {
+					if (fail) {
+						const int pos = top + __popcll(ballot & ((1ull << lane) - 1ull));
+						st.x0[pos] = x - hxb;
+						st.x1[pos] = x + hxb;
+						st.y0[pos] = y - hyb;
+						st.y1[pos] = y + hyb;
+						st.lvl[pos] = (uint8_t)(level + 1);
+					}
+					top += n;
+				}
+			}
+		}
+		__syncthreads();
+	}
+	return wave_sum(total);
+}
+
+template <typename TI, bool FP32>
+__global__ __launch_bounds__(64) void refine_kernel(const PkJob *__restrict__ jobs, const PkComp *__restrict__ comps,
+                                                    const RefineEntry *__restrict__ entries,
+                                                    PkCounters *__restrict__ counters, TI *__restrict__ img_nc,
+                                                    TI *__restrict__ img_c)
+{
+	__shared__ RefStack st;
+	const uint32_t n_entries = counters->n_entries;
+	uint32_t overflow = 0;
+	unsigned long long nev = 0;
+	for (;;) {
+		uint32_t e = 0;
+		if (lane_id() == 0) e = atomicAdd(&counters->next_entry, 1u);
+		e = __shfl(e, 0);
+		if (e >= n_entries) break;
+		const RefineEntry en = entries[e];
+		const PkJob &jb = jobs[en.job];
+		const uint32_t i = en.pix % jb.dw, j = en.pix / jb.dw;
+		const double hx = jb.sx / 2., hy = jb.sy / 2.;
+		const double x = hx + i * jb.sx, y = hy + j * jb.sy;
+		double add_nc = 0, add_c = 0;
+		uint32_t cm = en.cmask;
+		while (cm) {
+			const int c = __ffs(cm) - 1;
+			cm &= cm - 1;
+			const PkComp &cp = comps[jb.comp_off + c];
+			int res = cp.res, mr = cp.max_rec;
+			if (cp.center_res) {
+				// SersicProfile::subsampling_params (src/sersic.cpp:331-347)
+				const bool center = fabs(x - cp.xcen) < cp.win_x && fabs(y - cp.ycen) < cp.win_y;
+				if (center) {
+					res = cp.center_res;
+					mr = cp.center_max_rec;
+				}
+			}
+			const double v = refine_pixel<FP32>(cp, st, x - hx, x + hx, y - hy, y + hy, res, mr, overflow, nev);
+			if (cp.convolve) add_c += cp.flux_scale * v;
+			else add_nc += cp.flux_scale * v;
+		}
+		if (lane_id() == 0) {
+			if (jb.has_nc) img_nc[jb.img_off + en.pix] += (TI)add_nc;
+			if (jb.has_conv) img_c[jb.img_off + en.pix] += (TI)add_c;
+		}
+	}
+	// statistics and overflow flag, one atomic per wavefront
+	for (int off = 32; off > 0; off >>= 1) nev += __shfl_xor(nev, off);
+	const uint64_t ov = __ballot(overflow != 0);
+	if (lane_id() == 0) {
+		atomicAdd(&counters->n_subcells, nev);
+		if (ov) atomicOr(&counters->overflow, 1u);
+	}
+}
+
+// ---------------------------------------------------------------------------
+// PSF profile (src/psf.cpp:58-145); one lane per image pixel of the item's box
+// ---------------------------------------------------------------------------
+__device__ inline uint32_t bind_u(double value, uint32_t max)
+{
+	int iv = (int)floor(value);
+	if (iv < 0) return 0;
+	uint32_t u = (uint32_t)iv;
+	return u < max ? u : max;
+}
+
+template <typename TI>
+__global__ void psf_kernel(const PkPsfItem *__restrict__ items, const PkJob *__restrict__ jobs,
+                           const double *__restrict__ psfs, TI *__restrict__ img_nc, TI *__restrict__ img_c)
+{
+	const PkPsfItem it = items[blockIdx.y];
+	const PkJob &jb = jobs[it.job];
+	const uint32_t bw = it.x1 - it.x0 + 1, bh = it.y1 - it.y0 + 1;
+	const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+	if (it.x1 < it.x0 || it.y1 < it.y0 || t >= bw * bh) return;
+	const uint32_t px = it.x0 + t % bw, py = it.y0 + t / bw;
+	const double sx = jb.sx, sy = jb.sy, psx = it.psx, psy = it.psy;
+	const double x = px * sx, y = py * sy;
+	const int qx0 = (int)bind_u(floor((x - it.origin_x) / psx), it.pw - 1);
+	const int qx1 = (int)bind_u(floor((x - it.origin_x + sx) / psx), it.ph - 1);
+	const int qy0 = (int)bind_u(floor((y - it.origin_y) / psy), it.pw - 1);
+	const int qy1 = (int)bind_u(floor((y - it.origin_y + sy) / psy), it.ph - 1);
+	const double *psf = psfs + it.psf_off;
+	double val = 0;
+	for (int qy = qy0; qy <= qy1; qy++) {
+		const double ppy = qy * psy + it.origin_y;
+		for (int qx = qx0; qx <= qx1; qx++) {
+			const double ppx = qx * psx + it.origin_x;
+			const double ix = fmin(x + sx, ppx + psx) - fmax(x, ppx);
+			const double iy = fmin(y + sy, ppy + psy) - fmax(y, ppy);
+			const uint32_t q = (uint32_t)qx + (uint32_t)qy * it.pw;
+			if (q < it.pw * it.ph) val += psf[q] * (ix * iy) / (psx * psy);
+		}
+	}
+	TI *img = it.to_conv ? img_c : img_nc;
+	img[jb.img_off + px + (uint64_t)py * jb.dw] += (TI)(val * it.scale);
+}
+
+// ---------------------------------------------------------------------------
+// direct convolution
+// ---------------------------------------------------------------------------
+constexpr int CT_W = 64;   // tile width (one wavefront)
+constexpr int CT_R = 4;    // output rows per lane
+constexpr int CT_H = 16;   // tile height (4 wavefronts x CT_R rows)
+
+template <typename T> struct Acc;
+template <> struct Acc<double> {
+	double s = 0;
+	__device__ void add_col(double v) { s += v; }
+	__device__ double get() const { return s; }
+};
+template <> struct Acc<float> {
+	// Kahan-compensated sum of per-column FP32 partial sums
+	float s = 0.f, c = 0.f;
+	__device__ void add_col(float v)
+	{
+		float y = v - c;
+		float t = s + y;
+		c = (t - s) - y;
+		s = t;
+	}
+	__device__ float get() const { return s; }
+};
+
+__device__ inline double fma_t(double a, double b, double c) { return fma(a, b, c); }
+__device__ inline float fma_t(float a, float b, float c) { return fmaf(a, b, c); }
+
+template <typename T>
+__global__ __launch_bounds__(256) void conv_kernel(const PkJob *__restrict__ jobs, uint32_t njobs,
+                                                   const T *__restrict__ src_all, const T *__restrict__ krn_all,
+                                                   const uint8_t *__restrict__ masks, T *__restrict__ dst_all)
+{
+	extern __shared__ unsigned char smem_raw[];
+	T *tile = reinterpret_cast<T *>(smem_raw);
+	const uint32_t jn = find_job(jobs, njobs, blockIdx.x, 1);
+	const PkJob &jb = jobs[jn];
+	const uint32_t t = blockIdx.x - jb.ctile_off;
+	const int W = (int)jb.dw, H = (int)jb.dh, kw = (int)jb.kw, kh = (int)jb.kh;
+	const int hw = kw / 2, hh = kh / 2;
+	const int bx = (int)(t % jb.ctiles_x) * CT_W, by = (int)(t / jb.ctiles_x) * CT_H;
+	const int ox = bx + jb.csx, oy = by + jb.csy;  // origin in the same-size frame
+	const int TW = CT_W + kw - 1, TH = CT_H + kh - 1;
+	const T *src = src_all + jb.img_off;
+	const T *krn = krn_all + jb.psf_off;
+	// source tile with zero padding outside the image
+	for (int e = threadIdx.x; e < TW * TH; e += blockDim.x) {
+		const int ty = e / TW, tx = e % TW;
+		const int si = ox + tx - hw, sj = oy + ty - hh;
+		T v = 0;
+		if (si >= 0 && si < W && sj >= 0 && sj < H) v = src[si + (size_t)sj * W];
+		tile[e] = v;
+	}
+	__syncthreads();
+	const int lx = threadIdx.x & 63, ly = (threadIdx.x >> 6) * CT_R;
+	Acc<T> acc[CT_R];
+	// out(i,j) = sum_{l,k} src(i+k-hw, j+l-hh) * krn(kw-1-k, kh-1-l)
+	for (int k = 0; k < kw; k++) {
+		T col[CT_R];
+#pragma unroll
+		for (int r = 0; r < CT_R; r++) col[r] = 0;
+		const T *tcol = tile + (size_t)ly * TW + lx + k;
+		const T *kcol = krn + (kw - 1 - k) + (size_t)(kh - 1) * kw;  // krn(kw-1-k, kh-1-l) at l = 0
+		for (int yy = 0; yy < kh + CT_R - 1; yy++) {
+			const T s = tcol[(size_t)yy * TW];
+#pragma unroll
+			for (int r = 0; r < CT_R; r++) {
+				const int l = yy - r;
+				if (l >= 0 && l < kh) col[r] = fma_t(s, kcol[-(ptrdiff_t)l * kw], col[r]);
+			}
+		}
+#pragma unroll
+		for (int r = 0; r < CT_R; r++) acc[r].add_col(col[r]);
+	}
+	T *dst = dst_all + jb.conv_off;
+#pragma unroll
+	for (int r = 0; r < CT_R; r++) {
+		const int X = bx + lx, Y = by + ly + r;
+		if (X < (int)jb.cw && Y < (int)jb.ch) {
+			T v = acc[r].get();
+			if (jb.mask_off != NONE) {
+				const int mi = X - jb.cmx, mj = Y - jb.cmy;
+				if (mi < 0 || mj < 0 || mi >= W || mj >= H || !masks[jb.mask_off + mi + (size_t)mj * W]) v = 0;
+			}
+			dst[X + (size_t)Y * jb.cw] = v;
+		}
+	}
+}
+
+// ---------------------------------------------------------------------------
+// composition: returned(i,j) = sum over the fs x fs block of
+//   conv(X,Y) + model(X - mox, Y - moy), (X,Y) = crop origin + fine position
+// ---------------------------------------------------------------------------
+template <typename TI, typename TO>
+__global__ __launch_bounds__(256) void compose_kernel(const PkJob *__restrict__ jobs, uint32_t njobs,
+                                                      const TI *__restrict__ img_nc,
+                                                      const TI *__restrict__ conv_all,
+                                                      const uint8_t *__restrict__ omasks, TO *__restrict__ out_all)
+{
+	const uint32_t jn = find_job(jobs, njobs, blockIdx.x, 2);
+	const PkJob &jb = jobs[jn];
+	const uint32_t t = blockIdx.x - jb.otile_off;
+	const uint32_t i = (t % jb.otiles_x) * 64 + (threadIdx.x & 63);
+	const uint32_t j = (t / jb.otiles_x) * 4 + (threadIdx.x >> 6);
+	if (i >= jb.ow || j >= jb.oh) return;
+	double v = 0;
+	for (uint32_t b = 0; b < jb.fs; b++) {
+		const uint32_t fy = j * jb.fs + b;
+		if (fy >= jb.crh) break;
+		for (uint32_t a = 0; a < jb.fs; a++) {
+			const uint32_t fx = i * jb.fs + a;
+			if (fx >= jb.crw) break;
+			const int X = jb.cox + (int)fx, Y = jb.coy + (int)fy;
+			if (X < 0 || Y < 0 || X >= (int)jb.cw || Y >= (int)jb.ch) continue;
+			double p = 0;
+			if (jb.has_conv) p += (double)conv_all[jb.conv_off + X + (size_t)Y * jb.cw];
+			if (jb.has_nc) {
+				const int mx = X - jb.mox, my = Y - jb.moy;
+				if (mx >= 0 && my >= 0 && mx < (int)jb.dw && my < (int)jb.dh)
+					p += (double)img_nc[jb.img_off + mx + (size_t)my * jb.dw];
+			}
+			v += p;
+		}
+	}
+	if (jb.omask_off != NONE) {
+		const uint32_t mi = i / jb.omask_f, mj = j / jb.omask_f;
+		if (mi < jb.omask_w && mj < jb.omask_h && !omasks[jb.omask_off + mi + (size_t)mj * jb.omask_w]) v = 0;
+	}
+	out_all[jb.out_off + i + (size_t)j * jb.ow] = (TO)v;
+}
+
+// ---------------------------------------------------------------------------
+// set-up of all components on the device (batched path)
+// ---------------------------------------------------------------------------
+__global__ void setup_kernel(const SetupIn *__restrict__ in, PkComp *__restrict__ out, uint32_t n)
+{
+	const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+	if (t >= n) return;
+	PkComp c;
+	setup_component(in[t], c);
+	out[t] = c;
+}
+
+}  // namespace pk

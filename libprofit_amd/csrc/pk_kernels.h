@@ -1,0 +1,65 @@
+// Device work descriptors shared by the host orchestration and the kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "pk_profiles.h"
+
+namespace pk {
+
+constexpr uint64_t NONE = ~0ull;
+
+// One model image to produce (a libprofit Model::evaluate call).
+struct PkJob {
+	uint32_t dw, dh;             // drawing dimensions (src/model.cpp:183)
+	uint32_t comp_off, ncomp;    // radial / sky components in the component table
+	uint64_t img_off;            // offset of the drawing image in the image buffers
+	uint64_t mask_off;           // drawing-dims calculation mask (NONE = no mask)
+	double sx, sy;               // pixel scale handed to the profiles (scale / finesampling)
+	uint32_t tile_off, tiles_x;  // coarse tiles of this job (prefix over jobs)
+	uint32_t has_conv, has_nc;   // any convolved / non-convolved component
+	// convolution (BRUTE family): output = drawing dims
+	uint64_t psf_off;            // normalised PSF in the PSF buffer
+	uint32_t kw, kh;
+	uint32_t ctile_off, ctiles_x;  // convolution tiles (prefix over jobs)
+	uint64_t conv_off;           // convolved image (cw x ch) in the conv buffer
+	uint32_t cw, ch;             // dims of the convolved image
+	int32_t csx, csy;            // convolved(X,Y) = same-size convolution at (X+csx, Y+csy)
+	int32_t cmx, cmy;            // calculation mask pixel of convolved (X,Y) is (X-cmx, Y-cmy)
+	int32_t mox, moy;            // position of the drawing image inside the convolved frame
+	// composition of the returned image (crop, SUM downsample, final mask)
+	int32_t cox, coy;            // crop origin in the convolved frame
+	uint32_t crw, crh;           // crop dims
+	uint32_t fs;                 // downsampling factor (1: none)
+	uint32_t ow, oh;             // returned dims
+	uint64_t out_off;            // offset of the returned image in the output buffer
+	uint64_t omask_off;          // user's mask applied at the end (NONE = none)
+	uint32_t omask_w, omask_h, omask_f;  // its dims and upsampling factor
+	uint32_t otile_off, otiles_x;  // compose tiles (prefix over jobs)
+};
+
+struct RefineEntry {
+	uint32_t job, pix, cmask;
+};
+
+struct PkCounters {
+	uint32_t n_entries;    // refinement entries appended by the coarse pass
+	uint32_t next_entry;   // work-queue head of the refinement pass
+	uint32_t overflow;     // set when a refinement stack exceeded its capacity
+	uint32_t pad_;
+	unsigned long long n_subcells;  // evaluate_at calls made by the refinement
+};
+
+// PSF-profile work item (src/psf.cpp:58-145), geometry resolved on the host.
+struct PkPsfItem {
+	uint32_t job;
+	uint32_t to_conv;
+	uint32_t x0, y0, x1, y1;     // image pixel range (inclusive)
+	double origin_x, origin_y, scale;
+	double psx, psy;
+	uint32_t pw, ph;
+	uint64_t psf_off;
+};
+
+}  // namespace pk

@@ -1,0 +1,238 @@
+// FP64 special functions and quadrature used by the per-component set-up.
+//
+// They take the place of libprofit's math back-end entry points
+// (include/profit/utils.h:52-113, implemented with GSL or Rmath in
+// src/utils.cpp:101-259).  Every function is __host__ __device__: the host
+// copy backs the exported pk_qgamma/... symbols and the single-model path,
+// the device copy runs inside the batched set-up kernel, so both paths produce
+// the same constants.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <math.h>
+
+#ifndef PK_PI
+#define PK_PI 3.14159265358979323846
+#endif
+
+namespace pk {
+
+#define PK_HD __host__ __device__ inline
+
+// Gamma function with libprofit's error semantics (src/utils.cpp:113-129):
+// NaN at 0 and the negative integers, 0 on underflow, +Inf on overflow.
+PK_HD double gammafn(double x)
+{
+	if (x != x) return NAN;
+	if (x <= 0 && x == floor(x)) return NAN;
+	double g = tgamma(x);
+	if (isinf(g)) return x > 0 ? INFINITY : NAN;
+	return g;
+}
+
+// Beta function with R semantics (include/profit/utils.h:79-92).
+PK_HD double beta(double a, double b)
+{
+	if (a < 0 || b < 0) return NAN;
+	if (a == 0 || b == 0) return INFINITY;
+	if (a + b < 170.0) return tgamma(a) * tgamma(b) / tgamma(a + b);
+	return exp(lgamma(a) + lgamma(b) - lgamma(a + b));
+}
+
+// Regularised incomplete gamma P(shape, x) (upper = false) or
+// Q(shape, x) = 1 - P (upper = true): power series below x = shape + 1,
+// Lentz continued fraction above, so that neither is formed as a difference of
+// nearly equal numbers.
+PK_HD double incgamma(double x, double a, bool upper)
+{
+	if (x != x || a != a || a <= 0) return NAN;
+	if (x <= 0) return upper ? 1 : 0;
+	if (isinf(x)) return upper ? 0 : 1;
+	double lpre = a * log(x) - x - lgamma(a);
+	if (x < a + 1) {
+		double term = 1.0 / a, sum = term;
+		for (int n = 1; n < 100000; n++) {
+			term *= x / (a + n);
+			sum += term;
+			if (fabs(term) < fabs(sum) * 1e-17) break;
+		}
+		double pv = sum * exp(lpre);
+		return upper ? 1 - pv : pv;
+	}
+	const double tiny = 1e-300;
+	double b = x + 1 - a, c = 1 / tiny, d = 1 / b, h = d;
+	for (int i = 1; i < 100000; i++) {
+		double an = -i * (i - a);
+		b += 2;
+		d = an * d + b;
+		if (fabs(d) < tiny) d = tiny;
+		c = b + an / c;
+		if (fabs(c) < tiny) c = tiny;
+		d = 1 / d;
+		double del = d * c;
+		h *= del;
+		if (fabs(del - 1) < 1e-17) break;
+	}
+	double qv = exp(lpre) * h;
+	return upper ? qv : 1 - qv;
+}
+
+// Regularised lower incomplete gamma P(shape, x) (gsl_cdf_gamma_P(x, shape, 1)).
+PK_HD double pgamma(double x, double a) { return incgamma(x, a, false); }
+
+// Starting point for the gamma quantile: inverse normal by a rational
+// approximation (Acklam), only used to seed the Halley iteration.
+PK_HD double norm_quantile_seed(double p)
+{
+	const double a0 = -3.969683028665376e+01, a1 = 2.209460984245205e+02, a2 = -2.759285104469687e+02,
+	             a3 = 1.383577518672690e+02, a4 = -3.066479806614716e+01, a5 = 2.506628277459239e+00;
+	const double b0 = -5.447609879822406e+01, b1 = 1.615858368580409e+02, b2 = -1.556989798598866e+02,
+	             b3 = 6.680131188771972e+01, b4 = -1.328068155288572e+01;
+	const double c0 = -7.784894002430293e-03, c1 = -3.223964580411365e-01, c2 = -2.400758277161838e+00,
+	             c3 = -2.549732539343734e+00, c4 = 4.374664141464968e+00, c5 = 2.938163982698783e+00;
+	const double d0 = 7.784695709041462e-03, d1 = 3.224671290700398e-01, d2 = 2.445134137142996e+00,
+	             d3 = 3.754408661907416e+00;
+	if (p < 0.02425 || p > 1 - 0.02425) {
+		double q = sqrt(-2 * log(p < 0.5 ? p : 1 - p));
+		double v = (((((c0 * q + c1) * q + c2) * q + c3) * q + c4) * q + c5) /
+		           ((((d0 * q + d1) * q + d2) * q + d3) * q + 1);
+		return p < 0.5 ? v : -v;
+	}
+	double q = p - 0.5, r = q * q;
+	return (((((a0 * r + a1) * r + a2) * r + a3) * r + a4) * r + a5) * q /
+	       (((((b0 * r + b1) * r + b2) * r + b3) * r + b4) * r + 1);
+}
+
+// Quantile of the unit-scale gamma distribution: x with P(shape, x) = p.
+// Halley steps on P from a Wilson-Hilferty (shape >= 1) or small-x seed.
+PK_HD double qgamma(double p, double a)
+{
+	if (p != p || a != a || a <= 0 || p < 0 || p > 1) return NAN;
+	if (p == 0) return 0;
+	if (p == 1) return INFINITY;
+	double x;
+	if (a < 1) {
+		x = pow(p * tgamma(a + 1), 1 / a);
+		if (!(x > 0)) x = 1e-300;
+	} else {
+		double z = norm_quantile_seed(p);
+		double t = 1 / (9 * a);
+		double w = 1 - t + z * sqrt(t);
+		x = a * w * w * w;
+		if (!(x > 0)) x = a * 1e-3;
+	}
+	double lg = lgamma(a);
+	const bool upper = p > 0.5;
+	const double q = 1 - p;  // exact for p > 0.5
+	for (int it = 0; it < 200; it++) {
+		double f = upper ? q - incgamma(x, a, true) : incgamma(x, a, false) - p;
+		double pdf = exp((a - 1) * log(x) - x - lg);
+		if (pdf == 0) break;
+		double u = f / pdf;
+		double h = u / (1 - 0.5 * u * ((a - 1) / x - 1));
+		if (!isfinite(h)) h = u;
+		double xn = x - h;
+		if (xn <= 0) xn = x / 2;
+		if (fabs(xn - x) <= 4e-16 * x) {
+			x = xn;
+			break;
+		}
+		x = xn;
+	}
+	return x;
+}
+
+// ---- adaptive Gauss-Kronrod quadrature (K21 with embedded G10) ----------
+// Nodes and weights are the published QUADPACK constants (Piessens et al.
+// 1983).  The interval with the largest error estimate is bisected until the
+// summed estimate falls below 1e-13 relative; libprofit asks GSL/R for 1e-4
+// (src/utils.cpp:155-186), so this is at least as accurate.
+struct GK21 {
+	static PK_HD double x(int i)
+	{
+		const double v[11] = {0.995657163025808080735527280689003, 0.973906528517171720077964012084452,
+		                      0.930157491355708226001207180059508, 0.865063366688984510732096688423493,
+		                      0.780817726586416897063717578345042, 0.679409568299024406234327365114874,
+		                      0.562757134668604683339000099272694, 0.433395394129247190799265943165784,
+		                      0.294392862701460198131126603103866, 0.148874338981631210884826001129720,
+		                      0.0};
+		return v[i];
+	}
+	static PK_HD double wk(int i)
+	{
+		const double v[11] = {0.011694638867371874278064396062192, 0.032558162307964727478818972459390,
+		                      0.054755896574351996031381300244580, 0.075039674810919952767043140916190,
+		                      0.093125454583697605535065465083366, 0.109387158802297641899210590325805,
+		                      0.123491976262065851077958109831074, 0.134709217311473325928054001771707,
+		                      0.142775938577060080797094273138717, 0.147739104901338491374841515972068,
+		                      0.149445554002916905664936468389821};
+		return v[i];
+	}
+	static PK_HD double wg(int i)
+	{
+		const double v[5] = {0.066671344308688137593568809893332, 0.149451349150580593145776339657697,
+		                     0.219086362515982043995534934228163, 0.269266719309996355091226921569469,
+		                     0.295524224714752870173892994651338};
+		return v[i];
+	}
+};
+
+template <typename F>
+PK_HD void gk21(const F &f, double a, double b, double &res, double &err)
+{
+	double c = 0.5 * (a + b), h = 0.5 * (b - a);
+	double rk = f(c) * GK21::wk(10), rg = 0;
+	for (int i = 0; i < 10; i++) {
+		double dx = h * GK21::x(i);
+		double s = f(c - dx) + f(c + dx);
+		rk += GK21::wk(i) * s;
+		if (i & 1) rg += GK21::wg(i >> 1) * s;
+	}
+	res = rk * h;
+	err = fabs((rk - rg) * h);
+}
+
+constexpr int QUAD_LIMIT = 160;
+
+template <typename F>
+PK_HD double integrate_adaptive(const F &f, double a, double b)
+{
+	double lo[QUAD_LIMIT], hi[QUAD_LIMIT], r[QUAD_LIMIT], e[QUAD_LIMIT];
+	int n = 1;
+	lo[0] = a;
+	hi[0] = b;
+	gk21(f, a, b, r[0], e[0]);
+	for (;;) {
+		double tot = 0, terr = 0;
+		int worst = 0;
+		for (int i = 0; i < n; i++) {
+			tot += r[i];
+			terr += e[i];
+			if (e[i] > e[worst]) worst = i;
+		}
+		if (terr <= 1e-13 * fabs(tot) || terr < 1e-300 || n >= QUAD_LIMIT) return tot;
+		double m = 0.5 * (lo[worst] + hi[worst]);
+		if (m <= lo[worst] || m >= hi[worst]) return tot;
+		lo[n] = m;
+		hi[n] = hi[worst];
+		hi[worst] = m;
+		gk21(f, lo[worst], hi[worst], r[worst], e[worst]);
+		gk21(f, lo[n], hi[n], r[n], e[n]);
+		n++;
+	}
+}
+
+// Integral over (a, +Inf) through the QAGI change of variable x = a + (1-t)/t.
+template <typename F>
+PK_HD double integrate_to_inf(const F &f, double a)
+{
+	auto g = [&](double t) -> double {
+		if (t <= 0) return 0;
+		double x = a + (1 - t) / t;
+		double v = f(x) / (t * t);
+		return isfinite(v) ? v : 0;
+	};
+	return integrate_adaptive(g, 0.0, 1.0);
+}
+
+}  // namespace pk

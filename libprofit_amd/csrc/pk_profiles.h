@@ -1,0 +1,445 @@
+// Radial-profile constants and evaluation, shared by host set-up and device
+// kernels.
+//
+// PkComp holds everything libprofit's RadialProfile computes before its pixel
+// loop (RadialProfile::initial_calculations, src/radial.cpp:137-199, and the
+// profile overrides), in FP64, plus FP32 copies for the FP32 bulk path.
+// Sersic values in the FP32 path are computed as exp(-bn * rf): the factor
+// exp(bn) is folded into `unfold` (FP64), which keeps the FP32 exponent
+// argument below ~14 for every pixel brighter than 1e-6 of the peak.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "pk_math.h"
+#include "profit_hip.h"
+
+namespace pk {
+
+enum SersicVariant { SV_GENERAL = 0, SV_HALF, SV_ONE, SV_TWO, SV_THREE, SV_FOUR, SV_EIGHT, SV_SIXTEEN };
+
+struct PkComp {
+	int32_t kind;       // PK_PROFILE_*
+	int32_t convolve;
+	int32_t rough;
+	int32_t adjust;
+	int32_t res, max_rec;        // sub-pixel resolution / max recursions after adjustment
+	int32_t center_res, center_max_rec;  // Sersic centre-pixel override (src/sersic.cpp:331-347), 0 = none
+	int32_t sv, boxy;            // Sersic evaluation variant (src/sersic.cpp:225-255)
+	int32_t fp32;                // evaluated in FP32 in the FP32 mode
+	int32_t pad_;
+	double xcen, ycen;           // _xcen/_ycen: centre in the drawing frame (src/radial.cpp:254-255)
+	double cos_ang, sin_ang, axrat, box;
+	double rscale, rscale_switch, rscale_max, acc;
+	double flux_scale;           // pixel area * Ie (* rescale factor), src/radial.cpp:219-222
+	double unfold;               // FP32 values are multiplied by this to reach libprofit's units
+	double win_x, win_y;         // Model image pixel scale (Sersic centre window, src/sersic.cpp:340-341)
+	double q[8];                 // profile constants (FP64)
+	float f[8];                  // profile constants (FP32)
+	// values after adjustment, reported back (libprofit writes them into the profile)
+	double bn, lumtot, ie;
+};
+
+PK_HD bool almost_equals(double x, double y) { return fabs(x - y) < 1e-10; }
+
+// ---------------------------------------------------------------------------
+// FP64 evaluation, operation for operation as libprofit's CPU code.
+// ---------------------------------------------------------------------------
+
+PK_HD double boxy_r(const PkComp &c, double x, double y)
+{
+	// include/profit/radial.h:93-102
+	if (c.box == 0) return sqrt(x * x + y * y);
+	double b2 = c.box + 2.;
+	return pow(pow(fabs(x), b2) + pow(fabs(y), b2), 1. / b2);
+}
+
+PK_HD double sersic_at(const PkComp &c, double x, double y)
+{
+	// src/sersic.cpp:79-205
+	const double re = c.q[0], bn = c.q[2], invexp = c.q[3];
+	double base, rf;
+	if (c.boxy) {
+		double B = c.box + 2;
+		base = pow(fabs(x / re), B) + pow(fabs(y / re), B);
+		switch (c.sv) {
+		case SV_HALF: rf = base * base; break;
+		case SV_ONE: rf = base; break;
+		case SV_TWO: rf = sqrt(base); break;
+		case SV_THREE: rf = cbrt(base); break;
+		case SV_FOUR: rf = sqrt(sqrt(base)); break;
+		case SV_EIGHT: rf = sqrt(sqrt(sqrt(base))); break;
+		case SV_SIXTEEN: rf = sqrt(sqrt(sqrt(sqrt(base)))); break;
+		default: rf = pow(base, 1 / invexp);
+		}
+	} else {
+		base = (x * x + y * y) / (re * re);
+		switch (c.sv) {
+		case SV_HALF: rf = base; break;
+		case SV_ONE: rf = sqrt(base); break;
+		case SV_TWO: rf = sqrt(sqrt(base)); break;
+		case SV_THREE: rf = cbrt(sqrt(base)); break;
+		case SV_FOUR: rf = sqrt(sqrt(sqrt(base))); break;
+		case SV_EIGHT: rf = sqrt(sqrt(sqrt(sqrt(base)))); break;
+		case SV_SIXTEEN: rf = sqrt(sqrt(sqrt(sqrt(sqrt(base))))); break;
+		default: rf = pow(sqrt(base), 1 / invexp);
+		}
+	}
+	return exp(-bn * (rf - 1));
+}
+
+PK_HD double broken_exponential(double r, double h1, double h2, double rb, double a)
+{
+	// src/brokenexponential.cpp:39-76
+	double base = r - rb;
+	double expo = 1 / h1 - 1 / h2;
+	if (a * base < 40) base = log(1 + exp(a * base)) / a;
+	return exp(-r / h1 + expo * base);
+}
+
+PK_HD double king_r(const PkComp &c, double r)
+{
+	// src/king.cpp:51-59
+	const double rc = c.q[0], rt = c.q[1], a = c.q[2];
+	if (r < rt) return pow(1 / pow(1 + pow(r / rc, 2), 1 / a) - 1 / pow(1 + pow(rt / rc, 2), 1 / a), a);
+	return 0;
+}
+
+PK_HD double coresersic_r(const PkComp &c, double r)
+{
+	// src/coresersic.cpp:51-60
+	const double re = c.q[0], rb = c.q[1], nser = c.q[2], a = c.q[3], b = c.q[4], bn = c.q[5];
+	return pow(1 + pow(r / rb, -a), b / a) * exp(-bn * pow((pow(r, a) + pow(rb, a)) / pow(re, a), 1 / (nser * a)));
+}
+
+PK_HD double evaluate_at(const PkComp &c, double x, double y)
+{
+	switch (c.kind) {
+	case PK_PROFILE_SERSIC: return sersic_at(c, x, y);
+	case PK_PROFILE_MOFFAT: {
+		// src/moffat.cpp:52-58
+		double r = boxy_r(c, x, y);
+		double rf = r / c.rscale;
+		return pow(1 + rf * rf, -c.q[1]);
+	}
+	case PK_PROFILE_FERRER: {
+		// src/ferrer.cpp:50-61
+		double r = boxy_r(c, x, y);
+		double rf = r / c.rscale;
+		if (rf < 1) return pow(1 - pow(rf, 2 - c.q[2]), c.q[1]);
+		return 0;
+	}
+	case PK_PROFILE_KING: return king_r(c, boxy_r(c, x, y));
+	case PK_PROFILE_BROKENEXP: return broken_exponential(boxy_r(c, x, y), c.q[0], c.q[1], c.q[2], c.q[3]);
+	case PK_PROFILE_CORESERSIC: return coresersic_r(c, boxy_r(c, x, y));
+	default: return 0;
+	}
+}
+
+// ---------------------------------------------------------------------------
+// FP32 evaluation (sersic, moffat, brokenexp, coresersic).  Values are in
+// units of 1/unfold of the FP64 ones.
+// ---------------------------------------------------------------------------
+
+__device__ inline float boxy_r32(const PkComp &c, float x, float y)
+{
+	if (!c.boxy) return sqrtf(x * x + y * y);
+	float b2 = c.f[7];
+	return exp2f(log2f(exp2f(b2 * log2f(fabsf(x))) + exp2f(b2 * log2f(fabsf(y)))) / b2);
+}
+
+__device__ inline float evaluate_at32(const PkComp &c, float x, float y)
+{
+	switch (c.kind) {
+	case PK_PROFILE_SERSIC: {
+		// f[0] = 1/re^2 (round) or 1/re (boxy), f[1] = exponent on base, f[2] = bn, f[7] = box + 2
+		float base;
+		if (!c.boxy) {
+			base = (x * x + y * y) * c.f[0];
+		} else {
+			float b2 = c.f[7];
+			base = exp2f(b2 * log2f(fabsf(x) * c.f[0])) + exp2f(b2 * log2f(fabsf(y) * c.f[0]));
+		}
+		float rf = base > 0.f ? exp2f(c.f[1] * log2f(base)) : 0.f;
+		return expf(-c.f[2] * rf);
+	}
+	case PK_PROFILE_MOFFAT: {
+		// f[0] = 1/rscale^2, f[1] = con
+		float r2;
+		if (!c.boxy) {
+			r2 = (x * x + y * y) * c.f[0];
+		} else {
+			float r = boxy_r32(c, x, y);
+			r2 = r * r * c.f[0];
+		}
+		return exp2f(-c.f[1] * log2f(1.f + r2));
+	}
+	case PK_PROFILE_BROKENEXP: {
+		// f[0] = 1/h1, f[1] = 1/h1 - 1/h2, f[2] = rb, f[3] = a
+		float r = boxy_r32(c, x, y);
+		float base = r - c.f[2];
+		if (c.f[3] * base < 40.f) base = log1pf(expf(c.f[3] * base)) / c.f[3];
+		return expf(-r * c.f[0] + c.f[1] * base);
+	}
+	case PK_PROFILE_CORESERSIC: {
+		// f[0] = 1/rb, f[1] = a, f[2] = b/a, f[3] = bn, f[4] = rb^a, f[5] = 1/re^a, f[6] = 1/(nser a)
+		float r = boxy_r32(c, x, y);
+		float a = c.f[1];
+		float ra = exp2f(a * log2f(r));
+		float core = exp2f(c.f[2] * log2f(1.f + exp2f(-a * log2f(r * c.f[0]))));
+		float sers = expf(-c.f[3] * exp2f(c.f[6] * log2f((ra + c.f[4]) * c.f[5])));
+		return core * sers;
+	}
+	default: return 0.f;
+	}
+}
+
+// ---------------------------------------------------------------------------
+// Set-up: RadialProfile::initial_calculations + profile overrides, FP64.
+// `sx`, `sy` are the (finesampled) pixel scales passed to Profile::evaluate,
+// (pad_x, pad_y) the PSF padding offset (src/model.cpp:336-342).
+// ---------------------------------------------------------------------------
+
+struct SetupIn {
+	pk_component c;
+	double magzero, sx, sy, img_sx, img_sy, pad_x, pad_y;
+	int32_t fp32_mode;
+	uint32_t fs;               // finesampling factor (sky bg is divided by fs^2, src/sky.cpp:42-45)
+};
+
+PK_HD double sersic_fluxfrac(double re, double nser, double bn, double fraction)
+{
+	// src/sersic.cpp:257-260
+	double ratio = qgamma(fraction, 2 * nser) / bn;
+	return re * pow(ratio, nser);
+}
+
+PK_HD double moffat_fluxfrac(double rscale, double con, double fraction)
+{
+	// src/moffat.cpp:72-75
+	return rscale * sqrt(pow(1 - fraction, 1 / (1 - con)) - 1);
+}
+
+PK_HD void setup_component(const SetupIn &in, PkComp &o)
+{
+	const pk_component &c = in.c;
+	o = PkComp{};
+	o.kind = c.kind;
+	o.convolve = c.convolve;
+	o.rough = c.rough;
+	o.adjust = c.adjust;
+	o.axrat = c.axrat;
+	o.box = c.box;
+	o.acc = c.acc;
+	o.rscale_switch = c.rscale_switch;
+	o.rscale_max = c.rscale_max;
+	o.res = (int32_t)(c.resolution > 4096u ? 4096u : c.resolution);
+	// deeper levels than 64 split cells below the resolution of a double
+	o.max_rec = (int32_t)(c.max_recursions > 64u ? 64u : c.max_recursions);
+	o.win_x = in.img_sx;
+	o.win_y = in.img_sy;
+	o.unfold = 1;
+	if (c.kind == PK_PROFILE_SKY) {
+		o.q[0] = c.p[PK_SKY_BG] / (double)(in.fs * in.fs);
+		return;
+	}
+	if (c.kind == PK_PROFILE_NULL || c.kind == PK_PROFILE_PSF) return;
+
+	const double *p = c.p;
+	double rscale = 0, lum = 0, bn = 0;
+	o.boxy = c.box != 0;
+	switch (c.kind) {
+	case PK_PROFILE_SERSIC: {
+		double re = p[PK_SERSIC_RE], n = p[PK_SERSIC_NSER];
+		bn = qgamma(0.5, 2 * n);  // src/sersic.cpp:297
+		rscale = re;
+		double inv = n;
+		if (o.boxy) inv *= (c.box + 2);
+		o.sv = almost_equals(inv, 0.5) ? SV_HALF : almost_equals(inv, 1) ? SV_ONE : almost_equals(inv, 2) ? SV_TWO :
+		       almost_equals(inv, 3) ? SV_THREE : almost_equals(inv, 4) ? SV_FOUR : almost_equals(inv, 8) ? SV_EIGHT :
+		       almost_equals(inv, 16) ? SV_SIXTEEN : SV_GENERAL;
+		o.q[0] = re;
+		o.q[1] = n;
+		o.q[2] = bn;
+		o.q[3] = inv;
+		lum = pow(re, 2) * 2 * PK_PI * n * gammafn(2 * n) * exp(bn) / pow(bn, 2 * n);  // src/sersic.cpp:283-289
+		break;
+	}
+	case PK_PROFILE_MOFFAT: {
+		double fwhm = p[PK_MOFFAT_FWHM], con = p[PK_MOFFAT_CON];
+		rscale = fwhm / (2 * sqrt(pow(2, (1 / con)) - 1));  // src/moffat.cpp:81-83
+		o.q[0] = fwhm;
+		o.q[1] = con;
+		lum = pow(rscale, 2) * PK_PI / (con - 1);
+		break;
+	}
+	case PK_PROFILE_FERRER: {
+		double rout = p[PK_FERRER_ROUT], a = p[PK_FERRER_A], b = p[PK_FERRER_B];
+		rscale = rout;
+		o.q[0] = rout;
+		o.q[1] = a;
+		o.q[2] = b;
+		lum = pow(rout, 2) * PK_PI * (a * beta(a, 1 + 2 / (2 - b)));  // src/ferrer.cpp:85-101
+		break;
+	}
+	case PK_PROFILE_KING: {
+		o.q[0] = p[PK_KING_RC];
+		o.q[1] = p[PK_KING_RT];
+		o.q[2] = p[PK_KING_A];
+		rscale = o.q[1];
+		const PkComp &k = o;
+		lum = 2 * PK_PI * integrate_adaptive([&](double r) { return r * king_r(k, r); }, 0.0, o.q[1]);
+		break;
+	}
+	case PK_PROFILE_BROKENEXP: {
+		o.q[0] = p[PK_BROKENEXP_H1];
+		o.q[1] = p[PK_BROKENEXP_H2];
+		o.q[2] = p[PK_BROKENEXP_RB];
+		o.q[3] = p[PK_BROKENEXP_A];
+		rscale = o.q[0] * 4;
+		const double h1 = o.q[0], h2 = o.q[1], rb = o.q[2], a = o.q[3];
+		lum = 2 * PK_PI * integrate_to_inf([&](double r) { return r * broken_exponential(r, h1, h2, rb, a); }, 0.0);
+		break;
+	}
+	case PK_PROFILE_CORESERSIC: {
+		o.q[0] = p[PK_CORESERSIC_RE];
+		o.q[1] = p[PK_CORESERSIC_RB];
+		o.q[2] = p[PK_CORESERSIC_NSER];
+		o.q[3] = p[PK_CORESERSIC_A];
+		o.q[4] = p[PK_CORESERSIC_B];
+		bn = qgamma(0.5, 2 * o.q[2]);
+		o.q[5] = bn;
+		rscale = o.q[0];
+		const PkComp &k = o;
+		lum = 2 * PK_PI * integrate_to_inf([&](double r) { return r * coresersic_r(k, r); }, 0.0);
+		break;
+	}
+	default: break;
+	}
+	o.rscale = rscale;
+	o.bn = bn;
+
+	// src/radial.cpp:150-153
+	double b2 = c.box + 2;
+	double r_box = PK_PI * b2 / (2 * beta(1 / b2, 1 / b2));
+	double lumtot = lum * c.axrat / r_box;
+	double ie = pow(10, -0.4 * (c.mag - in.magzero)) / lumtot;
+	o.lumtot = lumtot;
+	o.ie = ie;
+
+	if (c.adjust) {
+		// src/radial.cpp:159-186 with the profile overrides
+		double rsw = 0;
+		switch (c.kind) {
+		case PK_PROFILE_SERSIC:
+			rsw = ceil(sersic_fluxfrac(o.q[0], o.q[1], bn, 1. - o.q[1] * o.q[1] / 2e3));
+			rsw = fmax(fmin(rsw, 20.), 2.);
+			rsw = rsw / o.q[0];
+			break;
+		case PK_PROFILE_MOFFAT:
+			rsw = fmax(moffat_fluxfrac(rscale, o.q[1], 0.9999), 1.823 * o.q[0]);
+			rsw = fmax(fmin(rsw, 20.), 2.) / rscale;
+			break;
+		case PK_PROFILE_FERRER:
+		case PK_PROFILE_KING: rsw = 0.5; break;
+		case PK_PROFILE_BROKENEXP:
+		case PK_PROFILE_CORESERSIC: rsw = 1; break;
+		}
+		o.rscale_switch = rsw;
+		unsigned res = (unsigned)ceil(160 / (rsw * rscale));
+		res += res % 2;
+		res = res > 16u ? 16u : res;
+		res = res < 4u ? 4u : res;
+		o.res = (int32_t)res;
+		if (o.rscale_max == 0) {
+			double rmax = 0;
+			switch (c.kind) {
+			case PK_PROFILE_SERSIC:
+				rmax = ceil(fmax(sersic_fluxfrac(o.q[0], o.q[1], bn, 0.9999), 2.) / o.q[0]);
+				break;
+			case PK_PROFILE_MOFFAT: rmax = ceil(fmax(moffat_fluxfrac(rscale, o.q[1], 0.9999), 2.) / rscale); break;
+			case PK_PROFILE_FERRER: rmax = 1; break;
+			case PK_PROFILE_KING: rmax = 1.5; break;
+			case PK_PROFILE_BROKENEXP:
+			case PK_PROFILE_CORESERSIC: rmax = 100; break;
+			}
+			o.rscale_max = rmax;
+		}
+		double acc = o.acc;
+		if (c.kind == PK_PROFILE_SERSIC) {
+			acc /= sqrt(o.q[1]);  // src/sersic.cpp:278-281
+			acc = fmax(0.1, acc) / c.axrat;
+		} else if (c.kind == PK_PROFILE_MOFFAT) {
+			acc = 0.1 / c.axrat;  // src/moffat.cpp:97-99
+		}
+		o.acc = acc;
+	}
+
+	double angrad = fmod(c.ang + 90, 360.) * PK_PI / 180.;
+	o.cos_ang = cos(angrad);
+	o.sin_ang = sin(angrad);
+	o.xcen = c.xcen + in.pad_x * in.sx;
+	o.ycen = c.ycen + in.pad_y * in.sy;
+
+	double flux_scale = in.sx * in.sy * ie;
+	if (c.kind == PK_PROFILE_SERSIC && c.rescale_flux) {
+		// src/sersic.cpp:303-325; libprofit leaves the factor unset without
+		// adjust, here it is 1 then.
+		double rescale = 1;
+		if (c.adjust) {
+			double flux_r = bn * pow(o.rscale_max / o.q[0], 1 / o.q[1]);
+			rescale = 1 / pgamma(flux_r, 2 * o.q[1]);
+		}
+		flux_scale *= rescale;
+	}
+	o.flux_scale = flux_scale;
+
+	if (c.kind == PK_PROFILE_SERSIC && c.adjust && o.q[1] > 1) {
+		o.center_res = 8;
+		o.center_max_rec = 10;
+	}
+
+	// FP32 constants
+	o.fp32 = 0;
+	if (in.fp32_mode) {
+		switch (c.kind) {
+		case PK_PROFILE_SERSIC:
+			o.fp32 = 1;
+			o.f[0] = (float)(o.boxy ? 1 / o.q[0] : 1 / (o.q[0] * o.q[0]));
+			o.f[1] = (float)(o.boxy ? 1 / o.q[3] : 0.5 / o.q[1]);
+			o.f[2] = (float)bn;
+			o.f[7] = (float)(c.box + 2);
+			o.unfold = exp(bn);
+			break;
+		case PK_PROFILE_MOFFAT:
+			o.fp32 = 1;
+			o.f[0] = (float)(1 / (rscale * rscale));
+			o.f[1] = (float)o.q[1];
+			o.f[7] = (float)(c.box + 2);
+			break;
+		case PK_PROFILE_BROKENEXP:
+			o.fp32 = 1;
+			o.f[0] = (float)(1 / o.q[0]);
+			o.f[1] = (float)(1 / o.q[0] - 1 / o.q[1]);
+			o.f[2] = (float)o.q[2];
+			o.f[3] = (float)o.q[3];
+			o.f[7] = (float)(c.box + 2);
+			break;
+		case PK_PROFILE_CORESERSIC:
+			o.fp32 = 1;
+			o.f[0] = (float)(1 / o.q[1]);
+			o.f[1] = (float)o.q[3];
+			o.f[2] = (float)(o.q[4] / o.q[3]);
+			o.f[3] = (float)bn;
+			o.f[4] = (float)pow(o.q[1], o.q[3]);
+			o.f[5] = (float)(1 / pow(o.q[0], o.q[3]));
+			o.f[6] = (float)(1 / (o.q[2] * o.q[3]));
+			o.f[7] = (float)(c.box + 2);
+			break;
+		default: break;  // ferrer and king stay in FP64 (cancellation near rout / rt)
+		}
+	}
+}
+
+}  // namespace pk

@@ -1,0 +1,167 @@
+"""Device parity: pk_evaluate / pk_convolve against the FP64 oracle.
+
+FP64 mode must agree to <= 1e-12 per pixel and FP32 mode to <= 1e-5 on pixels
+above 1e-6 of the peak (relative error), at sizes the oracle finishes in
+seconds.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from libprofit_amd import abi
+from libprofit_amd.model import evaluate_spec
+from parity_util import TOL_FP32, TOL_FP64, c1a, c1b, c2_params, comp, moffat_psf, rel_errors, sersic
+
+pytestmark = pytest.mark.gpu
+
+
+def announce(name):
+    print("CASE", name, flush=True)
+    sys.stdout.flush()
+
+
+def run_both(spec, ctx, oracle):
+    got, goff, res = evaluate_spec(ctx, spec)
+    ref, roff = oracle.evaluate(spec)
+    assert got.shape == ref.shape
+    assert goff == roff
+    return got, ref, res
+
+
+def test_c1a_fp64(ctx64, oracle):
+    announce("c1a_fp64")
+    got, ref, res = run_both(c1a(), ctx64, oracle)
+    pix, flux = rel_errors(got, ref)
+    nref, nsub = oracle.stats()
+    assert res.n_refined_pixels == nref
+    assert res.n_subcells == nsub
+    assert pix <= TOL_FP64, pix
+    assert flux <= TOL_FP64, flux
+
+
+def test_c1b_fp64(ctx64, oracle):
+    announce("c1b_fp64")
+    got, ref, _ = run_both(c1b(), ctx64, oracle)
+    pix, flux = rel_errors(got, ref)
+    assert pix <= TOL_FP64, pix
+
+
+def test_c1_fp32(ctx32, oracle):
+    for name, spec in (("c1a", c1a()), ("c1b", c1b())):
+        announce(name + "_fp32")
+        got, ref, _ = run_both(spec, ctx32, oracle)
+        pix, flux = rel_errors(got, ref)
+        assert pix <= TOL_FP32, (name, pix)
+        assert flux <= TOL_FP32, (name, flux)
+
+
+PROFILES = [
+    ("sersic", dict(re=5.0, nser=2.5)),
+    ("sersic", dict(re=3.0, nser=0.5)),
+    ("sersic", dict(re=4.0, nser=1.0)),
+    ("sersic", dict(re=6.0, nser=8.0)),
+    ("moffat", dict(fwhm=4.0, con=3.0)),
+    ("ferrer", dict(rout=12.0, a=1.5, b=0.5)),
+    ("king", dict(rc=3.0, rt=20.0, a=2.0)),
+    ("brokenexp", dict(h1=6.0, h2=3.0, rb=8.0, a=0.5)),
+    ("coresersic", dict(re=8.0, rb=2.0, nser=3.0, a=2.0, b=0.8)),
+]
+
+
+@pytest.mark.parametrize("box", [0.0, 0.4, -0.3])
+@pytest.mark.parametrize("name,params", PROFILES)
+def test_profiles_fp64(ctx64, oracle, name, params, box):
+    announce("profile_fp64 %s %s box=%s" % (name, params, box))
+    spec = abi.ModelSpec(64, 64, [comp(name, xcen=31.3, ycen=32.6, mag=16, ang=25, axrat=0.7, box=box, **params)])
+    got, ref, _ = run_both(spec, ctx64, oracle)
+    pix, flux = rel_errors(got, ref)
+    assert pix <= TOL_FP64, pix
+
+
+@pytest.mark.parametrize("name,params", PROFILES)
+def test_profiles_fp32(ctx32, oracle, name, params):
+    announce("profile_fp32 %s %s" % (name, params))
+    spec = abi.ModelSpec(64, 64, [comp(name, xcen=31.3, ycen=32.6, mag=16, ang=25, axrat=0.7, **params)])
+    got, ref, _ = run_both(spec, ctx32, oracle)
+    pix, flux = rel_errors(got, ref)
+    assert pix <= TOL_FP32, pix
+
+
+def test_sky_and_mask(ctx64, oracle):
+    announce("sky_mask")
+    mask = np.ones((2, 2), dtype=np.uint8)
+    mask[1, 1] = 0
+    spec = abi.ModelSpec(2, 2, [comp("sky", bg=5.0)], mask=mask)
+    got, _, _ = evaluate_spec(ctx64, spec)
+    # tests/test_sky.h:48-61
+    assert list(got.ravel()) == [5, 5, 5, 0]
+
+
+def test_psf_profile_exact(ctx64):
+    announce("psf_profile")
+    # tests/test_psf.h:36-50
+    spec = abi.ModelSpec(10, 10, [comp("psf", xcen=2.0, ycen=2.0, mag=0.0)], psf=np.ones((2, 2)))
+    got, _, _ = evaluate_spec(ctx64, spec)
+    exp = np.zeros((10, 10))
+    exp[1:3, 1:3] = 0.25
+    assert np.allclose(got, exp, rtol=1e-9, atol=0)
+
+
+def test_mask_finesampling(ctx64, oracle):
+    announce("mask_finesampling")
+    rng = np.random.Generator(np.random.PCG64(3))
+    mask = (rng.uniform(size=(40, 50)) > 0.3).astype(np.uint8)
+    for fs in (1, 2):
+        for rf in (True, False):
+            spec = abi.ModelSpec(50, 40, [sersic(xcen=20.2, ycen=22.7, re=6, nser=3, axrat=0.5, ang=70)], mask=mask,
+                                 finesampling=fs, return_finesampled=rf)
+            got, ref, _ = run_both(spec, ctx64, oracle)
+            pix, _ = rel_errors(got, ref)
+            assert pix <= TOL_FP64, (fs, rf, pix)
+
+
+@pytest.mark.parametrize("conv", ["brute", "brute-old", "fft"])
+def test_convolved_model_fp64(ctx64, oracle, conv):
+    announce("convolved %s" % conv)
+    psf, _ = oracle.evaluate(moffat_psf(15, 3.0, 2.5))
+    spec = abi.ModelSpec(96, 80, [sersic(xcen=47.1, ycen=41.4, re=7, nser=2, axrat=0.6, ang=10, convolve=True),
+                                  comp("sky", bg=1e-9)], psf=psf, convolver=conv)
+    got, ref, _ = run_both(spec, ctx64, oracle)
+    pix, flux = rel_errors(got, ref)
+    tol = TOL_FP64 if conv != "fft" else 1e-9
+    assert pix <= tol, pix
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_c2_fp64_fp32(ctx64, ctx32, oracle, seed):
+    announce("c2 seed %d" % seed)
+    psf, _ = oracle.evaluate(moffat_psf(25, 3.0, 2.5))
+    p = c2_params(seed)
+    spec = abi.ModelSpec(400, 400, [sersic(convolve=True, **p)], psf=psf)
+    ref, _ = oracle.evaluate(spec)
+    g64, _, _ = evaluate_spec(ctx64, spec)
+    g32, _, _ = evaluate_spec(ctx32, spec)
+    pix64, flux64 = rel_errors(g64, ref)
+    pix32, flux32 = rel_errors(g32, ref)
+    print("c2 seed", seed, "fp64", pix64, flux64, "fp32", pix32, flux32)
+    assert pix64 <= TOL_FP64
+    assert pix32 <= TOL_FP32
+    assert flux32 <= TOL_FP32
+
+
+def test_convolver_api(ctx64, oracle):
+    announce("convolver_api")
+    rng = np.random.Generator(np.random.PCG64(7))
+    src = rng.uniform(size=(37, 41))
+    krn = rng.uniform(size=(9, 6))
+    mask = (rng.uniform(size=src.shape) > 0.2).astype(np.uint8)
+    from libprofit_amd.model import Convolver
+    for t in (abi.BRUTE_OLD, abi.BRUTE, abi.FFT):
+        for crop in (True, False):
+            for mk in (None, mask):
+                conv = Convolver(t, context=ctx64)
+                got, goff = conv.convolve(src, krn, mask=mk, crop=crop)
+                ref, roff = oracle.convolve(src, krn, mask=mk, crop=crop, conv=t)
+                assert got.shape == ref.shape and goff == roff, (t, crop)
+                assert np.max(np.abs(got - ref)) <= 1e-12 * np.max(np.abs(ref)), (t, crop)
