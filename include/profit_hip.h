@@ -198,6 +198,44 @@ int pk_convolve(pk_ctx *ctx, int type, const double *src, uint32_t src_w, uint32
 int pk_convolver_padding(int type, uint32_t src_w, uint32_t src_h, uint32_t krn_w, uint32_t krn_h,
                          int32_t *lo_x, int32_t *lo_y, int32_t *hi_x, int32_t *hi_y);
 
+/* ---- batched, device-resident evaluation --------------------------------
+ * No counterpart in libprofit, which evaluates one Model per call: a batch
+ * holds the work list of many models (each with its own dimensions,
+ * profiles and PSF) resident in device memory and produces all their images
+ * in one sequence of launches (optionally captured in a hipGraph), with the
+ * per-component set-up (bn, normalisation integrals, adjusted subsampling
+ * constants) computed in FP64 on the device.  Supported per model: radial,
+ * sky and null profiles, BRUTE/BRUTE_OLD convolution, masks, finesampling.
+ * Output images are FP32 in a PK_FP32 context and FP64 in a PK_FP64 one. */
+typedef struct pk_batch pk_batch;
+
+enum { PK_BATCH_GRAPH = 1 };  /* capture the launch sequence once, replay it */
+
+typedef struct pk_batch_info {
+	uint64_t n_refine_entries;  /* pixels that entered sub-pixel refinement */
+	uint64_t n_subcells;        /* evaluate_at calls made by the refinement */
+	/* per-stage device time of the last evaluation (not available in graph mode: 0) */
+	double ms_setup, ms_coarse, ms_refine, ms_conv, ms_compose, ms_total;
+} pk_batch_info;
+
+int pk_batch_create(pk_ctx *ctx, const pk_model *models, uint32_t n_models, uint32_t flags, pk_batch **out);
+/* Replaces the parameters of all components (the models' components in
+ * order, same kinds as at creation); e.g. one step of a fit. */
+int pk_batch_set_components(pk_batch *b, const pk_component *components, uint32_t n_components);
+/* Enqueues one evaluation of every model on the context's stream. */
+int pk_batch_evaluate(pk_batch *b);
+/* Waits for the last evaluation; fails with PK_ERANGE if a refinement stack
+ * overflowed.  `info` may be NULL. */
+int pk_batch_sync(pk_batch *b, pk_batch_info *info);
+int pk_batch_image_dims(const pk_batch *b, uint32_t i, uint32_t *width, uint32_t *height);
+/* Copies image i (converted to FP64) to host memory. */
+int pk_batch_get_image(pk_batch *b, uint32_t i, double *out, size_t out_capacity);
+/* Device address of all output images (packed, model order), their element
+ * size in bytes (4 or 8) and the element offset of image i. */
+int pk_batch_device_images(const pk_batch *b, void **base, size_t *elem_size);
+int pk_batch_image_offset(const pk_batch *b, uint32_t i, uint64_t *offset);
+int pk_batch_destroy(pk_batch *b);
+
 /* ---- FP64 special functions of the math back-end ------------------------
  * include/profit/utils.h:52-113 (GSL/Rmath in libprofit, own implementations
  * here; the same code runs on the device during per-component set-up). */

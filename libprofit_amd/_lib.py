@@ -52,6 +52,15 @@ def load():
         "pk_convolve": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, u32, u32, C.c_void_p, u32, u32, C.c_void_p,
                                   C.c_int, C.c_void_p, C.c_size_t, P(u32), P(u32), P(i32), P(i32)]),
         "pk_convolver_padding": (C.c_int, [C.c_int, u32, u32, u32, u32, P(i32), P(i32), P(i32), P(i32)]),
+        "pk_batch_create": (C.c_int, [C.c_void_p, P(abi.Model), u32, u32, P(C.c_void_p)]),
+        "pk_batch_set_components": (C.c_int, [C.c_void_p, P(abi.Component), u32]),
+        "pk_batch_evaluate": (C.c_int, [C.c_void_p]),
+        "pk_batch_sync": (C.c_int, [C.c_void_p, P(abi.BatchInfo)]),
+        "pk_batch_image_dims": (C.c_int, [C.c_void_p, u32, P(u32), P(u32)]),
+        "pk_batch_get_image": (C.c_int, [C.c_void_p, u32, C.c_void_p, C.c_size_t]),
+        "pk_batch_device_images": (C.c_int, [C.c_void_p, P(C.c_void_p), P(C.c_size_t)]),
+        "pk_batch_image_offset": (C.c_int, [C.c_void_p, u32, P(C.c_uint64)]),
+        "pk_batch_destroy": (C.c_int, [C.c_void_p]),
         "pk_qgamma": (dbl, [dbl, dbl]),
         "pk_pgamma": (dbl, [dbl, dbl]),
         "pk_gammafn": (dbl, [dbl]),

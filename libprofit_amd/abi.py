@@ -107,6 +107,22 @@ class Result(C.Structure):
     ]
 
 
+class BatchInfo(C.Structure):
+    _fields_ = [
+        ("n_refine_entries", C.c_uint64),
+        ("n_subcells", C.c_uint64),
+        ("ms_setup", C.c_double),
+        ("ms_coarse", C.c_double),
+        ("ms_refine", C.c_double),
+        ("ms_conv", C.c_double),
+        ("ms_compose", C.c_double),
+        ("ms_total", C.c_double),
+    ]
+
+
+PK_BATCH_GRAPH = 1
+
+
 RADIAL_DEFAULTS = dict(
     xcen=0.0, ycen=0.0, mag=15.0, ang=0.0, axrat=1.0, box=0.0,
     rough=False, adjust=True, rescale_flux=False, force_cpu=False,

@@ -972,3 +972,391 @@ double pk_integrate_qagi(pk_integration_func_t f, double a, void *params)
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// batched, device-resident evaluation
+// ---------------------------------------------------------------------------
+struct pk_batch {
+	pk_ctx *ctx = nullptr;
+	uint32_t flags = 0;
+	uint32_t n_models = 0, n_comps = 0;
+	size_t elem = 8;  // bytes per image element (4: FP32 context)
+	std::vector<PkJob> jobs;
+	std::vector<SetupIn> setup;  // one per radial/sky component
+	std::vector<uint32_t> comp_index;  // user component index -> setup slot (or ~0 for null)
+	std::vector<int32_t> comp_kind;
+	uint32_t ntiles = 0, nctiles = 0, notiles = 0;
+	uint32_t max_kw = 0, max_kh = 0;
+	bool any_conv = false;
+	// device memory
+	void *dmem = nullptr;
+	size_t o_jobs = 0, o_setup = 0, o_comps = 0, o_cnt = 0, o_psfT = 0, o_mask = 0, o_nc = 0, o_c = 0, o_conv = 0,
+	       o_ent = 0, o_out = 0;
+	uint64_t total_out = 0;
+	hipGraph_t graph = nullptr;
+	hipGraphExec_t exec = nullptr;
+	hipEvent_t ev[7] = {};
+	bool ran = false;
+	template <typename T> T *d(size_t off) const
+	{
+		return reinterpret_cast<T *>(static_cast<char *>(dmem) + off);
+	}
+};
+
+namespace {
+
+int batch_launch(pk_batch *b)
+{
+	pk_ctx *ctx = b->ctx;
+	hipStream_t s = ctx->stream;
+	const bool fp32 = ctx->precision == PK_FP32;
+	const bool timed = !(b->flags & PK_BATCH_GRAPH);
+	const PkJob *djobs = b->d<PkJob>(b->o_jobs);
+	PkCounters *dcnt = b->d<PkCounters>(b->o_cnt);
+	PkComp *dcomp = b->d<PkComp>(b->o_comps);
+	const uint8_t *dmask = b->d<uint8_t>(b->o_mask);
+	RefineEntry *dent = b->d<RefineEntry>(b->o_ent);
+	const uint32_t nj = b->n_models;
+	if (timed) PK_HIP(hipEventRecord(b->ev[0], s));
+	hipLaunchKernelGGL(reset_counters_kernel, dim3(1), dim3(64), 0, s, dcnt);
+	if (b->n_comps)
+		hipLaunchKernelGGL(setup_kernel, dim3(cdiv(b->n_comps, 64)), dim3(64), 0, s, b->d<SetupIn>(b->o_setup), dcomp,
+		                   b->n_comps);
+	PK_HIP(hipGetLastError());
+	if (timed) PK_HIP(hipEventRecord(b->ev[1], s));
+	if (fp32) {
+		float *nc = b->d<float>(b->o_nc), *c = b->d<float>(b->o_c);
+		hipLaunchKernelGGL((coarse_kernel<float, true>), dim3(b->ntiles), dim3(256), 0, s, djobs, nj, dcomp, dmask,
+		                   nc, c, dent, dcnt);
+		PK_HIP(hipGetLastError());
+		if (timed) PK_HIP(hipEventRecord(b->ev[2], s));
+		hipLaunchKernelGGL((refine_kernel<float, true>), dim3(4096), dim3(64), 0, s, djobs, dcomp, dent, dcnt, nc, c);
+		PK_HIP(hipGetLastError());
+		if (timed) PK_HIP(hipEventRecord(b->ev[3], s));
+		if (b->any_conv) {
+			const size_t lds = conv_lds_bytes(b->max_kw, b->max_kh, sizeof(float));
+			hipLaunchKernelGGL(conv_kernel<float>, dim3(b->nctiles), dim3(256), lds, s, djobs, nj, c,
+			                   b->d<float>(b->o_psfT), dmask, b->d<float>(b->o_conv));
+			PK_HIP(hipGetLastError());
+		}
+		if (timed) PK_HIP(hipEventRecord(b->ev[4], s));
+		hipLaunchKernelGGL((compose_kernel<float, float>), dim3(b->notiles), dim3(256), 0, s, djobs, nj, nc,
+		                   b->d<float>(b->o_conv), (const uint8_t *)nullptr, b->d<float>(b->o_out));
+	} else {
+		double *nc = b->d<double>(b->o_nc), *c = b->d<double>(b->o_c);
+		hipLaunchKernelGGL((coarse_kernel<double, false>), dim3(b->ntiles), dim3(256), 0, s, djobs, nj, dcomp, dmask,
+		                   nc, c, dent, dcnt);
+		PK_HIP(hipGetLastError());
+		if (timed) PK_HIP(hipEventRecord(b->ev[2], s));
+		hipLaunchKernelGGL((refine_kernel<double, false>), dim3(4096), dim3(64), 0, s, djobs, dcomp, dent, dcnt, nc, c);
+		PK_HIP(hipGetLastError());
+		if (timed) PK_HIP(hipEventRecord(b->ev[3], s));
+		if (b->any_conv) {
+			const size_t lds = conv_lds_bytes(b->max_kw, b->max_kh, sizeof(double));
+			hipLaunchKernelGGL(conv_kernel<double>, dim3(b->nctiles), dim3(256), lds, s, djobs, nj, c,
+			                   b->d<double>(b->o_psfT), dmask, b->d<double>(b->o_conv));
+			PK_HIP(hipGetLastError());
+		}
+		if (timed) PK_HIP(hipEventRecord(b->ev[4], s));
+		hipLaunchKernelGGL((compose_kernel<double, double>), dim3(b->notiles), dim3(256), 0, s, djobs, nj, nc,
+		                   b->d<double>(b->o_conv), (const uint8_t *)nullptr, b->d<double>(b->o_out));
+	}
+	PK_HIP(hipGetLastError());
+	if (timed) PK_HIP(hipEventRecord(b->ev[5], s));
+	return PK_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int pk_batch_destroy(pk_batch *b)
+{
+	if (!b) return PK_OK;
+	(void)hipSetDevice(b->ctx->device);
+	(void)hipStreamSynchronize(b->ctx->stream);
+	if (b->exec) (void)hipGraphExecDestroy(b->exec);
+	if (b->graph) (void)hipGraphDestroy(b->graph);
+	for (auto &e : b->ev)
+		if (e) (void)hipEventDestroy(e);
+	if (b->dmem) (void)hipFree(b->dmem);
+	delete b;
+	return PK_OK;
+}
+
+int pk_batch_create(pk_ctx *ctx, const pk_model *models, uint32_t n_models, uint32_t flags, pk_batch **out)
+{
+	if (!ctx || !out || (!models && n_models)) return set_err(PK_EINVAL, "NULL argument");
+	if (n_models == 0) return set_err(PK_EINVAL, "empty batch");
+	std::lock_guard<std::mutex> lock(ctx->mu);
+	PK_HIP(hipSetDevice(ctx->device));
+	const bool fp32 = ctx->precision == PK_FP32;
+	const size_t elem = fp32 ? 4 : 8;
+	pk_batch *b = new pk_batch();
+	b->ctx = ctx;
+	b->flags = flags;
+	b->n_models = n_models;
+	b->elem = elem;
+	std::vector<double> psf_all;
+	std::vector<uint8_t> mask_all;
+	uint64_t img_off = 0, out_off = 0;
+	auto fail = [&](int code) {
+		pk_batch_destroy(b);
+		return code;
+	};
+	for (uint32_t mi = 0; mi < n_models; mi++) {
+		const pk_model *m = &models[mi];
+		Analysis a;
+		int st = analyze(m, a);
+		if (st) return fail(set_err(st, "model %u: %s", mi, g_err.c_str()));
+		if (m->dry_run) return fail(set_err(PK_EINVAL, "model %u: dry_run is not supported in a batch", mi));
+		if (a.conv_required && m->convolver == PK_CONV_FFT)
+			return fail(set_err(PK_EINVAL, "model %u: the FFT convolver is not supported in a batch", mi));
+		if (has_mask(m))
+			return fail(set_err(PK_EINVAL, "model %u: calculation masks are not supported in a batch", mi));
+		if (a.padx == 0 && a.pady == 0 && a.conv_required && m->psf_width > 1 && m->psf_height > 1)
+			return fail(set_err(PK_EINVAL, "model %u: unsupported padding", mi));
+		const uint32_t fs = m->finesampling;
+		const double sx = m->scale_x / fs, sy = m->scale_y / fs;
+		PkJob jb{};
+		jb.dw = a.dw;
+		jb.dh = a.dh;
+		jb.comp_off = (uint32_t)b->setup.size();
+		jb.img_off = img_off;
+		jb.mask_off = NONE;
+		jb.sx = sx;
+		jb.sy = sy;
+		jb.tile_off = b->ntiles;
+		jb.tiles_x = cdiv(a.dw, 64);
+		b->ntiles += jb.tiles_x * cdiv(a.dh, 4);
+		bool any_nc = false;
+		for (uint32_t c = 0; c < m->n_components; c++) {
+			const pk_component &pc = m->components[c];
+			b->comp_kind.push_back(pc.kind);
+			if (pc.kind == PK_PROFILE_NULL) {
+				b->comp_index.push_back(~0u);
+				continue;
+			}
+			if (pc.kind == PK_PROFILE_PSF)
+				return fail(set_err(PK_EINVAL, "model %u: psf profiles are not supported in a batch", mi));
+			SetupIn in{};
+			in.c = pc;
+			in.magzero = m->magzero;
+			in.sx = sx;
+			in.sy = sy;
+			in.img_sx = m->scale_x;
+			in.img_sy = m->scale_y;
+			in.pad_x = a.padx;
+			in.pad_y = a.pady;
+			in.fp32_mode = fp32 ? 1 : 0;
+			in.fs = fs;
+			b->comp_index.push_back((uint32_t)b->setup.size());
+			b->setup.push_back(in);
+			if (!pc.convolve) any_nc = true;
+		}
+		jb.ncomp = (uint32_t)b->setup.size() - jb.comp_off;
+		if (jb.ncomp > 32) return fail(set_err(PK_EINVAL, "model %u: at most 32 radial/sky profiles per model", mi));
+		jb.has_conv = a.conv_required ? 1 : 0;
+		jb.has_nc = (any_nc || !a.conv_required) ? 1 : 0;
+		jb.kw = a.conv_required ? m->psf_width : 0;
+		jb.kh = a.conv_required ? m->psf_height : 0;
+		jb.psf_off = psf_all.size();
+		if (a.conv_required) {
+			const size_t pn = (size_t)m->psf_width * m->psf_height;
+			double sum = 0;
+			for (size_t t = 0; t < pn; t++) sum += m->psf[t];
+			for (size_t t = 0; t < pn; t++) psf_all.push_back(sum > 0 ? m->psf[t] / sum : m->psf[t]);
+			b->max_kw = std::max(b->max_kw, m->psf_width);
+			b->max_kh = std::max(b->max_kh, m->psf_height);
+			b->any_conv = true;
+			jb.ctile_off = b->nctiles;
+			jb.ctiles_x = cdiv(a.dw, CT_W);
+			b->nctiles += jb.ctiles_x * cdiv(a.dh, CT_H);
+		} else {
+			jb.ctile_off = b->nctiles;
+			jb.ctiles_x = 0;
+		}
+		jb.conv_off = img_off;
+		jb.cw = a.dw;
+		jb.ch = a.dh;
+		jb.cox = (a.padx > 0 && a.pady > 0) ? (int)a.padx : 0;
+		jb.coy = (a.padx > 0 && a.pady > 0) ? (int)a.pady : 0;
+		jb.crw = a.dw - 2 * (uint32_t)jb.cox;
+		jb.crh = a.dh - 2 * (uint32_t)jb.coy;
+		jb.fs = 1;
+		uint32_t ow = jb.crw, oh = jb.crh;
+		if (fs > 1 && !m->return_finesampled) {
+			jb.fs = fs;
+			ow = cdiv(ow, fs);
+			oh = cdiv(oh, fs);
+		}
+		jb.ow = ow;
+		jb.oh = oh;
+		jb.out_off = out_off;
+		jb.otile_off = b->notiles;
+		jb.otiles_x = cdiv(ow, 64);
+		b->notiles += jb.otiles_x * cdiv(oh, 4);
+		jb.omask_off = NONE;
+		jb.omask_f = 1;
+		b->jobs.push_back(jb);
+		img_off += (uint64_t)a.dw * a.dh;
+		out_off += (uint64_t)ow * oh;
+	}
+	b->n_comps = (uint32_t)b->setup.size();
+	b->total_out = out_off;
+	if (conv_lds_bytes(b->max_kw, b->max_kh, elem) > LDS_LIMIT)
+		return fail(set_err(PK_EINVAL, "PSF of %ux%u exceeds the direct convolution tile", b->max_kw, b->max_kh));
+	if (img_off >= (1ull << 32)) return fail(set_err(PK_EINVAL, "batch too large (%llu pixels)", (unsigned long long)img_off));
+
+	Layout L;
+	b->o_jobs = L.take(sizeof(PkJob) * b->jobs.size());
+	b->o_setup = L.take(sizeof(SetupIn) * std::max<size_t>(1, b->setup.size()));
+	b->o_psfT = L.take(elem * std::max<size_t>(1, psf_all.size()));
+	const size_t in_bytes = L.off;
+	b->o_mask = L.take(1);
+	b->o_comps = L.take(sizeof(PkComp) * std::max<size_t>(1, b->setup.size()));
+	b->o_cnt = L.take(sizeof(PkCounters));
+	b->o_nc = L.take(elem * img_off);
+	b->o_c = L.take(elem * (b->any_conv ? img_off : 1));
+	b->o_conv = L.take(elem * (b->any_conv ? img_off : 1));
+	b->o_ent = L.take(sizeof(RefineEntry) * img_off);
+	b->o_out = L.take(elem * out_off);
+	if (hipMalloc(&b->dmem, L.off) != hipSuccess) {
+		(void)hipGetLastError();
+		return fail(set_err(PK_ENOMEM, "device allocation of %zu bytes failed", L.off));
+	}
+	std::vector<unsigned char> stage(in_bytes, 0);
+	std::memcpy(stage.data() + b->o_jobs, b->jobs.data(), sizeof(PkJob) * b->jobs.size());
+	if (!b->setup.empty()) std::memcpy(stage.data() + b->o_setup, b->setup.data(), sizeof(SetupIn) * b->setup.size());
+	for (size_t t = 0; t < psf_all.size(); t++) {
+		if (fp32) reinterpret_cast<float *>(stage.data() + b->o_psfT)[t] = (float)psf_all[t];
+		else reinterpret_cast<double *>(stage.data() + b->o_psfT)[t] = psf_all[t];
+	}
+	PK_HIP(hipMemcpy(b->dmem, stage.data(), in_bytes, hipMemcpyHostToDevice));
+	for (auto &e : b->ev)
+		if (hipEventCreate(&e) != hipSuccess) return fail(set_err(PK_EDEVICE, "hipEventCreate failed"));
+	*out = b;
+	return PK_OK;
+}
+
+int pk_batch_set_components(pk_batch *b, const pk_component *comps, uint32_t n)
+{
+	if (!b || !comps) return set_err(PK_EINVAL, "NULL argument");
+	if (n != b->comp_index.size()) return set_err(PK_EINVAL, "expected %zu components, got %u", b->comp_index.size(), n);
+	for (uint32_t i = 0; i < n; i++) {
+		if (comps[i].kind != b->comp_kind[i]) return set_err(PK_EINVAL, "component %u changed its kind", i);
+		if (b->comp_index[i] == ~0u) continue;
+		SetupIn &in = b->setup[b->comp_index[i]];
+		if ((comps[i].convolve != 0) != (in.c.convolve != 0))
+			return set_err(PK_EINVAL, "component %u changed its convolve flag", i);
+		PK_TRY(validate_component(comps[i], true));
+		in.c = comps[i];
+	}
+	std::lock_guard<std::mutex> lock(b->ctx->mu);
+	PK_HIP(hipSetDevice(b->ctx->device));
+	PK_HIP(hipMemcpyAsync(b->d<void>(b->o_setup), b->setup.data(), sizeof(SetupIn) * b->setup.size(),
+	                      hipMemcpyHostToDevice, b->ctx->stream));
+	PK_HIP(hipStreamSynchronize(b->ctx->stream));
+	return PK_OK;
+}
+
+int pk_batch_evaluate(pk_batch *b)
+{
+	if (!b) return set_err(PK_EINVAL, "batch is NULL");
+	std::lock_guard<std::mutex> lock(b->ctx->mu);
+	PK_HIP(hipSetDevice(b->ctx->device));
+	b->ran = true;
+	if (!(b->flags & PK_BATCH_GRAPH)) return batch_launch(b);
+	if (!b->exec) {
+		PK_HIP(hipStreamBeginCapture(b->ctx->stream, hipStreamCaptureModeThreadLocal));
+		int st = batch_launch(b);
+		hipGraph_t g = nullptr;
+		hipError_t e = hipStreamEndCapture(b->ctx->stream, &g);
+		if (st) {
+			if (g) (void)hipGraphDestroy(g);
+			return st;
+		}
+		if (e != hipSuccess) return set_err(PK_EDEVICE, "hipStreamEndCapture failed: %s", hipGetErrorString(e));
+		b->graph = g;
+		PK_HIP(hipGraphInstantiate(&b->exec, g, nullptr, nullptr, 0));
+	}
+	PK_HIP(hipGraphLaunch(b->exec, b->ctx->stream));
+	return PK_OK;
+}
+
+int pk_batch_sync(pk_batch *b, pk_batch_info *info)
+{
+	if (!b) return set_err(PK_EINVAL, "batch is NULL");
+	std::lock_guard<std::mutex> lock(b->ctx->mu);
+	PK_HIP(hipSetDevice(b->ctx->device));
+	PK_HIP(hipStreamSynchronize(b->ctx->stream));
+	PkCounters cnt{};
+	PK_HIP(hipMemcpy(&cnt, b->d<void>(b->o_cnt), sizeof(cnt), hipMemcpyDeviceToHost));
+	if (info) {
+		std::memset(info, 0, sizeof(*info));
+		info->n_refine_entries = cnt.n_entries;
+		info->n_subcells = cnt.n_subcells;
+		if (b->ran && !(b->flags & PK_BATCH_GRAPH)) {
+			float t[6] = {};
+			(void)hipEventElapsedTime(&t[0], b->ev[0], b->ev[1]);
+			(void)hipEventElapsedTime(&t[1], b->ev[1], b->ev[2]);
+			(void)hipEventElapsedTime(&t[2], b->ev[2], b->ev[3]);
+			(void)hipEventElapsedTime(&t[3], b->ev[3], b->ev[4]);
+			(void)hipEventElapsedTime(&t[4], b->ev[4], b->ev[5]);
+			(void)hipEventElapsedTime(&t[5], b->ev[0], b->ev[5]);
+			info->ms_setup = t[0];
+			info->ms_coarse = t[1];
+			info->ms_refine = t[2];
+			info->ms_conv = t[3];
+			info->ms_compose = t[4];
+			info->ms_total = t[5];
+		}
+	}
+	if (cnt.overflow)
+		return set_err(PK_ERANGE, "sub-pixel refinement needed more than %d pending cells for one pixel", RQ_CAP);
+	return PK_OK;
+}
+
+int pk_batch_image_dims(const pk_batch *b, uint32_t i, uint32_t *w, uint32_t *h)
+{
+	if (!b || i >= b->n_models) return set_err(PK_EINVAL, "bad batch or image index");
+	if (w) *w = b->jobs[i].ow;
+	if (h) *h = b->jobs[i].oh;
+	return PK_OK;
+}
+
+int pk_batch_get_image(pk_batch *b, uint32_t i, double *out, size_t cap)
+{
+	if (!b || !out || i >= b->n_models) return set_err(PK_EINVAL, "bad argument");
+	const PkJob &jb = b->jobs[i];
+	const size_t n = (size_t)jb.ow * jb.oh;
+	if (n > cap) return set_err(PK_ERANGE, "output buffer holds %zu doubles, %zu needed", cap, n);
+	std::lock_guard<std::mutex> lock(b->ctx->mu);
+	PK_HIP(hipSetDevice(b->ctx->device));
+	PK_HIP(hipStreamSynchronize(b->ctx->stream));
+	if (b->elem == 8) {
+		PK_HIP(hipMemcpy(out, b->d<double>(b->o_out) + jb.out_off, n * 8, hipMemcpyDeviceToHost));
+	} else {
+		std::vector<float> tmp(n);
+		PK_HIP(hipMemcpy(tmp.data(), b->d<float>(b->o_out) + jb.out_off, n * 4, hipMemcpyDeviceToHost));
+		for (size_t t = 0; t < n; t++) out[t] = tmp[t];
+	}
+	return PK_OK;
+}
+
+int pk_batch_device_images(const pk_batch *b, void **base, size_t *elem_size)
+{
+	if (!b) return set_err(PK_EINVAL, "batch is NULL");
+	if (base) *base = b->d<void>(b->o_out);
+	if (elem_size) *elem_size = b->elem;
+	return PK_OK;
+}
+
+int pk_batch_image_offset(const pk_batch *b, uint32_t i, uint64_t *offset)
+{
+	if (!b || i >= b->n_models) return set_err(PK_EINVAL, "bad batch or image index");
+	if (offset) *offset = b->jobs[i].out_off;
+	return PK_OK;
+}
+
+}  // extern "C"

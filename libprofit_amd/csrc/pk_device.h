@@ -479,6 +479,19 @@ __global__ __launch_bounds__(256) void compose_kernel(const PkJob *__restrict__ 
 	out_all[jb.out_off + i + (size_t)j * jb.ow] = (TO)v;
 }
 
+// counters of one evaluation are cleared by a kernel (not a memset node) so
+// that the launch sequence can be captured into a graph and replayed
+__global__ void reset_counters_kernel(PkCounters *c)
+{
+	if (threadIdx.x == 0) {
+		c->n_entries = 0;
+		c->next_entry = 0;
+		c->overflow = 0;
+		c->pad_ = 0;
+		c->n_subcells = 0;
+	}
+}
+
 // ---------------------------------------------------------------------------
 // set-up of all components on the device (batched path)
 // ---------------------------------------------------------------------------

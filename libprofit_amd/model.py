@@ -277,3 +277,54 @@ def evaluate_spec(ctx, spec):
     check(L.pk_evaluate(ctx.handle, spec.ref, out.ctypes.data, cap, C.byref(res)))
     img = out[: res.width * res.height].reshape(res.height, res.width).copy()
     return img, (res.offset_x, res.offset_y), res
+
+
+class Batch:
+    """Device-resident batch of models (pk_batch_* of include/profit_hip.h):
+    all images produced by one launch sequence, optionally a hipGraph."""
+
+    def __init__(self, ctx, specs, graph=True):
+        self.ctx = ctx
+        self.L = load()
+        self.specs = list(specs)  # keeps the host buffers alive
+        arr = (abi.Model * len(self.specs))(*[s.m for s in self.specs])
+        self._arr = arr
+        h = C.c_void_p()
+        check(self.L.pk_batch_create(ctx.handle, arr, len(self.specs), abi.PK_BATCH_GRAPH if graph else 0,
+                                     C.byref(h)))
+        self.handle = h
+        self.n = len(self.specs)
+
+    def evaluate(self):
+        check(self.L.pk_batch_evaluate(self.handle))
+
+    def sync(self):
+        info = abi.BatchInfo()
+        check(self.L.pk_batch_sync(self.handle, C.byref(info)))
+        return info
+
+    def set_components(self, comps):
+        arr = (abi.Component * len(comps))(*comps)
+        check(self.L.pk_batch_set_components(self.handle, arr, len(comps)))
+
+    def dims(self, i):
+        w, h = C.c_uint32(), C.c_uint32()
+        check(self.L.pk_batch_image_dims(self.handle, i, C.byref(w), C.byref(h)))
+        return w.value, h.value
+
+    def image(self, i):
+        w, h = self.dims(i)
+        out = np.zeros(w * h, dtype=np.float64)
+        check(self.L.pk_batch_get_image(self.handle, i, out.ctypes.data, out.size))
+        return out.reshape(h, w)
+
+    def close(self):
+        if self.handle:
+            self.L.pk_batch_destroy(self.handle)
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,87 @@
+"""Synthetic inputs of the benchmark configurations (seeded, deterministic).
+
+  single_sersic_psf(seed)   400x400 image, one Sersic convolved with a 25x25
+                            Moffat PSF (fwhm 3, con 2.5)
+  galaxy_batch(n, seed)     n square stamps with side ~U{32..512}, 1-3
+                            components drawn from the six radial profiles plus a
+                            sky level, each convolved with its own Moffat PSF of
+                            odd side ~U{15..41}
+All draws use numpy's PCG64 in the order written here.
+"""
+import numpy as np
+
+from . import abi
+
+RADIAL = ("sersic", "coresersic", "moffat", "ferrer", "king", "brokenexp")
+
+
+def moffat_psf_image(size, fwhm, con):
+    """A size x size Moffat PSF sampled at pixel centres, unit sum."""
+    rs = fwhm / (2 * np.sqrt(2 ** (1 / con) - 1))
+    c = (size - 1) / 2.0
+    y, x = np.mgrid[0:size, 0:size]
+    r2 = ((x - c) ** 2 + (y - c) ** 2) / rs ** 2
+    img = (1 + r2) ** (-con)
+    return img / img.sum()
+
+
+def single_sersic_psf(seed, psf=None):
+    rng = np.random.Generator(np.random.PCG64(seed))
+    nser = rng.uniform(0.5, 8)
+    re = rng.uniform(2, 50)
+    axrat = rng.uniform(0.1, 1)
+    ang = rng.uniform(0, 180)
+    mag = rng.uniform(12, 20)
+    xcen, ycen = rng.uniform(150, 250, size=2)
+    if psf is None:
+        psf = moffat_psf_image(25, 3.0, 2.5)
+    comp = abi.make_component("sersic", xcen=xcen, ycen=ycen, mag=mag, re=re, nser=nser, ang=ang, axrat=axrat,
+                              convolve=True)
+    return abi.ModelSpec(400, 400, [comp], psf=psf)
+
+
+def _profile_params(rng, name, s):
+    u = rng.uniform
+    if name == "sersic":
+        return dict(nser=u(0.5, 6), re=u(2, max(2.0, 0.2 * s)))
+    if name == "moffat":
+        return dict(fwhm=u(2, 10), con=u(1.5, 10))
+    if name == "ferrer":
+        return dict(rout=u(5, max(5.0, 0.3 * s)), a=u(0.3, 2), b=u(0, 1.5))
+    if name == "king":
+        return dict(rc=u(2, 8), rt=u(15, max(15.0, 0.3 * s)), a=u(1, 3))
+    if name == "brokenexp":
+        h1 = u(5, 15)
+        return dict(h1=h1, h2=u(2, h1), rb=u(5, 15), a=u(0.1, 1))
+    if name == "coresersic":
+        return dict(re=u(5, 20), rb=u(1, 5), nser=u(1, 6), a=u(1, 3), b=u(0, 1.5))
+    raise ValueError(name)
+
+
+def galaxy_batch(n=4096, seed=0, smin=32, smax=512):
+    """List of abi.ModelSpec, one stamp per galaxy."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    specs = []
+    for _ in range(n):
+        s = int(rng.integers(smin, smax + 1))
+        ncomp = int(rng.integers(1, 4))
+        comps = []
+        for _c in range(ncomp):
+            name = RADIAL[int(rng.integers(0, len(RADIAL)))]
+            xc, yc = rng.uniform(0.4 * s, 0.6 * s, size=2)
+            mag = rng.uniform(14, 20)
+            ang = rng.uniform(0, 180)
+            axrat = rng.uniform(0.2, 1)
+            params = _profile_params(rng, name, s)
+            comps.append(abi.make_component(name, xcen=xc, ycen=yc, mag=mag, ang=ang, axrat=axrat, box=0.0,
+                                            convolve=True, **params))
+        comps.append(abi.make_component("sky", bg=rng.uniform(0, 10)))
+        k = 2 * int(rng.integers(7, 21)) + 1  # odd, 15..41
+        fwhm = rng.uniform(2, 4)
+        con = rng.uniform(2, 5)
+        specs.append(abi.ModelSpec(s, s, comps, psf=moffat_psf_image(k, fwhm, con)))
+    return specs
+
+
+def pixel_count(specs):
+    return sum(int(s.m.width) * int(s.m.height) for s in specs)
