@@ -176,8 +176,19 @@ int pk_model_defaults(uint32_t width, uint32_t height, pk_model *m);
 
 /* Validates a model exactly as Model::analyze_inputs + Profile::validate do
  * (src/model.cpp:186-219, src/radial.cpp:204-214 and the per-profile validate
- * methods) and reports the dimensions and offset evaluate() would return. */
+ * methods) and reports its drawing dimensions (requested dimensions times
+ * finesampling plus the PSF padding, src/model.cpp:183). */
 int pk_model_output_dims(const pk_model *m, uint32_t *width, uint32_t *height);
+
+/* Per-component constants after libprofit's set-up
+ * (RadialProfile::initial_calculations, src/radial.cpp:137-199, with the
+ * profile overrides): normalisation and the adjusted sub-pixel integration
+ * parameters that libprofit writes back into the profile.  Host-side only. */
+typedef struct pk_constants {
+	double rscale, lumtot, ie, bn, acc, rscale_switch, rscale_max, flux_scale, cos_ang, sin_ang;
+	uint32_t resolution, max_recursions;
+} pk_constants;
+int pk_component_constants(const pk_model *m, uint32_t index, pk_constants *out);
 
 /* Model::evaluate(Point &offset_out) (src/model.cpp:259-320).
  * Writes the model image (FP64, row-major) into `out`, which must hold at

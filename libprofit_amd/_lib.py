@@ -52,6 +52,7 @@ def load():
         "pk_convolve": (C.c_int, [C.c_void_p, C.c_int, C.c_void_p, u32, u32, C.c_void_p, u32, u32, C.c_void_p,
                                   C.c_int, C.c_void_p, C.c_size_t, P(u32), P(u32), P(i32), P(i32)]),
         "pk_convolver_padding": (C.c_int, [C.c_int, u32, u32, u32, u32, P(i32), P(i32), P(i32), P(i32)]),
+        "pk_component_constants": (C.c_int, [P(abi.Model), u32, P(abi.Constants)]),
         "pk_batch_create": (C.c_int, [C.c_void_p, P(abi.Model), u32, u32, P(C.c_void_p)]),
         "pk_batch_set_components": (C.c_int, [C.c_void_p, P(abi.Component), u32]),
         "pk_batch_evaluate": (C.c_int, [C.c_void_p]),

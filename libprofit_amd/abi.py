@@ -107,6 +107,12 @@ class Result(C.Structure):
     ]
 
 
+class Constants(C.Structure):
+    _fields_ = [(n, C.c_double) for n in ("rscale", "lumtot", "ie", "bn", "acc", "rscale_switch", "rscale_max",
+                                          "flux_scale", "cos_ang", "sin_ang")] + [
+        ("resolution", C.c_uint32), ("max_recursions", C.c_uint32)]
+
+
 class BatchInfo(C.Structure):
     _fields_ = [
         ("n_refine_entries", C.c_uint64),

@@ -465,21 +465,107 @@ T *hptr(pk_ctx *ctx, size_t off) { return reinterpret_cast<T *>(static_cast<char
 
 uint32_t cdiv(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
 
-size_t conv_lds_bytes(uint32_t kw, uint32_t kh, size_t elem)
+size_t conv_lds_bytes(uint32_t kw, uint32_t rows, size_t elem)
 {
-	return (size_t)(CT_W + kw - 1) * (CT_H + kh - 1) * elem;
+	return (size_t)(CT_W + kw - 1) * (CT_H + rows - 1) * elem;
 }
 
 constexpr size_t LDS_LIMIT = 64 * 1024;
 
-template <typename T>
-int launch_conv(pk_ctx *ctx, const PkJob *djob, uint32_t ntiles, const T *src, const T *krn, const uint8_t *masks,
-                T *dst, uint32_t kw, uint32_t kh)
+// PSF rows per convolution launch so that the source tile fits in LDS
+// (all rows in one launch for every PSF up to ~41x41 in FP64).
+int conv_rows_per_launch(uint32_t kw, uint32_t kh, size_t elem, uint32_t &rows)
 {
-	size_t lds = conv_lds_bytes(kw, kh, sizeof(T));
-	if (lds > LDS_LIMIT)
-		return set_err(PK_EINVAL, "PSF of %ux%u exceeds the direct convolution tile (%zu bytes of LDS)", kw, kh, lds);
-	hipLaunchKernelGGL(conv_kernel<T>, dim3(ntiles), dim3(256), lds, ctx->stream, djob, 1u, src, krn, masks, dst);
+	rows = kh;
+	while (rows > 1 && conv_lds_bytes(kw, rows, elem) > LDS_LIMIT) rows = (rows + 1) / 2;
+	if (conv_lds_bytes(kw, rows, elem) > LDS_LIMIT)
+		return set_err(PK_EINVAL, "PSF width %u exceeds the direct convolution tile", kw);
+	return PK_OK;
+}
+
+// Reversed, transposed copy of a kw x kh PSF: kt[k * kh + l] = krn(kw-1-k, kh-1-l).
+template <typename T>
+void psf_conv_layout(const double *krn, uint32_t kw, uint32_t kh, T *kt)
+{
+	for (uint32_t k = 0; k < kw; k++)
+		for (uint32_t l = 0; l < kh; l++) kt[(size_t)k * kh + l] = (T)krn[(kw - 1 - k) + (size_t)(kh - 1 - l) * kw];
+}
+
+template <typename T>
+int launch_conv(hipStream_t s, const PkJob *djobs, uint32_t njobs, uint32_t ntiles, const T *src, const T *kt,
+                const uint8_t *masks, T *dst, uint32_t max_kw, uint32_t max_kh)
+{
+	uint32_t rows;
+	PK_TRY(conv_rows_per_launch(max_kw, max_kh, sizeof(T), rows));
+	const size_t lds = conv_lds_bytes(max_kw, rows, sizeof(T));
+	for (uint32_t l0 = 0; l0 < max_kh; l0 += rows) {
+		hipLaunchKernelGGL(conv_kernel<T>, dim3(ntiles), dim3(256), lds, s, djobs, njobs, src, kt, masks, dst,
+		                   (int)l0, (int)rows);
+		PK_HIP(hipGetLastError());
+	}
+	return PK_OK;
+}
+
+constexpr uint32_t SPILL_CAP_SINGLE = 1u << 20;
+constexpr uint32_t SPILL_CAP_BATCH = 1u << 22;
+
+// Fixed-point scale of the FP32-mode refinement accumulators of one model:
+// 2^60 over the summed total flux of its radial profiles (no pixel can hold
+// more than a profile's total flux, so the sums stay below 2^63).
+double fixed_point_scale(const pk_model *m)
+{
+	double f = 0;
+	for (uint32_t c = 0; c < m->n_components; c++) {
+		const int k = m->components[c].kind;
+		if (k == PK_PROFILE_SKY || k == PK_PROFILE_NULL || k == PK_PROFILE_PSF) continue;
+		f += std::pow(10, -0.4 * (m->components[c].mag - m->magzero));
+	}
+	if (!(f > 0) || !std::isfinite(f)) return 1.0;
+	return std::ldexp(1.0, 60) / f;
+}
+
+// The FP32-mode refinement stores level, resolution and recursion depth in
+// 8 bits each.
+int check_fp32_limits(const pk_model *m)
+{
+	for (uint32_t c = 0; c < m->n_components; c++) {
+		const pk_component &pc = m->components[c];
+		const int k = pc.kind;
+		if (k == PK_PROFILE_SKY || k == PK_PROFILE_NULL || k == PK_PROFILE_PSF || pc.adjust) continue;
+		if (pc.resolution == 0 || pc.resolution > 255)
+			return set_err(PK_EINVAL, "resolution %u: the FP32 path supports 1..255", pc.resolution);
+	}
+	return PK_OK;
+}
+
+// coarse pass + refinement of all radial profiles; FP32 mode uses the
+// fixed-point wavefront-packed refinement, FP64 mode the exact one.
+template <typename TI>
+int launch_profiles(hipStream_t s, bool fp32, const PkJob *djobs, uint32_t njobs, uint32_t ntiles,
+                    const PkComp *dcomp, const CompLite *dlite, const uint8_t *dmask, TI *img_nc, TI *img_c,
+                    RefineEntry *dent, PkCounters *dcnt, unsigned long long *dacc, RCell *dspill, uint32_t spill_cap,
+                    uint32_t ref_blocks, bool any_comp, hipEvent_t ev_coarse)
+{
+	if (fp32) {
+		hipLaunchKernelGGL((coarse_kernel<TI, true>), dim3(ntiles), dim3(256), 0, s, djobs, njobs, dcomp, dmask, img_nc,
+		                   img_c, dent, dcnt, dacc);
+	} else {
+		hipLaunchKernelGGL((coarse_kernel<TI, false>), dim3(ntiles), dim3(256), 0, s, djobs, njobs, dcomp, dmask,
+		                   img_nc, img_c, dent, dcnt, (unsigned long long *)nullptr);
+	}
+	PK_HIP(hipGetLastError());
+	if (ev_coarse) PK_HIP(hipEventRecord(ev_coarse, s));
+	if (!any_comp) return PK_OK;
+	if (fp32) {
+		hipLaunchKernelGGL(refine32_kernel<0>, dim3(ref_blocks), dim3(64), 0, s, djobs, dcomp, dlite, dent, dcnt,
+		                   dspill, spill_cap, dacc);
+		hipLaunchKernelGGL(refine32_kernel<1>, dim3(ref_blocks), dim3(64), 0, s, djobs, dcomp, dlite, dent, dcnt,
+		                   dspill, spill_cap, dacc);
+		hipLaunchKernelGGL(finalize32_kernel<TI>, dim3(1024), dim3(256), 0, s, djobs, dent, dcnt, dacc, img_nc, img_c);
+	} else {
+		hipLaunchKernelGGL((refine_kernel<TI, false>), dim3(ref_blocks), dim3(64), 0, s, djobs, dcomp, dent, dcnt,
+		                   img_nc, img_c);
+	}
 	PK_HIP(hipGetLastError());
 	return PK_OK;
 }
@@ -544,6 +630,7 @@ int evaluate_impl(pk_ctx *ctx, const pk_model *m, const Analysis &a, double *out
 	// components: radial + sky go to the coarse/refine kernels, psf profiles
 	// become rebinning items, null profiles do nothing
 	std::vector<PkComp> comps;
+	std::vector<CompLite> lites;
 	std::vector<PkPsfItem> items;
 	bool any_nc = false;
 	for (uint32_t c = 0; c < m->n_components; c++) {
@@ -591,10 +678,12 @@ int evaluate_impl(pk_ctx *ctx, const pk_model *m, const Analysis &a, double *out
 		in.pad_y = a.pady;
 		in.fp32_mode = fp32 ? 1 : 0;
 		in.fs = fs;
+		in.fx_scale = fixed_point_scale(m);
 		PkComp pcmp;
 		setup_component(in, pcmp);
 		if (!pc.convolve) any_nc = true;
 		comps.push_back(pcmp);
+		lites.push_back(make_lite(pcmp, in));
 	}
 	if (comps.size() > 32) return set_err(PK_EINVAL, "at most 32 radial/sky profiles per model are supported");
 
@@ -684,6 +773,7 @@ int evaluate_impl(pk_ctx *ctx, const pk_model *m, const Analysis &a, double *out
 	const size_t o_job = L.take(sizeof(PkJob));
 	const size_t o_cnt = L.take(sizeof(PkCounters));
 	const size_t o_comp = L.take(sizeof(PkComp) * std::max<size_t>(1, comps.size()));
+	const size_t o_lite = L.take(sizeof(CompLite) * std::max<size_t>(1, lites.size()));
 	const size_t o_items = L.take(sizeof(PkPsfItem) * std::max<size_t>(1, items.size()));
 	const size_t o_psf64 = L.take(sizeof(double) * std::max<size_t>(1, psf.size()));
 	const size_t o_psfT = L.take(sizeof(TI) * std::max<size_t>(1, psf.size()));
@@ -694,6 +784,9 @@ int evaluate_impl(pk_ctx *ctx, const pk_model *m, const Analysis &a, double *out
 	const size_t o_c = L.take(sizeof(TI) * (a.conv_required ? npix : 1));
 	const size_t o_conv = L.take(sizeof(TI) * (a.conv_required ? ncpix : 1));
 	const size_t o_ent = L.take(sizeof(RefineEntry) * npix);
+	const size_t o_acc = L.take(fp32 ? 2 * sizeof(unsigned long long) * npix : 8);
+	const uint32_t spill_cap = fp32 ? SPILL_CAP_SINGLE : 0;
+	const size_t o_spill = L.take(sizeof(RCell) * std::max<uint32_t>(1, spill_cap));
 	const size_t o_out = L.take(sizeof(double) * (size_t)ow * oh);
 	PK_TRY(ensure_dev(ctx, L.off));
 	PK_TRY(ensure_host(ctx, std::max(in_bytes, sizeof(double) * (size_t)ow * oh + 256 + sizeof(PkCounters))));
@@ -701,6 +794,7 @@ int evaluate_impl(pk_ctx *ctx, const pk_model *m, const Analysis &a, double *out
 	jb.mask_off = use_mask ? 0 : NONE;
 	jb.psf_off = 0;
 	jb.conv_off = 0;
+	jb.fx_scale = fixed_point_scale(m);
 	if (adjusted) jb.omask_off = 0;
 	for (auto &it : items) it.psf_off = 0;
 
@@ -708,11 +802,12 @@ int evaluate_impl(pk_ctx *ctx, const pk_model *m, const Analysis &a, double *out
 	std::memset(ctx->hmem, 0, in_bytes);
 	*hptr<PkJob>(ctx, o_job) = jb;
 	if (!comps.empty()) std::memcpy(hptr<void>(ctx, o_comp), comps.data(), sizeof(PkComp) * comps.size());
+	if (!lites.empty()) std::memcpy(hptr<void>(ctx, o_lite), lites.data(), sizeof(CompLite) * lites.size());
 	if (!items.empty()) std::memcpy(hptr<void>(ctx, o_items), items.data(), sizeof(PkPsfItem) * items.size());
 	if (!psf.empty()) {
 		std::memcpy(hptr<void>(ctx, o_psf64), psf.data(), sizeof(double) * psf.size());
 		TI *pt = hptr<TI>(ctx, o_psfT);
-		for (size_t t = 0; t < psf.size(); t++) pt[t] = (TI)psf[t];
+		psf_conv_layout<TI>(psf.data(), m->psf_width, m->psf_height, pt);
 	}
 	if (use_mask) std::memcpy(hptr<void>(ctx, o_mask), dmask.data(), npix);
 	if (adjusted) {
@@ -733,25 +828,10 @@ int evaluate_impl(pk_ctx *ctx, const pk_model *m, const Analysis &a, double *out
 	const PkComp *dcomp = dptr<PkComp>(ctx, o_comp);
 
 	const uint32_t ntiles = jb.tiles_x * cdiv(a.dh, 4);
-	if (fp32) {
-		hipLaunchKernelGGL((coarse_kernel<TI, true>), dim3(ntiles), dim3(256), 0, s, djob, 1u, dcomp, dm, img_nc, img_c,
-		                   dent, dcnt);
-	} else {
-		hipLaunchKernelGGL((coarse_kernel<TI, false>), dim3(ntiles), dim3(256), 0, s, djob, 1u, dcomp, dm, img_nc,
-		                   img_c, dent, dcnt);
-	}
-	PK_HIP(hipGetLastError());
 	const uint32_t nref_blocks = (uint32_t)std::min<size_t>(npix, 2048);
-	if (!comps.empty()) {
-		if (fp32) {
-			hipLaunchKernelGGL((refine_kernel<TI, true>), dim3(nref_blocks), dim3(64), 0, s, djob, dcomp, dent, dcnt,
-			                   img_nc, img_c);
-		} else {
-			hipLaunchKernelGGL((refine_kernel<TI, false>), dim3(nref_blocks), dim3(64), 0, s, djob, dcomp, dent, dcnt,
-			                   img_nc, img_c);
-		}
-		PK_HIP(hipGetLastError());
-	}
+	PK_TRY(launch_profiles<TI>(s, fp32, djob, 1u, ntiles, dcomp, dptr<CompLite>(ctx, o_lite), dm, img_nc, img_c, dent, dcnt,
+	                           dptr<unsigned long long>(ctx, o_acc), dptr<RCell>(ctx, o_spill), spill_cap, nref_blocks,
+	                           !comps.empty(), nullptr));
 	for (size_t k = 0; k < items.size(); k++) {
 		const auto &it = items[k];
 		if (it.x1 < it.x0 || it.y1 < it.y0) continue;
@@ -764,7 +844,7 @@ int evaluate_impl(pk_ctx *ctx, const pk_model *m, const Analysis &a, double *out
 	PK_HIP(hipEventRecord(ctx->ev[1], s));
 	if (a.conv_required) {
 		const uint32_t nct = jb.ctiles_x * cdiv(jb.ch, CT_H);
-		PK_TRY(launch_conv<TI>(ctx, djob, nct, img_c, dptr<TI>(ctx, o_psfT), dm, dptr<TI>(ctx, o_conv), jb.kw, jb.kh));
+		PK_TRY(launch_conv<TI>(s, djob, 1u, nct, img_c, dptr<TI>(ctx, o_psfT), dm, dptr<TI>(ctx, o_conv), jb.kw, jb.kh));
 	}
 	PK_HIP(hipEventRecord(ctx->ev[2], s));
 	const uint32_t notiles = jb.otiles_x * cdiv(oh, 4);
@@ -781,7 +861,8 @@ int evaluate_impl(pk_ctx *ctx, const pk_model *m, const Analysis &a, double *out
 	PK_HIP(hipStreamSynchronize(s));
 	const PkCounters cnt = *hptr<PkCounters>(ctx, ho_cnt);
 	if (cnt.overflow)
-		return set_err(PK_ERANGE, "sub-pixel refinement needed more than %d pending cells for one pixel", RQ_CAP);
+		return set_err(PK_ERANGE, "sub-pixel refinement work queue capacity exceeded");
+	if (cnt.n_bad) return set_err(PK_ERANGE, "sub-pixel refinement produced %u non-finite values", cnt.n_bad);
 	std::memcpy(out, ctx->hmem, out_bytes);
 	if (res) {
 		res->width = ow;
@@ -817,6 +898,43 @@ int pk_model_output_dims(const pk_model *m, uint32_t *width, uint32_t *height)
 	return PK_OK;
 }
 
+int pk_component_constants(const pk_model *m, uint32_t index, pk_constants *out)
+{
+	if (!m || !out) return set_err(PK_EINVAL, "NULL argument");
+	Analysis a;
+	PK_TRY(analyze(m, a));
+	if (index >= m->n_components) return set_err(PK_EINVAL, "component index %u out of range", index);
+	const pk_component &pc = m->components[index];
+	if (pc.kind == PK_PROFILE_NULL || pc.kind == PK_PROFILE_SKY || pc.kind == PK_PROFILE_PSF)
+		return set_err(PK_EINVAL, "component %u is not a radial profile", index);
+	const uint32_t fs = m->finesampling;
+	SetupIn in{};
+	in.c = pc;
+	in.magzero = m->magzero;
+	in.sx = m->scale_x / fs;
+	in.sy = m->scale_y / fs;
+	in.img_sx = m->scale_x;
+	in.img_sy = m->scale_y;
+	in.pad_x = a.padx;
+	in.pad_y = a.pady;
+	in.fs = fs;
+	PkComp c;
+	setup_component(in, c);
+	out->rscale = c.rscale;
+	out->lumtot = c.lumtot;
+	out->ie = c.ie;
+	out->bn = c.bn;
+	out->acc = c.acc;
+	out->rscale_switch = c.rscale_switch;
+	out->rscale_max = c.rscale_max;
+	out->flux_scale = c.flux_scale;
+	out->cos_ang = c.cos_ang;
+	out->sin_ang = c.sin_ang;
+	out->resolution = (uint32_t)c.res;
+	out->max_recursions = (uint32_t)c.max_rec;
+	return PK_OK;
+}
+
 int pk_evaluate(pk_ctx *ctx, const pk_model *m, double *out, size_t out_capacity, pk_result *res)
 {
 	if (!ctx) return set_err(PK_EINVAL, "context is NULL");
@@ -838,7 +956,10 @@ int pk_evaluate(pk_ctx *ctx, const pk_model *m, double *out, size_t out_capacity
 		}
 		return PK_OK;
 	}
-	if (ctx->precision == PK_FP32) return evaluate_impl<float>(ctx, m, a, out, out_capacity, res);
+	if (ctx->precision == PK_FP32) {
+		PK_TRY(check_fp32_limits(m));
+		return evaluate_impl<float>(ctx, m, a, out, out_capacity, res);
+	}
 	return evaluate_impl<double>(ctx, m, a, out, out_capacity, res);
 }
 
@@ -906,7 +1027,7 @@ int convolve_impl(pk_ctx *ctx, int type, const double *src, uint32_t sw, uint32_
 	T *hs = hptr<T>(ctx, o_src);
 	for (size_t t = 0; t < ns; t++) hs[t] = (T)src[t];
 	T *hk = hptr<T>(ctx, o_krn);
-	for (size_t t = 0; t < nk; t++) hk[t] = (T)krn[t];
+	psf_conv_layout<T>(krn, kw, kh, hk);
 	if (mask) {
 		uint8_t *hm = hptr<uint8_t>(ctx, o_mask);
 		for (size_t t = 0; t < ns; t++) hm[t] = mask[t] ? 1 : 0;
@@ -915,7 +1036,7 @@ int convolve_impl(pk_ctx *ctx, int type, const double *src, uint32_t sw, uint32_
 	PK_HIP(hipMemcpyAsync(ctx->dmem, ctx->hmem, in_bytes, hipMemcpyHostToDevice, s));
 	const PkJob *djob = dptr<PkJob>(ctx, o_job);
 	const uint32_t nct = jb.ctiles_x * cdiv(g.ch, CT_H);
-	PK_TRY(launch_conv<T>(ctx, djob, nct, dptr<T>(ctx, o_src), dptr<T>(ctx, o_krn), dptr<uint8_t>(ctx, o_mask),
+	PK_TRY(launch_conv<T>(s, djob, 1u, nct, dptr<T>(ctx, o_src), dptr<T>(ctx, o_krn), dptr<uint8_t>(ctx, o_mask),
 	                      dptr<T>(ctx, o_conv), kw, kh));
 	hipLaunchKernelGGL((compose_kernel<T, double>), dim3(jb.otiles_x * cdiv(g.ch, 4)), dim3(256), 0, s, djob, 1u,
 	                   dptr<T>(ctx, o_conv), dptr<T>(ctx, o_conv), (const uint8_t *)nullptr,
@@ -991,7 +1112,9 @@ struct pk_batch {
 	// device memory
 	void *dmem = nullptr;
 	size_t o_jobs = 0, o_setup = 0, o_comps = 0, o_cnt = 0, o_psfT = 0, o_mask = 0, o_nc = 0, o_c = 0, o_conv = 0,
-	       o_ent = 0, o_out = 0;
+	       o_ent = 0, o_acc = 0, o_spill = 0, o_out = 0, o_lite = 0;
+	uint32_t spill_cap = 0;
+	std::vector<double> fx_scales;
 	uint64_t total_out = 0;
 	hipGraph_t graph = nullptr;
 	hipGraphExec_t exec = nullptr;
@@ -1021,22 +1144,20 @@ int batch_launch(pk_batch *b)
 	hipLaunchKernelGGL(reset_counters_kernel, dim3(1), dim3(64), 0, s, dcnt);
 	if (b->n_comps)
 		hipLaunchKernelGGL(setup_kernel, dim3(cdiv(b->n_comps, 64)), dim3(64), 0, s, b->d<SetupIn>(b->o_setup), dcomp,
+		                   fp32 ? b->d<CompLite>(b->o_lite) : (CompLite *)nullptr,
 		                   b->n_comps);
 	PK_HIP(hipGetLastError());
 	if (timed) PK_HIP(hipEventRecord(b->ev[1], s));
 	if (fp32) {
 		float *nc = b->d<float>(b->o_nc), *c = b->d<float>(b->o_c);
-		hipLaunchKernelGGL((coarse_kernel<float, true>), dim3(b->ntiles), dim3(256), 0, s, djobs, nj, dcomp, dmask,
-		                   nc, c, dent, dcnt);
-		PK_HIP(hipGetLastError());
-		if (timed) PK_HIP(hipEventRecord(b->ev[2], s));
-		hipLaunchKernelGGL((refine_kernel<float, true>), dim3(4096), dim3(64), 0, s, djobs, dcomp, dent, dcnt, nc, c);
-		PK_HIP(hipGetLastError());
+		PK_TRY(launch_profiles<float>(s, true, djobs, nj, b->ntiles, dcomp, b->d<CompLite>(b->o_lite), dmask, nc, c,
+		                              dent, dcnt,
+		                              b->d<unsigned long long>(b->o_acc), b->d<RCell>(b->o_spill), b->spill_cap, 4096,
+		                              b->n_comps > 0, timed ? b->ev[2] : nullptr));
 		if (timed) PK_HIP(hipEventRecord(b->ev[3], s));
 		if (b->any_conv) {
-			const size_t lds = conv_lds_bytes(b->max_kw, b->max_kh, sizeof(float));
-			hipLaunchKernelGGL(conv_kernel<float>, dim3(b->nctiles), dim3(256), lds, s, djobs, nj, c,
-			                   b->d<float>(b->o_psfT), dmask, b->d<float>(b->o_conv));
+			PK_TRY(launch_conv<float>(s, djobs, nj, b->nctiles, c, b->d<float>(b->o_psfT), dmask, b->d<float>(b->o_conv),
+			                          b->max_kw, b->max_kh));
 			PK_HIP(hipGetLastError());
 		}
 		if (timed) PK_HIP(hipEventRecord(b->ev[4], s));
@@ -1044,17 +1165,13 @@ int batch_launch(pk_batch *b)
 		                   b->d<float>(b->o_conv), (const uint8_t *)nullptr, b->d<float>(b->o_out));
 	} else {
 		double *nc = b->d<double>(b->o_nc), *c = b->d<double>(b->o_c);
-		hipLaunchKernelGGL((coarse_kernel<double, false>), dim3(b->ntiles), dim3(256), 0, s, djobs, nj, dcomp, dmask,
-		                   nc, c, dent, dcnt);
-		PK_HIP(hipGetLastError());
-		if (timed) PK_HIP(hipEventRecord(b->ev[2], s));
-		hipLaunchKernelGGL((refine_kernel<double, false>), dim3(4096), dim3(64), 0, s, djobs, dcomp, dent, dcnt, nc, c);
-		PK_HIP(hipGetLastError());
+		PK_TRY(launch_profiles<double>(s, false, djobs, nj, b->ntiles, dcomp, nullptr, dmask, nc, c, dent, dcnt, nullptr,
+		                               nullptr,
+		                               0, 4096, b->n_comps > 0, timed ? b->ev[2] : nullptr));
 		if (timed) PK_HIP(hipEventRecord(b->ev[3], s));
 		if (b->any_conv) {
-			const size_t lds = conv_lds_bytes(b->max_kw, b->max_kh, sizeof(double));
-			hipLaunchKernelGGL(conv_kernel<double>, dim3(b->nctiles), dim3(256), lds, s, djobs, nj, c,
-			                   b->d<double>(b->o_psfT), dmask, b->d<double>(b->o_conv));
+			PK_TRY(launch_conv<double>(s, djobs, nj, b->nctiles, c, b->d<double>(b->o_psfT), dmask,
+			                           b->d<double>(b->o_conv), b->max_kw, b->max_kh));
 			PK_HIP(hipGetLastError());
 		}
 		if (timed) PK_HIP(hipEventRecord(b->ev[4], s));
@@ -1110,6 +1227,7 @@ int pk_batch_create(pk_ctx *ctx, const pk_model *models, uint32_t n_models, uint
 		int st = analyze(m, a);
 		if (st) return fail(set_err(st, "model %u: %s", mi, g_err.c_str()));
 		if (m->dry_run) return fail(set_err(PK_EINVAL, "model %u: dry_run is not supported in a batch", mi));
+		if (fp32 && check_fp32_limits(m)) return fail(set_err(PK_EINVAL, "model %u: %s", mi, g_err.c_str()));
 		if (a.conv_required && m->convolver == PK_CONV_FFT)
 			return fail(set_err(PK_EINVAL, "model %u: the FFT convolver is not supported in a batch", mi));
 		if (has_mask(m))
@@ -1198,14 +1316,18 @@ int pk_batch_create(pk_ctx *ctx, const pk_model *models, uint32_t n_models, uint
 		b->notiles += jb.otiles_x * cdiv(oh, 4);
 		jb.omask_off = NONE;
 		jb.omask_f = 1;
+		jb.fx_scale = fixed_point_scale(m);
+		for (uint32_t k = jb.comp_off; k < jb.comp_off + jb.ncomp; k++) b->setup[k].fx_scale = jb.fx_scale;
 		b->jobs.push_back(jb);
 		img_off += (uint64_t)a.dw * a.dh;
 		out_off += (uint64_t)ow * oh;
 	}
 	b->n_comps = (uint32_t)b->setup.size();
 	b->total_out = out_off;
-	if (conv_lds_bytes(b->max_kw, b->max_kh, elem) > LDS_LIMIT)
-		return fail(set_err(PK_EINVAL, "PSF of %ux%u exceeds the direct convolution tile", b->max_kw, b->max_kh));
+	{
+		uint32_t rows;
+		if (b->any_conv && conv_rows_per_launch(b->max_kw, b->max_kh, elem, rows)) return fail(PK_EINVAL);
+	}
 	if (img_off >= (1ull << 32)) return fail(set_err(PK_EINVAL, "batch too large (%llu pixels)", (unsigned long long)img_off));
 
 	Layout L;
@@ -1215,11 +1337,15 @@ int pk_batch_create(pk_ctx *ctx, const pk_model *models, uint32_t n_models, uint
 	const size_t in_bytes = L.off;
 	b->o_mask = L.take(1);
 	b->o_comps = L.take(sizeof(PkComp) * std::max<size_t>(1, b->setup.size()));
+	b->o_lite = L.take(sizeof(CompLite) * std::max<size_t>(1, b->setup.size()));
 	b->o_cnt = L.take(sizeof(PkCounters));
 	b->o_nc = L.take(elem * img_off);
 	b->o_c = L.take(elem * (b->any_conv ? img_off : 1));
 	b->o_conv = L.take(elem * (b->any_conv ? img_off : 1));
 	b->o_ent = L.take(sizeof(RefineEntry) * img_off);
+	b->o_acc = L.take(fp32 ? 2 * sizeof(unsigned long long) * img_off : 8);
+	b->spill_cap = fp32 ? SPILL_CAP_BATCH : 0;
+	b->o_spill = L.take(sizeof(RCell) * std::max<uint32_t>(1, b->spill_cap));
 	b->o_out = L.take(elem * out_off);
 	if (hipMalloc(&b->dmem, L.off) != hipSuccess) {
 		(void)hipGetLastError();
@@ -1228,9 +1354,14 @@ int pk_batch_create(pk_ctx *ctx, const pk_model *models, uint32_t n_models, uint
 	std::vector<unsigned char> stage(in_bytes, 0);
 	std::memcpy(stage.data() + b->o_jobs, b->jobs.data(), sizeof(PkJob) * b->jobs.size());
 	if (!b->setup.empty()) std::memcpy(stage.data() + b->o_setup, b->setup.data(), sizeof(SetupIn) * b->setup.size());
-	for (size_t t = 0; t < psf_all.size(); t++) {
-		if (fp32) reinterpret_cast<float *>(stage.data() + b->o_psfT)[t] = (float)psf_all[t];
-		else reinterpret_cast<double *>(stage.data() + b->o_psfT)[t] = psf_all[t];
+	for (const auto &jb : b->jobs) {
+		if (!jb.has_conv) continue;
+		if (fp32)
+			psf_conv_layout<float>(psf_all.data() + jb.psf_off, jb.kw, jb.kh,
+			                       reinterpret_cast<float *>(stage.data() + b->o_psfT) + jb.psf_off);
+		else
+			psf_conv_layout<double>(psf_all.data() + jb.psf_off, jb.kw, jb.kh,
+			                        reinterpret_cast<double *>(stage.data() + b->o_psfT) + jb.psf_off);
 	}
 	PK_HIP(hipMemcpy(b->dmem, stage.data(), in_bytes, hipMemcpyHostToDevice));
 	for (auto &e : b->ev)
@@ -1250,10 +1381,25 @@ int pk_batch_set_components(pk_batch *b, const pk_component *comps, uint32_t n)
 		if ((comps[i].convolve != 0) != (in.c.convolve != 0))
 			return set_err(PK_EINVAL, "component %u changed its convolve flag", i);
 		PK_TRY(validate_component(comps[i], true));
+		if (b->elem == 4 && !comps[i].adjust && (comps[i].resolution == 0 || comps[i].resolution > 255))
+			return set_err(PK_EINVAL, "resolution %u: the FP32 path supports 1..255", comps[i].resolution);
 		in.c = comps[i];
+	}
+	// fixed-point scales follow the new magnitudes
+	for (auto &jb : b->jobs) {
+		double f = 0;
+		for (uint32_t k = jb.comp_off; k < jb.comp_off + jb.ncomp; k++) {
+			const SetupIn &in = b->setup[k];
+			if (in.c.kind == PK_PROFILE_SKY) continue;
+			f += std::pow(10, -0.4 * (in.c.mag - in.magzero));
+		}
+		jb.fx_scale = (f > 0 && std::isfinite(f)) ? std::ldexp(1.0, 60) / f : 1.0;
+		for (uint32_t k = jb.comp_off; k < jb.comp_off + jb.ncomp; k++) b->setup[k].fx_scale = jb.fx_scale;
 	}
 	std::lock_guard<std::mutex> lock(b->ctx->mu);
 	PK_HIP(hipSetDevice(b->ctx->device));
+	PK_HIP(hipMemcpyAsync(b->d<void>(b->o_jobs), b->jobs.data(), sizeof(PkJob) * b->jobs.size(),
+	                      hipMemcpyHostToDevice, b->ctx->stream));
 	PK_HIP(hipMemcpyAsync(b->d<void>(b->o_setup), b->setup.data(), sizeof(SetupIn) * b->setup.size(),
 	                      hipMemcpyHostToDevice, b->ctx->stream));
 	PK_HIP(hipStreamSynchronize(b->ctx->stream));
@@ -1312,8 +1458,8 @@ int pk_batch_sync(pk_batch *b, pk_batch_info *info)
 			info->ms_total = t[5];
 		}
 	}
-	if (cnt.overflow)
-		return set_err(PK_ERANGE, "sub-pixel refinement needed more than %d pending cells for one pixel", RQ_CAP);
+	if (cnt.overflow) return set_err(PK_ERANGE, "sub-pixel refinement work queue capacity exceeded");
+	if (cnt.n_bad) return set_err(PK_ERANGE, "sub-pixel refinement produced %u non-finite values", cnt.n_bad);
 	return PK_OK;
 }
 

@@ -59,7 +59,8 @@ __global__ __launch_bounds__(256) void coarse_kernel(const PkJob *__restrict__ j
                                                      const PkComp *__restrict__ comps,
                                                      const uint8_t *__restrict__ masks, TI *__restrict__ img_nc,
                                                      TI *__restrict__ img_c, RefineEntry *__restrict__ entries,
-                                                     PkCounters *__restrict__ counters)
+                                                     PkCounters *__restrict__ counters,
+                                                     unsigned long long *__restrict__ acc)
 {
 	const uint32_t jn = find_job(jobs, njobs, blockIdx.x, 0);
 	const PkJob &jb = jobs[jn];
@@ -87,7 +88,8 @@ __global__ __launch_bounds__(256) void coarse_kernel(const PkJob *__restrict__ j
 					double r = sqrt(xp * xp + yp * yp);
 					if (cp.rscale_max > 0 && r / cp.rscale > cp.rscale_max) continue;
 					if (cp.rough || r / cp.rscale > cp.rscale_switch) {
-						if (FP32 && cp.fp32) v = (double)evaluate_at32(cp, (float)xp, (float)yp) * cp.unfold;
+						if (FP32 && cp.fp32)
+							v = (double)evaluate_at32(cp, (float)xp, (float)yp, radius2_64(cp, xp, yp)) * cp.unfold;
 						else v = evaluate_at(cp, xp, yp);
 						v *= cp.flux_scale;
 					} else {
@@ -112,7 +114,13 @@ __global__ __launch_bounds__(256) void coarse_kernel(const PkJob *__restrict__ j
 		uint32_t base = 0;
 		if (lane_id() == leader) base = atomicAdd(&counters->n_entries, n);
 		base = __shfl(base, (int)leader);
-		if (need) entries[base + rank] = RefineEntry{jn, pix, cmask};
+		if (need) {
+			entries[base + rank] = RefineEntry{jn, pix, cmask};
+			if (acc) {
+				acc[2 * (size_t)(base + rank)] = 0;
+				acc[2 * (size_t)(base + rank) + 1] = 0;
+			}
+		}
 	}
 }
 
@@ -120,6 +128,7 @@ __global__ __launch_bounds__(256) void coarse_kernel(const PkJob *__restrict__ j
 // refinement
 // ---------------------------------------------------------------------------
 constexpr int RQ_CAP = 768;
+constexpr uint32_t REF_CHUNK = 16;
 
 struct RefStack {
 	double x0[RQ_CAP], x1[RQ_CAP], y0[RQ_CAP], y1[RQ_CAP];
@@ -203,8 +212,8 @@ __device__ double refine_pixel(const PkComp &cp, RefStack &st, double x0, double
 					const double txp = fabs(xp), typ = fabs(yp) + fabs(dyp);
 					bool decided = false;
 					if (lane_fp32) {
-						const float v32 = evaluate_at32(cp, (float)xp, (float)yp);
-						const float t32 = evaluate_at32(cp, (float)txp, (float)typ);
+						const float v32 = evaluate_at32(cp, (float)xp, (float)yp, radius2_64(cp, xp, yp));
+						const float t32 = evaluate_at32(cp, (float)txp, (float)typ, radius2_64(cp, txp, typ));
 						const float m = fabsf(t32 / v32 - 1.f);
 						const double guard = 1e-4 * (1.0 + acc_thr);
 						if (v32 > 0.f && isfinite(m) && fabs((double)m - acc_thr) > guard) {
@@ -221,7 +230,8 @@ __device__ double refine_pixel(const PkComp &cp, RefStack &st, double x0, double
 						nev += 2;
 					}
 				} else {
-					if (lane_fp32) val = (double)evaluate_at32(cp, (float)xp, (float)yp) * cp.unfold;
+					if (lane_fp32)
+						val = (double)evaluate_at32(cp, (float)xp, (float)yp, radius2_64(cp, xp, yp)) * cp.unfold;
 					else val = evaluate_at(cp, xp, yp);
 					nev += 1;
 				}
@@ -261,11 +271,20 @@ __global__ __launch_bounds__(64) void refine_kernel(const PkJob *__restrict__ jo
 	const uint32_t n_entries = counters->n_entries;
 	uint32_t overflow = 0;
 	unsigned long long nev = 0;
+	uint32_t chunk_next = 0, chunk_end = 0;
+	const uint32_t chunk = max(1u, min(REF_CHUNK, n_entries / (8u * gridDim.x)));
 	for (;;) {
-		uint32_t e = 0;
-		if (lane_id() == 0) e = atomicAdd(&counters->next_entry, 1u);
-		e = __shfl(e, 0);
-		if (e >= n_entries) break;
+		if (chunk_next >= chunk_end) {
+			// claim a chunk of entries with one atomic on the queue head: up to
+			// REF_CHUNK, fewer when there are few entries per wavefront
+			uint32_t base = 0;
+			if (lane_id() == 0) base = atomicAdd(&counters->next_entry, chunk);
+			base = __shfl(base, 0);
+			if (base >= n_entries) break;
+			chunk_next = base;
+			chunk_end = min(base + chunk, n_entries);
+		}
+		const uint32_t e = chunk_next++;
 		const RefineEntry en = entries[e];
 		const PkJob &jb = jobs[en.job];
 		const uint32_t i = en.pix % jb.dw, j = en.pix / jb.dw;
@@ -351,90 +370,133 @@ __global__ void psf_kernel(const PkPsfItem *__restrict__ items, const PkJob *__r
 // direct convolution
 // ---------------------------------------------------------------------------
 constexpr int CT_W = 64;   // tile width (one wavefront)
-constexpr int CT_R = 4;    // output rows per lane
-constexpr int CT_H = 16;   // tile height (4 wavefronts x CT_R rows)
+constexpr int CT_R = 8;    // output rows per lane
+constexpr int CT_H = 32;   // tile height (4 wavefronts x CT_R rows)
+constexpr int CT_KMIN = 5, CT_KMAX = 49;  // square odd PSFs with a compile-time loop nest
 
-template <typename T> struct Acc;
-template <> struct Acc<double> {
+// Per-output sum of per-PSF-column partial sums: FP32 columns are combined in
+// FP64, so a column of <= kh FMAs is the only FP32 rounding chain.
+template <typename T> struct ColSum;
+template <> struct ColSum<float> {
 	double s = 0;
-	__device__ void add_col(double v) { s += v; }
-	__device__ double get() const { return s; }
+	__device__ void add(float v) { s += (double)v; }
+	__device__ float get() const { return (float)s; }
 };
-template <> struct Acc<float> {
-	// Kahan-compensated sum of per-column FP32 partial sums
-	float s = 0.f, c = 0.f;
-	__device__ void add_col(float v)
-	{
-		float y = v - c;
-		float t = s + y;
-		c = (t - s) - y;
-		s = t;
-	}
-	__device__ float get() const { return s; }
+template <> struct ColSum<double> {
+	double s = 0;
+	__device__ void add(double v) { s += v; }
+	__device__ double get() const { return s; }
 };
 
 __device__ inline double fma_t(double a, double b, double c) { return fma(a, b, c); }
 __device__ inline float fma_t(float a, float b, float c) { return fmaf(a, b, c); }
 
-template <typename T>
-__global__ __launch_bounds__(256) void conv_kernel(const PkJob *__restrict__ jobs, uint32_t njobs,
-                                                   const T *__restrict__ src_all, const T *__restrict__ krn_all,
-                                                   const uint8_t *__restrict__ masks, T *__restrict__ dst_all)
+// One CT_W x CT_H tile of out(i,j) = sum_{l in [l0, l0+nl), k} src(i+k-hw, j+l-hh) krn(kw-1-k, kh-1-l).
+// `kt` holds the PSF reversed and transposed: kt[k * kh + l] = krn(kw-1-k, kh-1-l),
+// so the values of one PSF column are contiguous and uniform across the
+// wavefront (scalar loads).  NL > 0 fixes nl (and l0 = 0) at compile time.
+template <typename T, int NL>
+__device__ inline void conv_tile(const PkJob &jb, const T *__restrict__ src, const T *__restrict__ kt, T *tile,
+                                 int bx, int by, int l0, int nl_rt, bool accumulate, const uint8_t *__restrict__ masks,
+                                 T *__restrict__ dst)
 {
-	extern __shared__ unsigned char smem_raw[];
-	T *tile = reinterpret_cast<T *>(smem_raw);
-	const uint32_t jn = find_job(jobs, njobs, blockIdx.x, 1);
-	const PkJob &jb = jobs[jn];
-	const uint32_t t = blockIdx.x - jb.ctile_off;
+	const int nl = NL > 0 ? NL : nl_rt;
 	const int W = (int)jb.dw, H = (int)jb.dh, kw = (int)jb.kw, kh = (int)jb.kh;
 	const int hw = kw / 2, hh = kh / 2;
-	const int bx = (int)(t % jb.ctiles_x) * CT_W, by = (int)(t / jb.ctiles_x) * CT_H;
-	const int ox = bx + jb.csx, oy = by + jb.csy;  // origin in the same-size frame
-	const int TW = CT_W + kw - 1, TH = CT_H + kh - 1;
-	const T *src = src_all + jb.img_off;
-	const T *krn = krn_all + jb.psf_off;
-	// source tile with zero padding outside the image
+	const int ox = bx + jb.csx, oy = by + jb.csy;  // tile origin in the same-size frame
+	const int TW = CT_W + kw - 1, TH = CT_H + nl - 1;
 	for (int e = threadIdx.x; e < TW * TH; e += blockDim.x) {
-		const int ty = e / TW, tx = e % TW;
-		const int si = ox + tx - hw, sj = oy + ty - hh;
+		const int ty = e / TW, tx = e - ty * TW;
+		const int si = ox + tx - hw, sj = oy + ty + l0 - hh;
 		T v = 0;
 		if (si >= 0 && si < W && sj >= 0 && sj < H) v = src[si + (size_t)sj * W];
 		tile[e] = v;
 	}
 	__syncthreads();
 	const int lx = threadIdx.x & 63, ly = (threadIdx.x >> 6) * CT_R;
-	Acc<T> acc[CT_R];
-	// out(i,j) = sum_{l,k} src(i+k-hw, j+l-hh) * krn(kw-1-k, kh-1-l)
+	ColSum<T> acc[CT_R];
+	const T *trow = tile + (size_t)ly * TW + lx;
 	for (int k = 0; k < kw; k++) {
+		const T *kc = kt + (size_t)k * kh + l0;
 		T col[CT_R];
 #pragma unroll
 		for (int r = 0; r < CT_R; r++) col[r] = 0;
-		const T *tcol = tile + (size_t)ly * TW + lx + k;
-		const T *kcol = krn + (kw - 1 - k) + (size_t)(kh - 1) * kw;  // krn(kw-1-k, kh-1-l) at l = 0
-		for (int yy = 0; yy < kh + CT_R - 1; yy++) {
-			const T s = tcol[(size_t)yy * TW];
+		if (NL > 0) {
 #pragma unroll
-			for (int r = 0; r < CT_R; r++) {
-				const int l = yy - r;
-				if (l >= 0 && l < kh) col[r] = fma_t(s, kcol[-(ptrdiff_t)l * kw], col[r]);
+			for (int yy = 0; yy < NL + CT_R - 1; yy++) {
+				const T sv = trow[(size_t)yy * TW + k];
+#pragma unroll
+				for (int r = 0; r < CT_R; r++) {
+					const int m = yy - r;
+					if (m >= 0 && m < NL) col[r] = fma_t(sv, kc[m], col[r]);
+				}
+			}
+		} else {
+			for (int yy = 0; yy < nl + CT_R - 1; yy++) {
+				const T sv = trow[(size_t)yy * TW + k];
+#pragma unroll
+				for (int r = 0; r < CT_R; r++) {
+					const int m = yy - r;
+					if (m >= 0 && m < nl) col[r] = fma_t(sv, kc[m], col[r]);
+				}
 			}
 		}
 #pragma unroll
-		for (int r = 0; r < CT_R; r++) acc[r].add_col(col[r]);
+		for (int r = 0; r < CT_R; r++) acc[r].add(col[r]);
 	}
-	T *dst = dst_all + jb.conv_off;
 #pragma unroll
 	for (int r = 0; r < CT_R; r++) {
 		const int X = bx + lx, Y = by + ly + r;
 		if (X < (int)jb.cw && Y < (int)jb.ch) {
+			const size_t p = X + (size_t)Y * jb.cw;
 			T v = acc[r].get();
+			if (accumulate) v += dst[p];
 			if (jb.mask_off != NONE) {
 				const int mi = X - jb.cmx, mj = Y - jb.cmy;
 				if (mi < 0 || mj < 0 || mi >= W || mj >= H || !masks[jb.mask_off + mi + (size_t)mj * W]) v = 0;
 			}
-			dst[X + (size_t)Y * jb.cw] = v;
+			dst[p] = v;
 		}
 	}
+}
+
+template <typename T, int K>
+__device__ inline bool conv_dispatch(int kw, const PkJob &jb, const T *src, const T *kt, T *tile, int bx, int by,
+                                     const uint8_t *masks, T *dst)
+{
+	if (kw == K) {
+		conv_tile<T, K>(jb, src, kt, tile, bx, by, 0, K, false, masks, dst);
+		return true;
+	}
+	if constexpr (K + 2 <= CT_KMAX) return conv_dispatch<T, K + 2>(kw, jb, src, kt, tile, bx, by, masks, dst);
+	return false;
+}
+
+// Direct "same" convolution of every job (src/convolve.cpp:101-231).  With
+// l0/nl the launch covers PSF rows [l0, l0+nl) only and, if l0 > 0, adds to
+// dst (large PSFs are worked through in row chunks that fit in LDS).
+template <typename T>
+__global__ __launch_bounds__(256) void conv_kernel(const PkJob *__restrict__ jobs, uint32_t njobs,
+                                                   const T *__restrict__ src_all, const T *__restrict__ krn_all,
+                                                   const uint8_t *__restrict__ masks, T *__restrict__ dst_all,
+                                                   int l0, int nl)
+{
+	extern __shared__ unsigned char smem_raw[];
+	T *tile = reinterpret_cast<T *>(smem_raw);
+	const uint32_t jn = find_job(jobs, njobs, blockIdx.x, 1);
+	const PkJob &jb = jobs[jn];
+	const uint32_t t = blockIdx.x - jb.ctile_off;
+	const int bx = (int)(t % jb.ctiles_x) * CT_W, by = (int)(t / jb.ctiles_x) * CT_H;
+	const T *src = src_all + jb.img_off;
+	const T *kt = krn_all + jb.psf_off;
+	T *dst = dst_all + jb.conv_off;
+	const int kw = (int)jb.kw, kh = (int)jb.kh;
+	if (l0 == 0 && nl >= kh && kw == kh && (kw & 1) && kw >= CT_KMIN && kw <= CT_KMAX &&
+	    conv_dispatch<T, CT_KMIN>(kw, jb, src, kt, tile, bx, by, masks, dst))
+		return;
+	const int n = nl < kh - l0 ? nl : kh - l0;
+	if (n <= 0) return;
+	conv_tile<T, 0>(jb, src, kt, tile, bx, by, l0, n, l0 > 0, masks, dst);
 }
 
 // ---------------------------------------------------------------------------
@@ -479,6 +541,12 @@ __global__ __launch_bounds__(256) void compose_kernel(const PkJob *__restrict__ 
 	out_all[jb.out_off + i + (size_t)j * jb.ow] = (TO)v;
 }
 
+}  // namespace pk
+
+#include "pk_refine32.h"
+
+namespace pk {
+
 // counters of one evaluation are cleared by a kernel (not a memset node) so
 // that the launch sequence can be captured into a graph and replayed
 __global__ void reset_counters_kernel(PkCounters *c)
@@ -487,21 +555,25 @@ __global__ void reset_counters_kernel(PkCounters *c)
 		c->n_entries = 0;
 		c->next_entry = 0;
 		c->overflow = 0;
-		c->pad_ = 0;
+		c->n_spill = 0;
 		c->n_subcells = 0;
+		c->next_spill = 0;
+		c->n_bad = 0;
 	}
 }
 
 // ---------------------------------------------------------------------------
 // set-up of all components on the device (batched path)
 // ---------------------------------------------------------------------------
-__global__ void setup_kernel(const SetupIn *__restrict__ in, PkComp *__restrict__ out, uint32_t n)
+__global__ void setup_kernel(const SetupIn *__restrict__ in, PkComp *__restrict__ out, CompLite *__restrict__ lite,
+                             uint32_t n)
 {
 	const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
 	if (t >= n) return;
 	PkComp c;
 	setup_component(in[t], c);
 	out[t] = c;
+	if (lite) lite[t] = make_lite(c, in[t]);
 }
 
 }  // namespace pk

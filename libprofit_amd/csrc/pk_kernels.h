@@ -37,6 +37,11 @@ struct PkJob {
 	uint64_t omask_off;          // user's mask applied at the end (NONE = none)
 	uint32_t omask_w, omask_h, omask_f;  // its dims and upsampling factor
 	uint32_t otile_off, otiles_x;  // compose tiles (prefix over jobs)
+	// FP32-mode refinement accumulates pixel contributions as exact 64-bit
+	// fixed-point integers: value * fx_scale, fx_scale = 2^60 / (sum of the
+	// total fluxes of the job's components), so the order of the additions
+	// cannot change the result
+	double fx_scale;
 };
 
 struct RefineEntry {
@@ -47,8 +52,20 @@ struct PkCounters {
 	uint32_t n_entries;    // refinement entries appended by the coarse pass
 	uint32_t next_entry;   // work-queue head of the refinement pass
 	uint32_t overflow;     // set when a refinement stack exceeded its capacity
-	uint32_t pad_;
+	uint32_t n_spill;      // cells handed to the second refinement pass
 	unsigned long long n_subcells;  // evaluate_at calls made by the refinement
+	uint32_t next_spill;   // work-queue head of the second refinement pass
+	uint32_t n_bad;        // non-finite contributions (dropped)
+};
+
+// A cell waiting to be subdivided (FP32-mode refinement).
+struct RCell {
+	uint32_t gcomp;        // index in the component table
+	uint32_t entry;        // pixel accumulator
+	uint32_t meta;         // level | resolution << 8 | max recursions << 16 | convolved << 24
+	uint32_t pad_;
+	double dx0, dy0;       // lower corner relative to the component centre
+	double f;              // cell size in pixels: resolution^-level
 };
 
 // PSF-profile work item (src/psf.cpp:58-145), geometry resolved on the host.

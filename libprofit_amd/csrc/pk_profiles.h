@@ -39,6 +39,23 @@ struct PkComp {
 	float f[8];                  // profile constants (FP32)
 	// values after adjustment, reported back (libprofit writes them into the profile)
 	double bn, lumtot, ie;
+	double inv_axrat;            // 1 / axrat (FP32-mode refinement)
+	double total_flux;           // 10^(-0.4 (mag - magzero)), bound of any pixel value
+};
+
+// What one FP32 sub-cell evaluation of the refinement reads: one 128-byte
+// record (a single cache line) per component.
+struct alignas(128) CompLite {
+	double cos_ang, sin_ang, inv_axrat;
+	double wflux;    // flux_scale * fixed-point scale of the job
+	double unfold;
+	double acc;
+	double sx, sy;   // pixel scale of the job
+	double q4, q5;   // king / ferrer FP64 constants (PkComp::q[4], q[5])
+	double b2;       // box + 2
+	float f[8];
+	int8_t kind, boxy, fp32, convolve;
+	int32_t pad_;
 };
 
 PK_HD bool almost_equals(double x, double y) { return fabs(x - y) < 1e-10; }
@@ -142,16 +159,59 @@ PK_HD double evaluate_at(const PkComp &c, double x, double y)
 // units of 1/unfold of the FP64 ones.
 // ---------------------------------------------------------------------------
 
-__device__ inline float boxy_r32(const PkComp &c, float x, float y)
+PK_HD double kq4(const PkComp &c) { return c.q[4]; }
+PK_HD double kq5(const PkComp &c) { return c.q[5]; }
+PK_HD double kb2(const PkComp &c) { return c.box + 2.; }
+PK_HD double kq4(const CompLite &c) { return c.q4; }
+PK_HD double kq5(const CompLite &c) { return c.q5; }
+PK_HD double kb2(const CompLite &c) { return c.b2; }
+
+template <typename CR>
+__device__ inline float boxy_r32(const CR &c, float x, float y)
 {
 	if (!c.boxy) return sqrtf(x * x + y * y);
 	float b2 = c.f[7];
 	return exp2f(log2f(exp2f(b2 * log2f(fabsf(x))) + exp2f(b2 * log2f(fabsf(y)))) / b2);
 }
 
-__device__ inline float evaluate_at32(const PkComp &c, float x, float y)
+// FP64 squared radius for the FP32 king and ferrer evaluation, whose value
+// near the truncation radius depends on rt^2 - r^2; 0 for the other kinds.
+template <typename CR>
+PK_HD double radius2_64(const CR &c, double x, double y)
+{
+	if (c.kind != PK_PROFILE_KING && c.kind != PK_PROFILE_FERRER) return 0;
+	if (!c.boxy) return x * x + y * y;
+	const double b2 = kb2(c);
+	const double r = pow(pow(fabs(x), b2) + pow(fabs(y), b2), 1. / b2);
+	return r * r;
+}
+
+// `r2` is radius2_64 at the same point.
+template <typename CR>
+__device__ inline float evaluate_at32(const CR &c, float x, float y, double r2)
 {
 	switch (c.kind) {
+	case PK_PROFILE_KING: {
+		// f[0] = 1/rc^2, f[1] = 1/a, f[2] = a; q[4] = rt^2, q[5] = rc^2 + rt^2.
+		// (t(r) - t(rt))^a with t(r) = (1 + r^2/rc^2)^(-1/a), written as
+		// t(r) (1 - (1 - d)^(1/a)), d = (rt^2 - r^2) / (rc^2 + rt^2), so that
+		// no two nearly equal numbers are subtracted near rt.
+		if (!(r2 < kq4(c))) return 0.f;
+		const float lt = -c.f[1] * log2f(1.f + (float)r2 * c.f[0]);
+		const float d = (float)((kq4(c) - r2) / kq5(c));
+		const float diff = -expm1f(log1pf(-d) * c.f[1]);
+		if (!(diff > 0.f)) return c.f[2] == 0.f ? 1.f : 0.f;
+		return exp2f(c.f[2] * (lt + log2f(diff)));
+	}
+	case PK_PROFILE_FERRER: {
+		// f[0] = a, f[1] = (2 - b) / 2; q[4] = rout^2.
+		// 1 - (r/rout)^(2-b) = -expm1((2-b)/2 log1p(-d)), d = 1 - r^2/rout^2
+		if (!(r2 < kq4(c))) return 0.f;
+		const float d = (float)((kq4(c) - r2) / kq4(c));
+		const float base = -expm1f(c.f[1] * log1pf(-d));
+		if (!(base > 0.f)) return c.f[0] == 0.f ? 1.f : 0.f;
+		return exp2f(c.f[0] * log2f(base));
+	}
 	case PK_PROFILE_SERSIC: {
 		// f[0] = 1/re^2 (round) or 1/re (boxy), f[1] = exponent on base, f[2] = bn, f[7] = box + 2
 		float base;
@@ -206,6 +266,7 @@ struct SetupIn {
 	double magzero, sx, sy, img_sx, img_sy, pad_x, pad_y;
 	int32_t fp32_mode;
 	uint32_t fs;               // finesampling factor (sky bg is divided by fs^2, src/sky.cpp:42-45)
+	double fx_scale;           // fixed-point scale of the FP32-mode refinement of the model
 };
 
 PK_HD double sersic_fluxfrac(double re, double nser, double bn, double fraction)
@@ -377,6 +438,8 @@ PK_HD void setup_component(const SetupIn &in, PkComp &o)
 	}
 
 	double angrad = fmod(c.ang + 90, 360.) * PK_PI / 180.;
+	o.inv_axrat = 1 / c.axrat;
+	o.total_flux = pow(10, -0.4 * (c.mag - in.magzero));
 	o.cos_ang = cos(angrad);
 	o.sin_ang = sin(angrad);
 	o.xcen = c.xcen + in.pad_x * in.sx;
@@ -437,9 +500,45 @@ PK_HD void setup_component(const SetupIn &in, PkComp &o)
 			o.f[6] = (float)(1 / (o.q[2] * o.q[3]));
 			o.f[7] = (float)(c.box + 2);
 			break;
-		default: break;  // ferrer and king stay in FP64 (cancellation near rout / rt)
+		case PK_PROFILE_KING:
+			o.fp32 = 1;
+			o.f[0] = (float)(1 / (o.q[0] * o.q[0]));
+			o.f[1] = (float)(1 / o.q[2]);
+			o.f[2] = (float)o.q[2];
+			o.q[4] = o.q[1] * o.q[1];
+			o.q[5] = o.q[0] * o.q[0] + o.q[1] * o.q[1];
+			break;
+		case PK_PROFILE_FERRER:
+			o.fp32 = 1;
+			o.f[0] = (float)o.q[1];
+			o.f[1] = (float)((2 - o.q[2]) / 2);
+			o.q[4] = o.q[0] * o.q[0];
+			break;
+		default: break;
 		}
 	}
+}
+
+PK_HD CompLite make_lite(const PkComp &c, const SetupIn &in)
+{
+	CompLite l{};
+	l.cos_ang = c.cos_ang;
+	l.sin_ang = c.sin_ang;
+	l.inv_axrat = c.inv_axrat;
+	l.wflux = c.flux_scale * in.fx_scale;
+	l.unfold = c.unfold;
+	l.acc = c.acc;
+	l.sx = in.sx;
+	l.sy = in.sy;
+	l.q4 = c.q[4];
+	l.q5 = c.q[5];
+	l.b2 = c.box + 2;
+	for (int k = 0; k < 8; k++) l.f[k] = c.f[k];
+	l.kind = (int8_t)c.kind;
+	l.boxy = (int8_t)c.boxy;
+	l.fp32 = (int8_t)c.fp32;
+	l.convolve = (int8_t)(c.convolve ? 1 : 0);
+	return l;
 }
 
 }  // namespace pk

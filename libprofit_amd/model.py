@@ -29,8 +29,13 @@ class Context:
         h = C.c_void_p()
         check(self.lib.pk_ctx_create(int(device), self.precision, C.byref(h)))
         self.handle = h
+        import weakref
+        self._batches = weakref.WeakSet()
 
     def close(self):
+        # a batch must not outlive its context
+        for b in list(getattr(self, "_batches", ())):
+            b.close()
         if self.handle:
             self.lib.pk_ctx_destroy(self.handle)
             self.handle = None
@@ -294,6 +299,7 @@ class Batch:
                                      C.byref(h)))
         self.handle = h
         self.n = len(self.specs)
+        ctx._batches.add(self)
 
     def evaluate(self):
         check(self.L.pk_batch_evaluate(self.handle))

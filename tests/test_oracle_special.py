@@ -166,3 +166,46 @@ def test_adjust_values_config1(oracle):
     assert relerr(s["ie"], 10 ** (-0.4 * 15) / (lum * 0.6)) < 1e-13
     assert s["cos_ang"] == pytest.approx(math.cos(math.radians(120)), abs=1e-15)
     assert s["sin_ang"] == pytest.approx(math.sin(math.radians(120)), abs=1e-15)
+
+
+def _product_constants(name, **kw):
+    import ctypes as C
+    from libprofit_amd import _lib, abi
+    spec = abi.ModelSpec(10, 10, [abi.make_component(name, mag=0.0, axrat=1.0, box=0.0, **kw)])
+    out = abi.Constants()
+    _lib.check(_lib.load().pk_component_constants(spec.ref, 0, C.byref(out)))
+    return out
+
+
+def test_product_lumtot_golden():
+    """The product's set-up (host copy of the device code) against mpmath."""
+    for re, n, v in GOLD["sersic_lumtot"]:
+        assert relerr(_product_constants("sersic", re=re, nser=n).lumtot, v) < 1e-13, (re, n)
+    for rc, rt, a, v in GOLD["king_lumtot"]:
+        assert relerr(_product_constants("king", rc=rc, rt=rt, a=a).lumtot, v) < 1e-12, (rc, rt, a)
+    for h1, h2, rb, a, v in GOLD["brokenexp_lumtot"]:
+        assert relerr(_product_constants("brokenexp", h1=h1, h2=h2, rb=rb, a=a).lumtot, v) < 1e-12, (h1, h2, rb, a)
+    for re, rb, n, a, b, v in GOLD["coresersic_lumtot"]:
+        got = _product_constants("coresersic", re=re, rb=rb, nser=n, a=a, b=b).lumtot
+        assert relerr(got, v) < 1e-12, (re, rb, n, a, b, got, v)
+
+
+def test_product_constants_match_oracle(oracle):
+    """All adjusted set-up constants of the product equal the oracle's."""
+    from libprofit_amd import abi
+    rng = np.random.Generator(np.random.PCG64(1))
+    cases = [("sersic", dict(re=rng.uniform(1, 40), nser=rng.uniform(0.3, 9), axrat=rng.uniform(0.1, 1)))
+             for _ in range(30)]
+    cases += [("moffat", dict(fwhm=rng.uniform(1, 9), con=rng.uniform(1.2, 9))) for _ in range(10)]
+    cases += [("ferrer", dict(rout=rng.uniform(3, 60), a=rng.uniform(0.2, 2), b=rng.uniform(0, 1.9)))
+              for _ in range(10)]
+    for name, kw in cases:
+        spec = abi.ModelSpec(10, 10, [abi.make_component(name, mag=17.0, **kw)], magzero=2.0)
+        o = oracle.component_setup(spec)
+        import ctypes as C
+        from libprofit_amd import _lib
+        p = abi.Constants()
+        _lib.check(_lib.load().pk_component_constants(spec.ref, 0, C.byref(p)))
+        for k in ("rscale", "ie", "acc", "rscale_switch", "rscale_max", "bn", "cos_ang", "sin_ang"):
+            assert relerr(getattr(p, k), o[k]) < 1e-13 or abs(getattr(p, k) - o[k]) < 1e-15, (name, kw, k)
+        assert p.resolution == o["resolution"], (name, kw)
