@@ -28,11 +28,13 @@ constexpr int R32_CAP_MAIN = 192;
 constexpr int R32_CAP_SPILL = 1024;
 constexpr uint32_t WQ_CHUNK = 64;
 
+constexpr int R32_TASKS = 16;  // cells per round at most (their records are staged in LDS)
+
 template <int CAP>
 struct R32Stack {
+	CompLite lite[R32_TASKS];
 	uint32_t gcomp[CAP], entry[CAP], meta[CAP];
 	double dx0[CAP], dy0[CAP], f[CAP];
-	unsigned long long task_acc[64];
 };
 
 __device__ inline uint32_t wave_sum_u32(uint32_t v)
@@ -73,7 +75,7 @@ __device__ __attribute__((noinline)) double refine_fp64_decision(const PkComp &c
 }
 
 template <int SOURCE>  // 0: pixels appended by the coarse pass, 1: the spill queue
-__global__ __launch_bounds__(64, 5) void refine32_kernel(const PkJob *__restrict__ jobs, const PkComp *__restrict__ comps,
+__global__ __launch_bounds__(64) void refine32_kernel(const PkJob *__restrict__ jobs, const PkComp *__restrict__ comps,
                                                       const CompLite *__restrict__ lite,
                                                       const RefineEntry *__restrict__ entries,
                                                       PkCounters *__restrict__ cnt, RCell *__restrict__ spill,
@@ -175,7 +177,7 @@ __global__ __launch_bounds__(64, 5) void refine32_kernel(const PkJob *__restrict
 			r2 = res * res;
 		}
 		const uint32_t incl = wave_incl_scan_u32(r2);
-		const uint64_t fit = __ballot(lane < top && incl <= 64u);
+		const uint64_t fit = __ballot(lane < top && lane < R32_TASKS && incl <= 64u);
 		const int ntask = fit ? __popcll(fit) : 1;
 		const uint32_t lanes_needed = __shfl(incl, ntask - 1);
 		const int rounds = (int)((lanes_needed + 63u) / 64u);
@@ -183,23 +185,20 @@ __global__ __launch_bounds__(64, 5) void refine32_kernel(const PkJob *__restrict
 		int t_local = 0;
 		for (int t = 0; t < ntask; t++) t_local += (__shfl(incl, t) <= (uint32_t)lane) ? 1 : 0;
 		if (rounds > 1 || t_local >= ntask) t_local = rounds > 1 ? 0 : ntask - 1;
-		// every lane takes part in the shuffle (a lane that is inactive in a
+		// every lane takes part in the shuffles (a lane that is inactive in a
 		// branch would be read as 0)
 		const uint32_t prev = __shfl(incl, t_local > 0 ? t_local - 1 : 0);
-		const uint32_t before = t_local > 0 ? prev : 0u;
+		const uint32_t seg_end1 = __shfl(incl, t_local);
+		const uint32_t before = t_local > 0 ? prev : 0u;  // first lane of the task (single round)
 		const int slot = top - 1 - t_local;
 		const uint32_t gcomp = st.gcomp[slot], entry = st.entry[slot], meta = st.meta[slot];
 		const double dx0 = st.dx0[slot], dy0 = st.dy0[slot], f = st.f[slot];
-		// lane t < ntask owns the accumulator target of task t
-		uint32_t flush_target = 0;
-		if (lane < ntask) {
-			const int ts = top - 1 - lane;
-			flush_target = 2u * st.entry[ts] + cm_conv(st.meta[ts]);
-		}
+		// lane t < ntask stages the component record of task t in LDS
+		if (lane < ntask) st.lite[lane] = lite[st.gcomp[top - 1 - lane]];
 		__syncthreads();
 		top -= ntask;
 
-		const CompLite &cl = lite[gcomp];
+		const CompLite &cl = st.lite[t_local];
 		const int level = (int)cm_level(meta), res = (int)cm_res(meta), mr = (int)cm_maxrec(meta);
 		const double fr = f / res;
 		const double hx = 0.5 * fr * cl.sx, hy = 0.5 * fr * cl.sy;
@@ -207,14 +206,14 @@ __global__ __launch_bounds__(64, 5) void refine32_kernel(const PkJob *__restrict
 		const bool recurse = res > 1 && level < mr;
 		const double acc_thr = cl.acc;
 		const double ddy = (-2 * hx * cl.sin_ang + 2 * hy * cl.cos_ang) * cl.inv_axrat;
+		const size_t target = 2 * (size_t)entry + cm_conv(meta);
 		for (int rd = 0; rd < rounds; rd++) {
-			st.task_acc[lane] = 0;
 			const uint32_t g = (uint32_t)lane + 64u * (uint32_t)rd;
 			const bool active = g < lanes_needed;
 			const uint32_t s = g - before;
-			__syncthreads();
 			bool fail = false;
 			double dx = 0, dy = 0;
+			unsigned long long fxv = 0;
 			if (active) {
 				const int si = (int)s / res, sj = (int)s % res;
 				dx = dx0 + (2 * si + 1) * hx;
@@ -247,14 +246,20 @@ __global__ __launch_bounds__(64, 5) void refine32_kernel(const PkJob *__restrict
 				}
 				if (!fail) {
 					const double q = val * wflux;
-					if (q >= 0 && q < 9.2e18) {
-						const unsigned long long fx = (unsigned long long)__double2ll_rn(q);
-						if (fx) atomicAdd(&st.task_acc[t_local], fx);
-					} else {
-						bad++;
-					}
+					if (q >= 0 && q < 9.2e18) fxv = (unsigned long long)__double2ll_rn(q);
+					else bad++;
 				}
 			}
+			// exact per-cell sums: segmented scan over the cell's contiguous lanes,
+			// the last lane adds the sum to the pixel accumulator
+			const uint32_t sbeg = rounds > 1 ? 0u : before;
+			const uint32_t send = rounds > 1 ? min(64u, lanes_needed - 64u * (uint32_t)rd) : seg_end1;
+			unsigned long long v = fxv;
+			for (int off = 1; off < 64; off <<= 1) {
+				const unsigned long long t = __shfl_up(v, (unsigned int)off);
+				if ((uint32_t)lane >= sbeg + (uint32_t)off) v += t;
+			}
+			if (active && (uint32_t)lane == send - 1 && v) atomicAdd(&acc[target], v);
 			// failing sub-cells: onto the LIFO, or the spill queue when it is full
 			const uint64_t ballot = __ballot(fail);
 			if (ballot) {
@@ -296,12 +301,6 @@ __global__ __launch_bounds__(64, 5) void refine32_kernel(const PkJob *__restrict
 				} else {
 					overflow = 1;
 				}
-			}
-			__syncthreads();
-			// one atomic per cell into the pixel accumulator
-			if (lane < ntask) {
-				const unsigned long long v = st.task_acc[lane];
-				if (v) atomicAdd(&acc[flush_target], v);
 			}
 			__syncthreads();
 		}
