@@ -1,6 +1,8 @@
 """Loader of the HIP library libprofit_amd/_lib/libprofit_hip.so.
 
-There is no fallback: if the library is missing, loading raises.
+There is no fallback: if the library is missing, loading raises.  The
+environment variable LIBPROFIT_HIP_LIBRARY names another build of the same
+library (for instance one made with different compile-time options).
 """
 import ctypes as C
 import os
@@ -9,7 +11,7 @@ import re
 from . import abi
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "_lib", "libprofit_hip.so")
+LIB_PATH = os.environ.get("LIBPROFIT_HIP_LIBRARY") or os.path.join(HERE, "_lib", "libprofit_hip.so")
 HEADER = os.path.join(os.path.dirname(HERE), "include", "profit_hip.h")
 
 _lib = None

@@ -63,6 +63,9 @@ struct pk_ctx {
 	void *hmem = nullptr;  // pinned staging
 	size_t hcap = 0;
 	hipEvent_t ev[4] = {};
+	// grids of the FP32 refinement passes: as many wavefronts as fit on the
+	// device at once (the hand-over rule counts them)
+	uint32_t r32_grid_main = 0, r32_grid_last = 0;
 };
 
 static int ensure_dev(pk_ctx *ctx, size_t bytes)
@@ -152,6 +155,13 @@ int pk_ctx_create(int device, int precision, pk_ctx **ctx_out)
 		return set_err(PK_EDEVICE, "hipStreamCreate failed");
 	}
 	for (auto &e : ctx->ev) PK_HIP(hipEventCreate(&e));
+	hipDeviceProp_t prop;
+	PK_HIP(hipGetDeviceProperties(&prop, device));
+	int nb_main = 0, nb_last = 0;
+	PK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb_main, refine32_kernel<0, false>, 64, 0));
+	PK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb_last, refine32_kernel<1, true>, 64, 0));
+	ctx->r32_grid_main = (uint32_t)std::max(1, nb_main * prop.multiProcessorCount);
+	ctx->r32_grid_last = (uint32_t)std::max(1, nb_last * prop.multiProcessorCount);
 	*ctx_out = ctx;
 	return PK_OK;
 }
@@ -506,8 +516,9 @@ int launch_conv(hipStream_t s, const PkJob *djobs, uint32_t njobs, uint32_t ntil
 	return PK_OK;
 }
 
-constexpr uint32_t SPILL_CAP_SINGLE = 1u << 20;
-constexpr uint32_t SPILL_CAP_BATCH = 1u << 22;
+// capacity (cells) of each of the two queues between FP32 refinement passes
+constexpr uint32_t R32_QCAP_SINGLE = 1u << 20;
+constexpr uint32_t R32_QCAP_BATCH = 1u << 22;
 
 // Fixed-point scale of the FP32-mode refinement accumulators of one model:
 // 2^60 over the summed total flux of its radial profiles (no pixel can hold
@@ -543,8 +554,8 @@ int check_fp32_limits(const pk_model *m)
 template <typename TI>
 int launch_profiles(hipStream_t s, bool fp32, const PkJob *djobs, uint32_t njobs, uint32_t ntiles,
                     const PkComp *dcomp, const CompLite *dlite, const uint8_t *dmask, TI *img_nc, TI *img_c,
-                    RefineEntry *dent, PkCounters *dcnt, unsigned long long *dacc, RCell *dspill, uint32_t spill_cap,
-                    uint32_t ref_blocks, bool any_comp, hipEvent_t ev_coarse)
+                    RefineEntry *dent, PkCounters *dcnt, unsigned long long *dacc, RCell *dq, uint32_t qcap,
+                    uint32_t ref_blocks, uint32_t grid_main, uint32_t grid_last, bool any_comp, hipEvent_t ev_coarse)
 {
 	if (fp32) {
 		hipLaunchKernelGGL((coarse_kernel<TI, true>), dim3(ntiles), dim3(256), 0, s, djobs, njobs, dcomp, dmask, img_nc,
@@ -557,10 +568,16 @@ int launch_profiles(hipStream_t s, bool fp32, const PkJob *djobs, uint32_t njobs
 	if (ev_coarse) PK_HIP(hipEventRecord(ev_coarse, s));
 	if (!any_comp) return PK_OK;
 	if (fp32) {
-		hipLaunchKernelGGL(refine32_kernel<0>, dim3(ref_blocks), dim3(64), 0, s, djobs, dcomp, dlite, dent, dcnt,
-		                   dspill, spill_cap, dacc);
-		hipLaunchKernelGGL(refine32_kernel<1>, dim3(ref_blocks), dim3(64), 0, s, djobs, dcomp, dlite, dent, dcnt,
-		                   dspill, spill_cap, dacc);
+		// pass p reads queue (p - 1) & 1 and hands over into queue p & 1
+		RCell *q0 = dq, *q1 = dq + qcap;
+		const uint32_t gm = std::min(grid_main, ref_blocks), gl = std::min(grid_last, ref_blocks);
+		hipLaunchKernelGGL((refine32_kernel<0, false>), dim3(gm), dim3(64), 0, s, djobs, dcomp, dlite, dent, dcnt,
+		                   (const RCell *)nullptr, q0, qcap, 0, dacc);
+		for (int p = 1; p < R32_PASSES - 1; p++)
+			hipLaunchKernelGGL((refine32_kernel<1, false>), dim3(gm), dim3(64), 0, s, djobs, dcomp, dlite, dent, dcnt,
+			                   (const RCell *)((p & 1) ? q0 : q1), (p & 1) ? q1 : q0, qcap, p, dacc);
+		hipLaunchKernelGGL((refine32_kernel<1, true>), dim3(gl), dim3(64), 0, s, djobs, dcomp, dlite, dent, dcnt,
+		                   (const RCell *)((R32_PASSES & 1) ? q1 : q0), (RCell *)nullptr, qcap, R32_PASSES - 1, dacc);
 		hipLaunchKernelGGL(finalize32_kernel<TI>, dim3(1024), dim3(256), 0, s, djobs, dent, dcnt, dacc, img_nc, img_c);
 	} else {
 		hipLaunchKernelGGL((refine_kernel<TI, false>), dim3(ref_blocks), dim3(64), 0, s, djobs, dcomp, dent, dcnt,
@@ -785,8 +802,8 @@ int evaluate_impl(pk_ctx *ctx, const pk_model *m, const Analysis &a, double *out
 	const size_t o_conv = L.take(sizeof(TI) * (a.conv_required ? ncpix : 1));
 	const size_t o_ent = L.take(sizeof(RefineEntry) * npix);
 	const size_t o_acc = L.take(fp32 ? 2 * sizeof(unsigned long long) * npix : 8);
-	const uint32_t spill_cap = fp32 ? SPILL_CAP_SINGLE : 0;
-	const size_t o_spill = L.take(sizeof(RCell) * std::max<uint32_t>(1, spill_cap));
+	const uint32_t qcap = fp32 ? R32_QCAP_SINGLE : 0;
+	const size_t o_q = L.take(2 * sizeof(RCell) * std::max<uint32_t>(1, qcap));
 	const size_t o_out = L.take(sizeof(double) * (size_t)ow * oh);
 	PK_TRY(ensure_dev(ctx, L.off));
 	PK_TRY(ensure_host(ctx, std::max(in_bytes, sizeof(double) * (size_t)ow * oh + 256 + sizeof(PkCounters))));
@@ -830,7 +847,8 @@ int evaluate_impl(pk_ctx *ctx, const pk_model *m, const Analysis &a, double *out
 	const uint32_t ntiles = jb.tiles_x * cdiv(a.dh, 4);
 	const uint32_t nref_blocks = (uint32_t)std::min<size_t>(npix, 2048);
 	PK_TRY(launch_profiles<TI>(s, fp32, djob, 1u, ntiles, dcomp, dptr<CompLite>(ctx, o_lite), dm, img_nc, img_c, dent, dcnt,
-	                           dptr<unsigned long long>(ctx, o_acc), dptr<RCell>(ctx, o_spill), spill_cap, nref_blocks,
+	                           dptr<unsigned long long>(ctx, o_acc), dptr<RCell>(ctx, o_q), qcap, nref_blocks,
+	                           ctx->r32_grid_main, ctx->r32_grid_last,
 	                           !comps.empty(), nullptr));
 	for (size_t k = 0; k < items.size(); k++) {
 		const auto &it = items[k];
@@ -1112,8 +1130,8 @@ struct pk_batch {
 	// device memory
 	void *dmem = nullptr;
 	size_t o_jobs = 0, o_setup = 0, o_comps = 0, o_cnt = 0, o_psfT = 0, o_mask = 0, o_nc = 0, o_c = 0, o_conv = 0,
-	       o_ent = 0, o_acc = 0, o_spill = 0, o_out = 0, o_lite = 0;
-	uint32_t spill_cap = 0;
+	       o_ent = 0, o_acc = 0, o_q = 0, o_out = 0, o_lite = 0;
+	uint32_t qcap = 0;
 	std::vector<double> fx_scales;
 	uint64_t total_out = 0;
 	hipGraph_t graph = nullptr;
@@ -1152,7 +1170,8 @@ int batch_launch(pk_batch *b)
 		float *nc = b->d<float>(b->o_nc), *c = b->d<float>(b->o_c);
 		PK_TRY(launch_profiles<float>(s, true, djobs, nj, b->ntiles, dcomp, b->d<CompLite>(b->o_lite), dmask, nc, c,
 		                              dent, dcnt,
-		                              b->d<unsigned long long>(b->o_acc), b->d<RCell>(b->o_spill), b->spill_cap, 4096,
+		                              b->d<unsigned long long>(b->o_acc), b->d<RCell>(b->o_q), b->qcap, UINT32_MAX,
+		                              b->ctx->r32_grid_main, b->ctx->r32_grid_last,
 		                              b->n_comps > 0, timed ? b->ev[2] : nullptr));
 		if (timed) PK_HIP(hipEventRecord(b->ev[3], s));
 		if (b->any_conv) {
@@ -1167,7 +1186,7 @@ int batch_launch(pk_batch *b)
 		double *nc = b->d<double>(b->o_nc), *c = b->d<double>(b->o_c);
 		PK_TRY(launch_profiles<double>(s, false, djobs, nj, b->ntiles, dcomp, nullptr, dmask, nc, c, dent, dcnt, nullptr,
 		                               nullptr,
-		                               0, 4096, b->n_comps > 0, timed ? b->ev[2] : nullptr));
+		                               0, 4096, 0, 0, b->n_comps > 0, timed ? b->ev[2] : nullptr));
 		if (timed) PK_HIP(hipEventRecord(b->ev[3], s));
 		if (b->any_conv) {
 			PK_TRY(launch_conv<double>(s, djobs, nj, b->nctiles, c, b->d<double>(b->o_psfT), dmask,
@@ -1344,8 +1363,8 @@ int pk_batch_create(pk_ctx *ctx, const pk_model *models, uint32_t n_models, uint
 	b->o_conv = L.take(elem * (b->any_conv ? img_off : 1));
 	b->o_ent = L.take(sizeof(RefineEntry) * img_off);
 	b->o_acc = L.take(fp32 ? 2 * sizeof(unsigned long long) * img_off : 8);
-	b->spill_cap = fp32 ? SPILL_CAP_BATCH : 0;
-	b->o_spill = L.take(sizeof(RCell) * std::max<uint32_t>(1, b->spill_cap));
+	b->qcap = fp32 ? R32_QCAP_BATCH : 0;
+	b->o_q = L.take(2 * sizeof(RCell) * std::max<uint32_t>(1, b->qcap));
 	b->o_out = L.take(elem * out_off);
 	if (hipMalloc(&b->dmem, L.off) != hipSuccess) {
 		(void)hipGetLastError();

@@ -555,10 +555,13 @@ __global__ void reset_counters_kernel(PkCounters *c)
 		c->n_entries = 0;
 		c->next_entry = 0;
 		c->overflow = 0;
-		c->n_spill = 0;
-		c->n_subcells = 0;
-		c->next_spill = 0;
 		c->n_bad = 0;
+		c->n_subcells = 0;
+		for (int p = 0; p < R32_PASSES; p++) {
+			c->claim[p] = 0;
+			c->n_cells[p] = 0;
+			c->n_done[p] = 0;
+		}
 	}
 }
 

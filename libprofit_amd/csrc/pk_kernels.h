@@ -48,24 +48,29 @@ struct RefineEntry {
 	uint32_t job, pix, cmask;
 };
 
+// The FP32-mode refinement runs in R32_PASSES launches: a launch hands what
+// its wavefronts still hold to the next one once a quarter of them have run
+// out of work, so that no long tail of single wavefronts remains.
+constexpr int R32_PASSES = 4;
+
 struct PkCounters {
 	uint32_t n_entries;    // refinement entries appended by the coarse pass
-	uint32_t next_entry;   // work-queue head of the refinement pass
-	uint32_t overflow;     // set when a refinement stack exceeded its capacity
-	uint32_t n_spill;      // cells handed to the second refinement pass
-	unsigned long long n_subcells;  // evaluate_at calls made by the refinement
-	uint32_t next_spill;   // work-queue head of the second refinement pass
+	uint32_t next_entry;   // work-queue head of the FP64 refinement pass
+	uint32_t overflow;     // set when a refinement work queue exceeded its capacity
 	uint32_t n_bad;        // non-finite contributions (dropped)
+	unsigned long long n_subcells;  // profile evaluations made by the refinement
+	uint32_t claim[R32_PASSES];    // work-queue head of FP32 refinement pass p
+	uint32_t n_cells[R32_PASSES];  // cells pass p hands to pass p + 1
+	uint32_t n_done[R32_PASSES];   // wavefronts of pass p that have stopped
 };
 
-// A cell waiting to be subdivided (FP32-mode refinement).
-struct RCell {
+// A cell waiting to be subdivided (FP32-mode refinement), 32 bytes.
+struct alignas(16) RCell {
+	double dx0, dy0;       // lower corner relative to the component centre
+	float f;               // cell size in pixels: resolution^-level
 	uint32_t gcomp;        // index in the component table
 	uint32_t entry;        // pixel accumulator
 	uint32_t meta;         // level | resolution << 8 | max recursions << 16 | convolved << 24
-	uint32_t pad_;
-	double dx0, dy0;       // lower corner relative to the component centre
-	double f;              // cell size in pixels: resolution^-level
 };
 
 // PSF-profile work item (src/psf.cpp:58-145), geometry resolved on the host.

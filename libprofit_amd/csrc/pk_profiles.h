@@ -47,10 +47,10 @@ struct PkComp {
 // record (a single cache line) per component.
 struct alignas(128) CompLite {
 	double cos_ang, sin_ang, inv_axrat;
-	double wflux;    // flux_scale * fixed-point scale of the job
-	double unfold;
+	double wflux;    // flux_scale * fixed-point scale of the job (weight of an FP64 value)
+	double wq;       // wflux * unfold (weight of an FP32 value)
 	double acc;
-	double sx, sy;   // pixel scale of the job
+	double hsx, hsy; // half the pixel scale of the job
 	double q4, q5;   // king / ferrer FP64 constants (PkComp::q[4], q[5])
 	double b2;       // box + 2
 	float f[8];
@@ -526,10 +526,10 @@ PK_HD CompLite make_lite(const PkComp &c, const SetupIn &in)
 	l.sin_ang = c.sin_ang;
 	l.inv_axrat = c.inv_axrat;
 	l.wflux = c.flux_scale * in.fx_scale;
-	l.unfold = c.unfold;
+	l.wq = l.wflux * c.unfold;
 	l.acc = c.acc;
-	l.sx = in.sx;
-	l.sy = in.sy;
+	l.hsx = 0.5 * in.sx;
+	l.hsy = 0.5 * in.sy;
 	l.q4 = c.q[4];
 	l.q5 = c.q[5];
 	l.b2 = c.box + 2;

@@ -6,50 +6,98 @@
 // sub-cells (res^2 each) fill the 64 lanes -- a cell with more than 64
 // sub-cells is spread over several rounds -- and every lane evaluates one
 // sub-cell and its accuracy test (src/radial.cpp:86-109).  Failing sub-cells
-// go back onto the LIFO, or onto a global spill queue worked off by a second
-// launch when the LIFO is full.  Accepted values of a cell are summed as exact
-// 64-bit fixed-point integers and added atomically to the pixel's
-// accumulator: neither the packing of cells into rounds nor the scheduling of
-// wavefronts can change the result.
+// go back onto the LIFO.  Accepted values of a cell are summed as exact 64-bit
+// fixed-point integers and added atomically to the pixel's accumulator:
+// neither the packing of cells into rounds nor the scheduling of wavefronts
+// can change the result.
 //
-// Latency: a round reads one 128-byte CompLite record per cell (one cache
-// line); refills claim WQ_CHUNK items with one atomic and load them with one
-// coalesced access, expanding the pixels of a chunk lane-parallel.
+// Load balance: the refinement runs as R32_PASSES launches.  Pass 0 takes
+// pixels from the coarse pass's entry list, pass p > 0 the cells handed over
+// by pass p - 1.  Work is claimed in chunks that shrink as the source runs
+// dry.  Once a quarter of a pass's wavefronts have stopped, the others hand
+// everything they still hold (their LIFO and unclaimed chunk) to the next
+// pass and stop too, so a few deep pixels never keep a lone wavefront busy
+// while the rest of the GPU idles.  The last pass finishes all that is left,
+// with a deeper LIFO.  A LIFO that fills up before that also hands its new
+// cells to the next pass.
+//
+// Latency: lane-crossing sums and scans use DPP (no LDS round trips); the
+// 128-byte CompLite record of a cell is staged in LDS and kept there while the
+// following rounds work on the same component.
 #pragma once
 
 namespace pk {
 
-// LIFO capacity: small in the main pass (occupancy), large in the pass over
-// spilled cells, which has nowhere further to spill.  A cell of resolution
-// res can push res^2 children, so depth-first processing needs at most
-// about (res^2 - 1) * levels + res^2 cells: 1024 covers res 16 with 3 levels
+// LIFO capacity: small in the handing-over passes (occupancy), large in the
+// last pass, which has nowhere further to go.  Depth-first processing holds at
+// most about res^2 * (max recursions) cells: 1024 covers res 16 with 3 levels
 // and the Sersic centre pixels (res 8, 10 levels).
 constexpr int R32_CAP_MAIN = 192;
-constexpr int R32_CAP_SPILL = 1024;
+constexpr int R32_CAP_LAST = 1024;
 constexpr uint32_t WQ_CHUNK = 64;
-
-constexpr int R32_TASKS = 16;  // cells per round at most (their records are staged in LDS)
+constexpr int R32_TASKS = 16;              // cells per round at most (their records are staged in LDS)
+constexpr int R32_POLL = 16;               // rounds between two checks of the hand-over condition
+constexpr uint32_t R32_HANDOVER_DIV = 4;   // hand over once 1/R32_HANDOVER_DIV of the wavefronts stopped
 
 template <int CAP>
 struct R32Stack {
 	CompLite lite[R32_TASKS];
-	uint32_t gcomp[CAP], entry[CAP], meta[CAP];
-	double dx0[CAP], dy0[CAP], f[CAP];
+	uint32_t lite_g[R32_TASKS];   // component whose record lite[t] holds
+	RCell cell[CAP];
 };
 
-__device__ inline uint32_t wave_sum_u32(uint32_t v)
+// ---- wavefront primitives on DPP ----------------------------------------------
+constexpr int DPP_ROW_SHR1 = 0x111, DPP_ROW_SHR2 = 0x112, DPP_ROW_SHR4 = 0x114, DPP_ROW_SHR8 = 0x118;
+constexpr int DPP_ROW_BCAST15 = 0x142, DPP_ROW_BCAST31 = 0x143;
+
+// v of the DPP source lane; 0 where the source lies outside the row, or the
+// lane's row is outside ROW_MASK.  Call with all 64 lanes active.
+template <int CTRL, int ROW_MASK = 0xf>
+__device__ inline uint32_t dpp0(uint32_t v)
 {
-	for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+	return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, ROW_MASK, 0xf, false);
+}
+
+template <int CTRL, int ROW_MASK = 0xf>
+__device__ inline unsigned long long dpp0_u64(unsigned long long v)
+{
+	const uint32_t lo = dpp0<CTRL, ROW_MASK>((uint32_t)v), hi = dpp0<CTRL, ROW_MASK>((uint32_t)(v >> 32));
+	return ((unsigned long long)hi << 32) | lo;
+}
+
+// inclusive prefix sum over the 64 lanes
+__device__ inline uint32_t wave_incl_scan_u32(uint32_t v)
+{
+	v += dpp0<DPP_ROW_SHR1>(v);
+	v += dpp0<DPP_ROW_SHR2>(v);
+	v += dpp0<DPP_ROW_SHR4>(v);
+	v += dpp0<DPP_ROW_SHR8>(v);
+	v += dpp0<DPP_ROW_BCAST15, 0xa>(v);  // row 0 -> row 1, row 2 -> row 3
+	v += dpp0<DPP_ROW_BCAST31, 0xc>(v);  // lane 31 -> rows 2, 3
 	return v;
 }
 
-__device__ inline uint32_t wave_incl_scan_u32(uint32_t v)
+__device__ inline uint32_t wave_read(uint32_t v, int lane) { return (uint32_t)__builtin_amdgcn_readlane((int)v, lane); }
+
+__device__ inline uint32_t wave_sum_u32(uint32_t v) { return wave_read(wave_incl_scan_u32(v), 63); }
+
+// Inclusive prefix sum within segments of consecutive lanes; `sbeg` is the
+// first lane of the calling lane's segment.
+__device__ inline unsigned long long wave_seg_scan_u64(unsigned long long v, uint32_t lane, uint32_t sbeg)
 {
-	const int lane = (int)lane_id();
-	for (int off = 1; off < 64; off <<= 1) {
-		const uint32_t t = __shfl_up(v, off);
-		if (lane >= off) v += t;
-	}
+	unsigned long long t;
+	t = dpp0_u64<DPP_ROW_SHR1>(v);
+	if (lane >= sbeg + 1) v += t;
+	t = dpp0_u64<DPP_ROW_SHR2>(v);
+	if (lane >= sbeg + 2) v += t;
+	t = dpp0_u64<DPP_ROW_SHR4>(v);
+	if (lane >= sbeg + 4) v += t;
+	t = dpp0_u64<DPP_ROW_SHR8>(v);
+	if (lane >= sbeg + 8) v += t;
+	t = dpp0_u64<DPP_ROW_BCAST15, 0xa>(v);  // source: last lane of the previous row
+	if (sbeg < (lane & ~15u)) v += t;
+	t = dpp0_u64<DPP_ROW_BCAST31, 0xc>(v);  // source: lane 31
+	if (sbeg <= 31u) v += t;
 	return v;
 }
 
@@ -74,233 +122,307 @@ __device__ __attribute__((noinline)) double refine_fp64_decision(const PkComp &c
 	return v;
 }
 
-template <int SOURCE>  // 0: pixels appended by the coarse pass, 1: the spill queue
+// Non-negative value below 2^63 -> 64-bit integer (truncated); false if out of range.
+__device__ inline bool fixed_point_f32(float q, unsigned long long *out)
+{
+	if (!(q >= 0.f && q < 9.2e18f)) return false;
+	const float hi = truncf(q * 0x1p-32f);
+	const float lo = fmaf(hi, -0x1p32f, q);  // exact
+	*out = ((unsigned long long)(uint32_t)hi << 32) | (uint32_t)lo;
+	return true;
+}
+
+// Expands refinement entries [next, end) into level-0 cells on the LIFO while
+// they fit, one pixel per lane (one cell per refined component).  Needs at
+// least 32 free slots, so that one pixel always fits.
+__device__ inline void r32_expand(const PkJob *__restrict__ jobs, const PkComp *__restrict__ comps,
+                                  const RefineEntry *__restrict__ entries, RCell *cells, int cap, int &top,
+                                  uint32_t &next, uint32_t end, uint32_t lane)
+{
+	const uint32_t avail = end - next;
+	const uint32_t space = (uint32_t)(cap - top);
+	const bool valid = lane < avail;
+	RefineEntry en{0, 0, 0};
+	if (valid) en = entries[next + lane];
+	const uint32_t nb = valid ? (uint32_t)__popc(en.cmask) : 0u;
+	const uint32_t incl = wave_incl_scan_u32(nb);
+	const int take = __popcll(__ballot(valid && incl <= space));
+	if ((int)lane < take) {
+		const PkJob &jb = jobs[en.job];
+		const uint32_t i = en.pix % jb.dw, j = en.pix / jb.dw;
+		const double hx = jb.sx / 2., hy = jb.sy / 2.;
+		const double x = hx + i * jb.sx, y = hy + j * jb.sy;  // pixel centre (src/radial.cpp:313-314)
+		int pos = top + (int)(incl - nb);
+		uint32_t m = en.cmask;
+		while (m) {
+			const int c = __ffs(m) - 1;
+			m &= m - 1;
+			const uint32_t gc = jb.comp_off + (uint32_t)c;
+			const PkComp &cp = comps[gc];
+			uint32_t res = (uint32_t)cp.res, mr = (uint32_t)cp.max_rec;
+			if (cp.center_res && fabs(x - cp.xcen) < cp.win_x && fabs(y - cp.ycen) < cp.win_y) {
+				// SersicProfile::subsampling_params (src/sersic.cpp:331-347)
+				res = (uint32_t)cp.center_res;
+				mr = (uint32_t)cp.center_max_rec;
+			}
+			RCell cl;
+			cl.dx0 = (x - hx) - cp.xcen;
+			cl.dy0 = (y - hy) - cp.ycen;
+			cl.f = 1.f;
+			cl.gcomp = gc;
+			cl.entry = next + lane;
+			cl.meta = (res << 8) | (mr << 16) | ((cp.convolve ? 1u : 0u) << 24);
+			cells[pos++] = cl;
+		}
+	}
+	if (take > 0) top += (int)wave_read(incl, take - 1);
+	next += (uint32_t)take;
+}
+
+// Claims the next chunk of a work queue of n items; the chunk shrinks as the
+// queue runs dry (`seen` is the head this wavefront saw last).
+__device__ inline bool r32_claim(uint32_t *head, uint32_t n, uint32_t &seen, uint32_t &next, uint32_t &end,
+                                 uint32_t lane)
+{
+	const uint32_t left = n > seen ? n - seen : 0u;
+	const uint32_t want = max(1u, min(WQ_CHUNK, left / (2u * gridDim.x)));
+	uint32_t base = 0;
+	if (lane == 0) base = atomicAdd(head, want);
+	base = wave_read(base, 0);
+	seen = base + want;
+	if (base >= n) return false;
+	next = base;
+	end = min(base + want, n);
+	return true;
+}
+
+// SOURCE 0: pixels from the coarse pass (pass 0); 1: cells of the previous pass.
+// LAST: the final pass, which hands nothing over.
+template <int SOURCE, bool LAST>
 __global__ __launch_bounds__(64) void refine32_kernel(const PkJob *__restrict__ jobs, const PkComp *__restrict__ comps,
                                                       const CompLite *__restrict__ lite,
                                                       const RefineEntry *__restrict__ entries,
-                                                      PkCounters *__restrict__ cnt, RCell *__restrict__ spill,
-                                                      uint32_t spill_cap, unsigned long long *__restrict__ acc)
+                                                      PkCounters *__restrict__ cnt, const RCell *__restrict__ qin,
+                                                      RCell *__restrict__ qout, uint32_t qcap, int pass,
+                                                      unsigned long long *__restrict__ acc)
 {
-	constexpr int R32_CAP = SOURCE == 0 ? R32_CAP_MAIN : R32_CAP_SPILL;
-	__shared__ R32Stack<R32_CAP> st;
-	const int lane = (int)lane_id();
+	constexpr int CAP = LAST ? R32_CAP_LAST : R32_CAP_MAIN;
+	__shared__ R32Stack<CAP> st;
+	const uint32_t lane = lane_id();
+	if (lane < (uint32_t)R32_TASKS) st.lite_g[lane] = ~0u;
 	int top = 0;
-	bool source_done = false;
-	const uint32_t n_source = SOURCE == 0 ? cnt->n_entries : min(cnt->n_spill, spill_cap);
+	bool source_done = false, handover = false;
+	const uint32_t n_source = SOURCE == 0 ? cnt->n_entries : min(cnt->n_cells[pass - 1], qcap);
+	uint32_t *head = &cnt->claim[pass];
 	uint32_t overflow = 0, bad = 0;
 	unsigned long long nev = 0;
-	uint32_t chunk_next = 0, chunk_end = 0;
+	uint32_t chunk_next = 0, chunk_end = 0, seen = 0;
+	int poll = 0;
+	__syncthreads();
 	for (;;) {
 		// ---- refill while the cells on the LIFO cannot fill the lanes ----
 		for (;;) {
 			uint32_t r2 = 0;
-			if (lane < top) {
-				const uint32_t res = cm_res(st.meta[top - 1 - lane]);
+			if ((int)lane < top) {
+				const uint32_t res = cm_res(st.cell[top - 1 - (int)lane].meta);
 				r2 = res * res;
 			}
 			if (wave_sum_u32(r2) >= 64u || source_done) break;
-			if (R32_CAP - top < (SOURCE == 0 ? 32 : 1)) break;
-			if (chunk_next >= chunk_end) {
-				uint32_t base = 0;
-				if (lane == 0) base = atomicAdd(SOURCE == 0 ? &cnt->next_entry : &cnt->next_spill, WQ_CHUNK);
-				base = __shfl(base, 0);
-				if (base >= n_source) {
-					source_done = true;
-					break;
-				}
-				chunk_next = base;
-				chunk_end = min(base + WQ_CHUNK, n_source);
+			if (CAP - top < (SOURCE == 0 ? 32 : 1)) break;
+			if (chunk_next >= chunk_end && !r32_claim(head, n_source, seen, chunk_next, chunk_end, lane)) {
+				source_done = true;
+				break;
 			}
-			const uint32_t avail = chunk_end - chunk_next;
-			const uint32_t space = (uint32_t)(R32_CAP - top);
 			if (SOURCE == 0) {
-				// lane L expands pixel chunk_next + L into one cell per refined component
-				const bool valid = (uint32_t)lane < avail;
-				RefineEntry en{0, 0, 0};
-				if (valid) en = entries[chunk_next + lane];
-				const uint32_t nb = valid ? (uint32_t)__popc(en.cmask) : 0u;
-				const uint32_t incl = wave_incl_scan_u32(nb);
-				const uint64_t ok = __ballot(valid && incl <= space);
-				const int take = __popcll(ok);  // >= 1: space >= 32 >= cells of one pixel
-				if ((uint64_t)lane < (uint64_t)take) {
-					const PkJob &jb = jobs[en.job];
-					const uint32_t i = en.pix % jb.dw, j = en.pix / jb.dw;
-					const double hx = jb.sx / 2., hy = jb.sy / 2.;
-					const double x = hx + i * jb.sx, y = hy + j * jb.sy;  // pixel centre (src/radial.cpp:313-314)
-					int pos = top + (int)(incl - nb);
-					uint32_t m = en.cmask;
-					while (m) {
-						const int c = __ffs(m) - 1;
-						m &= m - 1;
-						const uint32_t gc = jb.comp_off + (uint32_t)c;
-						const PkComp &cp = comps[gc];
-						uint32_t res = (uint32_t)cp.res, mr = (uint32_t)cp.max_rec;
-						if (cp.center_res && fabs(x - cp.xcen) < cp.win_x && fabs(y - cp.ycen) < cp.win_y) {
-							// SersicProfile::subsampling_params (src/sersic.cpp:331-347)
-							res = (uint32_t)cp.center_res;
-							mr = (uint32_t)cp.center_max_rec;
-						}
-						st.gcomp[pos] = gc;
-						st.entry[pos] = chunk_next + (uint32_t)lane;
-						st.meta[pos] = (res << 8) | (mr << 16) | ((cp.convolve ? 1u : 0u) << 24);
-						st.dx0[pos] = (x - hx) - cp.xcen;
-						st.dy0[pos] = (y - hy) - cp.ycen;
-						st.f[pos] = 1.0;
-						pos++;
-					}
-				}
-				top += (int)__shfl(incl, take - 1);
-				chunk_next += (uint32_t)take;
+				r32_expand(jobs, comps, entries, st.cell, CAP, top, chunk_next, chunk_end, lane);
 			} else {
-				const uint32_t n = min(min(avail, space), 64u);
-				if ((uint32_t)lane < n) {
-					const RCell c = spill[chunk_next + lane];
-					const int pos = top + lane;
-					st.gcomp[pos] = c.gcomp;
-					st.entry[pos] = c.entry;
-					st.meta[pos] = c.meta;
-					st.dx0[pos] = c.dx0;
-					st.dy0[pos] = c.dy0;
-					st.f[pos] = c.f;
-				}
+				const uint32_t n = min(min(chunk_end - chunk_next, (uint32_t)(CAP - top)), 64u);
+				if (lane < n) st.cell[top + (int)lane] = qin[chunk_next + lane];
 				top += (int)n;
 				chunk_next += n;
 			}
 			__syncthreads();
 		}
 		if (top == 0) break;
+		if (!LAST && ++poll >= R32_POLL) {
+			poll = 0;
+			uint32_t d = 0;
+			if (lane == 0) d = __hip_atomic_load(&cnt->n_done[pass], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+			if (wave_read(d, 0) * R32_HANDOVER_DIV >= gridDim.x) {
+				handover = true;
+				break;
+			}
+		}
 
 		// ---- cells on top of the LIFO -> lanes ----
 		uint32_t r2 = 0;
-		if (lane < top) {
-			const uint32_t res = cm_res(st.meta[top - 1 - lane]);
+		if ((int)lane < top && lane < (uint32_t)R32_TASKS) {
+			const uint32_t res = cm_res(st.cell[top - 1 - (int)lane].meta);
 			r2 = res * res;
 		}
 		const uint32_t incl = wave_incl_scan_u32(r2);
-		const uint64_t fit = __ballot(lane < top && lane < R32_TASKS && incl <= 64u);
+		const uint64_t fit = __ballot((int)lane < top && lane < (uint32_t)R32_TASKS && incl <= 64u);
 		const int ntask = fit ? __popcll(fit) : 1;
-		const uint32_t lanes_needed = __shfl(incl, ntask - 1);
+		const uint32_t lanes_needed = wave_read(incl, ntask - 1);
 		const int rounds = (int)((lanes_needed + 63u) / 64u);
-		// task of this lane (rounds > 1 only with a single task)
+		// task of this lane: the first whose inclusive end exceeds the lane
 		int t_local = 0;
-		for (int t = 0; t < ntask; t++) t_local += (__shfl(incl, t) <= (uint32_t)lane) ? 1 : 0;
-		if (rounds > 1 || t_local >= ntask) t_local = rounds > 1 ? 0 : ntask - 1;
-		// every lane takes part in the shuffles (a lane that is inactive in a
-		// branch would be read as 0)
-		const uint32_t prev = __shfl(incl, t_local > 0 ? t_local - 1 : 0);
-		const uint32_t seg_end1 = __shfl(incl, t_local);
-		const uint32_t before = t_local > 0 ? prev : 0u;  // first lane of the task (single round)
-		const int slot = top - 1 - t_local;
-		const uint32_t gcomp = st.gcomp[slot], entry = st.entry[slot], meta = st.meta[slot];
-		const double dx0 = st.dx0[slot], dy0 = st.dy0[slot], f = st.f[slot];
-		// lane t < ntask stages the component record of task t in LDS
-		if (lane < ntask) st.lite[lane] = lite[st.gcomp[top - 1 - lane]];
-		__syncthreads();
+		uint32_t before = 0, seg_end1 = 0;
+		for (int t = 0; t < ntask; t++) {
+			const uint32_t it = wave_read(incl, t);
+			if (it <= lane) {
+				t_local = t + 1;
+				before = it;
+			} else if (seg_end1 == 0) {
+				seg_end1 = it;
+			}
+		}
+		if (rounds > 1) {
+			t_local = 0;
+			before = 0;
+		} else if (t_local >= ntask) {
+			t_local = ntask - 1;  // lane beyond the last task: inactive
+		}
+		const RCell cell = st.cell[top - 1 - t_local];
+		// component records of the tasks -> LDS (kept while unchanged)
+		{
+			uint32_t g = 0;
+			bool miss = false;
+			if ((int)lane < ntask) {
+				g = st.cell[top - 1 - (int)lane].gcomp;
+				miss = st.lite_g[lane] != g;
+			}
+			if (__ballot(miss)) {
+				if (miss) {
+					st.lite[lane] = lite[g];
+					st.lite_g[lane] = g;
+				}
+				__syncthreads();
+			}
+		}
 		top -= ntask;
-
 		const CompLite &cl = st.lite[t_local];
+		const uint32_t meta = cell.meta;
 		const int level = (int)cm_level(meta), res = (int)cm_res(meta), mr = (int)cm_maxrec(meta);
-		const double fr = f / res;
-		const double hx = 0.5 * fr * cl.sx, hy = 0.5 * fr * cl.sy;
-		const double wflux = fr * fr * cl.wflux;
+		const float inv_res = 1.f / (float)res;
+		const float fr = cell.f * inv_res;
+		const double hx = (double)fr * cl.hsx, hy = (double)fr * cl.hsy;
+		const float wq = (float)((double)(fr * fr) * cl.wq);
 		const bool recurse = res > 1 && level < mr;
-		const double acc_thr = cl.acc;
+		const float acc_thr = (float)cl.acc;
+		const float band = 1e-4f * (1.f + acc_thr);
 		const double ddy = (-2 * hx * cl.sin_ang + 2 * hy * cl.cos_ang) * cl.inv_axrat;
-		const size_t target = 2 * (size_t)entry + cm_conv(meta);
+		const size_t target = 2 * (size_t)cell.entry + cm_conv(meta);
+		const bool fp32 = cl.fp32 != 0;
 		for (int rd = 0; rd < rounds; rd++) {
-			const uint32_t g = (uint32_t)lane + 64u * (uint32_t)rd;
+			const uint32_t g = lane + 64u * (uint32_t)rd;
 			const bool active = g < lanes_needed;
-			const uint32_t s = g - before;
+			const int s = (int)(g - before);
 			bool fail = false;
 			double dx = 0, dy = 0;
 			unsigned long long fxv = 0;
 			if (active) {
-				const int si = (int)s / res, sj = (int)s % res;
-				dx = dx0 + (2 * si + 1) * hx;
-				dy = dy0 + (2 * sj + 1) * hy;
-				const double xp = dx * cl.cos_ang + dy * cl.sin_ang;
-				const double yp = (-dx * cl.sin_ang + dy * cl.cos_ang) * cl.inv_axrat;
-				double val = 0;
+				// s = si * res + sj without an integer division
+				int si = (int)((float)s * inv_res);
+				int sj = s - si * res;
+				if (sj >= res) {
+					si++;
+					sj -= res;
+				} else if (sj < 0) {
+					si--;
+					sj += res;
+				}
+				dx = fma((double)(2 * si + 1), hx, cell.dx0);
+				dy = fma((double)(2 * sj + 1), hy, cell.dy0);
+				const double xp = fma(dx, cl.cos_ang, dy * cl.sin_ang);
+				const double yp = fma(dy, cl.cos_ang, -dx * cl.sin_ang) * cl.inv_axrat;
 				bool decided = false;
-				if (cl.fp32) {
+				if (fp32) {
 					const float v32 = evaluate_at32(cl, (float)xp, (float)yp, radius2_64(cl, xp, yp));
 					nev += 1;
 					if (recurse) {
 						const double txp = fabs(xp), typ = fabs(yp) + fabs(ddy);
 						const float t32 = evaluate_at32(cl, (float)txp, (float)typ, radius2_64(cl, txp, typ));
 						nev += 1;
-						const float m = fabsf(t32 / v32 - 1.f);
-						if (v32 > 0.f && isfinite(m) && fabs((double)m - acc_thr) > 1e-4 * (1.0 + acc_thr)) {
-							fail = (double)m > acc_thr;
-							val = (double)v32 * cl.unfold;
+						const float m = fabsf(t32 * __builtin_amdgcn_rcpf(v32) - 1.f);
+						if (v32 > 0.f && m <= 3e38f && fabsf(m - acc_thr) > band) {
+							fail = m > acc_thr;
 							decided = true;
 						}
 					} else {
-						val = (double)v32 * cl.unfold;
 						decided = true;
 					}
+					if (decided && !fail && !fixed_point_f32(v32 * wq, &fxv)) bad++;
 				}
 				if (!decided) {
-					val = refine_fp64_decision(comps[gcomp], xp, yp, ddy, recurse, &fail);
+					const double val = refine_fp64_decision(comps[cell.gcomp], xp, yp, ddy, recurse, &fail);
 					nev += recurse ? 2 : 1;
-				}
-				if (!fail) {
-					const double q = val * wflux;
-					if (q >= 0 && q < 9.2e18) fxv = (unsigned long long)__double2ll_rn(q);
-					else bad++;
+					if (!fail) {
+						const double q = val * ((double)(fr * fr) * cl.wflux);
+						if (q >= 0 && q < 9.2e18) fxv = (unsigned long long)__double2ll_rn(q);
+						else bad++;
+					}
 				}
 			}
 			// exact per-cell sums: segmented scan over the cell's contiguous lanes,
 			// the last lane adds the sum to the pixel accumulator
 			const uint32_t sbeg = rounds > 1 ? 0u : before;
 			const uint32_t send = rounds > 1 ? min(64u, lanes_needed - 64u * (uint32_t)rd) : seg_end1;
-			unsigned long long v = fxv;
-			for (int off = 1; off < 64; off <<= 1) {
-				const unsigned long long t = __shfl_up(v, (unsigned int)off);
-				if ((uint32_t)lane >= sbeg + (uint32_t)off) v += t;
-			}
-			if (active && (uint32_t)lane == send - 1 && v) atomicAdd(&acc[target], v);
-			// failing sub-cells: onto the LIFO, or the spill queue when it is full
+			const unsigned long long v = wave_seg_scan_u64(fxv, lane, sbeg);
+			if (active && lane == send - 1 && v) atomicAdd(&acc[target], v);
+			// failing sub-cells: onto the LIFO, or to the next pass when it is full
 			const uint64_t ballot = __ballot(fail);
 			if (ballot) {
 				const int n = __popcll(ballot);
 				const int rank = __popcll(ballot & ((1ull << lane) - 1ull));
-				const double cdx0 = dx - hx, cdy0 = dy - hy;
-				const uint32_t cmeta = (meta & ~255u) | (uint32_t)(level + 1);
-				if (top + n <= R32_CAP) {
-					if (fail) {
-						const int pos = top + rank;
-						st.gcomp[pos] = gcomp;
-						st.entry[pos] = entry;
-						st.meta[pos] = cmeta;
-						st.dx0[pos] = cdx0;
-						st.dy0[pos] = cdy0;
-						st.f[pos] = fr;
-					}
+				RCell c;
+				c.dx0 = dx - hx;
+				c.dy0 = dy - hy;
+				c.f = fr;
+				c.gcomp = cell.gcomp;
+				c.entry = cell.entry;
+				c.meta = (meta & ~255u) | (uint32_t)(level + 1);
+				if (top + n <= CAP) {
+					if (fail) st.cell[top + rank] = c;
 					top += n;
-				} else if (SOURCE == 0) {
+				} else if (!LAST) {
 					const int leader = __ffsll((long long)ballot) - 1;
 					uint32_t base = 0;
-					if (lane == leader) base = atomicAdd(&cnt->n_spill, (uint32_t)n);
-					base = __shfl(base, leader);
+					if ((int)lane == leader) base = atomicAdd(&cnt->n_cells[pass], (uint32_t)n);
+					base = wave_read(base, leader);
 					if (fail) {
-						if (base + rank < spill_cap) {
-							RCell c;
-							c.gcomp = gcomp;
-							c.entry = entry;
-							c.meta = cmeta;
-							c.pad_ = 0;
-							c.dx0 = cdx0;
-							c.dy0 = cdy0;
-							c.f = fr;
-							spill[base + rank] = c;
-						} else {
-							overflow = 1;
-						}
+						if (base + (uint32_t)rank < qcap) qout[base + rank] = c;
+						else overflow = 1;
 					}
 				} else {
 					overflow = 1;
 				}
+			}
+			__syncthreads();
+		}
+	}
+	if (!LAST && handover) {
+		// everything this wavefront holds goes to the next pass
+		for (;;) {
+			if (top > 0) {
+				uint32_t base = 0;
+				if (lane == 0) base = atomicAdd(&cnt->n_cells[pass], (uint32_t)top);
+				base = wave_read(base, 0);
+				for (int i = (int)lane; i < top; i += 64) {
+					if (base + (uint32_t)i < qcap) qout[base + i] = st.cell[i];
+					else overflow = 1;
+				}
+				top = 0;
+				__syncthreads();
+			}
+			if (chunk_next >= chunk_end) break;
+			if (SOURCE == 0) {
+				r32_expand(jobs, comps, entries, st.cell, CAP, top, chunk_next, chunk_end, lane);
+			} else {
+				const uint32_t n = min(min(chunk_end - chunk_next, (uint32_t)CAP), 64u);
+				if (lane < n) st.cell[lane] = qin[chunk_next + lane];
+				top = (int)n;
+				chunk_next += n;
 			}
 			__syncthreads();
 		}
@@ -312,6 +434,7 @@ __global__ __launch_bounds__(64) void refine32_kernel(const PkJob *__restrict__ 
 		atomicAdd(&cnt->n_subcells, nev);
 		if (ov) atomicOr(&cnt->overflow, 1u);
 		if (nbad) atomicAdd(&cnt->n_bad, nbad);
+		if (!LAST) atomicAdd(&cnt->n_done[pass], 1u);
 	}
 }
 
