@@ -88,8 +88,9 @@ __global__ __launch_bounds__(256) void coarse_kernel(const PkJob *__restrict__ j
 					double r = sqrt(xp * xp + yp * yp);
 					if (cp.rscale_max > 0 && r / cp.rscale > cp.rscale_max) continue;
 					if (cp.rough || r / cp.rscale > cp.rscale_switch) {
-						if (FP32 && cp.fp32)
-							v = (double)evaluate_at32(cp, (float)xp, (float)yp, radius2_64(cp, xp, yp)) * cp.unfold;
+						float v32 = -1.f;
+						if (FP32 && cp.fp32) v32 = evaluate_at32(cp, (float)xp, (float)yp, radius2_64(cp, xp, yp));
+						if (v32 >= 0.f && v32 <= 3e38f) v = (double)v32 * cp.unfold;
 						else v = evaluate_at(cp, xp, yp);
 						v *= cp.flux_scale;
 					} else {

@@ -155,9 +155,48 @@ PK_HD double evaluate_at(const PkComp &c, double x, double y)
 }
 
 // ---------------------------------------------------------------------------
-// FP32 evaluation (sersic, moffat, brokenexp, coresersic).  Values are in
-// units of 1/unfold of the FP64 ones.
+// FP32 evaluation (all radial kinds).  Values are in units of 1/unfold of the
+// FP64 ones.  Built on the hardware transcendental instructions (V_EXP_F32,
+// V_LOG_F32, V_RCP_F32, V_SQRT_F32: about 1 ulp; results below 2^-126 flush
+// to 0); log1p, expm1 and exp are composed from them so that no accuracy is
+// lost near 0.  A value that is not finite is left to the FP64 evaluation by
+// the callers.
 // ---------------------------------------------------------------------------
+
+__device__ inline float fexp2(float x) { return __builtin_amdgcn_exp2f(x); }
+__device__ inline float flog2(float x) { return __builtin_amdgcn_logf(x); }
+__device__ inline float frcp(float x) { return __builtin_amdgcn_rcpf(x); }
+__device__ inline float fsqrt(float x) { return __builtin_amdgcn_sqrtf(x); }
+
+constexpr float F_LN2 = 0.693147180559945309f;
+
+// e^x; the rounding error of x log2(e) is carried to first order
+__device__ inline float fexp(float x)
+{
+	constexpr float L2E_HI = 1.44269502162933349609375f;  // log2(e) = L2E_HI + L2E_LO
+	constexpr float L2E_LO = 1.925963033500011079e-8f;
+	const float h = x * L2E_HI;
+	const float l = fmaf(x, L2E_HI, -h) + x * L2E_LO;
+	return fexp2(h) * fmaf(l, F_LN2, 1.f);
+}
+
+// log(1 + x), x > -1: log(u) x / (u - 1) with u = 1 + x rounded (W. Kahan)
+__device__ inline float flog1p(float x)
+{
+	const float u = 1.f + x;
+	const float d = u - 1.f;
+	return d == 0.f ? x : flog2(u) * F_LN2 * (x * frcp(d));
+}
+
+// e^x - 1: (u - 1) x / log(u) with u = e^x rounded (W. Kahan)
+__device__ inline float fexpm1(float x)
+{
+	const float u = fexp(x);
+	const float um1 = u - 1.f;
+	if (um1 == 0.f) return x;
+	if (um1 == -1.f) return -1.f;
+	return um1 * x * frcp(flog2(u) * F_LN2);
+}
 
 PK_HD double kq4(const PkComp &c) { return c.q[4]; }
 PK_HD double kq5(const PkComp &c) { return c.q[5]; }
@@ -166,12 +205,36 @@ PK_HD double kq4(const CompLite &c) { return c.q4; }
 PK_HD double kq5(const CompLite &c) { return c.q5; }
 PK_HD double kb2(const CompLite &c) { return c.b2; }
 
+// Boxy radius (|x|^b2 + |y|^b2)^(1/b2) (include/profit/radial.h:93-102),
+// b2 = c.f[7], written as m (1 + (s/m)^b2)^(1/b2) with m = max(|x|, |y|),
+// s = min(|x|, |y|): the powers stay within [0, 2], so that the FP32 result
+// is good to a few ulps at any radius.  Returns m and log2(1 + (s/m)^b2) / b2.
+template <typename CR>
+__device__ inline float boxy_parts32(const CR &c, float x, float y, float &m)
+{
+	const float ax = fabsf(x), ay = fabsf(y);
+	m = fmaxf(ax, ay);
+	const float q = m > 0.f ? fminf(ax, ay) * frcp(m) : 0.f;
+	const float b2 = c.f[7];
+	return flog2(1.f + fexp2(b2 * flog2(q))) * frcp(b2);
+}
+
 template <typename CR>
 __device__ inline float boxy_r32(const CR &c, float x, float y)
 {
-	if (!c.boxy) return sqrtf(x * x + y * y);
-	float b2 = c.f[7];
-	return exp2f(log2f(exp2f(b2 * log2f(fabsf(x))) + exp2f(b2 * log2f(fabsf(y)))) / b2);
+	float m;
+	const float l = boxy_parts32(c, x, y, m);
+	return m * fexp2(l);
+}
+
+// log2 of the radius (round or boxy)
+template <typename CR>
+__device__ inline float log2_r32(const CR &c, float x, float y)
+{
+	if (!c.boxy) return 0.5f * flog2(x * x + y * y);
+	float m;
+	const float l = boxy_parts32(c, x, y, m);
+	return flog2(m) + l;
 }
 
 // FP64 squared radius for the FP32 king and ferrer evaluation, whose value
@@ -192,63 +255,56 @@ __device__ inline float evaluate_at32(const CR &c, float x, float y, double r2)
 {
 	switch (c.kind) {
 	case PK_PROFILE_KING: {
-		// f[0] = 1/rc^2, f[1] = 1/a, f[2] = a; q[4] = rt^2, q[5] = rc^2 + rt^2.
+		// f[0] = 1/rc^2, f[1] = 1/a, f[2] = a; q[4] = rt^2, q[5] = 1 / (rc^2 + rt^2).
 		// (t(r) - t(rt))^a with t(r) = (1 + r^2/rc^2)^(-1/a), written as
 		// t(r) (1 - (1 - d)^(1/a)), d = (rt^2 - r^2) / (rc^2 + rt^2), so that
 		// no two nearly equal numbers are subtracted near rt.
 		if (!(r2 < kq4(c))) return 0.f;
-		const float lt = -c.f[1] * log2f(1.f + (float)r2 * c.f[0]);
-		const float d = (float)((kq4(c) - r2) / kq5(c));
-		const float diff = -expm1f(log1pf(-d) * c.f[1]);
+		const float lt = -c.f[1] * flog2(1.f + (float)r2 * c.f[0]);
+		const float d = (float)((kq4(c) - r2) * kq5(c));
+		const float diff = -fexpm1(flog1p(-d) * c.f[1]);
 		if (!(diff > 0.f)) return c.f[2] == 0.f ? 1.f : 0.f;
-		return exp2f(c.f[2] * (lt + log2f(diff)));
+		return fexp2(c.f[2] * (lt + flog2(diff)));
 	}
 	case PK_PROFILE_FERRER: {
-		// f[0] = a, f[1] = (2 - b) / 2; q[4] = rout^2.
+		// f[0] = a, f[1] = (2 - b) / 2; q[4] = rout^2, q[5] = 1 / rout^2.
 		// 1 - (r/rout)^(2-b) = -expm1((2-b)/2 log1p(-d)), d = 1 - r^2/rout^2
 		if (!(r2 < kq4(c))) return 0.f;
-		const float d = (float)((kq4(c) - r2) / kq4(c));
-		const float base = -expm1f(c.f[1] * log1pf(-d));
+		const float d = (float)((kq4(c) - r2) * kq5(c));
+		const float base = -fexpm1(c.f[1] * flog1p(-d));
 		if (!(base > 0.f)) return c.f[0] == 0.f ? 1.f : 0.f;
-		return exp2f(c.f[0] * log2f(base));
+		return fexp2(c.f[0] * flog2(base));
 	}
 	case PK_PROFILE_SERSIC: {
-		// f[0] = 1/re^2 (round) or 1/re (boxy), f[1] = exponent on base, f[2] = bn, f[7] = box + 2
-		float base;
-		if (!c.boxy) {
-			base = (x * x + y * y) * c.f[0];
-		} else {
-			float b2 = c.f[7];
-			base = exp2f(b2 * log2f(fabsf(x) * c.f[0])) + exp2f(b2 * log2f(fabsf(y) * c.f[0]));
-		}
-		float rf = base > 0.f ? exp2f(c.f[1] * log2f(base)) : 0.f;
-		return expf(-c.f[2] * rf);
+		// f[0] = -log2(re), f[1] = 1/nser, f[2] = bn, f[7] = box + 2: exp(-bn (r/re)^(1/nser))
+		const float rf = fexp2(c.f[1] * (log2_r32(c, x, y) + c.f[0]));
+		return fexp(-c.f[2] * rf);
 	}
 	case PK_PROFILE_MOFFAT: {
 		// f[0] = 1/rscale^2, f[1] = con
-		float r2;
+		float r2f;
 		if (!c.boxy) {
-			r2 = (x * x + y * y) * c.f[0];
+			r2f = (x * x + y * y) * c.f[0];
 		} else {
-			float r = boxy_r32(c, x, y);
-			r2 = r * r * c.f[0];
+			const float r = boxy_r32(c, x, y);
+			r2f = r * r * c.f[0];
 		}
-		return exp2f(-c.f[1] * log2f(1.f + r2));
+		return fexp2(-c.f[1] * flog2(1.f + r2f));
 	}
 	case PK_PROFILE_BROKENEXP: {
-		// f[0] = 1/h1, f[1] = 1/h1 - 1/h2, f[2] = rb, f[3] = a
-		float r = boxy_r32(c, x, y);
+		// f[0] = 1/h1, f[1] = 1/h1 - 1/h2, f[2] = rb, f[3] = a, f[4] = 1/a
+		const float r = c.boxy ? boxy_r32(c, x, y) : fsqrt(x * x + y * y);
 		float base = r - c.f[2];
-		if (c.f[3] * base < 40.f) base = log1pf(expf(c.f[3] * base)) / c.f[3];
-		return expf(-r * c.f[0] + c.f[1] * base);
+		if (c.f[3] * base < 40.f) base = flog1p(fexp(c.f[3] * base)) * c.f[4];
+		return fexp(-r * c.f[0] + c.f[1] * base);
 	}
 	case PK_PROFILE_CORESERSIC: {
-		// f[0] = 1/rb, f[1] = a, f[2] = b/a, f[3] = bn, f[4] = rb^a, f[5] = 1/re^a, f[6] = 1/(nser a)
-		float r = boxy_r32(c, x, y);
-		float a = c.f[1];
-		float ra = exp2f(a * log2f(r));
-		float core = exp2f(c.f[2] * log2f(1.f + exp2f(-a * log2f(r * c.f[0]))));
-		float sers = expf(-c.f[3] * exp2f(c.f[6] * log2f((ra + c.f[4]) * c.f[5])));
+		// f[0] = log2(1/rb), f[1] = a, f[2] = b/a, f[3] = bn, f[4] = rb^a, f[5] = 1/re^a, f[6] = 1/(nser a)
+		const float lr = log2_r32(c, x, y);
+		const float a = c.f[1];
+		const float ra = fexp2(a * lr);
+		const float core = fexp2(c.f[2] * flog2(1.f + fexp2(-a * (lr + c.f[0]))));
+		const float sers = fexp(-c.f[3] * fexp2(c.f[6] * flog2((ra + c.f[4]) * c.f[5])));
 		return core * sers;
 	}
 	default: return 0.f;
@@ -469,8 +525,8 @@ PK_HD void setup_component(const SetupIn &in, PkComp &o)
 		switch (c.kind) {
 		case PK_PROFILE_SERSIC:
 			o.fp32 = 1;
-			o.f[0] = (float)(o.boxy ? 1 / o.q[0] : 1 / (o.q[0] * o.q[0]));
-			o.f[1] = (float)(o.boxy ? 1 / o.q[3] : 0.5 / o.q[1]);
+			o.f[0] = (float)(-log2(o.q[0]));
+			o.f[1] = (float)(1 / o.q[1]);
 			o.f[2] = (float)bn;
 			o.f[7] = (float)(c.box + 2);
 			o.unfold = exp(bn);
@@ -487,11 +543,12 @@ PK_HD void setup_component(const SetupIn &in, PkComp &o)
 			o.f[1] = (float)(1 / o.q[0] - 1 / o.q[1]);
 			o.f[2] = (float)o.q[2];
 			o.f[3] = (float)o.q[3];
+			o.f[4] = (float)(1 / o.q[3]);
 			o.f[7] = (float)(c.box + 2);
 			break;
 		case PK_PROFILE_CORESERSIC:
 			o.fp32 = 1;
-			o.f[0] = (float)(1 / o.q[1]);
+			o.f[0] = (float)(-log2(o.q[1]));
 			o.f[1] = (float)o.q[3];
 			o.f[2] = (float)(o.q[4] / o.q[3]);
 			o.f[3] = (float)bn;
@@ -506,13 +563,14 @@ PK_HD void setup_component(const SetupIn &in, PkComp &o)
 			o.f[1] = (float)(1 / o.q[2]);
 			o.f[2] = (float)o.q[2];
 			o.q[4] = o.q[1] * o.q[1];
-			o.q[5] = o.q[0] * o.q[0] + o.q[1] * o.q[1];
+			o.q[5] = 1 / (o.q[0] * o.q[0] + o.q[1] * o.q[1]);
 			break;
 		case PK_PROFILE_FERRER:
 			o.fp32 = 1;
 			o.f[0] = (float)o.q[1];
 			o.f[1] = (float)((2 - o.q[2]) / 2);
 			o.q[4] = o.q[0] * o.q[0];
+			o.q[5] = 1 / o.q[4];
 			break;
 		default: break;
 		}

@@ -34,7 +34,8 @@ namespace pk {
 // and the Sersic centre pixels (res 8, 10 levels).
 constexpr int R32_CAP_MAIN = 192;
 constexpr int R32_CAP_LAST = 1024;
-constexpr uint32_t WQ_CHUNK = 64;
+constexpr uint32_t WQ_CHUNK = 64;         // pixels per claim at most
+constexpr uint32_t WQ_CHUNK_CELLS = 16;   // handed-over cells per claim at most
 constexpr int R32_TASKS = 16;              // cells per round at most (their records are staged in LDS)
 constexpr int R32_POLL = 16;               // rounds between two checks of the hand-over condition
 constexpr uint32_t R32_HANDOVER_DIV = 4;   // hand over once 1/R32_HANDOVER_DIV of the wavefronts stopped
@@ -181,11 +182,11 @@ __device__ inline void r32_expand(const PkJob *__restrict__ jobs, const PkComp *
 
 // Claims the next chunk of a work queue of n items; the chunk shrinks as the
 // queue runs dry (`seen` is the head this wavefront saw last).
-__device__ inline bool r32_claim(uint32_t *head, uint32_t n, uint32_t &seen, uint32_t &next, uint32_t &end,
-                                 uint32_t lane)
+__device__ inline bool r32_claim(uint32_t *head, uint32_t n, uint32_t chunk, uint32_t &seen, uint32_t &next,
+                                 uint32_t &end, uint32_t lane)
 {
 	const uint32_t left = n > seen ? n - seen : 0u;
-	const uint32_t want = max(1u, min(WQ_CHUNK, left / (2u * gridDim.x)));
+	const uint32_t want = max(1u, min(chunk, left / (2u * gridDim.x)));
 	uint32_t base = 0;
 	if (lane == 0) base = atomicAdd(head, want);
 	base = wave_read(base, 0);
@@ -194,6 +195,22 @@ __device__ inline bool r32_claim(uint32_t *head, uint32_t n, uint32_t &seen, uin
 	next = base;
 	end = min(base + want, n);
 	return true;
+}
+
+// Copies handed-over cells of claim indices [next, end) onto the LIFO, as many
+// as fit (at most 64); claim index i holds cell (i % 64) * stride + i / 64 if
+// that is below n.
+__device__ inline void r32_take_cells(const RCell *__restrict__ qin, uint32_t n, uint32_t stride, RCell *cells,
+                                      int cap, int &top, uint32_t &next, uint32_t end, uint32_t lane)
+{
+	const uint32_t k = min(min(end - next, (uint32_t)(cap - top)), 64u);
+	const uint32_t i = next + lane;
+	const uint32_t c = (i & 63u) * stride + (i >> 6);
+	const bool valid = lane < k && c < n;
+	const uint64_t b = __ballot(valid);
+	if (valid) cells[top + __popcll(b & ((1ull << lane) - 1ull))] = qin[c];
+	top += __popcll(b);
+	next += k;
 }
 
 // SOURCE 0: pixels from the coarse pass (pass 0); 1: cells of the previous pass.
@@ -212,7 +229,12 @@ __global__ __launch_bounds__(64) void refine32_kernel(const PkJob *__restrict__ 
 	if (lane < (uint32_t)R32_TASKS) st.lite_g[lane] = ~0u;
 	int top = 0;
 	bool source_done = false, handover = false;
-	const uint32_t n_source = SOURCE == 0 ? cnt->n_entries : min(cnt->n_cells[pass - 1], qcap);
+	const uint32_t n_cells_in = SOURCE == 0 ? 0u : min(cnt->n_cells[pass - 1], qcap);
+	// Handed-over cells are claimed in a transposed order -- claim index i takes
+	// cell (i % 64) * stride + i / 64 -- so that the cells one wavefront handed
+	// over, often parts of the same deep pixel, go to many wavefronts.
+	const uint32_t stride = (n_cells_in + 63u) / 64u;
+	const uint32_t n_source = SOURCE == 0 ? cnt->n_entries : 64u * stride;
 	uint32_t *head = &cnt->claim[pass];
 	uint32_t overflow = 0, bad = 0;
 	unsigned long long nev = 0;
@@ -229,17 +251,16 @@ __global__ __launch_bounds__(64) void refine32_kernel(const PkJob *__restrict__ 
 			}
 			if (wave_sum_u32(r2) >= 64u || source_done) break;
 			if (CAP - top < (SOURCE == 0 ? 32 : 1)) break;
-			if (chunk_next >= chunk_end && !r32_claim(head, n_source, seen, chunk_next, chunk_end, lane)) {
+			if (chunk_next >= chunk_end &&
+			    !r32_claim(head, n_source, SOURCE == 0 ? WQ_CHUNK : WQ_CHUNK_CELLS, seen, chunk_next, chunk_end,
+			               lane)) {
 				source_done = true;
 				break;
 			}
 			if (SOURCE == 0) {
 				r32_expand(jobs, comps, entries, st.cell, CAP, top, chunk_next, chunk_end, lane);
 			} else {
-				const uint32_t n = min(min(chunk_end - chunk_next, (uint32_t)(CAP - top)), 64u);
-				if (lane < n) st.cell[top + (int)lane] = qin[chunk_next + lane];
-				top += (int)n;
-				chunk_next += n;
+				r32_take_cells(qin, n_cells_in, stride, st.cell, CAP, top, chunk_next, chunk_end, lane);
 			}
 			__syncthreads();
 		}
@@ -344,13 +365,13 @@ __global__ __launch_bounds__(64) void refine32_kernel(const PkJob *__restrict__ 
 						const double txp = fabs(xp), typ = fabs(yp) + fabs(ddy);
 						const float t32 = evaluate_at32(cl, (float)txp, (float)typ, radius2_64(cl, txp, typ));
 						nev += 1;
-						const float m = fabsf(t32 * __builtin_amdgcn_rcpf(v32) - 1.f);
-						if (v32 > 0.f && m <= 3e38f && fabsf(m - acc_thr) > band) {
+						const float m = fabsf(t32 * frcp(v32) - 1.f);
+						if (v32 > 0.f && v32 <= 3e38f && m <= 3e38f && fabsf(m - acc_thr) > band) {
 							fail = m > acc_thr;
 							decided = true;
 						}
 					} else {
-						decided = true;
+						decided = v32 >= 0.f && v32 <= 3e38f;  // else: FP64
 					}
 					if (decided && !fail && !fixed_point_f32(v32 * wq, &fxv)) bad++;
 				}
@@ -419,10 +440,7 @@ __global__ __launch_bounds__(64) void refine32_kernel(const PkJob *__restrict__ 
 			if (SOURCE == 0) {
 				r32_expand(jobs, comps, entries, st.cell, CAP, top, chunk_next, chunk_end, lane);
 			} else {
-				const uint32_t n = min(min(chunk_end - chunk_next, (uint32_t)CAP), 64u);
-				if (lane < n) st.cell[lane] = qin[chunk_next + lane];
-				top = (int)n;
-				chunk_next += n;
+				r32_take_cells(qin, n_cells_in, stride, st.cell, CAP, top, chunk_next, chunk_end, lane);
 			}
 			__syncthreads();
 		}

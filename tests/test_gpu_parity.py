@@ -79,10 +79,23 @@ def test_profiles_fp64(ctx64, oracle, name, params, box):
     assert pix <= TOL_FP64, pix
 
 
-@pytest.mark.parametrize("name,params", PROFILES)
-def test_profiles_fp32(ctx32, oracle, name, params):
-    announce("profile_fp32 %s %s" % (name, params))
-    spec = abi.ModelSpec(64, 64, [comp(name, xcen=31.3, ycen=32.6, mag=16, ang=25, axrat=0.7, **params)])
+# FP32 evaluation regimes: truncations (king, ferrer with small a), sharp breaks,
+# b = 0 cores, compact Sersic profiles with deep refinement
+PROFILES_FP32 = PROFILES + [
+    ("ferrer", dict(rout=10.0, a=0.05, b=1.0)),
+    ("ferrer", dict(rout=9.0, a=3.0, b=1.9)),
+    ("king", dict(rc=1.0, rt=10.0, a=0.5)),
+    ("brokenexp", dict(h1=4.0, h2=8.0, rb=10.0, a=5.0)),
+    ("coresersic", dict(re=6.0, rb=1.0, nser=2.0, a=3.0, b=0.0)),
+    ("sersic", dict(re=1.2, nser=4.0)),
+]
+
+
+@pytest.mark.parametrize("box", [0.0, 0.4, -0.3])
+@pytest.mark.parametrize("name,params", PROFILES_FP32)
+def test_profiles_fp32(ctx32, oracle, name, params, box):
+    announce("profile_fp32 %s %s box=%s" % (name, params, box))
+    spec = abi.ModelSpec(64, 64, [comp(name, xcen=31.3, ycen=32.6, mag=16, ang=25, axrat=0.7, box=box, **params)])
     got, ref, _ = run_both(spec, ctx32, oracle)
     pix, flux = rel_errors(got, ref)
     assert pix <= TOL_FP32, pix
