@@ -237,16 +237,94 @@ __device__ inline float log2_r32(const CR &c, float x, float y)
 	return flog2(m) + l;
 }
 
-// FP64 squared radius for the FP32 king and ferrer evaluation, whose value
-// near the truncation radius depends on rt^2 - r^2; 0 for the other kinds.
+// FP64 squared (boxy) radius, for the FP32 king and ferrer evaluation whose
+// value near the truncation radius depends on rt^2 - r^2.
 template <typename CR>
-PK_HD double radius2_64(const CR &c, double x, double y)
+PK_HD double radius2_64_kf(const CR &c, double x, double y)
 {
-	if (c.kind != PK_PROFILE_KING && c.kind != PK_PROFILE_FERRER) return 0;
 	if (!c.boxy) return x * x + y * y;
 	const double b2 = kb2(c);
 	const double r = pow(pow(fabs(x), b2) + pow(fabs(y), b2), 1. / b2);
 	return r * r;
+}
+
+// radius2_64_kf for king and ferrer, 0 for the other kinds
+template <typename CR>
+PK_HD double radius2_64(const CR &c, double x, double y)
+{
+	if (c.kind != PK_PROFILE_KING && c.kind != PK_PROFILE_FERRER) return 0;
+	return radius2_64_kf(c, x, y);
+}
+
+template <typename CR>
+__device__ inline float king32(const CR &c, double r2)
+{
+	// f[0] = 1/rc^2, f[1] = 1/a, f[2] = a; q[4] = rt^2, q[5] = 1 / (rc^2 + rt^2).
+	// (t(r) - t(rt))^a with t(r) = (1 + r^2/rc^2)^(-1/a), written as
+	// t(r) (1 - (1 - d)^(1/a)), d = (rt^2 - r^2) / (rc^2 + rt^2), so that
+	// no two nearly equal numbers are subtracted near rt.
+	if (!(r2 < kq4(c))) return 0.f;
+	const float lt = -c.f[1] * flog2(1.f + (float)r2 * c.f[0]);
+	const float d = (float)((kq4(c) - r2) * kq5(c));
+	const float diff = -fexpm1(flog1p(-d) * c.f[1]);
+	if (!(diff > 0.f)) return c.f[2] == 0.f ? 1.f : 0.f;
+	return fexp2(c.f[2] * (lt + flog2(diff)));
+}
+
+template <typename CR>
+__device__ inline float ferrer32(const CR &c, double r2)
+{
+	// f[0] = a, f[1] = (2 - b) / 2; q[4] = rout^2, q[5] = 1 / rout^2.
+	// 1 - (r/rout)^(2-b) = -expm1((2-b)/2 log1p(-d)), d = 1 - r^2/rout^2
+	if (!(r2 < kq4(c))) return 0.f;
+	const float d = (float)((kq4(c) - r2) * kq5(c));
+	const float base = -fexpm1(c.f[1] * flog1p(-d));
+	if (!(base > 0.f)) return c.f[0] == 0.f ? 1.f : 0.f;
+	return fexp2(c.f[0] * flog2(base));
+}
+
+template <typename CR>
+__device__ inline float sersic32(const CR &c, float x, float y)
+{
+	// f[0] = -log2(re), f[1] = 1/nser, f[2] = bn, f[7] = box + 2: exp(-bn (r/re)^(1/nser))
+	const float rf = fexp2(c.f[1] * (log2_r32(c, x, y) + c.f[0]));
+	return fexp(-c.f[2] * rf);
+}
+
+template <typename CR>
+__device__ inline float moffat32(const CR &c, float x, float y)
+{
+	// f[0] = 1/rscale^2, f[1] = con
+	float r2f;
+	if (!c.boxy) {
+		r2f = (x * x + y * y) * c.f[0];
+	} else {
+		const float r = boxy_r32(c, x, y);
+		r2f = r * r * c.f[0];
+	}
+	return fexp2(-c.f[1] * flog2(1.f + r2f));
+}
+
+template <typename CR>
+__device__ inline float brokenexp32(const CR &c, float x, float y)
+{
+	// f[0] = 1/h1, f[1] = 1/h1 - 1/h2, f[2] = rb, f[3] = a, f[4] = 1/a
+	const float r = c.boxy ? boxy_r32(c, x, y) : fsqrt(x * x + y * y);
+	float base = r - c.f[2];
+	if (c.f[3] * base < 40.f) base = flog1p(fexp(c.f[3] * base)) * c.f[4];
+	return fexp(-r * c.f[0] + c.f[1] * base);
+}
+
+template <typename CR>
+__device__ inline float coresersic32(const CR &c, float x, float y)
+{
+	// f[0] = log2(1/rb), f[1] = a, f[2] = b/a, f[3] = bn, f[4] = rb^a, f[5] = 1/re^a, f[6] = 1/(nser a)
+	const float lr = log2_r32(c, x, y);
+	const float a = c.f[1];
+	const float ra = fexp2(a * lr);
+	const float core = fexp2(c.f[2] * flog2(1.f + fexp2(-a * (lr + c.f[0]))));
+	const float sers = fexp(-c.f[3] * fexp2(c.f[6] * flog2((ra + c.f[4]) * c.f[5])));
+	return core * sers;
 }
 
 // `r2` is radius2_64 at the same point.
@@ -254,60 +332,52 @@ template <typename CR>
 __device__ inline float evaluate_at32(const CR &c, float x, float y, double r2)
 {
 	switch (c.kind) {
-	case PK_PROFILE_KING: {
-		// f[0] = 1/rc^2, f[1] = 1/a, f[2] = a; q[4] = rt^2, q[5] = 1 / (rc^2 + rt^2).
-		// (t(r) - t(rt))^a with t(r) = (1 + r^2/rc^2)^(-1/a), written as
-		// t(r) (1 - (1 - d)^(1/a)), d = (rt^2 - r^2) / (rc^2 + rt^2), so that
-		// no two nearly equal numbers are subtracted near rt.
-		if (!(r2 < kq4(c))) return 0.f;
-		const float lt = -c.f[1] * flog2(1.f + (float)r2 * c.f[0]);
-		const float d = (float)((kq4(c) - r2) * kq5(c));
-		const float diff = -fexpm1(flog1p(-d) * c.f[1]);
-		if (!(diff > 0.f)) return c.f[2] == 0.f ? 1.f : 0.f;
-		return fexp2(c.f[2] * (lt + flog2(diff)));
-	}
-	case PK_PROFILE_FERRER: {
-		// f[0] = a, f[1] = (2 - b) / 2; q[4] = rout^2, q[5] = 1 / rout^2.
-		// 1 - (r/rout)^(2-b) = -expm1((2-b)/2 log1p(-d)), d = 1 - r^2/rout^2
-		if (!(r2 < kq4(c))) return 0.f;
-		const float d = (float)((kq4(c) - r2) * kq5(c));
-		const float base = -fexpm1(c.f[1] * flog1p(-d));
-		if (!(base > 0.f)) return c.f[0] == 0.f ? 1.f : 0.f;
-		return fexp2(c.f[0] * flog2(base));
-	}
-	case PK_PROFILE_SERSIC: {
-		// f[0] = -log2(re), f[1] = 1/nser, f[2] = bn, f[7] = box + 2: exp(-bn (r/re)^(1/nser))
-		const float rf = fexp2(c.f[1] * (log2_r32(c, x, y) + c.f[0]));
-		return fexp(-c.f[2] * rf);
-	}
-	case PK_PROFILE_MOFFAT: {
-		// f[0] = 1/rscale^2, f[1] = con
-		float r2f;
-		if (!c.boxy) {
-			r2f = (x * x + y * y) * c.f[0];
-		} else {
-			const float r = boxy_r32(c, x, y);
-			r2f = r * r * c.f[0];
-		}
-		return fexp2(-c.f[1] * flog2(1.f + r2f));
-	}
-	case PK_PROFILE_BROKENEXP: {
-		// f[0] = 1/h1, f[1] = 1/h1 - 1/h2, f[2] = rb, f[3] = a, f[4] = 1/a
-		const float r = c.boxy ? boxy_r32(c, x, y) : fsqrt(x * x + y * y);
-		float base = r - c.f[2];
-		if (c.f[3] * base < 40.f) base = flog1p(fexp(c.f[3] * base)) * c.f[4];
-		return fexp(-r * c.f[0] + c.f[1] * base);
-	}
-	case PK_PROFILE_CORESERSIC: {
-		// f[0] = log2(1/rb), f[1] = a, f[2] = b/a, f[3] = bn, f[4] = rb^a, f[5] = 1/re^a, f[6] = 1/(nser a)
-		const float lr = log2_r32(c, x, y);
-		const float a = c.f[1];
-		const float ra = fexp2(a * lr);
-		const float core = fexp2(c.f[2] * flog2(1.f + fexp2(-a * (lr + c.f[0]))));
-		const float sers = fexp(-c.f[3] * fexp2(c.f[6] * flog2((ra + c.f[4]) * c.f[5])));
-		return core * sers;
-	}
+	case PK_PROFILE_KING: return king32(c, r2);
+	case PK_PROFILE_FERRER: return ferrer32(c, r2);
+	case PK_PROFILE_SERSIC: return sersic32(c, x, y);
+	case PK_PROFILE_MOFFAT: return moffat32(c, x, y);
+	case PK_PROFILE_BROKENEXP: return brokenexp32(c, x, y);
+	case PK_PROFILE_CORESERSIC: return coresersic32(c, x, y);
 	default: return 0.f;
+	}
+}
+
+// Values of a profile of kind `kind` at profile coordinates (xa, ya) and, if
+// PAIR, at (xb, yb); both evaluations sit in one branch, so that they
+// interleave.
+template <bool PAIR, typename CR>
+__device__ inline void evaluate_pair32(const CR &c, int kind, double xa, double ya, double xb, double yb, float &va,
+                                       float &vb)
+{
+	const float x1 = (float)xa, y1 = (float)ya, x2 = (float)xb, y2 = (float)yb;
+	switch (kind) {
+	case PK_PROFILE_KING:
+		va = king32(c, radius2_64_kf(c, xa, ya));
+		if (PAIR) vb = king32(c, radius2_64_kf(c, xb, yb));
+		break;
+	case PK_PROFILE_FERRER:
+		va = ferrer32(c, radius2_64_kf(c, xa, ya));
+		if (PAIR) vb = ferrer32(c, radius2_64_kf(c, xb, yb));
+		break;
+	case PK_PROFILE_SERSIC:
+		va = sersic32(c, x1, y1);
+		if (PAIR) vb = sersic32(c, x2, y2);
+		break;
+	case PK_PROFILE_MOFFAT:
+		va = moffat32(c, x1, y1);
+		if (PAIR) vb = moffat32(c, x2, y2);
+		break;
+	case PK_PROFILE_BROKENEXP:
+		va = brokenexp32(c, x1, y1);
+		if (PAIR) vb = brokenexp32(c, x2, y2);
+		break;
+	case PK_PROFILE_CORESERSIC:
+		va = coresersic32(c, x1, y1);
+		if (PAIR) vb = coresersic32(c, x2, y2);
+		break;
+	default:
+		va = -1.f;  // no FP32 form
+		vb = -1.f;
 	}
 }
 

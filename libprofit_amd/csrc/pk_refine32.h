@@ -82,6 +82,19 @@ __device__ inline uint32_t wave_read(uint32_t v, int lane) { return (uint32_t)__
 
 __device__ inline uint32_t wave_sum_u32(uint32_t v) { return wave_read(wave_incl_scan_u32(v), 63); }
 
+// sum over the 64 lanes (lane 63 holds it after the scan; read as a scalar)
+__device__ inline unsigned long long wave_sum_u64(unsigned long long v)
+{
+	v += dpp0_u64<DPP_ROW_SHR1>(v);
+	v += dpp0_u64<DPP_ROW_SHR2>(v);
+	v += dpp0_u64<DPP_ROW_SHR4>(v);
+	v += dpp0_u64<DPP_ROW_SHR8>(v);
+	v += dpp0_u64<DPP_ROW_BCAST15, 0xa>(v);
+	v += dpp0_u64<DPP_ROW_BCAST31, 0xc>(v);
+	const uint32_t lo = wave_read((uint32_t)v, 63), hi = wave_read((uint32_t)(v >> 32), 63);
+	return ((unsigned long long)hi << 32) | lo;
+}
+
 // Inclusive prefix sum within segments of consecutive lanes; `sbeg` is the
 // first lane of the calling lane's segment.
 __device__ inline unsigned long long wave_seg_scan_u64(unsigned long long v, uint32_t lane, uint32_t sbeg)
@@ -334,50 +347,53 @@ __global__ __launch_bounds__(64) void refine32_kernel(const PkJob *__restrict__ 
 		const float band = 1e-4f * (1.f + acc_thr);
 		const double ddy = (-2 * hx * cl.sin_ang + 2 * hy * cl.cos_ang) * cl.inv_axrat;
 		const size_t target = 2 * (size_t)cell.entry + cm_conv(meta);
-		const bool fp32 = cl.fp32 != 0;
+		const int kind = cl.fp32 ? (int)cl.kind : -1;
+		// wavefront-uniform facts of this group of rounds (lanes beyond the last
+		// task, which are inactive, share the last task's)
+		const bool any_rec = __ballot(recurse) != 0;
+		const int kind0 = (int)wave_read((uint32_t)kind, 0);
+		const bool one_kind = __ballot(kind != kind0) == 0;
 		for (int rd = 0; rd < rounds; rd++) {
 			const uint32_t g = lane + 64u * (uint32_t)rd;
 			const bool active = g < lanes_needed;
+			// sub-cell s = si * res + sj, found without an integer division
 			const int s = (int)(g - before);
-			bool fail = false;
-			double dx = 0, dy = 0;
+			int si = (int)((float)s * inv_res);
+			int sj = s - si * res;
+			if (sj >= res) {
+				si++;
+				sj -= res;
+			} else if (sj < 0) {
+				si--;
+				sj += res;
+			}
+			const double dx = fma((double)(2 * si + 1), hx, cell.dx0);
+			const double dy = fma((double)(2 * sj + 1), hy, cell.dy0);
+			const double xp = fma(dx, cl.cos_ang, dy * cl.sin_ang);
+			const double yp = fma(dy, cl.cos_ang, -dx * cl.sin_ang) * cl.inv_axrat;
+			const double txp = fabs(xp), typ = fabs(yp) + fabs(ddy);  // accuracy test point
+			float v32 = -1.f, t32 = -1.f;
+			if (one_kind) {
+				// the usual case: one profile kind in all lanes, scalar branch
+				if (any_rec) evaluate_pair32<true>(cl, kind0, xp, yp, txp, typ, v32, t32);
+				else evaluate_pair32<false>(cl, kind0, xp, yp, txp, typ, v32, t32);
+			} else {
+				evaluate_pair32<true>(cl, kind, xp, yp, txp, typ, v32, t32);
+			}
+			bool decided, fail = false;
+			if (recurse) {
+				const float m = fabsf(t32 * frcp(v32) - 1.f);
+				decided = v32 > 0.f && v32 <= 3e38f && m <= 3e38f && fabsf(m - acc_thr) > band;
+				fail = decided && m > acc_thr;
+			} else {
+				decided = v32 >= 0.f && v32 <= 3e38f;
+			}
 			unsigned long long fxv = 0;
-			if (active) {
-				// s = si * res + sj without an integer division
-				int si = (int)((float)s * inv_res);
-				int sj = s - si * res;
-				if (sj >= res) {
-					si++;
-					sj -= res;
-				} else if (sj < 0) {
-					si--;
-					sj += res;
-				}
-				dx = fma((double)(2 * si + 1), hx, cell.dx0);
-				dy = fma((double)(2 * sj + 1), hy, cell.dy0);
-				const double xp = fma(dx, cl.cos_ang, dy * cl.sin_ang);
-				const double yp = fma(dy, cl.cos_ang, -dx * cl.sin_ang) * cl.inv_axrat;
-				bool decided = false;
-				if (fp32) {
-					const float v32 = evaluate_at32(cl, (float)xp, (float)yp, radius2_64(cl, xp, yp));
-					nev += 1;
-					if (recurse) {
-						const double txp = fabs(xp), typ = fabs(yp) + fabs(ddy);
-						const float t32 = evaluate_at32(cl, (float)txp, (float)typ, radius2_64(cl, txp, typ));
-						nev += 1;
-						const float m = fabsf(t32 * frcp(v32) - 1.f);
-						if (v32 > 0.f && v32 <= 3e38f && m <= 3e38f && fabsf(m - acc_thr) > band) {
-							fail = m > acc_thr;
-							decided = true;
-						}
-					} else {
-						decided = v32 >= 0.f && v32 <= 3e38f;  // else: FP64
-					}
-					if (decided && !fail && !fixed_point_f32(v32 * wq, &fxv)) bad++;
-				}
-				if (!decided) {
+			if (active && decided && !fail && !fixed_point_f32(v32 * wq, &fxv)) bad++;
+			if (__ballot(active && !decided)) {
+				// FP64 value and decision within the guard band (or without an FP32 form)
+				if (active && !decided) {
 					const double val = refine_fp64_decision(comps[cell.gcomp], xp, yp, ddy, recurse, &fail);
-					nev += recurse ? 2 : 1;
 					if (!fail) {
 						const double q = val * ((double)(fr * fr) * cl.wflux);
 						if (q >= 0 && q < 9.2e18) fxv = (unsigned long long)__double2ll_rn(q);
@@ -385,12 +401,18 @@ __global__ __launch_bounds__(64) void refine32_kernel(const PkJob *__restrict__ 
 					}
 				}
 			}
-			// exact per-cell sums: segmented scan over the cell's contiguous lanes,
-			// the last lane adds the sum to the pixel accumulator
-			const uint32_t sbeg = rounds > 1 ? 0u : before;
-			const uint32_t send = rounds > 1 ? min(64u, lanes_needed - 64u * (uint32_t)rd) : seg_end1;
-			const unsigned long long v = wave_seg_scan_u64(fxv, lane, sbeg);
-			if (active && lane == send - 1 && v) atomicAdd(&acc[target], v);
+			fail = fail && active;
+			if (active) nev += recurse ? 2 : 1;
+			// exact per-cell sums; the last lane of a cell adds its sum to the
+			// pixel accumulator
+			if (ntask == 1) {
+				const unsigned long long v = wave_sum_u64(fxv);
+				if (lane == 0 && v) atomicAdd(&acc[target], v);
+			} else {
+				// several cells: segmented scan over each cell's contiguous lanes
+				const unsigned long long v = wave_seg_scan_u64(fxv, lane, before);
+				if (active && lane == seg_end1 - 1 && v) atomicAdd(&acc[target], v);
+			}
 			// failing sub-cells: onto the LIFO, or to the next pass when it is full
 			const uint64_t ballot = __ballot(fail);
 			if (ballot) {
