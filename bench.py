@@ -59,6 +59,20 @@ def conv_flops(specs):
     return tot
 
 
+def conv_traffic():
+    """HBM bytes per launch of the convolution kernel on this workload, as
+    measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE and recorded in
+    profiles/conv_kernel_galaxy4096_traffic.txt (null if absent)."""
+    path = os.path.join(ROOT, "profiles", "conv_kernel_galaxy4096_traffic.txt")
+    try:
+        for line in open(path):
+            if line.startswith("hbm_bytes_per_launch:"):
+                return int(float(line.split(":", 1)[1]))
+    except OSError:
+        pass
+    return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -69,7 +83,19 @@ def main():
     ap.add_argument("--sample", type=int, default=96, help="galaxies timed on the plain FP64 path")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU baseline")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
+                    help="only run a plain device copy of BYTES bytes 3 times (calibrates the HBM byte counters)")
     args = ap.parse_args()
+    if args.copy_probe:
+        import torch
+        n = args.copy_probe // 4
+        a = torch.ones(n, dtype=torch.float32, device="cuda:0")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        log("copy probe: 3 copies of %d bytes" % (4 * n))
+        return
     if args.gpus != 1:
         print("bench.py measures one GPU: the batched path shards galaxies across GPUs with no exchange, "
               "and only --gpus 1 is run here", file=sys.stderr)
@@ -127,12 +153,14 @@ def main():
     achieved_tflops = flops / (conv_ms * 1e-3) / 1e12
     roofline = {
         "kernel": "conv_kernel<float> (direct PSF convolution)",
-        "bound": "valu_fp32",
+        # 2*kw*kh flop per pixel over ~8 B per pixel: far right of the HBM ridge,
+        # bound by FP32 vector FMA issue
+        "bound": "valu",
         "achieved": round(achieved_tflops, 3),
         "peak": FP32_PEAK_TFLOPS,
         "unit": "TFLOP/s",
         "frac": round(achieved_tflops / FP32_PEAK_TFLOPS, 4),
-        "traffic": None,
+        "traffic": conv_traffic(),
         "algorithmic_flops_per_launch": flops,
         "avg_launch_ms": round(conv_ms, 4),
         "stage_ms": {k: round(v, 4) for k, v in stages.items()},
@@ -165,28 +193,27 @@ def main():
     log("fp32 vs fp64 on the sample: max pixel rel err %.3g, max flux rel err %.3g" % (worst_pix, worst_flux))
 
     # ---- CPU baseline: FP64 oracle with OpenMP on a bounded sample -------------------
+    # galaxies of the same batch in a seeded order until --cpu-seconds of work
     cpu = None
     if not args.no_cpu:
-        try:
-            from oracle import oracle as O
-            O.build()
-            threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
-            O.set_threads(threads)
-            done, tpix, t0 = 0, 0, time.perf_counter()
-            for i in sample:
-                O.evaluate(specs[i])
-                done += 1
-                tpix += int(specs[i].m.width) * int(specs[i].m.height)
-                if time.perf_counter() - t0 > args.cpu_seconds:
-                    break
-            dt = time.perf_counter() - t0
-            cpu_ips = done / dt * (tpix / done) / (npix / len(specs))
-            cpu = {"value": round(cpu_ips, 3), "unit": "images/s", "cores": threads, "kind": "port",
-                   "sample": "%d of the same galaxies (FP64 oracle, OpenMP, BRUTE), normalised to the batch mean "
-                             "image size; %.1f s" % (done, dt)}
-            log("cpu baseline: %s" % cpu)
-        except Exception as e:  # the baseline is reported, never required
-            log("cpu baseline failed: %r" % e)
+        from oracle import oracle as O
+        O.build()
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
+        O.set_threads(threads)
+        order = np.random.Generator(np.random.PCG64(777)).permutation(len(specs)).tolist()
+        done, tpix, t0 = 0, 0, time.perf_counter()
+        for i in order:
+            O.evaluate(specs[i])
+            done += 1
+            tpix += int(specs[i].m.width) * int(specs[i].m.height)
+            if time.perf_counter() - t0 > args.cpu_seconds:
+                break
+        dt = time.perf_counter() - t0
+        cpu_ips = done / dt * (tpix / done) / (npix / len(specs))
+        cpu = {"value": round(cpu_ips, 3), "unit": "images/s", "cores": threads, "kind": "port",
+               "sample": "%d of the %d galaxies in a seeded order (FP64 oracle, OpenMP, BRUTE), %.1f s; "
+                         "images/s normalised to the batch mean image size" % (done, len(specs), dt)}
+        log("cpu baseline: %s" % cpu)
 
     out = {
         "metric": METRIC,
