@@ -187,6 +187,9 @@ int pk_model_output_dims(const pk_model *m, uint32_t *width, uint32_t *height);
 typedef struct pk_constants {
 	double rscale, lumtot, ie, bn, acc, rscale_switch, rscale_max, flux_scale, cos_ang, sin_ang;
 	uint32_t resolution, max_recursions;
+	/* Sersic only, for the chi^2 gradient: d bn / d nser and the derivatives
+	 * of log(flux_scale) with respect to re, nser, axrat and box */
+	double dbn_dnser, dlogflux_dre, dlogflux_dnser, dlogflux_daxrat, dlogflux_dbox;
 } pk_constants;
 int pk_component_constants(const pk_model *m, uint32_t index, pk_constants *out);
 
@@ -220,13 +223,17 @@ int pk_convolver_padding(int type, uint32_t src_w, uint32_t src_h, uint32_t krn_
  * Output images are FP32 in a PK_FP32 context and FP64 in a PK_FP64 one. */
 typedef struct pk_batch pk_batch;
 
-enum { PK_BATCH_GRAPH = 1 };  /* capture the launch sequence once, replay it */
+enum {
+	PK_BATCH_GRAPH = 1,  /* capture the launch sequence once, replay it */
+	PK_BATCH_GRAD = 2    /* also allocate what pk_batch_evaluate_grad needs */
+};
 
 typedef struct pk_batch_info {
 	uint64_t n_refine_entries;  /* pixels that entered sub-pixel refinement */
 	uint64_t n_subcells;        /* evaluate_at calls made by the refinement */
 	/* per-stage device time of the last evaluation (not available in graph mode: 0) */
 	double ms_setup, ms_coarse, ms_refine, ms_conv, ms_compose, ms_total;
+	double ms_grad;             /* adjoint part of the last pk_batch_evaluate_grad */
 } pk_batch_info;
 
 int pk_batch_create(pk_ctx *ctx, const pk_model *models, uint32_t n_models, uint32_t flags, pk_batch **out);
@@ -246,6 +253,38 @@ int pk_batch_get_image(pk_batch *b, uint32_t i, double *out, size_t out_capacity
 int pk_batch_device_images(const pk_batch *b, void **base, size_t *elem_size);
 int pk_batch_image_offset(const pk_batch *b, uint32_t i, uint64_t *offset);
 int pk_batch_destroy(pk_batch *b);
+
+/* ---- chi^2 and its gradient ----------------------------------------------
+ * No counterpart in libprofit (fits with it use finite differences, e.g. in
+ * ProFit).  For every model of a PK_BATCH_GRAD batch:
+ *   chi2 = sum_p ((data_p - model_p) / sigma_p)^2 over its returned image,
+ * and d chi2 / d parameter for every component, by a hand-written adjoint of
+ * the evaluation: returned image -> crop/downsample -> PSF correlation ->
+ * directly evaluated pixels and the sub-pixel refinement.  The refinement
+ * pattern (which sub-cells are refined, the adjusted resolution,
+ * rscale_switch, rscale_max and acc) is that of the evaluation at the given
+ * parameters and is held fixed: the gradient is that of the piecewise-smooth
+ * image with this pattern.  Gradients are available for sersic and sky
+ * profiles (null profiles: zeros) with odd PSF dimensions.
+ * Gradient slots per component (PK_GRAD_SLOTS doubles, unused slots 0):
+ *   sersic: xcen, ycen, mag, re, nser, ang (degrees), axrat, box
+ *   sky:    bg */
+#define PK_GRAD_SLOTS 8
+enum {
+	PK_GRAD_XCEN = 0, PK_GRAD_YCEN = 1, PK_GRAD_MAG = 2, PK_GRAD_RE = 3, PK_GRAD_NSER = 4,
+	PK_GRAD_ANG = 5, PK_GRAD_AXRAT = 6, PK_GRAD_BOX = 7,
+	PK_GRAD_SKY_BG = 0
+};
+/* Observed images and their per-pixel sigma (> 0), host arrays of n doubles
+ * packed like the batch's returned images (pk_batch_image_offset); sigma
+ * NULL means 1 everywhere.  Kept in device memory until replaced. */
+int pk_batch_set_data(pk_batch *b, const double *data, const double *sigma, size_t n);
+/* Enqueues one evaluation of every model followed by chi2 and its gradient
+ * (in a PK_BATCH_GRAPH batch: one more captured graph). */
+int pk_batch_evaluate_grad(pk_batch *b);
+/* Waits and copies chi2[n_models] and grad[n_components * PK_GRAD_SLOTS]
+ * (components in the order given at creation). */
+int pk_batch_get_chi2_grad(pk_batch *b, double *chi2, double *grad);
 
 /* ---- FP64 special functions of the math back-end ------------------------
  * include/profit/utils.h:52-113 (GSL/Rmath in libprofit, own implementations

@@ -110,7 +110,9 @@ class Result(C.Structure):
 class Constants(C.Structure):
     _fields_ = [(n, C.c_double) for n in ("rscale", "lumtot", "ie", "bn", "acc", "rscale_switch", "rscale_max",
                                           "flux_scale", "cos_ang", "sin_ang")] + [
-        ("resolution", C.c_uint32), ("max_recursions", C.c_uint32)]
+        ("resolution", C.c_uint32), ("max_recursions", C.c_uint32)] + [
+        (n, C.c_double) for n in ("dbn_dnser", "dlogflux_dre", "dlogflux_dnser", "dlogflux_daxrat",
+                                  "dlogflux_dbox")]
 
 
 class BatchInfo(C.Structure):
@@ -123,10 +125,20 @@ class BatchInfo(C.Structure):
         ("ms_conv", C.c_double),
         ("ms_compose", C.c_double),
         ("ms_total", C.c_double),
+        ("ms_grad", C.c_double),
     ]
 
 
 PK_BATCH_GRAPH = 1
+PK_BATCH_GRAD = 2
+
+# gradient slots of a component (pk_batch_get_chi2_grad)
+GRAD_SLOTS = 8
+GRAD_PARAMS = {
+    "sersic": ["xcen", "ycen", "mag", "re", "nser", "ang", "axrat", "box"],
+    "sky": ["bg"],
+    "null": [],
+}
 
 
 RADIAL_DEFAULTS = dict(

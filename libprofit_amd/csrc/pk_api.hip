@@ -66,6 +66,7 @@ struct pk_ctx {
 	// grids of the FP32 refinement passes: as many wavefronts as fit on the
 	// device at once (the hand-over rule counts them)
 	uint32_t r32_grid_main = 0, r32_grid_last = 0;
+	uint32_t r32g_grid_main = 0, r32g_grid_last = 0;  // the same for the adjoint variants
 };
 
 static int ensure_dev(pk_ctx *ctx, size_t bytes)
@@ -162,6 +163,10 @@ int pk_ctx_create(int device, int precision, pk_ctx **ctx_out)
 	PK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb_last, refine32_kernel<1, true>, 64, 0));
 	ctx->r32_grid_main = (uint32_t)std::max(1, nb_main * prop.multiProcessorCount);
 	ctx->r32_grid_last = (uint32_t)std::max(1, nb_last * prop.multiProcessorCount);
+	PK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb_main, refine32_kernel<1, false, true>, 64, 0));
+	PK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb_last, refine32_kernel<1, true, true>, 64, 0));
+	ctx->r32g_grid_main = (uint32_t)std::max(1, nb_main * prop.multiProcessorCount);
+	ctx->r32g_grid_last = (uint32_t)std::max(1, nb_last * prop.multiProcessorCount);
 	*ctx_out = ctx;
 	return PK_OK;
 }
@@ -936,8 +941,14 @@ int pk_component_constants(const pk_model *m, uint32_t index, pk_constants *out)
 	in.pad_x = a.padx;
 	in.pad_y = a.pady;
 	in.fs = fs;
+	in.grad = pc.kind == PK_PROFILE_SERSIC;
 	PkComp c;
 	setup_component(in, c);
+	out->dbn_dnser = c.dbn;
+	out->dlogflux_dre = c.glnf_re;
+	out->dlogflux_dnser = c.glnf_n;
+	out->dlogflux_daxrat = c.glnf_q;
+	out->dlogflux_dbox = c.glnf_b;
 	out->rscale = c.rscale;
 	out->lumtot = c.lumtot;
 	out->ie = c.ie;
@@ -1138,6 +1149,12 @@ struct pk_batch {
 	hipGraphExec_t exec = nullptr;
 	hipEvent_t ev[7] = {};
 	bool ran = false;
+	// chi^2 adjoint (PK_BATCH_GRAD)
+	size_t o_psfF = 0, o_gd = 0, o_gc = 0, o_data = 0, o_ivar = 0, o_gacc = 0, o_gout = 0, o_chi2 = 0, o_gcnt = 0;
+	bool have_data = false, ran_grad = false;
+	hipGraph_t graph_grad = nullptr;
+	hipGraphExec_t exec_grad = nullptr;
+	hipEvent_t ev_grad[2] = {};
 	template <typename T> T *d(size_t off) const
 	{
 		return reinterpret_cast<T *>(static_cast<char *>(dmem) + off);
@@ -1202,6 +1219,70 @@ int batch_launch(pk_batch *b)
 	return PK_OK;
 }
 
+// The adjoint part of pk_batch_evaluate_grad, after batch_launch: residuals
+// and chi^2 -> adjoint drawing images -> PSF correlation -> coarse and
+// refinement adjoints -> gradient slots.
+template <typename T>
+int grad_launch_t(pk_batch *b)
+{
+	pk_ctx *ctx = b->ctx;
+	hipStream_t s = ctx->stream;
+	const bool timed = !(b->flags & PK_BATCH_GRAPH);
+	const PkJob *djobs = b->d<PkJob>(b->o_jobs);
+	PkComp *dcomp = b->d<PkComp>(b->o_comps);
+	PkCounters *gcnt = b->d<PkCounters>(b->o_gcnt);
+	double *gacc = b->d<double>(b->o_gacc), *chi2 = b->d<double>(b->o_chi2);
+	const uint32_t nj = b->n_models;
+	T *Gd = b->d<T>(b->o_gd), *Gc = b->d<T>(b->o_gc);
+	if (timed) PK_HIP(hipEventRecord(b->ev_grad[0], s));
+	const uint32_t nacc = b->n_comps * GA_SLOTS;
+	hipLaunchKernelGGL(reset_grad_kernel, dim3(std::max(1u, std::min(256u, cdiv(std::max(nacc, nj), 256)))),
+	                   dim3(256), 0, s, gcnt, (const PkCounters *)b->d<PkCounters>(b->o_cnt), gacc, nacc, chi2, nj);
+	hipLaunchKernelGGL(resid_kernel<T>, dim3(b->ntiles), dim3(256), 0, s, djobs, nj, (const T *)b->d<T>(b->o_out),
+	                   (const T *)b->d<T>(b->o_data), (const T *)b->d<T>(b->o_ivar), Gd, chi2);
+	PK_HIP(hipGetLastError());
+	if (b->any_conv)
+		PK_TRY(launch_conv<T>(s, djobs, nj, b->nctiles, Gd, b->d<T>(b->o_psfF), b->d<uint8_t>(b->o_mask), Gc,
+		                      b->max_kw, b->max_kh));
+	const bool fp32 = sizeof(T) == 4;
+	hipLaunchKernelGGL((coarse_grad_kernel<T, fp32>), dim3(b->ntiles), dim3(256), 0, s, djobs, nj,
+	                   (const PkComp *)dcomp, (const uint8_t *)b->d<uint8_t>(b->o_mask), (const T *)Gd,
+	                   (const T *)Gc, gacc);
+	PK_HIP(hipGetLastError());
+	const RefineEntry *dent = b->d<RefineEntry>(b->o_ent);
+	if (fp32) {
+		const CompLite *dlite = b->d<CompLite>(b->o_lite);
+		RCell *q0 = b->d<RCell>(b->o_q), *q1 = q0 + b->qcap;
+		const uint32_t gm = ctx->r32g_grid_main, gl = ctx->r32g_grid_last;
+		const float *Gf = (const float *)Gd, *Gcf = (const float *)Gc;
+		hipLaunchKernelGGL((refine32_kernel<0, false, true>), dim3(gm), dim3(64), 0, s, djobs, dcomp, dlite, dent,
+		                   gcnt, (const RCell *)nullptr, q0, b->qcap, 0, (unsigned long long *)nullptr, Gf, Gcf, gacc);
+		for (int p = 1; p < R32_PASSES - 1; p++)
+			hipLaunchKernelGGL((refine32_kernel<1, false, true>), dim3(gm), dim3(64), 0, s, djobs, dcomp, dlite, dent,
+			                   gcnt, (const RCell *)((p & 1) ? q0 : q1), (p & 1) ? q1 : q0, b->qcap, p,
+			                   (unsigned long long *)nullptr, Gf, Gcf, gacc);
+		hipLaunchKernelGGL((refine32_kernel<1, true, true>), dim3(gl), dim3(64), 0, s, djobs, dcomp, dlite, dent, gcnt,
+		                   (const RCell *)((R32_PASSES & 1) ? q1 : q0), (RCell *)nullptr, b->qcap, R32_PASSES - 1,
+		                   (unsigned long long *)nullptr, Gf, Gcf, gacc);
+	} else {
+		hipLaunchKernelGGL((refine_kernel<double, false, true>), dim3(4096), dim3(64), 0, s, djobs,
+		                   (const PkComp *)dcomp, dent, gcnt, (double *)nullptr, (double *)nullptr,
+		                   (const double *)Gd, (const double *)Gc, gacc);
+	}
+	PK_HIP(hipGetLastError());
+	hipLaunchKernelGGL(grad_finalize_kernel, dim3(cdiv(std::max(1u, b->n_comps), 64)), dim3(64), 0, s,
+	                   (const PkComp *)dcomp, (const double *)gacc, b->d<double>(b->o_gout), b->n_comps);
+	PK_HIP(hipGetLastError());
+	if (timed) PK_HIP(hipEventRecord(b->ev_grad[1], s));
+	return PK_OK;
+}
+
+int grad_launch(pk_batch *b)
+{
+	PK_TRY(batch_launch(b));
+	return b->elem == 4 ? grad_launch_t<float>(b) : grad_launch_t<double>(b);
+}
+
 }  // namespace
 
 extern "C" {
@@ -1213,6 +1294,10 @@ int pk_batch_destroy(pk_batch *b)
 	(void)hipStreamSynchronize(b->ctx->stream);
 	if (b->exec) (void)hipGraphExecDestroy(b->exec);
 	if (b->graph) (void)hipGraphDestroy(b->graph);
+	if (b->exec_grad) (void)hipGraphExecDestroy(b->exec_grad);
+	if (b->graph_grad) (void)hipGraphDestroy(b->graph_grad);
+	for (auto &e : b->ev_grad)
+		if (e) (void)hipEventDestroy(e);
 	for (auto &e : b->ev)
 		if (e) (void)hipEventDestroy(e);
 	if (b->dmem) (void)hipFree(b->dmem);
@@ -1253,6 +1338,19 @@ int pk_batch_create(pk_ctx *ctx, const pk_model *models, uint32_t n_models, uint
 			return fail(set_err(PK_EINVAL, "model %u: calculation masks are not supported in a batch", mi));
 		if (a.padx == 0 && a.pady == 0 && a.conv_required && m->psf_width > 1 && m->psf_height > 1)
 			return fail(set_err(PK_EINVAL, "model %u: unsupported padding", mi));
+		if (flags & PK_BATCH_GRAD) {
+			// the adjoint of the PSF convolution is the convolution with the
+			// flipped PSF on the same grid only for odd PSF dimensions
+			if (a.conv_required && (m->psf_width % 2 == 0 || m->psf_height % 2 == 0))
+				return fail(set_err(PK_EINVAL, "model %u: gradients need odd PSF dimensions", mi));
+			for (uint32_t c = 0; c < m->n_components; c++) {
+				const pk_component &pc = m->components[c];
+				if (pc.kind != PK_PROFILE_SERSIC && pc.kind != PK_PROFILE_SKY && pc.kind != PK_PROFILE_NULL)
+					return fail(set_err(PK_EINVAL, "model %u: gradients are available for sersic and sky profiles", mi));
+				if (pc.kind == PK_PROFILE_SERSIC && pc.rescale_flux)
+					return fail(set_err(PK_EINVAL, "model %u: gradients do not support rescale_flux", mi));
+			}
+		}
 		const uint32_t fs = m->finesampling;
 		const double sx = m->scale_x / fs, sy = m->scale_y / fs;
 		PkJob jb{};
@@ -1287,6 +1385,7 @@ int pk_batch_create(pk_ctx *ctx, const pk_model *models, uint32_t n_models, uint
 			in.pad_y = a.pady;
 			in.fp32_mode = fp32 ? 1 : 0;
 			in.fs = fs;
+			in.grad = (flags & PK_BATCH_GRAD) ? 1 : 0;
 			b->comp_index.push_back((uint32_t)b->setup.size());
 			b->setup.push_back(in);
 			if (!pc.convolve) any_nc = true;
@@ -1366,6 +1465,17 @@ int pk_batch_create(pk_ctx *ctx, const pk_model *models, uint32_t n_models, uint
 	b->qcap = fp32 ? R32_QCAP_BATCH : 0;
 	b->o_q = L.take(2 * sizeof(RCell) * std::max<uint32_t>(1, b->qcap));
 	b->o_out = L.take(elem * out_off);
+	if (flags & PK_BATCH_GRAD) {
+		b->o_psfF = L.take(elem * std::max<size_t>(1, psf_all.size()));
+		b->o_gd = L.take(elem * img_off);
+		b->o_gc = L.take(elem * (b->any_conv ? img_off : 1));
+		b->o_data = L.take(elem * out_off);
+		b->o_ivar = L.take(elem * out_off);
+		b->o_gacc = L.take(sizeof(double) * GA_SLOTS * std::max<size_t>(1, b->setup.size()));
+		b->o_gout = L.take(sizeof(double) * PK_GRAD_SLOTS * std::max<size_t>(1, b->setup.size()));
+		b->o_chi2 = L.take(sizeof(double) * n_models);
+		b->o_gcnt = L.take(sizeof(PkCounters));
+	}
 	if (hipMalloc(&b->dmem, L.off) != hipSuccess) {
 		(void)hipGetLastError();
 		return fail(set_err(PK_ENOMEM, "device allocation of %zu bytes failed", L.off));
@@ -1383,9 +1493,106 @@ int pk_batch_create(pk_ctx *ctx, const pk_model *models, uint32_t n_models, uint
 			                        reinterpret_cast<double *>(stage.data() + b->o_psfT) + jb.psf_off);
 	}
 	PK_HIP(hipMemcpy(b->dmem, stage.data(), in_bytes, hipMemcpyHostToDevice));
+	if (flags & PK_BATCH_GRAD) {
+		// PSF layout of the adjoint convolution: kt[k * kh + l] = krn(k, l)
+		// (the flipped PSF in the layout conv_kernel reads)
+		std::vector<unsigned char> f(elem * std::max<size_t>(1, psf_all.size()), 0);
+		for (const auto &jb : b->jobs) {
+			if (!jb.has_conv) continue;
+			const double *k = psf_all.data() + jb.psf_off;
+			for (uint32_t a = 0; a < jb.kw; a++)
+				for (uint32_t l = 0; l < jb.kh; l++) {
+					const double v = k[a + (size_t)l * jb.kw];
+					const size_t o = jb.psf_off + (size_t)a * jb.kh + l;
+					if (fp32) reinterpret_cast<float *>(f.data())[o] = (float)v;
+					else reinterpret_cast<double *>(f.data())[o] = v;
+				}
+		}
+		PK_HIP(hipMemcpy(b->d<void>(b->o_psfF), f.data(), f.size(), hipMemcpyHostToDevice));
+		for (auto &e : b->ev_grad)
+			if (hipEventCreate(&e) != hipSuccess) return fail(set_err(PK_EDEVICE, "hipEventCreate failed"));
+	}
 	for (auto &e : b->ev)
 		if (hipEventCreate(&e) != hipSuccess) return fail(set_err(PK_EDEVICE, "hipEventCreate failed"));
 	*out = b;
+	return PK_OK;
+}
+
+int pk_batch_set_data(pk_batch *b, const double *data, const double *sigma, size_t n)
+{
+	if (!b || !data) return set_err(PK_EINVAL, "NULL argument");
+	if (!(b->flags & PK_BATCH_GRAD)) return set_err(PK_EINVAL, "the batch was not created with PK_BATCH_GRAD");
+	if (n != b->total_out) return set_err(PK_EINVAL, "expected %llu data values, got %zu",
+	                                      (unsigned long long)b->total_out, n);
+	std::vector<unsigned char> d(b->elem * n), iv(b->elem * n);
+	for (size_t t = 0; t < n; t++) {
+		const double sg = sigma ? sigma[t] : 1.0;
+		if (!(sg > 0) || !std::isfinite(sg)) return set_err(PK_EINVAL, "sigma[%zu] = %g, must be > 0", t, sg);
+		const double ivv = 1 / (sg * sg);
+		if (b->elem == 4) {
+			reinterpret_cast<float *>(d.data())[t] = (float)data[t];
+			reinterpret_cast<float *>(iv.data())[t] = (float)ivv;
+		} else {
+			reinterpret_cast<double *>(d.data())[t] = data[t];
+			reinterpret_cast<double *>(iv.data())[t] = ivv;
+		}
+	}
+	std::lock_guard<std::mutex> lock(b->ctx->mu);
+	PK_HIP(hipSetDevice(b->ctx->device));
+	PK_HIP(hipMemcpyAsync(b->d<void>(b->o_data), d.data(), d.size(), hipMemcpyHostToDevice, b->ctx->stream));
+	PK_HIP(hipMemcpyAsync(b->d<void>(b->o_ivar), iv.data(), iv.size(), hipMemcpyHostToDevice, b->ctx->stream));
+	PK_HIP(hipStreamSynchronize(b->ctx->stream));
+	b->have_data = true;
+	return PK_OK;
+}
+
+int pk_batch_evaluate_grad(pk_batch *b)
+{
+	if (!b) return set_err(PK_EINVAL, "batch is NULL");
+	if (!(b->flags & PK_BATCH_GRAD)) return set_err(PK_EINVAL, "the batch was not created with PK_BATCH_GRAD");
+	if (!b->have_data) return set_err(PK_EINVAL, "no data: call pk_batch_set_data first");
+	std::lock_guard<std::mutex> lock(b->ctx->mu);
+	PK_HIP(hipSetDevice(b->ctx->device));
+	b->ran = true;
+	b->ran_grad = true;
+	if (!(b->flags & PK_BATCH_GRAPH)) return grad_launch(b);
+	if (!b->exec_grad) {
+		PK_HIP(hipStreamBeginCapture(b->ctx->stream, hipStreamCaptureModeThreadLocal));
+		int st = grad_launch(b);
+		hipGraph_t g = nullptr;
+		hipError_t e = hipStreamEndCapture(b->ctx->stream, &g);
+		if (st) {
+			if (g) (void)hipGraphDestroy(g);
+			return st;
+		}
+		if (e != hipSuccess) return set_err(PK_EDEVICE, "hipStreamEndCapture failed: %s", hipGetErrorString(e));
+		b->graph_grad = g;
+		PK_HIP(hipGraphInstantiate(&b->exec_grad, g, nullptr, nullptr, 0));
+	}
+	PK_HIP(hipGraphLaunch(b->exec_grad, b->ctx->stream));
+	return PK_OK;
+}
+
+int pk_batch_get_chi2_grad(pk_batch *b, double *chi2, double *grad)
+{
+	if (!b) return set_err(PK_EINVAL, "batch is NULL");
+	if (!b->ran_grad) return set_err(PK_EINVAL, "no gradient evaluated yet");
+	std::lock_guard<std::mutex> lock(b->ctx->mu);
+	PK_HIP(hipSetDevice(b->ctx->device));
+	PK_HIP(hipStreamSynchronize(b->ctx->stream));
+	PkCounters cnt{};
+	PK_HIP(hipMemcpy(&cnt, b->d<void>(b->o_gcnt), sizeof(cnt), hipMemcpyDeviceToHost));
+	if (cnt.overflow) return set_err(PK_ERANGE, "sub-pixel refinement work queue capacity exceeded");
+	if (chi2) PK_HIP(hipMemcpy(chi2, b->d<void>(b->o_chi2), sizeof(double) * b->n_models, hipMemcpyDeviceToHost));
+	if (grad) {
+		std::vector<double> g((size_t)PK_GRAD_SLOTS * std::max<size_t>(1, b->n_comps));
+		PK_HIP(hipMemcpy(g.data(), b->d<void>(b->o_gout), sizeof(double) * g.size(), hipMemcpyDeviceToHost));
+		for (size_t i = 0; i < b->comp_index.size(); i++) {
+			double *o = grad + i * PK_GRAD_SLOTS;
+			const uint32_t slot = b->comp_index[i];
+			for (int k = 0; k < PK_GRAD_SLOTS; k++) o[k] = slot == ~0u ? 0.0 : g[(size_t)slot * PK_GRAD_SLOTS + k];
+		}
+	}
 	return PK_OK;
 }
 
@@ -1475,6 +1682,11 @@ int pk_batch_sync(pk_batch *b, pk_batch_info *info)
 			info->ms_conv = t[3];
 			info->ms_compose = t[4];
 			info->ms_total = t[5];
+			if (b->ran_grad) {
+				float tg = 0;
+				(void)hipEventElapsedTime(&tg, b->ev_grad[0], b->ev_grad[1]);
+				info->ms_grad = tg;
+			}
 		}
 	}
 	if (cnt.overflow) return set_err(PK_ERANGE, "sub-pixel refinement work queue capacity exceeded");

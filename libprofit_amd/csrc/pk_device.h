@@ -145,9 +145,10 @@ __device__ inline double wave_sum(double v)
 
 // Integrates one component over one pixel; returns the pixel value in
 // libprofit units (before flux_scale), identical on every lane.
-template <bool FP32>
+template <bool FP32, bool GRAD = false>
 __device__ double refine_pixel(const PkComp &cp, RefStack &st, double x0, double x1, double y0, double y1,
-                               int res, int max_rec, uint32_t &overflow, unsigned long long &nev)
+                               int res, int max_rec, uint32_t &overflow, unsigned long long &nev,
+                               double gw = 0, double *gacc = nullptr)
 {
 	const int lane = (int)lane_id();
 	if (res <= 0) return NAN;  // libprofit divides 0 by 0 (src/radial.cpp:134)
@@ -237,6 +238,12 @@ __device__ double refine_pixel(const PkComp &cp, RefStack &st, double x0, double
 					nev += 1;
 				}
 				if (!fail) total += val * w;
+				if (GRAD && !fail) {
+					// adjoint: the same accepted sub-cells, weight G * flux_scale * w * f
+					double g[GT_COUNT];
+					const double vg = sersic_grad64(cp, xp, yp, g);
+					grad_accumulate(gacc, gw * w * vg, g);
+				}
 			}
 			// compaction of failing sub-cells onto the LIFO
 			const uint64_t ballot = __ballot(fail);
@@ -262,11 +269,16 @@ __device__ double refine_pixel(const PkComp &cp, RefStack &st, double x0, double
 	return wave_sum(total);
 }
 
-template <typename TI, bool FP32>
+// GRAD: instead of adding the pixel values, accumulate the chi^2 adjoint of
+// the same sub-cells (G_nc / G_c: adjoint images of the non-convolved /
+// convolved drawing images) into gacc[component][GA_SLOTS].
+template <typename TI, bool FP32, bool GRAD = false>
 __global__ __launch_bounds__(64) void refine_kernel(const PkJob *__restrict__ jobs, const PkComp *__restrict__ comps,
                                                     const RefineEntry *__restrict__ entries,
                                                     PkCounters *__restrict__ counters, TI *__restrict__ img_nc,
-                                                    TI *__restrict__ img_c)
+                                                    TI *__restrict__ img_c, const TI *__restrict__ G_nc = nullptr,
+                                                    const TI *__restrict__ G_c = nullptr,
+                                                    double *__restrict__ gacc = nullptr)
 {
 	__shared__ RefStack st;
 	const uint32_t n_entries = counters->n_entries;
@@ -306,11 +318,23 @@ __global__ __launch_bounds__(64) void refine_kernel(const PkJob *__restrict__ jo
 					mr = cp.center_max_rec;
 				}
 			}
+			if (GRAD) {
+				const size_t p = jb.img_off + en.pix;
+				const double gw = (double)(cp.convolve ? G_c[p] : G_nc[p]) * cp.flux_scale;
+				double ga[GA_SLOTS] = {0, 0, 0, 0, 0, 0, 0, 0};
+				(void)refine_pixel<FP32, true>(cp, st, x - hx, x + hx, y - hy, y + hy, res, mr, overflow, nev, gw, ga);
+#pragma unroll
+				for (int k = 0; k < GA_SLOTS; k++) {
+					const double v = wave_sum(ga[k]);
+					if (lane_id() == 0 && v != 0) atomicAdd(&gacc[(size_t)(jb.comp_off + c) * GA_SLOTS + k], v);
+				}
+				continue;
+			}
 			const double v = refine_pixel<FP32>(cp, st, x - hx, x + hx, y - hy, y + hy, res, mr, overflow, nev);
 			if (cp.convolve) add_c += cp.flux_scale * v;
 			else add_nc += cp.flux_scale * v;
 		}
-		if (lane_id() == 0) {
+		if (!GRAD && lane_id() == 0) {
 			if (jb.has_nc) img_nc[jb.img_off + en.pix] += (TI)add_nc;
 			if (jb.has_conv) img_c[jb.img_off + en.pix] += (TI)add_c;
 		}
@@ -540,6 +564,167 @@ __global__ __launch_bounds__(256) void compose_kernel(const PkJob *__restrict__ 
 		if (mi < jb.omask_w && mj < jb.omask_h && !omasks[jb.omask_off + mi + (size_t)mj * jb.omask_w]) v = 0;
 	}
 	out_all[jb.out_off + i + (size_t)j * jb.ow] = (TO)v;
+}
+
+// ---------------------------------------------------------------------------
+// chi^2 adjoint
+// ---------------------------------------------------------------------------
+
+// Sums v[0..NV) over a 256-lane workgroup; thread 0 gets the totals.
+template <int NV>
+__device__ inline void block_sum_256(double v[NV])
+{
+	__shared__ double red[4][NV];
+#pragma unroll
+	for (int k = 0; k < NV; k++) v[k] = wave_sum(v[k]);
+	const int w = threadIdx.x >> 6;
+	if ((threadIdx.x & 63) == 0)
+#pragma unroll
+		for (int k = 0; k < NV; k++) red[w][k] = v[k];
+	__syncthreads();
+	if (threadIdx.x == 0)
+#pragma unroll
+		for (int k = 0; k < NV; k++) v[k] = red[0][k] + red[1][k] + red[2][k] + red[3][k];
+	__syncthreads();
+}
+
+// Residuals of the returned images and their adjoint on the drawing frame:
+//   chi^2 = sum_p (data_p - model_p)^2 ivar_p,   G = d chi^2 / d drawing pixel.
+// The returned image is the crop of the drawing frame, SUM-downsampled by fs
+// (src/model.cpp:289-311), so every drawing pixel of the crop gets
+// -2 (data - model) ivar of its returned pixel and pixels outside get 0.
+template <typename TI>
+__global__ __launch_bounds__(256) void resid_kernel(const PkJob *__restrict__ jobs, uint32_t njobs,
+                                                    const TI *__restrict__ out_all, const TI *__restrict__ data,
+                                                    const TI *__restrict__ ivar, TI *__restrict__ G,
+                                                    double *__restrict__ chi2)
+{
+	const uint32_t jn = find_job(jobs, njobs, blockIdx.x, 0);
+	const PkJob &jb = jobs[jn];
+	const uint32_t t = blockIdx.x - jb.tile_off;
+	const uint32_t i = (t % jb.tiles_x) * 64 + (threadIdx.x & 63);
+	const uint32_t j = (t / jb.tiles_x) * 4 + (threadIdx.x >> 6);
+	double c2[1] = {0};
+	if (i < jb.dw && j < jb.dh) {
+		double g = 0;
+		const int fx = (int)i - jb.cox, fy = (int)j - jb.coy;
+		if (fx >= 0 && fy >= 0 && fx < (int)jb.crw && fy < (int)jb.crh) {
+			const uint32_t oi = (uint32_t)fx / jb.fs, oj = (uint32_t)fy / jb.fs;
+			const size_t p = jb.out_off + oi + (size_t)oj * jb.ow;
+			const double r = (double)data[p] - (double)out_all[p];
+			const double iv = (double)ivar[p];
+			g = -2 * r * iv;
+			if ((uint32_t)fx % jb.fs == 0 && (uint32_t)fy % jb.fs == 0) c2[0] = r * r * iv;
+		}
+		G[jb.img_off + i + (size_t)j * jb.dw] = (TI)g;
+	}
+	block_sum_256<1>(c2);
+	if (threadIdx.x == 0 && c2[0] != 0) atomicAdd(&chi2[jn], c2[0]);
+}
+
+// Adjoint of the coarse pass: every pixel a component evaluates directly
+// (src/radial.cpp:316-324) contributes G * flux_scale * f * dL/dθ; the
+// pixels refined by the forward pass are left to the refinement adjoint.
+template <typename TI, bool FP32>
+__global__ __launch_bounds__(256) void coarse_grad_kernel(const PkJob *__restrict__ jobs, uint32_t njobs,
+                                                          const PkComp *__restrict__ comps,
+                                                          const uint8_t *__restrict__ masks,
+                                                          const TI *__restrict__ G_nc, const TI *__restrict__ G_c,
+                                                          double *__restrict__ gacc)
+{
+	const uint32_t jn = find_job(jobs, njobs, blockIdx.x, 0);
+	const PkJob &jb = jobs[jn];
+	const uint32_t t = blockIdx.x - jb.tile_off;
+	const uint32_t i = (t % jb.tiles_x) * 64 + (threadIdx.x & 63);
+	const uint32_t j = (t / jb.tiles_x) * 4 + (threadIdx.x >> 6);
+	bool live = i < jb.dw && j < jb.dh;
+	double gn = 0, gc = 0;
+	if (live) {
+		const size_t pix = jb.img_off + j * jb.dw + i;
+		if (jb.mask_off != NONE && !masks[jb.mask_off + j * jb.dw + i]) live = false;
+		gn = (double)G_nc[pix];
+		if (jb.has_conv) gc = (double)G_c[pix];
+	}
+	const double x = jb.sx / 2. + i * jb.sx, y = jb.sy / 2. + j * jb.sy;
+	for (uint32_t c = 0; c < jb.ncomp; c++) {
+		const PkComp &cp = comps[jb.comp_off + c];
+		double acc[GA_SLOTS] = {0, 0, 0, 0, 0, 0, 0, 0};
+		if (live) {
+			const double G = cp.convolve ? gc : gn;
+			if (cp.kind == PK_PROFILE_SKY) {
+				acc[GA_W] = G;
+			} else if (cp.kind == PK_PROFILE_SERSIC && G != 0) {
+				double xp, yp;
+				to_prof(cp, x, y, xp, yp);
+				const double r = sqrt(xp * xp + yp * yp);
+				const bool zero = cp.rscale_max > 0 && r / cp.rscale > cp.rscale_max;
+				if (!zero && (cp.rough || r / cp.rscale > cp.rscale_switch)) {
+					if (FP32) {
+						float g[GT_COUNT];
+						const float v = sersic_grad32(cp, (float)xp, (float)yp, g);
+						grad_accumulate(acc, G * cp.flux_scale * cp.unfold * (double)v, g);
+					} else {
+						double g[GT_COUNT];
+						const double v = sersic_grad64(cp, xp, yp, g);
+						grad_accumulate(acc, G * cp.flux_scale * v, g);
+					}
+				}
+			}
+		}
+		block_sum_256<GA_SLOTS>(acc);
+		if (threadIdx.x == 0)
+#pragma unroll
+			for (int k = 0; k < GA_SLOTS; k++)
+				if (acc[k] != 0) atomicAdd(&gacc[(size_t)(jb.comp_off + c) * GA_SLOTS + k], acc[k]);
+	}
+}
+
+// Raw accumulators -> gradient slots of include/profit_hip.h (PK_GRAD_*).
+__global__ void grad_finalize_kernel(const PkComp *__restrict__ comps, const double *__restrict__ gacc,
+                                     double *__restrict__ gout, uint32_t n)
+{
+	const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+	if (t >= n) return;
+	const PkComp &c = comps[t];
+	const double *a = gacc + (size_t)t * GA_SLOTS;
+	double *o = gout + (size_t)t * PK_GRAD_SLOTS;
+	for (int k = 0; k < PK_GRAD_SLOTS; k++) o[k] = 0;
+	if (c.kind == PK_PROFILE_SKY) {
+		o[PK_GRAD_SKY_BG] = a[GA_W] * c.q[1];
+	} else if (c.kind == PK_PROFILE_SERSIC) {
+		const double w = a[GA_W];
+		o[PK_GRAD_XCEN] = a[GA_XC];
+		o[PK_GRAD_YCEN] = a[GA_YC];
+		o[PK_GRAD_MAG] = -0.4 * 2.302585092994045684 * w;  // d log(10^(-0.4 mag)) / d mag
+		o[PK_GRAD_RE] = a[GA_RE] + c.glnf_re * w;
+		o[PK_GRAD_NSER] = a[GA_N] + c.glnf_n * w;
+		o[PK_GRAD_ANG] = a[GA_PHI] * (PK_PI / 180.);  // the angle is in degrees (src/radial.cpp:191-196)
+		o[PK_GRAD_AXRAT] = a[GA_Q] + c.glnf_q * w;
+		o[PK_GRAD_BOX] = a[GA_B] + c.glnf_b * w;
+	}
+}
+
+// Clears the adjoint accumulators and the refinement-adjoint counters (a
+// kernel rather than memset nodes, so that the sequence can be captured in a
+// graph); the refinement entries of the forward pass are reused.
+__global__ void reset_grad_kernel(PkCounters *__restrict__ g, const PkCounters *__restrict__ fwd,
+                                  double *__restrict__ gacc, uint32_t nacc, double *__restrict__ chi2, uint32_t nj)
+{
+	const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+	for (uint32_t k = t; k < nacc; k += gridDim.x * blockDim.x) gacc[k] = 0;
+	for (uint32_t k = t; k < nj; k += gridDim.x * blockDim.x) chi2[k] = 0;
+	if (t == 0) {
+		g->n_entries = fwd->n_entries;
+		g->next_entry = 0;
+		g->overflow = 0;
+		g->n_bad = 0;
+		g->n_subcells = 0;
+		for (int p = 0; p < R32_PASSES; p++) {
+			g->claim[p] = 0;
+			g->n_cells[p] = 0;
+			g->n_done[p] = 0;
+		}
+	}
 }
 
 }  // namespace pk

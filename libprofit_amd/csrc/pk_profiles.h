@@ -41,6 +41,9 @@ struct PkComp {
 	double bn, lumtot, ie;
 	double inv_axrat;            // 1 / axrat (FP32-mode refinement)
 	double total_flux;           // 10^(-0.4 (mag - magzero)), bound of any pixel value
+	// gradient constants (set-up with SetupIn::grad): d bn / d nser and the
+	// derivatives of log(flux_scale) with respect to re, nser, axrat, box
+	double dbn, glnf_re, glnf_n, glnf_q, glnf_b;
 };
 
 // What one FP32 sub-cell evaluation of the refinement reads: one 128-byte
@@ -382,6 +385,131 @@ __device__ inline void evaluate_pair32(const CR &c, int kind, double xa, double 
 }
 
 // ---------------------------------------------------------------------------
+// Gradients of the Sersic profile value for the chi^2 adjoint.
+//
+// With x', y' the profile coordinates (src/radial.cpp:47-53),
+//   x' = dx cos + dy sin,  y' = (-dx sin + dy cos) / q,  dx = x - xcen, dy = y - ycen,
+// R = (|x'|^B + |y'|^B)^(1/B) / re (B = box + 2) and rf = R^(1/n), the value is
+// f = exp(-bn (rf - 1)) (src/sersic.cpp:79-205), and with L = log f,
+//   dL/dθ = -bn' (rf - 1) [θ = n] + K d log R / dθ,  K = -bn rf / n.
+// The functions return the value and dL/dθ for θ in the order of the
+// GradTerm enum (phi = angle in radians; the caller converts to degrees).
+// The derivatives of log(flux_scale) are the per-component glnf_* constants.
+// ---------------------------------------------------------------------------
+enum GradTerm { GT_XC = 0, GT_YC, GT_RE, GT_N, GT_PHI, GT_Q, GT_B, GT_COUNT };
+
+// Raw adjoint accumulators of one component: GA_W sums the weights
+// a = G w flux_scale f, GA_<θ> sums a dL/dθ.
+enum GradAcc { GA_XC = 0, GA_YC, GA_W, GA_RE, GA_N, GA_PHI, GA_Q, GA_B, GA_SLOTS };
+
+// FP32: f[0] = -log2 re, f[1] = 1/n, f[2] = bn, f[3] = dbn/dn, f[4] = 1/re,
+// f[7] = B; value in units of 1/unfold = e^-bn.
+template <typename CR>
+__device__ inline float sersic_grad32(const CR &c, float x, float y, float g[GT_COUNT])
+{
+	const float ax = fabsf(x), ay = fabsf(y);
+	const float m = fmaxf(ax, ay);
+	const float qq = m > 0.f ? fminf(ax, ay) * frcp(m) : 0.f;
+	const bool xmax = ax >= ay;
+	float l2R, dlx, dly, dlb;  // log2 R; d log R / d x', d y', d B
+	if (!c.boxy) {
+		const float s2 = x * x + y * y;
+		const float inv = s2 > 0.f ? frcp(s2) : 0.f;
+		l2R = 0.5f * flog2(s2) + c.f[0];
+		dlx = x * inv;
+		dly = y * inv;
+		// B = 2: -log(1 + q^2) / 4 + q^2 log q / (2 (1 + q^2))
+		const float q2 = qq * qq;
+		const float lq = qq > 0.f ? flog2(qq) * F_LN2 : 0.f;
+		dlb = -0.25f * flog1p(q2) + 0.5f * q2 * lq * frcp(1.f + q2);
+	} else {
+		// R re = m (1 + t)^(1/B), t = q^B, m = max(|x'|, |y'|), q = min/max
+		const float B = c.f[7];
+		const float lq2 = qq > 0.f ? flog2(qq) : -INFINITY;
+		const float t = qq > 0.f ? fexp2(B * lq2) : 0.f;
+		const float l1t = flog1p(t);
+		const float iB = frcp(B);
+		l2R = (m > 0.f ? flog2(m) : -INFINITY) + l1t * (iB / F_LN2) + c.f[0];
+		const float den = m > 0.f ? frcp(m * (1.f + t)) : 0.f;
+		const float wmin = qq > 0.f ? t * frcp(qq) : 0.f;  // q^(B-1)
+		dlx = copysignf((xmax ? 1.f : wmin) * den, x);
+		dly = copysignf((xmax ? wmin : 1.f) * den, y);
+		dlb = -l1t * iB * iB + (qq > 0.f ? t * (lq2 * F_LN2) * frcp(B * (1.f + t)) : 0.f);
+	}
+	const float rf = fexp2(c.f[1] * l2R);
+	const float v = fexp(-c.f[2] * rf);
+	const float K = -c.f[2] * rf * c.f[1];
+	const float cs = (float)c.cos_ang, sn = (float)c.sin_ang, iq = (float)c.inv_axrat, q = 1.f / iq;
+	const float Lx = K * dlx, Ly = K * dly;
+	g[GT_XC] = -cs * Lx + sn * iq * Ly;
+	g[GT_YC] = -sn * Lx - cs * iq * Ly;
+	g[GT_RE] = -K * c.f[4];
+	g[GT_N] = -c.f[3] * (rf - 1.f) - K * (l2R * F_LN2) * c.f[1];
+	g[GT_PHI] = q * y * Lx - x * iq * Ly;
+	g[GT_Q] = -y * iq * Ly;
+	g[GT_B] = K * dlb;
+	return v;
+}
+
+// FP64 counterpart (libprofit's units: value exp(-bn (rf - 1))).
+PK_HD double sersic_grad64(const PkComp &c, double x, double y, double g[GT_COUNT])
+{
+	const double re = c.q[0], n = c.q[1], bn = c.q[2], B = c.box + 2;
+	const double ax = fabs(x), ay = fabs(y);
+	const double m = fmax(ax, ay);
+	const double qq = m > 0 ? fmin(ax, ay) / m : 0;
+	const bool xmax = ax >= ay;
+	double lR, dlx, dly, dlb;
+	if (!c.boxy) {
+		const double s2 = x * x + y * y;
+		const double inv = s2 > 0 ? 1 / s2 : 0;
+		lR = 0.5 * log(s2) - log(re);
+		dlx = x * inv;
+		dly = y * inv;
+		const double q2 = qq * qq;
+		const double lq = qq > 0 ? log(qq) : 0;
+		dlb = -0.25 * log1p(q2) + 0.5 * q2 * lq / (1 + q2);
+	} else {
+		const double lq = qq > 0 ? log(qq) : -INFINITY;
+		const double t = qq > 0 ? exp(B * lq) : 0;
+		const double l1t = log1p(t);
+		lR = (m > 0 ? log(m) : -INFINITY) + l1t / B - log(re);
+		const double den = m > 0 ? 1 / (m * (1 + t)) : 0;
+		const double wmin = qq > 0 ? t / qq : 0;
+		dlx = copysign((xmax ? 1. : wmin) * den, x);
+		dly = copysign((xmax ? wmin : 1.) * den, y);
+		dlb = -l1t / (B * B) + (qq > 0 ? t * lq / (B * (1 + t)) : 0.);
+	}
+	const double rf = exp(lR / n);
+	const double v = exp(-bn * (rf - 1));
+	const double K = -bn * rf / n;
+	const double cs = c.cos_ang, sn = c.sin_ang, iq = c.inv_axrat, q = c.axrat;
+	const double Lx = K * dlx, Ly = K * dly;
+	g[GT_XC] = -cs * Lx + sn * iq * Ly;
+	g[GT_YC] = -sn * Lx - cs * iq * Ly;
+	g[GT_RE] = -K / re;
+	g[GT_N] = -c.dbn * (rf - 1) - K * lR / n;
+	g[GT_PHI] = q * y * Lx - x * iq * Ly;
+	g[GT_Q] = -y * iq * Ly;
+	g[GT_B] = K * dlb;
+	return v;
+}
+
+// a = weight of one evaluation point, g = dL/dθ: adds into the raw accumulators.
+template <typename TG>
+PK_HD void grad_accumulate(double acc[GA_SLOTS], double a, const TG g[GT_COUNT])
+{
+	acc[GA_W] += a;
+	acc[GA_XC] += a * (double)g[GT_XC];
+	acc[GA_YC] += a * (double)g[GT_YC];
+	acc[GA_RE] += a * (double)g[GT_RE];
+	acc[GA_N] += a * (double)g[GT_N];
+	acc[GA_PHI] += a * (double)g[GT_PHI];
+	acc[GA_Q] += a * (double)g[GT_Q];
+	acc[GA_B] += a * (double)g[GT_B];
+}
+
+// ---------------------------------------------------------------------------
 // Set-up: RadialProfile::initial_calculations + profile overrides, FP64.
 // `sx`, `sy` are the (finesampled) pixel scales passed to Profile::evaluate,
 // (pad_x, pad_y) the PSF padding offset (src/model.cpp:336-342).
@@ -393,6 +521,8 @@ struct SetupIn {
 	int32_t fp32_mode;
 	uint32_t fs;               // finesampling factor (sky bg is divided by fs^2, src/sky.cpp:42-45)
 	double fx_scale;           // fixed-point scale of the FP32-mode refinement of the model
+	int32_t grad;              // also compute the gradient constants
+	int32_t pad_;
 };
 
 PK_HD double sersic_fluxfrac(double re, double nser, double bn, double fraction)
@@ -429,6 +559,7 @@ PK_HD void setup_component(const SetupIn &in, PkComp &o)
 	o.unfold = 1;
 	if (c.kind == PK_PROFILE_SKY) {
 		o.q[0] = c.p[PK_SKY_BG] / (double)(in.fs * in.fs);
+		o.q[1] = 1 / (double)(in.fs * in.fs);  // d value / d bg
 		return;
 	}
 	if (c.kind == PK_PROFILE_NULL || c.kind == PK_PROFILE_PSF) return;
@@ -589,6 +720,18 @@ PK_HD void setup_component(const SetupIn &in, PkComp &o)
 		o.center_max_rec = 10;
 	}
 
+	if (in.grad && c.kind == PK_PROFILE_SERSIC) {
+		// flux_scale = sx sy 10^(-0.4 (mag - magzero)) r_box(B) / (lum axrat), B = box + 2,
+		// lum = re^2 2 pi n Gamma(2n) e^bn / bn^(2n)  (src/sersic.cpp:283-289, src/radial.cpp:150-153)
+		const double n = o.q[1], re = o.q[0], B = c.box + 2;
+		o.dbn = sersic_dbn_dn(n);
+		o.glnf_re = -2 / re;
+		o.glnf_n = -(1 / n + 2 * digamma(2 * n) + o.dbn - 2 * log(bn) - 2 * n * o.dbn / bn);
+		o.glnf_q = -1 / c.axrat;
+		// d log r_box / dB, r_box = pi B / (2 Beta(1/B, 1/B))
+		o.glnf_b = 1 / B + 2 * (digamma(1 / B) - digamma(2 / B)) / (B * B);
+	}
+
 	// FP32 constants
 	o.fp32 = 0;
 	if (in.fp32_mode) {
@@ -598,6 +741,8 @@ PK_HD void setup_component(const SetupIn &in, PkComp &o)
 			o.f[0] = (float)(-log2(o.q[0]));
 			o.f[1] = (float)(1 / o.q[1]);
 			o.f[2] = (float)bn;
+			o.f[3] = (float)o.dbn;
+			o.f[4] = (float)(1 / o.q[0]);
 			o.f[7] = (float)(c.box + 2);
 			o.unfold = exp(bn);
 			break;

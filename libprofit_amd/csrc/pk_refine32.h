@@ -226,15 +226,41 @@ __device__ inline void r32_take_cells(const RCell *__restrict__ qin, uint32_t n,
 	next += k;
 }
 
+// Adds the adjoint terms of this round's accepted sub-cells (lanes with `ok`)
+// to gacc[component]: one wavefront sum per distinct component in the round
+// (usually one).
+__device__ inline void r32_flush_grad(bool ok, uint32_t gcomp, double ga[GA_SLOTS], double *__restrict__ gacc,
+                                      uint32_t lane)
+{
+	uint64_t todo = __ballot(ok);
+	while (todo) {
+		const int l0 = __ffsll((long long)todo) - 1;
+		const uint32_t g0 = wave_read(gcomp, l0);
+		const bool mine = ok && gcomp == g0;
+		todo &= ~__ballot(mine);
+#pragma unroll
+		for (int k = 0; k < GA_SLOTS; k++) {
+			const double v = wave_sum(mine ? ga[k] : 0.0);
+			if (lane == 0 && v != 0) atomicAdd(&gacc[(size_t)g0 * GA_SLOTS + k], v);
+		}
+	}
+}
+
 // SOURCE 0: pixels from the coarse pass (pass 0); 1: cells of the previous pass.
 // LAST: the final pass, which hands nothing over.
-template <int SOURCE, bool LAST>
+// GRAD: the chi^2 adjoint of the refinement -- the same cells and decisions as
+// the forward pass, but every accepted sub-cell adds G w flux_scale f dL/dθ
+// to gacc (G_nc / G_c: adjoint images) instead of its value to the pixel.
+template <int SOURCE, bool LAST, bool GRAD = false>
 __global__ __launch_bounds__(64) void refine32_kernel(const PkJob *__restrict__ jobs, const PkComp *__restrict__ comps,
                                                       const CompLite *__restrict__ lite,
                                                       const RefineEntry *__restrict__ entries,
                                                       PkCounters *__restrict__ cnt, const RCell *__restrict__ qin,
                                                       RCell *__restrict__ qout, uint32_t qcap, int pass,
-                                                      unsigned long long *__restrict__ acc)
+                                                      unsigned long long *__restrict__ acc,
+                                                      const float *__restrict__ G_nc = nullptr,
+                                                      const float *__restrict__ G_c = nullptr,
+                                                      double *__restrict__ gacc = nullptr)
 {
 	constexpr int CAP = LAST ? R32_CAP_LAST : R32_CAP_MAIN;
 	__shared__ R32Stack<CAP> st;
@@ -389,12 +415,12 @@ __global__ __launch_bounds__(64) void refine32_kernel(const PkJob *__restrict__ 
 				decided = v32 >= 0.f && v32 <= 3e38f;
 			}
 			unsigned long long fxv = 0;
-			if (active && decided && !fail && !fixed_point_f32(v32 * wq, &fxv)) bad++;
+			if (!GRAD && active && decided && !fail && !fixed_point_f32(v32 * wq, &fxv)) bad++;
 			if (__ballot(active && !decided)) {
 				// FP64 value and decision within the guard band (or without an FP32 form)
 				if (active && !decided) {
 					const double val = refine_fp64_decision(comps[cell.gcomp], xp, yp, ddy, recurse, &fail);
-					if (!fail) {
+					if (!GRAD && !fail) {
 						const double q = val * ((double)(fr * fr) * cl.wflux);
 						if (q >= 0 && q < 9.2e18) fxv = (unsigned long long)__double2ll_rn(q);
 						else bad++;
@@ -403,9 +429,22 @@ __global__ __launch_bounds__(64) void refine32_kernel(const PkJob *__restrict__ 
 			}
 			fail = fail && active;
 			if (active) nev += recurse ? 2 : 1;
-			// exact per-cell sums; the last lane of a cell adds its sum to the
-			// pixel accumulator
-			if (ntask == 1) {
+			if (GRAD) {
+				const bool ok = active && !fail;
+				double ga[GA_SLOTS] = {0, 0, 0, 0, 0, 0, 0, 0};
+				if (ok) {
+					const RefineEntry en = entries[cell.entry];
+					const PkJob &jb = jobs[en.job];
+					const size_t p = jb.img_off + en.pix;
+					const double G = (double)(cm_conv(meta) ? G_c[p] : G_nc[p]);
+					float g[GT_COUNT];
+					const float vg = sersic_grad32(cl, (float)xp, (float)yp, g);
+					grad_accumulate(ga, G * ((double)(fr * fr) * cl.wq / jb.fx_scale) * (double)vg, g);
+				}
+				r32_flush_grad(ok, cell.gcomp, ga, gacc, lane);
+			} else if (ntask == 1) {
+				// exact per-cell sums; the last lane of a cell adds its sum to the
+				// pixel accumulator
 				const unsigned long long v = wave_sum_u64(fxv);
 				if (lane == 0 && v) atomicAdd(&acc[target], v);
 			} else {

@@ -286,20 +286,55 @@ def evaluate_spec(ctx, spec):
 
 class Batch:
     """Device-resident batch of models (pk_batch_* of include/profit_hip.h):
-    all images produced by one launch sequence, optionally a hipGraph."""
+    all images produced by one launch sequence, optionally a hipGraph; with
+    grad=True also chi^2 against set data and its gradient."""
 
-    def __init__(self, ctx, specs, graph=True):
+    def __init__(self, ctx, specs, graph=True, grad=False):
         self.ctx = ctx
         self.L = load()
         self.specs = list(specs)  # keeps the host buffers alive
         arr = (abi.Model * len(self.specs))(*[s.m for s in self.specs])
         self._arr = arr
         h = C.c_void_p()
-        check(self.L.pk_batch_create(ctx.handle, arr, len(self.specs), abi.PK_BATCH_GRAPH if graph else 0,
-                                     C.byref(h)))
+        flags = (abi.PK_BATCH_GRAPH if graph else 0) | (abi.PK_BATCH_GRAD if grad else 0)
+        check(self.L.pk_batch_create(ctx.handle, arr, len(self.specs), flags, C.byref(h)))
         self.handle = h
         self.n = len(self.specs)
+        self.n_components = sum(int(s.m.n_components) for s in self.specs)
         ctx._batches.add(self)
+
+    def offsets(self):
+        """Element offset of every returned image in the packed output."""
+        out = []
+        for i in range(self.n):
+            o = C.c_uint64()
+            check(self.L.pk_batch_image_offset(self.handle, i, C.byref(o)))
+            out.append(o.value)
+        return out
+
+    def set_data(self, images, sigmas=None):
+        """Observed images (one array per model, shaped like the returned
+        image) and their sigma (arrays, scalars or None = 1)."""
+        data = np.concatenate([np.asarray(d, dtype=np.float64).ravel() for d in images])
+        sig = None
+        if sigmas is not None:
+            parts = []
+            for d, sg in zip(images, sigmas):
+                parts.append(np.broadcast_to(np.asarray(sg, dtype=np.float64), np.shape(d)).ravel())
+            sig = np.ascontiguousarray(np.concatenate(parts))
+        data = np.ascontiguousarray(data)
+        check(self.L.pk_batch_set_data(self.handle, data.ctypes.data, None if sig is None else sig.ctypes.data,
+                                       data.size))
+
+    def evaluate_grad(self):
+        check(self.L.pk_batch_evaluate_grad(self.handle))
+
+    def chi2_grad(self):
+        """(chi2[n_models], grad[n_components, GRAD_SLOTS]) of the last evaluate_grad."""
+        chi2 = np.zeros(self.n)
+        grad = np.zeros((max(1, self.n_components), abi.GRAD_SLOTS))
+        check(self.L.pk_batch_get_chi2_grad(self.handle, chi2.ctypes.data, grad.ctypes.data))
+        return chi2, grad[: self.n_components]
 
     def evaluate(self):
         check(self.L.pk_batch_evaluate(self.handle))

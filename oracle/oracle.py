@@ -94,6 +94,17 @@ def evaluate(spec):
     return out[: w.value * h.value].reshape(h.value, w.value).copy(), (ox.value, oy.value)
 
 
+def pattern(mode):
+    """Frozen refinement pattern: "record" (the next evaluations log their
+    refinement pattern), "replay" (the next evaluations reuse the log in the
+    same order) or "off"."""
+    lib().or_pattern(C.c_int({"off": 0, "record": 1, "replay": 2}[mode]))
+
+
+def pattern_ok():
+    return lib().or_pattern_status() == 0
+
+
 def stats():
     return lib().or_stat_refined(), lib().or_stat_subcells()
 

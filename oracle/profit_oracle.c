@@ -680,6 +680,49 @@ typedef struct {
 	uint64_t n_subcells;
 } or_stats;
 
+/* ---- frozen refinement pattern (finite differences of the chi^2 gradient) --
+ * Mode 1 records, for every radial component and pixel in evaluation order,
+ * the pixel's class (0 beyond rscale_max, 1 evaluated at its centre, 2
+ * refined), the resolution and recursion depth it used, and every accuracy
+ * decision of subsample_pixel; mode 2 replays them, so that an evaluation at
+ * other parameters keeps the recorded refinement pattern.  Both run on one
+ * thread (the log is sequential). */
+static int or_pat_mode = 0;
+static uint32_t *or_pat = NULL;
+static size_t or_pat_len = 0, or_pat_cap = 0, or_pat_pos = 0;
+static int or_pat_bad = 0;
+
+void or_pattern(int mode)
+{
+	or_pat_mode = mode;
+	if (mode == 1) or_pat_len = 0;
+	or_pat_pos = 0;
+	or_pat_bad = 0;
+}
+
+/* 0: the replay consumed exactly the recorded log */
+int or_pattern_status(void) { return or_pat_bad || (or_pat_mode == 2 && or_pat_pos != or_pat_len); }
+
+static inline uint32_t or_pat_io(uint32_t v)
+{
+	if (or_pat_mode == 1) {
+		if (or_pat_len == or_pat_cap) {
+			or_pat_cap = or_pat_cap ? 2 * or_pat_cap : 1 << 16;
+			or_pat = (uint32_t *)realloc(or_pat, or_pat_cap * sizeof(uint32_t));
+		}
+		or_pat[or_pat_len++] = v;
+		return v;
+	}
+	if (or_pat_mode == 2) {
+		if (or_pat_pos >= or_pat_len) {
+			or_pat_bad = 1;
+			return v;
+		}
+		return or_pat[or_pat_pos++];
+	}
+	return v;
+}
+
 /* RadialProfile::subsample_pixel (src/radial.cpp:55-135) */
 static double or_subsample(const or_prof *p, double x0, double x1, double y0, double y1, unsigned level,
                            unsigned max_rec, unsigned res, uint64_t *nev)
@@ -704,7 +747,7 @@ static double or_subsample(const or_prof *p, double x0, double x1, double y0, do
 				double dyp = (-xbin * p->sin_ang + ybin * p->cos_ang) / p->axrat;
 				double testval = or_evaluate_at(p, fabs(xp), fabs(yp) + fabs(dyp));
 				*nev += 2;
-				if (fabs(testval / subval - 1.0) > p->acc) {
+				if (or_pat_io(fabs(testval / subval - 1.0) > p->acc)) {
 					if (npts == cap) {
 						cap = cap ? 2 * cap : 16;
 						pts = (double *)realloc(pts, cap * 2 * sizeof(double));
@@ -749,7 +792,8 @@ static void or_radial_evaluate(const or_prof *p, double *image, unsigned W, unsi
 	double half_xbin = sx / 2., half_ybin = sy / 2.;
 	uint64_t nref = 0, nsub = 0;
 	long total = (long)W * H;
-#pragma omp parallel for schedule(dynamic, 10) num_threads(or_threads) reduction(+ : nref, nsub)
+	const int nthreads = or_pat_mode ? 1 : or_threads;
+#pragma omp parallel for schedule(dynamic, 10) num_threads(nthreads) reduction(+ : nref, nsub)
 	for (long idx = 0; idx < total; idx++) {
 		unsigned i = (unsigned)(idx % W), j = (unsigned)(idx / W);
 		if (mask && !mask[idx]) continue;
@@ -759,9 +803,12 @@ static void or_radial_evaluate(const or_prof *p, double *image, unsigned W, unsi
 		or_to_prof(p, x, y, &xp, &yp);
 		double r = sqrt(xp * xp + yp * yp);
 		double val;
-		if (p->rscale_max > 0 && r / p->rscale > p->rscale_max) {
+		const uint32_t cls = or_pat_io((p->rscale_max > 0 && r / p->rscale > p->rscale_max) ? 0u
+		                               : (p->rough || r / p->rscale > p->rscale_switch) ? 1u
+		                                                                                : 2u);
+		if (cls == 0) {
 			val = 0.;
-		} else if (p->rough || r / p->rscale > p->rscale_switch) {
+		} else if (cls == 1) {
 			val = or_evaluate_at(p, xp, yp);
 		} else {
 			unsigned res = p->resolution, mr = p->max_recursions;
@@ -773,6 +820,8 @@ static void or_radial_evaluate(const or_prof *p, double *image, unsigned W, unsi
 					mr = 10;
 				}
 			}
+			res = or_pat_io(res);
+			mr = or_pat_io(mr);
 			uint64_t ne = 0;
 			val = or_subsample(p, x - half_xbin, x + half_xbin, y - half_ybin, y + half_ybin, 0, mr, res, &ne);
 			nref += 1;
