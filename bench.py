@@ -73,6 +73,144 @@ def conv_traffic():
     return None
 
 
+def fd_plain_gradient(ctx64, spec, data, sigma, evaluate_spec):
+    """chi^2 gradient of one fit by central differences of the plain FP64 path
+    (one pk_evaluate per point: 2 per parameter + 1), the way a caller of
+    libprofit's Model::evaluate gets it."""
+    from libprofit_amd import abi
+    comps = [abi.Component.from_buffer_copy(spec.m.components[c]) for c in range(spec.m.n_components)]
+    img, _, _ = evaluate_spec(ctx64, spec)
+    chi0 = float(np.sum(((data - img) / sigma) ** 2))
+    g = []
+    for ci, c in enumerate(comps):
+        names = abi.GRAD_PARAMS["sersic" if c.kind == abi.PROFILE_KINDS["sersic"] else "sky"]
+        for n in names:
+            vals = []
+            for sgn in (1, -1):
+                cc = [abi.Component.from_buffer_copy(x) for x in comps]
+                if n in abi.PROFILE_PARAMS["sersic"] and c.kind == abi.PROFILE_KINDS["sersic"]:
+                    idx = abi.PROFILE_PARAMS["sersic"].index(n)
+                    h = 1e-4 * abs(cc[ci].p[idx]) or 1e-4
+                    cc[ci].p[idx] += sgn * h
+                elif n == "bg":
+                    h = 1e-4 * abs(cc[ci].p[0]) or 1e-4
+                    cc[ci].p[0] += sgn * h
+                else:
+                    v = getattr(cc[ci], n)
+                    h = 1e-4 * abs(v) or 1e-4
+                    setattr(cc[ci], n, v + sgn * h)
+                sp = abi.ModelSpec(spec.m.width, spec.m.height, cc, psf=spec._psf, magzero=spec.m.magzero)
+                im, _, _ = evaluate_spec(ctx64, sp)
+                vals.append(float(np.sum(((data - im) / sigma) ** 2)))
+            g.append((vals[0] - vals[1]) / (2 * h))
+    return chi0, np.array(g)
+
+
+def grad_bench(args, Batch, Context, evaluate_spec, ctx32, ctx64):
+    """chi^2-gradient throughput: 1024 bulge+disk+sky fits of 128x128 with a
+    31x31 PSF (one FP32 forward + adjoint of all fits per step, graph) and
+    the 256x256 bulge+disk fit with a 31x31 PSF, beside the plain FP64 path
+    (finite differences of pk_evaluate) on a sample of the fits."""
+    from libprofit_amd import workloads
+    out = {}
+    start, true = workloads.fit_batch(args.fits, seed=args.seed)
+    # data: the true models (FP64 batch) + 1 % noise
+    tb = Batch(ctx64, true, graph=False)
+    tb.evaluate()
+    tb.sync()
+    rng = np.random.Generator(np.random.PCG64(args.seed))
+    datas, sigmas = [], []
+    for i in range(len(true)):
+        d, sg = workloads.noisy(tb.image(i), rng)
+        datas.append(d)
+        sigmas.append(sg)
+    tb.close()
+    gb = Batch(ctx32, start, graph=True, grad=True)
+    gb.set_data(datas, sigmas)
+    for _ in range(args.warmup):
+        gb.evaluate_grad()
+    gb.chi2_grad()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        gb.evaluate_grad()
+    chi2_32, grad_32 = gb.chi2_grad()
+    dt = (time.perf_counter() - t0) / args.steps
+    out["fits_chi2_grads_per_s"] = round(len(start) / dt, 2)
+    out["fits_ms_per_step"] = round(dt * 1e3, 4)
+    log("fits: %d x (128^2 bulge+disk+sky, PSF 31) forward+gradient in %.3f ms -> %.1f chi2-grads/s"
+        % (len(start), dt * 1e3, len(start) / dt))
+    # stage split (eager launches, HIP events)
+    eb = Batch(ctx32, start, graph=False, grad=True)
+    eb.set_data(datas, sigmas)
+    eb.evaluate_grad()
+    eb.sync()
+    eb.evaluate_grad()
+    inf = eb.sync()
+    out["fits_stage_ms"] = {"forward": round(inf.ms_total, 4), "adjoint": round(inf.ms_grad, 4)}
+    eb.close()
+    # FP64 adjoint of the same fits: accuracy of the FP32 gradients
+    b64 = Batch(ctx64, start, graph=False, grad=True)
+    b64.set_data(datas, sigmas)
+    b64.evaluate_grad()
+    chi2_64, grad_64 = b64.chi2_grad()
+    b64.close()
+    nz = np.abs(grad_64) > 0
+    rel = np.abs(grad_32[nz] - grad_64[nz]) / np.abs(grad_64[nz])
+    out["fits_fp32_vs_fp64_grad_rel_err"] = {"median": float(np.median(rel)), "p99": float(np.percentile(rel, 99)),
+                                             "max": float(np.max(rel))}
+    # per fit, relative to the largest gradient component of the same parameter kind
+    nrm = []
+    for f in range(len(start)):
+        a, b = grad_32[3 * f:3 * f + 3], grad_64[3 * f:3 * f + 3]
+        scale = np.maximum(np.abs(b[:2]).max(axis=0), 1e-300)
+        nrm.append(float(np.max(np.abs(a[:2] - b[:2]) / scale)))
+    out["fits_fp32_vs_fp64_grad_max_err_vs_param_scale"] = float(np.max(nrm))
+    out["fits_fp32_vs_fp64_chi2_max_rel_err"] = float(np.max(np.abs(chi2_32 - chi2_64) / np.abs(chi2_64)))
+    # plain FP64 path: central differences of pk_evaluate on a sample of fits
+    ns = min(args.grad_sample, len(start))
+    t0 = time.perf_counter()
+    for i in range(ns):
+        fd_plain_gradient(ctx64, start[i], datas[i], sigmas[i], evaluate_spec)
+    dtp = (time.perf_counter() - t0) / ns
+    out["fits_plain_fp64_fd_chi2_grads_per_s"] = round(1.0 / dtp, 3)
+    out["fits_speedup_vs_plain_fp64"] = round(len(start) / dt * dtp, 2)
+    log("fits: plain FP64 FD gradient %.2f ms/fit -> speed-up %.1fx" % (dtp * 1e3, len(start) / dt * dtp))
+    gb.close()
+
+    # the single 256x256 bulge + disk fit
+    comps, psf = workloads.bulge_disk_fit(256)
+    tspec = workloads.abi.ModelSpec(256, 256, comps, psf=psf)
+    img, _, _ = evaluate_spec(ctx64, tspec)
+    d, sg = workloads.noisy(img, np.random.Generator(np.random.PCG64(0)))
+    scomps = []
+    r2 = np.random.Generator(np.random.PCG64(1))
+    for c in comps:
+        p = workloads.perturbed(dict(xcen=c.xcen, ycen=c.ycen, mag=c.mag, re=c.p[0], nser=c.p[1], ang=c.ang,
+                                     axrat=c.axrat, box=c.box), r2)
+        scomps.append(workloads.abi.make_component("sersic", convolve=True, **p))
+    sspec = workloads.abi.ModelSpec(256, 256, scomps, psf=psf)
+    one = Batch(ctx32, [sspec], graph=True, grad=True)
+    one.set_data([d], [sg])
+    for _ in range(args.warmup):
+        one.evaluate_grad()
+    one.chi2_grad()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        one.evaluate_grad()
+    one.chi2_grad()
+    dt1 = (time.perf_counter() - t0) / args.steps
+    one.close()
+    t0 = time.perf_counter()
+    fd_plain_gradient(ctx64, sspec, d, sg, evaluate_spec)
+    dtp1 = time.perf_counter() - t0
+    out["c256_chi2_grads_per_s"] = round(1.0 / dt1, 2)
+    out["c256_plain_fp64_fd_chi2_grads_per_s"] = round(1.0 / dtp1, 3)
+    out["c256_speedup_vs_plain_fp64"] = round(dtp1 / dt1, 2)
+    log("256^2 fit: %.3f ms per forward+gradient, plain FP64 FD %.1f ms -> %.1fx" % (dt1 * 1e3, dtp1 * 1e3,
+                                                                                  dtp1 / dt1))
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -83,6 +221,9 @@ def main():
     ap.add_argument("--sample", type=int, default=96, help="galaxies timed on the plain FP64 path")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU baseline")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--fits", type=int, default=1024, help="fits of the chi^2-gradient workload")
+    ap.add_argument("--grad-sample", type=int, default=8, help="fits timed on the plain FP64 FD gradient")
+    ap.add_argument("--no-grad", action="store_true")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run a plain device copy of BYTES bytes 3 times (calibrates the HBM byte counters)")
     args = ap.parse_args()
@@ -215,6 +356,10 @@ def main():
                          "images/s normalised to the batch mean image size" % (done, len(specs), dt)}
         log("cpu baseline: %s" % cpu)
 
+    grad = {}
+    if not args.no_grad:
+        grad = grad_bench(args, Batch, Context, evaluate_spec, ctx32, ctx64)
+
     out = {
         "metric": METRIC,
         "value": round(images_per_s, 2),
@@ -245,6 +390,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
+    out.update(grad)
     batch.close()
     print(json.dumps(out), flush=True)
 
