@@ -44,5 +44,55 @@ def build(verbose=False, csrc=CSRC, out=LIB, defines=()):
     return out
 
 
+HOST_LIB = os.path.join(OUT_DIR, "libprofit.so")
+CLI = os.path.join(OUT_DIR, "profit-cli")
+HOST_SOURCES = ["profit.cpp", "fits.cpp"]
+CXX = os.environ.get("CXX", "g++")
+HOST_FLAGS = ["-std=c++17", "-O2", "-fPIC", "-Wall", "-Wno-unused-function"]
+
+
+def build_host(verbose=False):
+    """The C++ host API (include/profit/*.h) over libprofit_hip.so, and
+    profit-cli on top of it."""
+    host = os.path.join(HERE, "host")
+    inc = ["-I", os.path.join(ROOT, "include")]
+    link = ["-L", OUT_DIR, "-l:libprofit_hip.so", "-Wl,-rpath,$ORIGIN"]
+    cmds = [
+        [CXX] + HOST_FLAGS + inc + ["-shared"] + [os.path.join(host, s) for s in HOST_SOURCES] + link +
+        ["-o", HOST_LIB + ".tmp"],
+        [CXX] + HOST_FLAGS + inc + [os.path.join(host, "profit_cli.cpp"), "-L", OUT_DIR, "-l:libprofit.so",
+                                    "-l:libprofit_hip.so", "-Wl,-rpath,$ORIGIN", "-o", CLI + ".tmp"],
+    ]
+    for out, cmd in zip((HOST_LIB, CLI), cmds):
+        if verbose:
+            print(" ".join(cmd))
+        subprocess.check_call(cmd)
+        os.replace(out + ".tmp", out)
+    return HOST_LIB
+
+
+def build_host_check(verbose=False):
+    """tests/cpp/host_api_check (test program of the C++ host API)."""
+    src = os.path.join(ROOT, "tests", "cpp", "host_api_check.cpp")
+    out_dir = os.path.join(ROOT, "tests", "cpp", "_build")
+    os.makedirs(out_dir, exist_ok=True)
+    out = os.path.join(out_dir, "host_api_check")
+    cmd = [CXX] + HOST_FLAGS + ["-I", os.path.join(ROOT, "include"), src, "-L", OUT_DIR, "-l:libprofit.so",
+                                "-l:libprofit_hip.so", "-Wl,-rpath," + OUT_DIR, "-Wl,-rpath,$ORIGIN/../../../libprofit_amd/_lib",
+                                "-o", out + ".tmp"]
+    if verbose:
+        print(" ".join(cmd))
+    subprocess.check_call(cmd)
+    os.replace(out + ".tmp", out)
+    return out
+
+
+def build_all(verbose=False):
+    build(verbose)
+    build_host(verbose)
+    build_host_check(verbose)
+
+
 if __name__ == "__main__":
-    print(build(verbose="-v" in sys.argv))
+    build_all(verbose="-v" in sys.argv)
+    print(LIB)
