@@ -18,6 +18,7 @@
 #include <vector>
 
 #include "pk_device.h"
+#include "pk_fft.h"
 #include "profit_hip.h"
 
 using namespace pk;
@@ -521,6 +522,81 @@ int launch_conv(hipStream_t s, const PkJob *djobs, uint32_t njobs, uint32_t ntil
 	return PK_OK;
 }
 
+// ---- FFT convolution ---------------------------------------------------------
+struct FftGeom {
+	uint32_t P = 0, Q = 0;
+	int lx = 0, ly = 0;
+};
+
+int fft_geometry(uint32_t sw, uint32_t sh, uint32_t kw, uint32_t kh, FftGeom &g)
+{
+	// smallest powers of two holding the linear convolution
+	const uint32_t lw = sw + kw - 1, lh = sh + kh - 1;
+	g.lx = 0;
+	while ((1u << g.lx) < lw) g.lx++;
+	g.ly = 0;
+	while ((1u << g.ly) < lh) g.ly++;
+	if (g.lx > FFT_MAX_LOG2 || g.ly > FFT_MAX_LOG2)
+		return set_err(PK_EFFT, "FFT convolution of %ux%u with %ux%u needs %ux%u points, at most %d per axis", sw, sh,
+		               kw, kh, 1u << g.lx, 1u << g.ly, 1 << FFT_MAX_LOG2);
+	g.P = 1u << g.lx;
+	g.Q = 1u << g.ly;
+	return PK_OK;
+}
+
+// exp(-2 pi i t / FFT_TWIDDLES), t < FFT_TWIDDLES, in FP64 (host, once)
+const double2 *fft_twiddles()
+{
+	static std::vector<double2> tw;
+	static std::once_flag once;
+	std::call_once(once, [] {
+		tw.resize(FFT_TWIDDLES);
+		for (int t = 0; t < FFT_TWIDDLES; t++) {
+			const double a = -2 * PK_PI * t / FFT_TWIDDLES;
+			tw[t] = make_double2(cos(a), sin(a));
+		}
+	});
+	return tw.data();
+}
+
+// One FFT convolution of the job's source (src + jb.img_off, sw x sh = dw x
+// dh) with the normalised kw x kh PSF (FP64, natural layout), into the job's
+// conv buffer with its geometry; z: P x Q complex work array, tw: twiddles.
+template <typename TI>
+int launch_fft_conv(hipStream_t s, const PkJob &jb, const PkJob *djob, const TI *src, const double *psf,
+                    const uint8_t *masks, TI *dst, double2 *z, const double2 *tw)
+{
+	FftGeom g;
+	PK_TRY(fft_geometry(jb.dw, jb.dh, jb.kw, jb.kh, g));
+	const FftPlan px = fft_plan(g.lx), py = fft_plan(g.ly);
+	const size_t n = (size_t)g.P * g.Q;
+	const uint32_t grid = (uint32_t)std::min<size_t>(cdiv((uint32_t)std::min<size_t>(n, 1u << 30), 256), 8192);
+	hipLaunchKernelGGL(fft_pack_kernel<TI>, dim3(grid), dim3(256), 0, s, src + jb.img_off, jb.dw, jb.dh, psf, jb.kw,
+	                   jb.kh, z, g.P, g.Q);
+	auto pass = [&](bool inverse, bool rows) {
+		const FftPlan &pl = rows ? px : py;
+		const uint32_t nlines = rows ? g.Q : g.P;
+		const uint32_t per = std::min<uint32_t>(std::max<uint32_t>(1, FFT_LDS_COMPLEX / (uint32_t)pl.n), nlines);
+		const size_t ls = rows ? g.P : 1, es = rows ? 1 : g.P;
+		if (inverse)
+			hipLaunchKernelGGL(fft_pass_kernel<true>, dim3(cdiv(nlines, per)), dim3(256), 0, s, z, pl, nlines, per, ls,
+			                   es, tw);
+		else
+			hipLaunchKernelGGL(fft_pass_kernel<false>, dim3(cdiv(nlines, per)), dim3(256), 0, s, z, pl, nlines, per,
+			                   ls, es, tw);
+	};
+	pass(false, true);
+	pass(false, false);
+	hipLaunchKernelGGL(fft_mul_kernel, dim3(grid), dim3(256), 0, s, z, px, py);
+	pass(true, false);
+	pass(true, true);
+	const uint32_t ng = (uint32_t)std::min<size_t>(cdiv((uint32_t)((size_t)jb.cw * jb.ch), 256), 8192);
+	hipLaunchKernelGGL(fft_sample_kernel<TI>, dim3(std::max(1u, ng)), dim3(256), 0, s, djob, (const double2 *)z, g.P,
+	                   masks, dst);
+	PK_HIP(hipGetLastError());
+	return PK_OK;
+}
+
 // capacity (cells) of each of the two queues between FP32 refinement passes
 constexpr uint32_t R32_QCAP_SINGLE = 1u << 20;
 constexpr uint32_t R32_QCAP_BATCH = 1u << 22;
@@ -810,6 +886,11 @@ int evaluate_impl(pk_ctx *ctx, const pk_model *m, const Analysis &a, double *out
 	const uint32_t qcap = fp32 ? R32_QCAP_SINGLE : 0;
 	const size_t o_q = L.take(2 * sizeof(RCell) * std::max<uint32_t>(1, qcap));
 	const size_t o_out = L.take(sizeof(double) * (size_t)ow * oh);
+	const bool use_fft = a.conv_required && m->convolver == PK_CONV_FFT;
+	FftGeom fg;
+	if (use_fft) PK_TRY(fft_geometry(a.dw, a.dh, m->psf_width, m->psf_height, fg));
+	const size_t o_fz = L.take(use_fft ? sizeof(double2) * (size_t)fg.P * fg.Q : 1);
+	const size_t o_ftw = L.take(use_fft ? sizeof(double2) * FFT_TWIDDLES : 1);
 	PK_TRY(ensure_dev(ctx, L.off));
 	PK_TRY(ensure_host(ctx, std::max(in_bytes, sizeof(double) * (size_t)ow * oh + 256 + sizeof(PkCounters))));
 
@@ -839,6 +920,9 @@ int evaluate_impl(pk_ctx *ctx, const pk_model *m, const Analysis &a, double *out
 
 	hipStream_t s = ctx->stream;
 	PK_HIP(hipMemcpyAsync(ctx->dmem, ctx->hmem, in_bytes, hipMemcpyHostToDevice, s));
+	if (use_fft)
+		PK_HIP(hipMemcpyAsync(dptr<void>(ctx, o_ftw), fft_twiddles(), sizeof(double2) * FFT_TWIDDLES,
+		                      hipMemcpyHostToDevice, s));
 	PK_HIP(hipEventRecord(ctx->ev[0], s));
 
 	const PkJob *djob = dptr<PkJob>(ctx, o_job);
@@ -865,7 +949,10 @@ int evaluate_impl(pk_ctx *ctx, const pk_model *m, const Analysis &a, double *out
 		PK_HIP(hipGetLastError());
 	}
 	PK_HIP(hipEventRecord(ctx->ev[1], s));
-	if (a.conv_required) {
+	if (use_fft) {
+		PK_TRY(launch_fft_conv<TI>(s, jb, djob, img_c, dptr<double>(ctx, o_psf64), dm, dptr<TI>(ctx, o_conv),
+		                           dptr<double2>(ctx, o_fz), dptr<double2>(ctx, o_ftw)));
+	} else if (a.conv_required) {
 		const uint32_t nct = jb.ctiles_x * cdiv(jb.ch, CT_H);
 		PK_TRY(launch_conv<TI>(s, djob, 1u, nct, img_c, dptr<TI>(ctx, o_psfT), dm, dptr<TI>(ctx, o_conv), jb.kw, jb.kh));
 	}
@@ -1050,6 +1137,12 @@ int convolve_impl(pk_ctx *ctx, int type, const double *src, uint32_t sw, uint32_
 	const size_t in_bytes = L.off;
 	const size_t o_conv = L.take(sizeof(T) * no);
 	const size_t o_out = L.take(sizeof(double) * no);
+	const bool use_fft = type == PK_CONV_FFT;
+	FftGeom fg;
+	if (use_fft) PK_TRY(fft_geometry(sw, sh, kw, kh, fg));
+	const size_t o_k64 = L.take(use_fft ? sizeof(double) * nk : 1);
+	const size_t o_fz = L.take(use_fft ? sizeof(double2) * (size_t)fg.P * fg.Q : 1);
+	const size_t o_ftw = L.take(use_fft ? sizeof(double2) * FFT_TWIDDLES : 1);
 	PK_TRY(ensure_dev(ctx, L.off));
 	PK_TRY(ensure_host(ctx, std::max(in_bytes, sizeof(double) * no)));
 	*hptr<PkJob>(ctx, o_job) = jb;
@@ -1064,9 +1157,19 @@ int convolve_impl(pk_ctx *ctx, int type, const double *src, uint32_t sw, uint32_
 	hipStream_t s = ctx->stream;
 	PK_HIP(hipMemcpyAsync(ctx->dmem, ctx->hmem, in_bytes, hipMemcpyHostToDevice, s));
 	const PkJob *djob = dptr<PkJob>(ctx, o_job);
-	const uint32_t nct = jb.ctiles_x * cdiv(g.ch, CT_H);
-	PK_TRY(launch_conv<T>(s, djob, 1u, nct, dptr<T>(ctx, o_src), dptr<T>(ctx, o_krn), dptr<uint8_t>(ctx, o_mask),
-	                      dptr<T>(ctx, o_conv), kw, kh));
+	if (use_fft) {
+		// the kernel as given (Convolver::convolve does not normalise it)
+		PK_HIP(hipMemcpyAsync(dptr<void>(ctx, o_k64), krn, sizeof(double) * nk, hipMemcpyHostToDevice, s));
+		PK_HIP(hipMemcpyAsync(dptr<void>(ctx, o_ftw), fft_twiddles(), sizeof(double2) * FFT_TWIDDLES,
+		                      hipMemcpyHostToDevice, s));
+		PK_TRY(launch_fft_conv<T>(s, jb, djob, dptr<T>(ctx, o_src), dptr<double>(ctx, o_k64),
+		                          dptr<uint8_t>(ctx, o_mask), dptr<T>(ctx, o_conv), dptr<double2>(ctx, o_fz),
+		                          dptr<double2>(ctx, o_ftw)));
+	} else {
+		const uint32_t nct = jb.ctiles_x * cdiv(g.ch, CT_H);
+		PK_TRY(launch_conv<T>(s, djob, 1u, nct, dptr<T>(ctx, o_src), dptr<T>(ctx, o_krn), dptr<uint8_t>(ctx, o_mask),
+		                      dptr<T>(ctx, o_conv), kw, kh));
+	}
 	hipLaunchKernelGGL((compose_kernel<T, double>), dim3(jb.otiles_x * cdiv(g.ch, 4)), dim3(256), 0, s, djob, 1u,
 	                   dptr<T>(ctx, o_conv), dptr<T>(ctx, o_conv), (const uint8_t *)nullptr,
 	                   dptr<double>(ctx, o_out));

@@ -487,7 +487,7 @@ pk_model Model::to_pk_model(std::vector<pk_component> &comps, std::vector<unsign
 	}
 	m.psf_scale_x = psf_scale.first;
 	m.psf_scale_y = psf_scale.second;
-	m.convolver = convolver ? convolver->type() : PK_CONV_BRUTE;
+	m.convolver = convolver ? (int32_t)convolver->type() : (int32_t)PK_CONV_BRUTE;
 	m.reuse_krn_fft = convolver ? convolver->reuse_krn_fft() : 0;
 	comps.clear();
 	for (auto &p : profiles) comps.push_back(p->get_component());
