@@ -439,11 +439,12 @@ struct ConvGeom {
 	int csx, csy;  // result(X,Y) = same(X + csx, Y + csy)
 	int mx, my;    // source pixel (X - mx, Y - my) is the mask pixel of result (X,Y)
 	int offx, offy;
+	uint32_t wrap_y;  // FFT: rows of the circular buffer
 };
 
 int conv_geometry(int type, uint32_t sw, uint32_t sh, uint32_t kw, uint32_t kh, bool crop, ConvGeom &g)
 {
-	g = ConvGeom{sw, sh, 0, 0, 0, 0, 0, 0};
+	g = ConvGeom{sw, sh, 0, 0, 0, 0, 0, 0, 0};
 	if (!(sw >= kw && sh >= kh)) {
 		// src/convolve.cpp:65-75: extend, convolve, crop the extension back
 		const uint32_t mxd = std::max(sw, kw);
@@ -460,8 +461,14 @@ int conv_geometry(int type, uint32_t sw, uint32_t sh, uint32_t kw, uint32_t kh, 
 	if (type != PK_CONV_FFT) return PK_OK;
 	int ox, oy;
 	fft_offset(sw, sh, kw, kh, ox, oy);
-	if (crop) return PK_OK;  // crop(src_dims, ext_offset) is the same-size convolution
 	const uint32_t mxd = std::max(sw, kw);
+	// libprofit's circular buffer has 2 max(src, krn) rows, the x component
+	// (include/profit/image.h:204-207), and the crop of mask_and_crop fails
+	// when it leaves it (include/profit/image.h:433-448)
+	g.wrap_y = 2 * mxd;
+	if (crop && ((uint32_t)ox + sw > 2 * mxd || (uint32_t)oy + sh > 2 * mxd))
+		return set_err(PK_EINVAL, "start_y + new_height should be <= image.height");
+	if (crop) return PK_OK;  // crop(src_dims, ext_offset) is the same-size convolution
 	g.cw = 2 * mxd;
 	g.ch = 2 * mxd;
 	g.csx = -ox;
@@ -562,17 +569,23 @@ const double2 *fft_twiddles()
 // One FFT convolution of the job's source (src + jb.img_off, sw x sh = dw x
 // dh) with the normalised kw x kh PSF (FP64, natural layout), into the job's
 // conv buffer with its geometry; z: P x Q complex work array, tw: twiddles.
+// z needs P * Q complex doubles plus 16 bytes (the source's peak).
 template <typename TI>
 int launch_fft_conv(hipStream_t s, const PkJob &jb, const PkJob *djob, const TI *src, const double *psf,
-                    const uint8_t *masks, TI *dst, double2 *z, const double2 *tw)
+                    double krn_max, const uint8_t *masks, TI *dst, double2 *z, const double2 *tw)
 {
 	FftGeom g;
 	PK_TRY(fft_geometry(jb.dw, jb.dh, jb.kw, jb.kh, g));
 	const FftPlan px = fft_plan(g.lx), py = fft_plan(g.ly);
 	const size_t n = (size_t)g.P * g.Q;
 	const uint32_t grid = (uint32_t)std::min<size_t>(cdiv((uint32_t)std::min<size_t>(n, 1u << 30), 256), 8192);
+	unsigned long long *mx = reinterpret_cast<unsigned long long *>(z + n);
+	const size_t ns = (size_t)jb.dw * jb.dh;
+	hipLaunchKernelGGL(fft_zero_kernel, dim3(1), dim3(1), 0, s, mx);
+	hipLaunchKernelGGL(fft_maxabs_kernel<TI>, dim3((uint32_t)std::min<size_t>(cdiv((uint32_t)std::min<size_t>(ns, 1u << 30), 256), 1024)),
+	                   dim3(256), 0, s, src + jb.img_off, ns, mx);
 	hipLaunchKernelGGL(fft_pack_kernel<TI>, dim3(grid), dim3(256), 0, s, src + jb.img_off, jb.dw, jb.dh, psf, jb.kw,
-	                   jb.kh, z, g.P, g.Q);
+	                   jb.kh, krn_max, (const unsigned long long *)mx, z, g.P, g.Q);
 	auto pass = [&](bool inverse, bool rows) {
 		const FftPlan &pl = rows ? px : py;
 		const uint32_t nlines = rows ? g.Q : g.P;
@@ -587,7 +600,7 @@ int launch_fft_conv(hipStream_t s, const PkJob &jb, const PkJob *djob, const TI 
 	};
 	pass(false, true);
 	pass(false, false);
-	hipLaunchKernelGGL(fft_mul_kernel, dim3(grid), dim3(256), 0, s, z, px, py);
+	hipLaunchKernelGGL(fft_mul_kernel, dim3(grid), dim3(256), 0, s, z, px, py, krn_max, (const unsigned long long *)mx);
 	pass(true, false);
 	pass(true, true);
 	const uint32_t ng = (uint32_t)std::min<size_t>(cdiv((uint32_t)((size_t)jb.cw * jb.ch), 256), 8192);
@@ -810,6 +823,7 @@ int evaluate_impl(pk_ctx *ctx, const pk_model *m, const Analysis &a, double *out
 	jb.ch = a.conv_required ? g.ch : a.dh;
 	jb.ctiles_x = cdiv(jb.cw, CT_W);
 	jb.csx = g.csx;
+	jb.wrap_y = g.wrap_y;
 	jb.csy = g.csy;
 	jb.cmx = g.mx;
 	jb.cmy = g.my;
@@ -889,7 +903,7 @@ int evaluate_impl(pk_ctx *ctx, const pk_model *m, const Analysis &a, double *out
 	const bool use_fft = a.conv_required && m->convolver == PK_CONV_FFT;
 	FftGeom fg;
 	if (use_fft) PK_TRY(fft_geometry(a.dw, a.dh, m->psf_width, m->psf_height, fg));
-	const size_t o_fz = L.take(use_fft ? sizeof(double2) * (size_t)fg.P * fg.Q : 1);
+	const size_t o_fz = L.take(use_fft ? sizeof(double2) * ((size_t)fg.P * fg.Q + 1) : 1);
 	const size_t o_ftw = L.take(use_fft ? sizeof(double2) * FFT_TWIDDLES : 1);
 	PK_TRY(ensure_dev(ctx, L.off));
 	PK_TRY(ensure_host(ctx, std::max(in_bytes, sizeof(double) * (size_t)ow * oh + 256 + sizeof(PkCounters))));
@@ -950,7 +964,9 @@ int evaluate_impl(pk_ctx *ctx, const pk_model *m, const Analysis &a, double *out
 	}
 	PK_HIP(hipEventRecord(ctx->ev[1], s));
 	if (use_fft) {
-		PK_TRY(launch_fft_conv<TI>(s, jb, djob, img_c, dptr<double>(ctx, o_psf64), dm, dptr<TI>(ctx, o_conv),
+		double kmax = 0;
+		for (double v : psf) kmax = std::max(kmax, std::fabs(v));
+		PK_TRY(launch_fft_conv<TI>(s, jb, djob, img_c, dptr<double>(ctx, o_psf64), kmax, dm, dptr<TI>(ctx, o_conv),
 		                           dptr<double2>(ctx, o_fz), dptr<double2>(ctx, o_ftw)));
 	} else if (a.conv_required) {
 		const uint32_t nct = jb.ctiles_x * cdiv(jb.ch, CT_H);
@@ -1112,6 +1128,7 @@ int convolve_impl(pk_ctx *ctx, int type, const double *src, uint32_t sw, uint32_
 	jb.cw = g.cw;
 	jb.ch = g.ch;
 	jb.csx = g.csx;
+	jb.wrap_y = g.wrap_y;
 	jb.csy = g.csy;
 	jb.cmx = g.mx;
 	jb.cmy = g.my;
@@ -1141,7 +1158,7 @@ int convolve_impl(pk_ctx *ctx, int type, const double *src, uint32_t sw, uint32_
 	FftGeom fg;
 	if (use_fft) PK_TRY(fft_geometry(sw, sh, kw, kh, fg));
 	const size_t o_k64 = L.take(use_fft ? sizeof(double) * nk : 1);
-	const size_t o_fz = L.take(use_fft ? sizeof(double2) * (size_t)fg.P * fg.Q : 1);
+	const size_t o_fz = L.take(use_fft ? sizeof(double2) * ((size_t)fg.P * fg.Q + 1) : 1);
 	const size_t o_ftw = L.take(use_fft ? sizeof(double2) * FFT_TWIDDLES : 1);
 	PK_TRY(ensure_dev(ctx, L.off));
 	PK_TRY(ensure_host(ctx, std::max(in_bytes, sizeof(double) * no)));
@@ -1162,7 +1179,9 @@ int convolve_impl(pk_ctx *ctx, int type, const double *src, uint32_t sw, uint32_
 		PK_HIP(hipMemcpyAsync(dptr<void>(ctx, o_k64), krn, sizeof(double) * nk, hipMemcpyHostToDevice, s));
 		PK_HIP(hipMemcpyAsync(dptr<void>(ctx, o_ftw), fft_twiddles(), sizeof(double2) * FFT_TWIDDLES,
 		                      hipMemcpyHostToDevice, s));
-		PK_TRY(launch_fft_conv<T>(s, jb, djob, dptr<T>(ctx, o_src), dptr<double>(ctx, o_k64),
+		double kmax = 0;
+		for (size_t t = 0; t < nk; t++) kmax = std::max(kmax, std::fabs(krn[t]));
+		PK_TRY(launch_fft_conv<T>(s, jb, djob, dptr<T>(ctx, o_src), dptr<double>(ctx, o_k64), kmax,
 		                          dptr<uint8_t>(ctx, o_mask), dptr<T>(ctx, o_conv), dptr<double2>(ctx, o_fz),
 		                          dptr<double2>(ctx, o_ftw)));
 	} else {
@@ -1281,7 +1300,7 @@ int batch_launch(pk_batch *b)
 	if (timed) PK_HIP(hipEventRecord(b->ev[0], s));
 	hipLaunchKernelGGL(reset_counters_kernel, dim3(1), dim3(64), 0, s, dcnt);
 	if (b->n_comps)
-		hipLaunchKernelGGL(setup_kernel, dim3(cdiv(b->n_comps, 64)), dim3(64), 0, s, b->d<SetupIn>(b->o_setup), dcomp,
+		hipLaunchKernelGGL(setup_kernel, dim3(b->n_comps), dim3(64), 0, s, b->d<SetupIn>(b->o_setup), dcomp,
 		                   fp32 ? b->d<CompLite>(b->o_lite) : (CompLite *)nullptr,
 		                   b->n_comps);
 	PK_HIP(hipGetLastError());

@@ -85,9 +85,9 @@ __global__ __launch_bounds__(256) void coarse_kernel(const PkJob *__restrict__ j
 				} else {
 					double xp, yp;
 					to_prof(cp, x, y, xp, yp);
-					double r = sqrt(xp * xp + yp * yp);
-					if (cp.rscale_max > 0 && r / cp.rscale > cp.rscale_max) continue;
-					if (cp.rough || r / cp.rscale > cp.rscale_switch) {
+					const double r2 = xp * xp + yp * yp;
+					if (cp.rscale_max > 0 && radius_exceeds(r2, cp.r2_max, cp.rscale, cp.rscale_max)) continue;
+					if (cp.rough || radius_exceeds(r2, cp.r2_switch, cp.rscale, cp.rscale_switch)) {
 						float v32 = -1.f;
 						if (FP32 && cp.fp32) v32 = evaluate_at32(cp, (float)xp, (float)yp, radius2_64(cp, xp, yp));
 						if (v32 >= 0.f && v32 <= 3e38f) v = (double)v32 * cp.unfold;
@@ -656,9 +656,9 @@ __global__ __launch_bounds__(256) void coarse_grad_kernel(const PkJob *__restric
 			} else if (cp.kind == PK_PROFILE_SERSIC && G != 0) {
 				double xp, yp;
 				to_prof(cp, x, y, xp, yp);
-				const double r = sqrt(xp * xp + yp * yp);
-				const bool zero = cp.rscale_max > 0 && r / cp.rscale > cp.rscale_max;
-				if (!zero && (cp.rough || r / cp.rscale > cp.rscale_switch)) {
+				const double r2 = xp * xp + yp * yp;
+				const bool zero = cp.rscale_max > 0 && radius_exceeds(r2, cp.r2_max, cp.rscale, cp.rscale_max);
+				if (!zero && (cp.rough || radius_exceeds(r2, cp.r2_switch, cp.rscale, cp.rscale_switch))) {
 					if (FP32) {
 						float g[GT_COUNT];
 						const float v = sersic_grad32(cp, (float)xp, (float)yp, g);
@@ -754,15 +754,32 @@ __global__ void reset_counters_kernel(PkCounters *c)
 // ---------------------------------------------------------------------------
 // set-up of all components on the device (batched path)
 // ---------------------------------------------------------------------------
-__global__ void setup_kernel(const SetupIn *__restrict__ in, PkComp *__restrict__ out, CompLite *__restrict__ lite,
-                             uint32_t n)
+// The lanes of a wavefront as the quadrature's Lanes policy (pk_math.h): the
+// butterfly leaves rounding-level differences between lanes, so the total
+// is taken from lane 0 for all.
+struct WaveLanes {
+	__device__ int lane() const { return (int)__lane_id(); }
+	__device__ int count() const { return 64; }
+	__device__ double sum(double v) const
+	{
+		for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+		return __shfl(v, 0);
+	}
+};
+
+// One wavefront per component: every lane runs the (sequential) set-up, the
+// luminosity quadratures spread their nodes over the lanes, lane 0 writes.
+__global__ __launch_bounds__(64) void setup_kernel(const SetupIn *__restrict__ in, PkComp *__restrict__ out,
+                                                   CompLite *__restrict__ lite, uint32_t n)
 {
-	const uint32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+	const uint32_t t = blockIdx.x;
 	if (t >= n) return;
 	PkComp c;
-	setup_component(in[t], c);
-	out[t] = c;
-	if (lite) lite[t] = make_lite(c, in[t]);
+	setup_component(in[t], c, WaveLanes());
+	if (__lane_id() == 0) {
+		out[t] = c;
+		if (lite) lite[t] = make_lite(c, in[t]);
+	}
 }
 
 }  // namespace pk

@@ -5,7 +5,8 @@
 // convolution L.  Here L is computed directly by 2-D FFTs of size P x Q
 // (powers of two with P >= sw + kw - 1, Q >= sh + kh - 1, so the circular
 // convolution does not wrap) and then sampled in the convolver's geometry:
-//   fft_pack_kernel    Z = src + i krn on the P x Q grid (both real, at the origin)
+//   fft_pack_kernel    Z = src + i c krn on the P x Q grid (both real, at the origin;
+//                      c a power of two that gives both the same peak)
 //   fft_pass_kernel    in-place radix-4 (+ one radix-2) FFT along one axis of
 //                      several rows or columns per workgroup, staged in LDS.
 //                      Forward passes are decimation in frequency, whose
@@ -17,7 +18,9 @@
 //                      inverse; k and -k are visited by one thread
 //   fft_sample_kernel  conv(X, Y) = Re L(X + csx + kw-1-kw/2, Y + csy + kh-1-kh/2),
 //                      the same-size convolution of conv_kernel at the shifted
-//                      position (0 outside L), then the mask
+//                      position (0 outside L), plus the rows libprofit's
+//                      circular buffer of 2 max(src, krn) rows wraps onto it
+//                      (images much taller than wide), then the mask
 // FFT rounding error is relative to the peak (~log2(PQ) eps max|L|): ~1e-15 of
 // the peak in FP64.
 #pragma once
@@ -91,17 +94,47 @@ __device__ inline double2 cconj(double2 a) { return make_double2(a.x, -a.y); }
 __device__ inline double2 cmul_mi(double2 a) { return make_double2(a.y, -a.x); }
 __device__ inline double2 cmul_pi(double2 a) { return make_double2(-a.y, a.x); }
 
+// max |src| as the bits of a non-negative double (their integer order is the
+// order of the values); `mx` must be 0 before the launch
+template <typename TI>
+__global__ __launch_bounds__(256) void fft_maxabs_kernel(const TI *__restrict__ src, size_t n,
+                                                         unsigned long long *__restrict__ mx)
+{
+	double m = 0;
+	for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) {
+		const double v = fabs((double)src[t]);
+		if (v > m) m = v;  // NaN never wins
+	}
+	for (int off = 32; off > 0; off >>= 1) m = fmax(m, __shfl_xor(m, off));
+	if (__lane_id() == 0 && m > 0) atomicMax(mx, (unsigned long long)__double_as_longlong(m));
+}
+
+__global__ void fft_zero_kernel(unsigned long long *mx) { *mx = 0; }
+
+// Power of two that brings the kernel's peak to the source's: src and krn are
+// transformed together as src + i c krn, and each of the two recovered
+// transforms carries rounding error relative to the larger of the pair, so
+// both are kept at the same magnitude (c is exact to apply and to undo).
+__device__ inline double fft_krn_scale(const unsigned long long *mx, double krn_max)
+{
+	const double sm = __longlong_as_double((long long)*mx);
+	if (!(sm > 0) || !(krn_max > 0) || !isfinite(sm)) return 1.0;
+	return ldexp(1.0, ilogb(sm) - ilogb(krn_max));
+}
+
 template <typename TI>
 __global__ __launch_bounds__(256) void fft_pack_kernel(const TI *__restrict__ src, uint32_t sw, uint32_t sh,
                                                        const double *__restrict__ krn, uint32_t kw, uint32_t kh,
+                                                       double krn_max, const unsigned long long *__restrict__ mx,
                                                        double2 *__restrict__ z, uint32_t P, uint32_t Q)
 {
 	const size_t n = (size_t)P * Q;
+	const double c = fft_krn_scale(mx, krn_max);
 	for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) {
 		const uint32_t x = (uint32_t)(t % P), y = (uint32_t)(t / P);
 		double re = 0, im = 0;
 		if (x < sw && y < sh) re = (double)src[x + (size_t)y * sw];
-		if (x < kw && y < kh) im = krn[x + (size_t)y * kw];
+		if (x < kw && y < kh) im = krn[x + (size_t)y * kw] * c;
 		z[t] = make_double2(re, im);
 	}
 }
@@ -181,12 +214,13 @@ __global__ __launch_bounds__(256) void fft_pass_kernel(double2 *__restrict__ z, 
 	}
 }
 
-// Z -> S K / (P Q) in place, in the digit-reversed order of both axes.
-__global__ __launch_bounds__(256) void fft_mul_kernel(double2 *__restrict__ z, FftPlan px, FftPlan py)
+// Z -> S K / (P Q c) in place, in the digit-reversed order of both axes.
+__global__ __launch_bounds__(256) void fft_mul_kernel(double2 *__restrict__ z, FftPlan px, FftPlan py, double krn_max,
+                                                      const unsigned long long *__restrict__ mx)
 {
 	const uint32_t P = (uint32_t)px.n, Q = (uint32_t)py.n;
 	const size_t n = (size_t)P * Q;
-	const double scale = 1.0 / ((double)P * (double)Q);
+	const double scale = 1.0 / ((double)P * (double)Q) / fft_krn_scale(mx, krn_max);
 	for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) {
 		const uint32_t p1 = (uint32_t)(t % P), p2 = (uint32_t)(t / P);
 		const uint32_t k1 = fft_freq_of_pos(px, p1), k2 = fft_freq_of_pos(py, p2);
@@ -217,8 +251,13 @@ __global__ __launch_bounds__(256) void fft_sample_kernel(const PkJob *__restrict
 	for (size_t t = blockIdx.x * (size_t)blockDim.x + threadIdx.x; t < n; t += (size_t)gridDim.x * blockDim.x) {
 		const int X = (int)(t % jb.cw), Y = (int)(t / jb.cw);
 		const int u = X + jb.csx + (kw - 1 - kw / 2), v = Y + jb.csy + (kh - 1 - kh / 2);
+		// the FFT convolver's buffer is circular in y with wrap_y rows: rows
+		// v + m wrap_y of L land on row v as well
 		double val = 0;
-		if (u >= 0 && v >= 0 && u < lw && v < lh) val = z[(size_t)u + (size_t)v * P].x;
+		if (u >= 0 && u < lw)
+			for (int vv = v; vv < lh; vv += jb.wrap_y ? (int)jb.wrap_y : lh) {
+				if (vv >= 0) val += z[(size_t)u + (size_t)vv * P].x;
+			}
 		if (jb.mask_off != NONE) {
 			const int mi = X - jb.cmx, mj = Y - jb.cmy;
 			if (mi < 0 || mj < 0 || mi >= W || mj >= H || !masks[jb.mask_off + mi + (size_t)mj * W]) val = 0;

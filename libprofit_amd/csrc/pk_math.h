@@ -180,16 +180,26 @@ PK_HD double sersic_dbn_dn(double n)
 // unchanged inside a device thread.  Levels halve the step until two
 // successive estimates agree to 1e-14.
 
+// The quadratures below spread their nodes over the lanes of a `Lanes`
+// policy: SerialLanes (one thread: host code) or a wavefront (device set-up
+// kernel, in pk_device.h).  lane()/count() select a lane's share of the
+// nodes; sum() adds the lanes' partial sums and gives every lane the same
+// total.
+struct SerialLanes {
+	PK_HD int lane() const { return 0; }
+	PK_HD int count() const { return 1; }
+	PK_HD double sum(double v) const { return v; }
+};
+
 // tanh-sinh on [a, b]: x = a + L q(t) / b - L q(t) with q = e/(1+e),
 // e = exp(-pi sinh t), weight L pi cosh(t) e / (1+e)^2.
-template <typename F>
-PK_HD double integrate_ts(const F &f, double a, double b)
+template <typename F, typename LN>
+PK_HD double integrate_ts(const F &f, double a, double b, const LN &ln)
 {
 	const double L = b - a;
 	if (!(L > 0)) return 0;
 	const double pi = PK_PI;
 	double h = 1.0;
-	double sum = 0.25 * L * pi * f(a + 0.5 * L);  // t = 0: e = 1, weight L pi / 4
 	auto add_pair = [&](double t) -> double {
 		const double e = exp(-pi * sinh(t));
 		const double q = e / (1 + e);
@@ -198,12 +208,21 @@ PK_HD double integrate_ts(const F &f, double a, double b)
 		double v = w * (f(a + L * q) + f(b - L * q));
 		return isfinite(v) ? v : 0;
 	};
-	// level 0: integer t
-	for (int k = 1; k <= 7; k++) sum += add_pair((double)k);
+	// level 0: t = 0 (weight L pi / 4) and the integers 1..7
+	double part = ln.lane() == 0 ? 0.25 * L * pi * f(a + 0.5 * L) : 0.0;
+	for (int k = 1 + ln.lane(); k <= 7; k += ln.count()) part += add_pair((double)k);
+	double sum = ln.sum(part);
 	double est = h * sum;
 	for (int level = 1; level <= 12; level++) {
 		h *= 0.5;
-		for (double t = h; t <= 7.0; t += 2 * h) sum += add_pair(t);
+		// new nodes t = h (2i + 1) <= 7
+		part = 0;
+		for (int i = ln.lane();; i += ln.count()) {
+			const double t = h * (2 * i + 1);
+			if (t > 7.0) break;
+			part += add_pair(t);
+		}
+		sum += ln.sum(part);
 		double nest = h * sum;
 		if (level >= 3 && fabs(nest - est) <= 1e-14 * fabs(nest)) return nest;
 		est = nest;
@@ -212,8 +231,8 @@ PK_HD double integrate_ts(const F &f, double a, double b)
 }
 
 // exp-sinh on [a, inf): x = a + exp(pi/2 sinh t), weight pi/2 cosh(t) (x - a).
-template <typename F>
-PK_HD double integrate_es(const F &f, double a)
+template <typename F, typename LN>
+PK_HD double integrate_es(const F &f, double a, const LN &ln)
 {
 	const double hp = 0.5 * PK_PI;
 	auto term = [&](double t) -> double {
@@ -224,12 +243,19 @@ PK_HD double integrate_es(const F &f, double a)
 		return isfinite(v) ? v : 0;
 	};
 	double h = 1.0;
-	double sum = term(0.0);
-	for (int k = 1; k <= 6; k++) sum += term((double)k) + term(-(double)k);
+	double part = ln.lane() == 0 ? term(0.0) : 0.0;
+	for (int k = 1 + ln.lane(); k <= 6; k += ln.count()) part += term((double)k) + term(-(double)k);
+	double sum = ln.sum(part);
 	double est = h * sum;
 	for (int level = 1; level <= 12; level++) {
 		h *= 0.5;
-		for (double t = h; t <= 6.0; t += 2 * h) sum += term(t) + term(-t);
+		part = 0;
+		for (int i = ln.lane();; i += ln.count()) {
+			const double t = h * (2 * i + 1);
+			if (t > 6.0) break;
+			part += term(t) + term(-t);
+		}
+		sum += ln.sum(part);
 		double nest = h * sum;
 		if (level >= 3 && fabs(nest - est) <= 1e-14 * fabs(nest)) return nest;
 		est = nest;
@@ -237,16 +263,16 @@ PK_HD double integrate_es(const F &f, double a)
 	return est;
 }
 
-template <typename F>
-PK_HD double integrate_adaptive(const F &f, double a, double b)
+template <typename F, typename LN = SerialLanes>
+PK_HD double integrate_adaptive(const F &f, double a, double b, const LN &ln = LN())
 {
-	return integrate_ts(f, a, b);
+	return integrate_ts(f, a, b, ln);
 }
 
-template <typename F>
-PK_HD double integrate_to_inf(const F &f, double a)
+template <typename F, typename LN = SerialLanes>
+PK_HD double integrate_to_inf(const F &f, double a, const LN &ln = LN())
 {
-	return integrate_es(f, a);
+	return integrate_es(f, a, ln);
 }
 
 }  // namespace pk

@@ -44,7 +44,21 @@ struct PkComp {
 	// gradient constants (set-up with SetupIn::grad): d bn / d nser and the
 	// derivatives of log(flux_scale) with respect to re, nser, axrat, box
 	double dbn, glnf_re, glnf_n, glnf_q, glnf_b;
+	// squared radii of the pixel classification: (rscale_max rscale)^2 and
+	// (rscale_switch rscale)^2
+	double r2_max, r2_switch;
 };
+
+// libprofit's test sqrt(x^2 + y^2) / rscale > limit (src/radial.cpp:316-321),
+// decided from r2 = x^2 + y^2 against r2_limit = (limit rscale)^2; only within
+// a relative 1e-10 of the limit (far wider than the rounding of either form)
+// is the reference expression itself evaluated, so the decision is the same.
+PK_HD bool radius_exceeds(double r2, double r2_limit, double rscale, double limit)
+{
+	if (r2 > r2_limit * (1 + 1e-10)) return true;
+	if (r2 < r2_limit * (1 - 1e-10)) return false;
+	return sqrt(r2) / rscale > limit;
+}
 
 // What one FP32 sub-cell evaluation of the refinement reads: one 128-byte
 // record (a single cache line) per component.
@@ -538,7 +552,8 @@ PK_HD double moffat_fluxfrac(double rscale, double con, double fraction)
 	return rscale * sqrt(pow(1 - fraction, 1 / (1 - con)) - 1);
 }
 
-PK_HD void setup_component(const SetupIn &in, PkComp &o)
+template <typename LN = SerialLanes>
+PK_HD void setup_component(const SetupIn &in, PkComp &o, const LN &ln = LN())
 {
 	const pk_component &c = in.c;
 	o = PkComp{};
@@ -607,7 +622,7 @@ PK_HD void setup_component(const SetupIn &in, PkComp &o)
 		o.q[2] = p[PK_KING_A];
 		rscale = o.q[1];
 		const PkComp &k = o;
-		lum = 2 * PK_PI * integrate_adaptive([&](double r) { return r * king_r(k, r); }, 0.0, o.q[1]);
+		lum = 2 * PK_PI * integrate_adaptive([&](double r) { return r * king_r(k, r); }, 0.0, o.q[1], ln);
 		break;
 	}
 	case PK_PROFILE_BROKENEXP: {
@@ -617,7 +632,7 @@ PK_HD void setup_component(const SetupIn &in, PkComp &o)
 		o.q[3] = p[PK_BROKENEXP_A];
 		rscale = o.q[0] * 4;
 		const double h1 = o.q[0], h2 = o.q[1], rb = o.q[2], a = o.q[3];
-		lum = 2 * PK_PI * integrate_to_inf([&](double r) { return r * broken_exponential(r, h1, h2, rb, a); }, 0.0);
+		lum = 2 * PK_PI * integrate_to_inf([&](double r) { return r * broken_exponential(r, h1, h2, rb, a); }, 0.0, ln);
 		break;
 	}
 	case PK_PROFILE_CORESERSIC: {
@@ -630,7 +645,7 @@ PK_HD void setup_component(const SetupIn &in, PkComp &o)
 		o.q[5] = bn;
 		rscale = o.q[0];
 		const PkComp &k = o;
-		lum = 2 * PK_PI * integrate_to_inf([&](double r) { return r * coresersic_r(k, r); }, 0.0);
+		lum = 2 * PK_PI * integrate_to_inf([&](double r) { return r * coresersic_r(k, r); }, 0.0, ln);
 		break;
 	}
 	default: break;
@@ -693,6 +708,9 @@ PK_HD void setup_component(const SetupIn &in, PkComp &o)
 		}
 		o.acc = acc;
 	}
+
+	o.r2_max = (o.rscale_max * rscale) * (o.rscale_max * rscale);
+	o.r2_switch = (o.rscale_switch * rscale) * (o.rscale_switch * rscale);
 
 	double angrad = fmod(c.ang + 90, 360.) * PK_PI / 180.;
 	o.inv_axrat = 1 / c.axrat;

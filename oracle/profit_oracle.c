@@ -1094,15 +1094,26 @@ static int or_convolve_impl(int type, const double *src, unsigned W, unsigned H,
 	double *L = (double *)malloc(lw * lh * sizeof(double));
 	or_linear_conv_fft(src, W, H, krn, kw, kh, L);
 	unsigned kx0 = (W - kw) / 2, ky0 = (H - kh) / 2;
+	int ox, oy;
+	or_fft_offset(W, H, kw, kh, &ox, &oy);
+	/* Image::crop (include/profit/image.h:433-448) of mask_and_crop throws
+	 * when the cropped block leaves the extended image: with ey = 2 max(W, kw)
+	 * (the x component, include/profit/image.h:204-207) this happens for images
+	 * taller than about 4/3 of their width */
+	if (crop && ((unsigned)ox + W > ex || (unsigned)oy + H > ey)) {
+		free(L);
+		return fail(PK_EINVAL, "start_y + new_height should be <= image.height");
+	}
+	/* the 1-D transform of length ex * ey is circular: the rows of the
+	 * placed linear convolution beyond ey wrap around to row 0 (no row
+	 * overflows in x, since ex >= W + kw - 1 + (W - kw) / 2) */
 	double *ext = (double *)calloc((size_t)ex * ey, sizeof(double));
 	for (size_t j = 0; j < lh; j++)
 		for (size_t i = 0; i < lw; i++) {
-			size_t X = i + kx0, Y = j + ky0;
-			if (X < ex && Y < ey) ext[X + Y * ex] = L[i + j * lw];
+			size_t X = i + kx0, Y = (j + ky0) % ey;
+			if (X < ex) ext[X + Y * ex] += L[i + j * lw];
 		}
 	free(L);
-	int ox, oy;
-	or_fft_offset(W, H, kw, kh, &ox, &oy);
 	/* Convolver::mask_and_crop (src/convolve.cpp:81-99) */
 	if (!crop) {
 		*offx = ox;

@@ -311,3 +311,39 @@ def test_total_flux_equals_magnitude(oracle):
         # near their edge, which costs a few 1e-3 of the flux
         tol = 1e-2 if name in ("king", "ferrer", "coresersic") else 2e-3
         assert abs(tot - exp) < tol * exp, (name, tot, exp)
+
+
+@pytest.mark.parametrize("W,H,kw,kh", [(40, 40, 9, 9), (37, 53, 13, 7), (53, 37, 7, 13), (30, 45, 8, 6),
+                                       (44, 50, 11, 12)])
+@pytest.mark.parametrize("crop", [True, False])
+def test_fft_convolver_flattened_semantics(oracle, W, H, kw, kh, crop):
+    """The oracle's FFT convolver against the reference algorithm run with
+    numpy's FFT (src/convolve.cpp:245-311): both images in a buffer of
+    2 max(src, krn) on each axis -- the x component for both
+    (include/profit/image.h:204-207) --, the kernel at (src - krn) / 2, a 1-D
+    real FFT of the flattened buffer (circular over ex * ey), product,
+    inverse, crop at offset_after_convolution.  Taller images wrap rows; a
+    crop past the buffer raises, as Image::crop does."""
+    rng = np.random.Generator(np.random.PCG64(W * 7 + H))
+    src = rng.uniform(size=(H, W))
+    krn = rng.uniform(size=(kh, kw))
+    e = 2 * max(W, kw)
+    ext_s = np.zeros((e, e))
+    ext_s[:H, :W] = src
+    ext_k = np.zeros((e, e))
+    ky, kx = (H - kh) // 2, (W - kw) // 2
+    ext_k[ky:ky + kh, kx:kx + kw] = krn
+    n = e * e
+    full = np.fft.irfft(np.fft.rfft(ext_s.ravel()) * np.fft.rfft(ext_k.ravel()), n).reshape(e, e)
+    ox = W // 2 - (1 if (W % 2 == 0 or kw % 2 == 0) else 0)
+    oy = H // 2 - (1 if (H % 2 == 0 or kh % 2 == 0) else 0)
+    if crop and (oy + H > e or ox + W > e):
+        with pytest.raises(oracle.OracleError, match="new_height"):
+            oracle.convolve(src, krn, crop=True, conv="fft")
+        return
+    got, off = oracle.convolve(src, krn, crop=crop, conv="fft")
+    exp = full[oy:oy + H, ox:ox + W] if crop else full
+    assert got.shape == exp.shape
+    assert np.max(np.abs(got - exp)) <= 1e-12 * np.max(np.abs(exp))
+    if not crop:
+        assert off == (ox, oy)
