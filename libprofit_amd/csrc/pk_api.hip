@@ -516,14 +516,14 @@ void psf_conv_layout(const double *krn, uint32_t kw, uint32_t kh, T *kt)
 
 template <typename T>
 int launch_conv(hipStream_t s, const PkJob *djobs, uint32_t njobs, uint32_t ntiles, const T *src, const T *kt,
-                const uint8_t *masks, T *dst, uint32_t max_kw, uint32_t max_kh)
+                const uint8_t *masks, T *dst, uint32_t max_kw, uint32_t max_kh, const uint32_t *tile_job = nullptr)
 {
 	uint32_t rows;
 	PK_TRY(conv_rows_per_launch(max_kw, max_kh, sizeof(T), rows));
 	const size_t lds = conv_lds_bytes(max_kw, rows, sizeof(T));
 	for (uint32_t l0 = 0; l0 < max_kh; l0 += rows) {
 		hipLaunchKernelGGL(conv_kernel<T>, dim3(ntiles), dim3(256), lds, s, djobs, njobs, src, kt, masks, dst,
-		                   (int)l0, (int)rows);
+		                   (int)l0, (int)rows, tile_job);
 		PK_HIP(hipGetLastError());
 	}
 	return PK_OK;
@@ -649,14 +649,15 @@ template <typename TI>
 int launch_profiles(hipStream_t s, bool fp32, const PkJob *djobs, uint32_t njobs, uint32_t ntiles,
                     const PkComp *dcomp, const CompLite *dlite, const uint8_t *dmask, TI *img_nc, TI *img_c,
                     RefineEntry *dent, PkCounters *dcnt, unsigned long long *dacc, RCell *dq, uint32_t qcap,
-                    uint32_t ref_blocks, uint32_t grid_main, uint32_t grid_last, bool any_comp, hipEvent_t ev_coarse)
+                    uint32_t ref_blocks, uint32_t grid_main, uint32_t grid_last, bool any_comp, hipEvent_t ev_coarse,
+                    const uint32_t *tile_job = nullptr)
 {
 	if (fp32) {
 		hipLaunchKernelGGL((coarse_kernel<TI, true>), dim3(ntiles), dim3(256), 0, s, djobs, njobs, dcomp, dmask, img_nc,
-		                   img_c, dent, dcnt, dacc);
+		                   img_c, dent, dcnt, dacc, tile_job);
 	} else {
 		hipLaunchKernelGGL((coarse_kernel<TI, false>), dim3(ntiles), dim3(256), 0, s, djobs, njobs, dcomp, dmask,
-		                   img_nc, img_c, dent, dcnt, (unsigned long long *)nullptr);
+		                   img_nc, img_c, dent, dcnt, (unsigned long long *)nullptr, tile_job);
 	}
 	PK_HIP(hipGetLastError());
 	if (ev_coarse) PK_HIP(hipEventRecord(ev_coarse, s));
@@ -947,7 +948,7 @@ int evaluate_impl(pk_ctx *ctx, const pk_model *m, const Analysis &a, double *out
 	RefineEntry *dent = dptr<RefineEntry>(ctx, o_ent);
 	const PkComp *dcomp = dptr<PkComp>(ctx, o_comp);
 
-	const uint32_t ntiles = jb.tiles_x * cdiv(a.dh, 4);
+	const uint32_t ntiles = jb.tiles_x * cdiv(a.dh, PX_ROWS);
 	const uint32_t nref_blocks = (uint32_t)std::min<size_t>(npix, 2048);
 	PK_TRY(launch_profiles<TI>(s, fp32, djob, 1u, ntiles, dcomp, dptr<CompLite>(ctx, o_lite), dm, img_nc, img_c, dent, dcnt,
 	                           dptr<unsigned long long>(ctx, o_acc), dptr<RCell>(ctx, o_q), qcap, nref_blocks,
@@ -973,9 +974,10 @@ int evaluate_impl(pk_ctx *ctx, const pk_model *m, const Analysis &a, double *out
 		PK_TRY(launch_conv<TI>(s, djob, 1u, nct, img_c, dptr<TI>(ctx, o_psfT), dm, dptr<TI>(ctx, o_conv), jb.kw, jb.kh));
 	}
 	PK_HIP(hipEventRecord(ctx->ev[2], s));
-	const uint32_t notiles = jb.otiles_x * cdiv(oh, 4);
+	const uint32_t notiles = jb.otiles_x * cdiv(oh, PX_ROWS);
 	hipLaunchKernelGGL((compose_kernel<TI, double>), dim3(notiles), dim3(256), 0, s, djob, 1u, img_nc,
-	                   dptr<TI>(ctx, o_conv), dptr<uint8_t>(ctx, o_omask), dptr<double>(ctx, o_out));
+	                   dptr<TI>(ctx, o_conv), dptr<uint8_t>(ctx, o_omask), dptr<double>(ctx, o_out),
+	                   (const uint32_t *)nullptr);
 	PK_HIP(hipGetLastError());
 	PK_HIP(hipEventRecord(ctx->ev[3], s));
 
@@ -1189,9 +1191,9 @@ int convolve_impl(pk_ctx *ctx, int type, const double *src, uint32_t sw, uint32_
 		PK_TRY(launch_conv<T>(s, djob, 1u, nct, dptr<T>(ctx, o_src), dptr<T>(ctx, o_krn), dptr<uint8_t>(ctx, o_mask),
 		                      dptr<T>(ctx, o_conv), kw, kh));
 	}
-	hipLaunchKernelGGL((compose_kernel<T, double>), dim3(jb.otiles_x * cdiv(g.ch, 4)), dim3(256), 0, s, djob, 1u,
+	hipLaunchKernelGGL((compose_kernel<T, double>), dim3(jb.otiles_x * cdiv(g.ch, PX_ROWS)), dim3(256), 0, s, djob, 1u,
 	                   dptr<T>(ctx, o_conv), dptr<T>(ctx, o_conv), (const uint8_t *)nullptr,
-	                   dptr<double>(ctx, o_out));
+	                   dptr<double>(ctx, o_out), (const uint32_t *)nullptr);
 	PK_HIP(hipGetLastError());
 	PK_HIP(hipMemcpyAsync(ctx->hmem, dptr<void>(ctx, o_out), sizeof(double) * no, hipMemcpyDeviceToHost, s));
 	PK_HIP(hipStreamSynchronize(s));
@@ -1264,6 +1266,7 @@ struct pk_batch {
 	void *dmem = nullptr;
 	size_t o_jobs = 0, o_setup = 0, o_comps = 0, o_cnt = 0, o_psfT = 0, o_mask = 0, o_nc = 0, o_c = 0, o_conv = 0,
 	       o_ent = 0, o_acc = 0, o_q = 0, o_out = 0, o_lite = 0;
+	size_t o_tj = 0;  // job of every tile: coarse tiles, then convolution tiles, then composition tiles
 	uint32_t qcap = 0;
 	std::vector<double> fx_scales;
 	uint64_t total_out = 0;
@@ -1297,6 +1300,8 @@ int batch_launch(pk_batch *b)
 	const uint8_t *dmask = b->d<uint8_t>(b->o_mask);
 	RefineEntry *dent = b->d<RefineEntry>(b->o_ent);
 	const uint32_t nj = b->n_models;
+	const uint32_t *tj_coarse = b->d<uint32_t>(b->o_tj), *tj_conv = tj_coarse + b->ntiles,
+	               *tj_out = tj_conv + b->nctiles;
 	if (timed) PK_HIP(hipEventRecord(b->ev[0], s));
 	hipLaunchKernelGGL(reset_counters_kernel, dim3(1), dim3(64), 0, s, dcnt);
 	if (b->n_comps)
@@ -1311,30 +1316,30 @@ int batch_launch(pk_batch *b)
 		                              dent, dcnt,
 		                              b->d<unsigned long long>(b->o_acc), b->d<RCell>(b->o_q), b->qcap, UINT32_MAX,
 		                              b->ctx->r32_grid_main, b->ctx->r32_grid_last,
-		                              b->n_comps > 0, timed ? b->ev[2] : nullptr));
+		                              b->n_comps > 0, timed ? b->ev[2] : nullptr, tj_coarse));
 		if (timed) PK_HIP(hipEventRecord(b->ev[3], s));
 		if (b->any_conv) {
 			PK_TRY(launch_conv<float>(s, djobs, nj, b->nctiles, c, b->d<float>(b->o_psfT), dmask, b->d<float>(b->o_conv),
-			                          b->max_kw, b->max_kh));
+			                          b->max_kw, b->max_kh, tj_conv));
 			PK_HIP(hipGetLastError());
 		}
 		if (timed) PK_HIP(hipEventRecord(b->ev[4], s));
 		hipLaunchKernelGGL((compose_kernel<float, float>), dim3(b->notiles), dim3(256), 0, s, djobs, nj, nc,
-		                   b->d<float>(b->o_conv), (const uint8_t *)nullptr, b->d<float>(b->o_out));
+		                   b->d<float>(b->o_conv), (const uint8_t *)nullptr, b->d<float>(b->o_out), tj_out);
 	} else {
 		double *nc = b->d<double>(b->o_nc), *c = b->d<double>(b->o_c);
 		PK_TRY(launch_profiles<double>(s, false, djobs, nj, b->ntiles, dcomp, nullptr, dmask, nc, c, dent, dcnt, nullptr,
 		                               nullptr,
-		                               0, 4096, 0, 0, b->n_comps > 0, timed ? b->ev[2] : nullptr));
+		                               0, 4096, 0, 0, b->n_comps > 0, timed ? b->ev[2] : nullptr, tj_coarse));
 		if (timed) PK_HIP(hipEventRecord(b->ev[3], s));
 		if (b->any_conv) {
 			PK_TRY(launch_conv<double>(s, djobs, nj, b->nctiles, c, b->d<double>(b->o_psfT), dmask,
-			                           b->d<double>(b->o_conv), b->max_kw, b->max_kh));
+			                           b->d<double>(b->o_conv), b->max_kw, b->max_kh, tj_conv));
 			PK_HIP(hipGetLastError());
 		}
 		if (timed) PK_HIP(hipEventRecord(b->ev[4], s));
 		hipLaunchKernelGGL((compose_kernel<double, double>), dim3(b->notiles), dim3(256), 0, s, djobs, nj, nc,
-		                   b->d<double>(b->o_conv), (const uint8_t *)nullptr, b->d<double>(b->o_out));
+		                   b->d<double>(b->o_conv), (const uint8_t *)nullptr, b->d<double>(b->o_out), tj_out);
 	}
 	PK_HIP(hipGetLastError());
 	if (timed) PK_HIP(hipEventRecord(b->ev[5], s));
@@ -1360,16 +1365,17 @@ int grad_launch_t(pk_batch *b)
 	const uint32_t nacc = b->n_comps * GA_SLOTS;
 	hipLaunchKernelGGL(reset_grad_kernel, dim3(std::max(1u, std::min(256u, cdiv(std::max(nacc, nj), 256)))),
 	                   dim3(256), 0, s, gcnt, (const PkCounters *)b->d<PkCounters>(b->o_cnt), gacc, nacc, chi2, nj);
+	const uint32_t *tj_coarse = b->d<uint32_t>(b->o_tj), *tj_conv = tj_coarse + b->ntiles;
 	hipLaunchKernelGGL(resid_kernel<T>, dim3(b->ntiles), dim3(256), 0, s, djobs, nj, (const T *)b->d<T>(b->o_out),
-	                   (const T *)b->d<T>(b->o_data), (const T *)b->d<T>(b->o_ivar), Gd, chi2);
+	                   (const T *)b->d<T>(b->o_data), (const T *)b->d<T>(b->o_ivar), Gd, chi2, tj_coarse);
 	PK_HIP(hipGetLastError());
 	if (b->any_conv)
 		PK_TRY(launch_conv<T>(s, djobs, nj, b->nctiles, Gd, b->d<T>(b->o_psfF), b->d<uint8_t>(b->o_mask), Gc,
-		                      b->max_kw, b->max_kh));
+		                      b->max_kw, b->max_kh, tj_conv));
 	const bool fp32 = sizeof(T) == 4;
 	hipLaunchKernelGGL((coarse_grad_kernel<T, fp32>), dim3(b->ntiles), dim3(256), 0, s, djobs, nj,
 	                   (const PkComp *)dcomp, (const uint8_t *)b->d<uint8_t>(b->o_mask), (const T *)Gd,
-	                   (const T *)Gc, gacc);
+	                   (const T *)Gc, gacc, tj_coarse);
 	PK_HIP(hipGetLastError());
 	const RefineEntry *dent = b->d<RefineEntry>(b->o_ent);
 	if (fp32) {
@@ -1485,7 +1491,7 @@ int pk_batch_create(pk_ctx *ctx, const pk_model *models, uint32_t n_models, uint
 		jb.sy = sy;
 		jb.tile_off = b->ntiles;
 		jb.tiles_x = cdiv(a.dw, 64);
-		b->ntiles += jb.tiles_x * cdiv(a.dh, 4);
+		b->ntiles += jb.tiles_x * cdiv(a.dh, PX_ROWS);
 		bool any_nc = false;
 		for (uint32_t c = 0; c < m->n_components; c++) {
 			const pk_component &pc = m->components[c];
@@ -1553,7 +1559,7 @@ int pk_batch_create(pk_ctx *ctx, const pk_model *models, uint32_t n_models, uint
 		jb.out_off = out_off;
 		jb.otile_off = b->notiles;
 		jb.otiles_x = cdiv(ow, 64);
-		b->notiles += jb.otiles_x * cdiv(oh, 4);
+		b->notiles += jb.otiles_x * cdiv(oh, PX_ROWS);
 		jb.omask_off = NONE;
 		jb.omask_f = 1;
 		jb.fx_scale = fixed_point_scale(m);
@@ -1570,10 +1576,22 @@ int pk_batch_create(pk_ctx *ctx, const pk_model *models, uint32_t n_models, uint
 	}
 	if (img_off >= (1ull << 32)) return fail(set_err(PK_EINVAL, "batch too large (%llu pixels)", (unsigned long long)img_off));
 
+	// job of every tile of the three tilings (one scalar load per workgroup)
+	std::vector<uint32_t> tj((size_t)b->ntiles + b->nctiles + b->notiles);
+	for (uint32_t jn = 0; jn < n_models; jn++) {
+		const PkJob &jb = b->jobs[jn];
+		const uint32_t nt = (jn + 1 < n_models ? b->jobs[jn + 1].tile_off : b->ntiles) - jb.tile_off;
+		const uint32_t nc = (jn + 1 < n_models ? b->jobs[jn + 1].ctile_off : b->nctiles) - jb.ctile_off;
+		const uint32_t no = (jn + 1 < n_models ? b->jobs[jn + 1].otile_off : b->notiles) - jb.otile_off;
+		std::fill_n(tj.begin() + jb.tile_off, nt, jn);
+		std::fill_n(tj.begin() + b->ntiles + jb.ctile_off, nc, jn);
+		std::fill_n(tj.begin() + b->ntiles + b->nctiles + jb.otile_off, no, jn);
+	}
 	Layout L;
 	b->o_jobs = L.take(sizeof(PkJob) * b->jobs.size());
 	b->o_setup = L.take(sizeof(SetupIn) * std::max<size_t>(1, b->setup.size()));
 	b->o_psfT = L.take(elem * std::max<size_t>(1, psf_all.size()));
+	b->o_tj = L.take(sizeof(uint32_t) * std::max<size_t>(1, tj.size()));
 	const size_t in_bytes = L.off;
 	b->o_mask = L.take(1);
 	b->o_comps = L.take(sizeof(PkComp) * std::max<size_t>(1, b->setup.size()));
@@ -1604,6 +1622,7 @@ int pk_batch_create(pk_ctx *ctx, const pk_model *models, uint32_t n_models, uint
 	}
 	std::vector<unsigned char> stage(in_bytes, 0);
 	std::memcpy(stage.data() + b->o_jobs, b->jobs.data(), sizeof(PkJob) * b->jobs.size());
+	if (!tj.empty()) std::memcpy(stage.data() + b->o_tj, tj.data(), sizeof(uint32_t) * tj.size());
 	if (!b->setup.empty()) std::memcpy(stage.data() + b->o_setup, b->setup.data(), sizeof(SetupIn) * b->setup.size());
 	for (const auto &jb : b->jobs) {
 		if (!jb.has_conv) continue;

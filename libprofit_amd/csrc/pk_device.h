@@ -37,8 +37,13 @@ __device__ inline void to_prof(const PkComp &c, double x, double y, double &xp, 
 	yp /= c.axrat;
 }
 
-__device__ inline uint32_t find_job(const PkJob *jobs, uint32_t njobs, uint32_t block, int which)
+// Job of workgroup `block` in the tiling `which` (0 coarse, 1 convolution,
+// 2 composition): from the batch's per-tile table when given (one scalar
+// load), else by binary search over the jobs' tile offsets.
+__device__ inline uint32_t find_job(const PkJob *jobs, uint32_t njobs, uint32_t block, int which,
+                                    const uint32_t *__restrict__ tile_job = nullptr)
 {
+	if (tile_job) return tile_job[block];
 	int lo = 0, hi = (int)njobs - 1;
 	while (lo < hi) {
 		int mid = (lo + hi + 1) >> 1;
@@ -52,6 +57,14 @@ __device__ inline uint32_t find_job(const PkJob *jobs, uint32_t njobs, uint32_t 
 __device__ inline uint32_t lane_id() { return __lane_id(); }
 
 // ---------------------------------------------------------------------------
+// pixel passes (coarse, residual, coarse adjoint, composition): a 256-lane
+// workgroup covers a 64 x PX_ROWS tile, each lane one column of PX_RPT rows,
+// so that the per-component constants are loaded once for PX_RPT pixels
+// ---------------------------------------------------------------------------
+constexpr int PX_ROWS = 16;  // tile rows
+constexpr int PX_RPT = 4;    // rows per lane
+
+// ---------------------------------------------------------------------------
 // coarse pass
 // ---------------------------------------------------------------------------
 template <typename TI, bool FP32>
@@ -60,66 +73,78 @@ __global__ __launch_bounds__(256) void coarse_kernel(const PkJob *__restrict__ j
                                                      const uint8_t *__restrict__ masks, TI *__restrict__ img_nc,
                                                      TI *__restrict__ img_c, RefineEntry *__restrict__ entries,
                                                      PkCounters *__restrict__ counters,
-                                                     unsigned long long *__restrict__ acc)
+                                                     unsigned long long *__restrict__ acc,
+                                                     const uint32_t *__restrict__ tile_job)
 {
-	const uint32_t jn = find_job(jobs, njobs, blockIdx.x, 0);
+	const uint32_t jn = find_job(jobs, njobs, blockIdx.x, 0, tile_job);
 	const PkJob &jb = jobs[jn];
 	const uint32_t t = blockIdx.x - jb.tile_off;
 	const uint32_t i = (t % jb.tiles_x) * 64 + (threadIdx.x & 63);
-	const uint32_t j = (t / jb.tiles_x) * 4 + (threadIdx.x >> 6);
-	const bool inside = i < jb.dw && j < jb.dh;
-	uint32_t cmask = 0;
-	uint32_t pix = 0;
-	if (inside) {
-		pix = j * jb.dw + i;
-		double acc_nc = 0, acc_c = 0;
-		const bool masked = jb.mask_off != NONE && !masks[jb.mask_off + pix];
-		if (!masked) {
-			const double hx = jb.sx / 2., hy = jb.sy / 2.;
-			const double x = hx + i * jb.sx, y = hy + j * jb.sy;
-			for (uint32_t c = 0; c < jb.ncomp; c++) {
-				const PkComp &cp = comps[jb.comp_off + c];
-				double v;
-				if (cp.kind == PK_PROFILE_SKY) {
-					v = cp.q[0];
-				} else {
-					double xp, yp;
-					to_prof(cp, x, y, xp, yp);
-					const double r2 = xp * xp + yp * yp;
-					if (cp.rscale_max > 0 && radius_exceeds(r2, cp.r2_max, cp.rscale, cp.rscale_max)) continue;
-					if (cp.rough || radius_exceeds(r2, cp.r2_switch, cp.rscale, cp.rscale_switch)) {
-						float v32 = -1.f;
-						if (FP32 && cp.fp32) v32 = evaluate_at32(cp, (float)xp, (float)yp, radius2_64(cp, xp, yp));
-						if (v32 >= 0.f && v32 <= 3e38f) v = (double)v32 * cp.unfold;
-						else v = evaluate_at(cp, xp, yp);
-						v *= cp.flux_scale;
-					} else {
-						cmask |= 1u << c;
-						continue;
-					}
-				}
-				if (cp.convolve) acc_c += v;
-				else acc_nc += v;
-			}
-		}
-		if (jb.has_nc) img_nc[jb.img_off + pix] = (TI)acc_nc;
-		if (jb.has_conv) img_c[jb.img_off + pix] = (TI)acc_c;
+	const uint32_t j0 = (t / jb.tiles_x) * PX_ROWS + (threadIdx.x >> 6) * PX_RPT;
+	bool inside[PX_RPT], live[PX_RPT];
+	double acc_nc[PX_RPT], acc_c[PX_RPT];
+	uint32_t cmask[PX_RPT];
+#pragma unroll
+	for (int r = 0; r < PX_RPT; r++) {
+		inside[r] = i < jb.dw && j0 + r < jb.dh;
+		live[r] = inside[r] && !(jb.mask_off != NONE && !masks[jb.mask_off + (j0 + r) * jb.dw + i]);
+		acc_nc[r] = acc_c[r] = 0;
+		cmask[r] = 0;
 	}
-	// wavefront-aggregated append of the pixels that need refinement
-	const bool need = cmask != 0;
-	const uint64_t ballot = __ballot(need);
-	if (ballot) {
-		const uint32_t n = __popcll(ballot);
-		const uint32_t rank = __popcll(ballot & ((1ull << lane_id()) - 1ull));
-		const uint32_t leader = __ffsll((long long)ballot) - 1;
-		uint32_t base = 0;
-		if (lane_id() == leader) base = atomicAdd(&counters->n_entries, n);
-		base = __shfl(base, (int)leader);
-		if (need) {
-			entries[base + rank] = RefineEntry{jn, pix, cmask};
-			if (acc) {
-				acc[2 * (size_t)(base + rank)] = 0;
-				acc[2 * (size_t)(base + rank) + 1] = 0;
+	const double hx = jb.sx / 2., hy = jb.sy / 2.;
+	const double x = hx + i * jb.sx;
+	for (uint32_t c = 0; c < jb.ncomp; c++) {
+		const PkComp &cp = comps[jb.comp_off + c];
+#pragma unroll
+		for (int r = 0; r < PX_RPT; r++) {
+			if (!live[r]) continue;
+			const double y = hy + (j0 + r) * jb.sy;
+			double v;
+			if (cp.kind == PK_PROFILE_SKY) {
+				v = cp.q[0];
+			} else {
+				double xp, yp;
+				to_prof(cp, x, y, xp, yp);
+				const double r2 = xp * xp + yp * yp;
+				if (cp.rscale_max > 0 && radius_exceeds(r2, cp.r2_max, cp.rscale, cp.rscale_max)) continue;
+				if (cp.rough || radius_exceeds(r2, cp.r2_switch, cp.rscale, cp.rscale_switch)) {
+					float v32 = -1.f;
+					if (FP32 && cp.fp32) v32 = evaluate_at32(cp, (float)xp, (float)yp, radius2_64(cp, xp, yp));
+					if (v32 >= 0.f && v32 <= 3e38f) v = (double)v32 * cp.unfold;
+					else v = evaluate_at(cp, xp, yp);
+					v *= cp.flux_scale;
+				} else {
+					cmask[r] |= 1u << c;
+					continue;
+				}
+			}
+			if (cp.convolve) acc_c[r] += v;
+			else acc_nc[r] += v;
+		}
+	}
+#pragma unroll
+	for (int r = 0; r < PX_RPT; r++) {
+		const uint32_t pix = (j0 + r) * jb.dw + i;
+		if (inside[r]) {
+			if (jb.has_nc) img_nc[jb.img_off + pix] = (TI)acc_nc[r];
+			if (jb.has_conv) img_c[jb.img_off + pix] = (TI)acc_c[r];
+		}
+		// wavefront-aggregated append of the pixels that need refinement
+		const bool need = cmask[r] != 0;
+		const uint64_t ballot = __ballot(need);
+		if (ballot) {
+			const uint32_t n = __popcll(ballot);
+			const uint32_t rank = __popcll(ballot & ((1ull << lane_id()) - 1ull));
+			const uint32_t leader = __ffsll((long long)ballot) - 1;
+			uint32_t base = 0;
+			if (lane_id() == leader) base = atomicAdd(&counters->n_entries, n);
+			base = __shfl(base, (int)leader);
+			if (need) {
+				entries[base + rank] = RefineEntry{jn, pix, cmask[r]};
+				if (acc) {
+					acc[2 * (size_t)(base + rank)] = 0;
+					acc[2 * (size_t)(base + rank) + 1] = 0;
+				}
 			}
 		}
 	}
@@ -504,11 +529,11 @@ template <typename T>
 __global__ __launch_bounds__(256) void conv_kernel(const PkJob *__restrict__ jobs, uint32_t njobs,
                                                    const T *__restrict__ src_all, const T *__restrict__ krn_all,
                                                    const uint8_t *__restrict__ masks, T *__restrict__ dst_all,
-                                                   int l0, int nl)
+                                                   int l0, int nl, const uint32_t *__restrict__ tile_job)
 {
 	extern __shared__ unsigned char smem_raw[];
 	T *tile = reinterpret_cast<T *>(smem_raw);
-	const uint32_t jn = find_job(jobs, njobs, blockIdx.x, 1);
+	const uint32_t jn = find_job(jobs, njobs, blockIdx.x, 1, tile_job);
 	const PkJob &jb = jobs[jn];
 	const uint32_t t = blockIdx.x - jb.ctile_off;
 	const int bx = (int)(t % jb.ctiles_x) * CT_W, by = (int)(t / jb.ctiles_x) * CT_H;
@@ -532,38 +557,44 @@ template <typename TI, typename TO>
 __global__ __launch_bounds__(256) void compose_kernel(const PkJob *__restrict__ jobs, uint32_t njobs,
                                                       const TI *__restrict__ img_nc,
                                                       const TI *__restrict__ conv_all,
-                                                      const uint8_t *__restrict__ omasks, TO *__restrict__ out_all)
+                                                      const uint8_t *__restrict__ omasks, TO *__restrict__ out_all,
+                                                      const uint32_t *__restrict__ tile_job)
 {
-	const uint32_t jn = find_job(jobs, njobs, blockIdx.x, 2);
+	const uint32_t jn = find_job(jobs, njobs, blockIdx.x, 2, tile_job);
 	const PkJob &jb = jobs[jn];
 	const uint32_t t = blockIdx.x - jb.otile_off;
 	const uint32_t i = (t % jb.otiles_x) * 64 + (threadIdx.x & 63);
-	const uint32_t j = (t / jb.otiles_x) * 4 + (threadIdx.x >> 6);
-	if (i >= jb.ow || j >= jb.oh) return;
-	double v = 0;
-	for (uint32_t b = 0; b < jb.fs; b++) {
-		const uint32_t fy = j * jb.fs + b;
-		if (fy >= jb.crh) break;
-		for (uint32_t a = 0; a < jb.fs; a++) {
-			const uint32_t fx = i * jb.fs + a;
-			if (fx >= jb.crw) break;
-			const int X = jb.cox + (int)fx, Y = jb.coy + (int)fy;
-			if (X < 0 || Y < 0 || X >= (int)jb.cw || Y >= (int)jb.ch) continue;
-			double p = 0;
-			if (jb.has_conv) p += (double)conv_all[jb.conv_off + X + (size_t)Y * jb.cw];
-			if (jb.has_nc) {
-				const int mx = X - jb.mox, my = Y - jb.moy;
-				if (mx >= 0 && my >= 0 && mx < (int)jb.dw && my < (int)jb.dh)
-					p += (double)img_nc[jb.img_off + mx + (size_t)my * jb.dw];
+	const uint32_t j0 = (t / jb.otiles_x) * PX_ROWS + (threadIdx.x >> 6) * PX_RPT;
+	if (i >= jb.ow) return;
+#pragma unroll
+	for (int r = 0; r < PX_RPT; r++) {
+		const uint32_t j = j0 + r;
+		if (j >= jb.oh) break;
+		double v = 0;
+		for (uint32_t b = 0; b < jb.fs; b++) {
+			const uint32_t fy = j * jb.fs + b;
+			if (fy >= jb.crh) break;
+			for (uint32_t a = 0; a < jb.fs; a++) {
+				const uint32_t fx = i * jb.fs + a;
+				if (fx >= jb.crw) break;
+				const int X = jb.cox + (int)fx, Y = jb.coy + (int)fy;
+				if (X < 0 || Y < 0 || X >= (int)jb.cw || Y >= (int)jb.ch) continue;
+				double p = 0;
+				if (jb.has_conv) p += (double)conv_all[jb.conv_off + X + (size_t)Y * jb.cw];
+				if (jb.has_nc) {
+					const int mx = X - jb.mox, my = Y - jb.moy;
+					if (mx >= 0 && my >= 0 && mx < (int)jb.dw && my < (int)jb.dh)
+						p += (double)img_nc[jb.img_off + mx + (size_t)my * jb.dw];
+				}
+				v += p;
 			}
-			v += p;
 		}
+		if (jb.omask_off != NONE) {
+			const uint32_t mi = i / jb.omask_f, mj = j / jb.omask_f;
+			if (mi < jb.omask_w && mj < jb.omask_h && !omasks[jb.omask_off + mi + (size_t)mj * jb.omask_w]) v = 0;
+		}
+		out_all[jb.out_off + i + (size_t)j * jb.ow] = (TO)v;
 	}
-	if (jb.omask_off != NONE) {
-		const uint32_t mi = i / jb.omask_f, mj = j / jb.omask_f;
-		if (mi < jb.omask_w && mj < jb.omask_h && !omasks[jb.omask_off + mi + (size_t)mj * jb.omask_w]) v = 0;
-	}
-	out_all[jb.out_off + i + (size_t)j * jb.ow] = (TO)v;
 }
 
 // ---------------------------------------------------------------------------
@@ -597,24 +628,27 @@ template <typename TI>
 __global__ __launch_bounds__(256) void resid_kernel(const PkJob *__restrict__ jobs, uint32_t njobs,
                                                     const TI *__restrict__ out_all, const TI *__restrict__ data,
                                                     const TI *__restrict__ ivar, TI *__restrict__ G,
-                                                    double *__restrict__ chi2)
+                                                    double *__restrict__ chi2, const uint32_t *__restrict__ tile_job)
 {
-	const uint32_t jn = find_job(jobs, njobs, blockIdx.x, 0);
+	const uint32_t jn = find_job(jobs, njobs, blockIdx.x, 0, tile_job);
 	const PkJob &jb = jobs[jn];
 	const uint32_t t = blockIdx.x - jb.tile_off;
 	const uint32_t i = (t % jb.tiles_x) * 64 + (threadIdx.x & 63);
-	const uint32_t j = (t / jb.tiles_x) * 4 + (threadIdx.x >> 6);
+	const uint32_t j0 = (t / jb.tiles_x) * PX_ROWS + (threadIdx.x >> 6) * PX_RPT;
 	double c2[1] = {0};
-	if (i < jb.dw && j < jb.dh) {
+#pragma unroll
+	for (int r = 0; r < PX_RPT; r++) {
+		const uint32_t j = j0 + r;
+		if (!(i < jb.dw && j < jb.dh)) continue;
 		double g = 0;
 		const int fx = (int)i - jb.cox, fy = (int)j - jb.coy;
 		if (fx >= 0 && fy >= 0 && fx < (int)jb.crw && fy < (int)jb.crh) {
 			const uint32_t oi = (uint32_t)fx / jb.fs, oj = (uint32_t)fy / jb.fs;
 			const size_t p = jb.out_off + oi + (size_t)oj * jb.ow;
-			const double r = (double)data[p] - (double)out_all[p];
+			const double rr = (double)data[p] - (double)out_all[p];
 			const double iv = (double)ivar[p];
-			g = -2 * r * iv;
-			if ((uint32_t)fx % jb.fs == 0 && (uint32_t)fy % jb.fs == 0) c2[0] = r * r * iv;
+			g = -2 * rr * iv;
+			if ((uint32_t)fx % jb.fs == 0 && (uint32_t)fy % jb.fs == 0) c2[0] += rr * rr * iv;
 		}
 		G[jb.img_off + i + (size_t)j * jb.dw] = (TI)g;
 	}
@@ -630,30 +664,40 @@ __global__ __launch_bounds__(256) void coarse_grad_kernel(const PkJob *__restric
                                                           const PkComp *__restrict__ comps,
                                                           const uint8_t *__restrict__ masks,
                                                           const TI *__restrict__ G_nc, const TI *__restrict__ G_c,
-                                                          double *__restrict__ gacc)
+                                                          double *__restrict__ gacc,
+                                                          const uint32_t *__restrict__ tile_job)
 {
-	const uint32_t jn = find_job(jobs, njobs, blockIdx.x, 0);
+	const uint32_t jn = find_job(jobs, njobs, blockIdx.x, 0, tile_job);
 	const PkJob &jb = jobs[jn];
 	const uint32_t t = blockIdx.x - jb.tile_off;
 	const uint32_t i = (t % jb.tiles_x) * 64 + (threadIdx.x & 63);
-	const uint32_t j = (t / jb.tiles_x) * 4 + (threadIdx.x >> 6);
-	bool live = i < jb.dw && j < jb.dh;
-	double gn = 0, gc = 0;
-	if (live) {
-		const size_t pix = jb.img_off + j * jb.dw + i;
-		if (jb.mask_off != NONE && !masks[jb.mask_off + j * jb.dw + i]) live = false;
-		gn = (double)G_nc[pix];
-		if (jb.has_conv) gc = (double)G_c[pix];
+	const uint32_t j0 = (t / jb.tiles_x) * PX_ROWS + (threadIdx.x >> 6) * PX_RPT;
+	bool live[PX_RPT];
+	double gn[PX_RPT], gc[PX_RPT];
+#pragma unroll
+	for (int r = 0; r < PX_RPT; r++) {
+		const uint32_t j = j0 + r;
+		live[r] = i < jb.dw && j < jb.dh;
+		gn[r] = gc[r] = 0;
+		if (live[r]) {
+			const size_t pix = jb.img_off + j * jb.dw + i;
+			if (jb.mask_off != NONE && !masks[jb.mask_off + j * jb.dw + i]) live[r] = false;
+			gn[r] = (double)G_nc[pix];
+			if (jb.has_conv) gc[r] = (double)G_c[pix];
+		}
 	}
-	const double x = jb.sx / 2. + i * jb.sx, y = jb.sy / 2. + j * jb.sy;
+	const double x = jb.sx / 2. + i * jb.sx;
 	for (uint32_t c = 0; c < jb.ncomp; c++) {
 		const PkComp &cp = comps[jb.comp_off + c];
 		double acc[GA_SLOTS] = {0, 0, 0, 0, 0, 0, 0, 0};
-		if (live) {
-			const double G = cp.convolve ? gc : gn;
+#pragma unroll
+		for (int r = 0; r < PX_RPT; r++) {
+			if (!live[r]) continue;
+			const double G = cp.convolve ? gc[r] : gn[r];
 			if (cp.kind == PK_PROFILE_SKY) {
-				acc[GA_W] = G;
+				acc[GA_W] += G;
 			} else if (cp.kind == PK_PROFILE_SERSIC && G != 0) {
+				const double y = jb.sy / 2. + (j0 + r) * jb.sy;
 				double xp, yp;
 				to_prof(cp, x, y, xp, yp);
 				const double r2 = xp * xp + yp * yp;
