@@ -125,7 +125,7 @@ def grad_bench(args, Batch, Context, evaluate_spec, ctx32, ctx64):
         datas.append(d)
         sigmas.append(sg)
     tb.close()
-    gb = Batch(ctx32, start, graph=True, grad=True)
+    gb = Batch(ctx32, start, graph=True, grad=True, autotune=not args.no_tune)
     gb.set_data(datas, sigmas)
     for _ in range(args.warmup):
         gb.evaluate_grad()
@@ -141,6 +141,8 @@ def grad_bench(args, Batch, Context, evaluate_spec, ctx32, ctx64):
         % (len(start), dt * 1e3, len(start) / dt))
     # stage split (eager launches, HIP events)
     eb = Batch(ctx32, start, graph=False, grad=True)
+    eb.tuning = gb.tuning
+    out["fits_tuning"] = gb.tuning.as_dict()
     eb.set_data(datas, sigmas)
     eb.evaluate_grad()
     eb.sync()
@@ -224,6 +226,7 @@ def main():
     ap.add_argument("--fits", type=int, default=1024, help="fits of the chi^2-gradient workload")
     ap.add_argument("--grad-sample", type=int, default=8, help="fits timed on the plain FP64 FD gradient")
     ap.add_argument("--no-grad", action="store_true")
+    ap.add_argument("--no-tune", action="store_true", help="default launch settings instead of the autotuner's")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run a plain device copy of BYTES bytes 3 times (calibrates the HBM byte counters)")
     args = ap.parse_args()
@@ -255,8 +258,13 @@ def main():
 
     # ---- batched FP32 path (graph) --------------------------------------------------
     t0 = time.time()
-    batch = Batch(ctx32, specs, graph=True)
+    batch = Batch(ctx32, specs, graph=True, autotune=not args.no_tune)
     log("batch created in %.2fs" % (time.time() - t0))
+    t0 = time.time()
+    batch.evaluate()  # the autotuner runs here, before the graph is captured
+    batch.sync()
+    tuning = batch.tuning.as_dict()
+    log("first evaluation (autotuning included) %.2fs; settings %s" % (time.time() - t0, tuning))
     for _ in range(args.warmup):
         batch.evaluate()
     info = batch.sync()
@@ -277,6 +285,7 @@ def main():
     # ---- per-kernel timing (same kernels, launched without the graph so that HIP
     # events sit between them on the launch stream) ---------------------------------
     ev_batch = Batch(ctx32, specs, graph=False)
+    ev_batch.tuning = batch.tuning
     stages = {"setup": 0.0, "coarse": 0.0, "refine": 0.0, "conv": 0.0, "compose": 0.0, "total": 0.0}
     nrep = max(3, min(args.steps, 10))
     for _ in range(2):
@@ -293,7 +302,8 @@ def main():
     conv_ms = stages["conv"]
     achieved_tflops = flops / (conv_ms * 1e-3) / 1e12
     roofline = {
-        "kernel": "conv_kernel<float> (direct PSF convolution)",
+        "kernel": ("conv_mfma_kernel (direct PSF convolution, MFMA tiles)" if tuning["conv_tiles"]
+                   else "conv_kernel<float> (direct PSF convolution)"),
         # 2*kw*kh flop per pixel over ~8 B per pixel: far right of the HBM ridge,
         # bound by FP32 vector FMA issue
         "bound": "valu",
@@ -380,6 +390,7 @@ def main():
             "output_pixels": npix,
             "seed": args.seed,
             "graph": True,
+            "tuning": tuning,
         },
         "plain_fp64_images_per_s": round(plain_images_per_s, 2),
         "speedup_vs_plain_fp64": round(images_per_s / plain_images_per_s, 2),

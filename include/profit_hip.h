@@ -152,7 +152,8 @@ typedef struct pk_ctx pk_ctx;
 /* ---- library ----------------------------------------------------------- */
 /* profit::version() (include/profit/library.h:38-60) */
 const char *pk_version(void);
-/* profit::init() / profit::finish() (include/profit/library.h:67-80) */
+/* profit::init() / profit::finish() (include/profit/library.h:67-80):
+ * device check plus loading / saving the autotuner cache (below). */
 int pk_init(void);
 void pk_finish(void);
 /* Message of the last error raised on this thread ("" if none). */
@@ -224,9 +225,28 @@ int pk_convolver_padding(int type, uint32_t src_w, uint32_t src_h, uint32_t krn_
 typedef struct pk_batch pk_batch;
 
 enum {
-	PK_BATCH_GRAPH = 1,  /* capture the launch sequence once, replay it */
-	PK_BATCH_GRAD = 2    /* also allocate what pk_batch_evaluate_grad needs */
+	PK_BATCH_GRAPH = 1,    /* capture the launch sequence once, replay it */
+	PK_BATCH_GRAD = 2,     /* also allocate what pk_batch_evaluate_grad needs */
+	PK_BATCH_AUTOTUNE = 4  /* tune the launch settings at the first evaluation
+	                          (pk_batch_autotune, or the cached result for the
+	                          same device and workload) */
 };
+
+/* Launch settings of a batch.  None of them changes which pixels are refined
+ * or how; the refinement settings do not change the result at all (its sums
+ * are exact fixed-point), the convolution variant changes the FP32 rounding
+ * of convolved pixels only.  Defaults: VALU tiles, 100, 4, 64, 16, 16. */
+enum { PK_CONV_TILES_VALU = 0, PK_CONV_TILES_MFMA = 1 };
+typedef struct pk_tuning {
+	uint32_t conv_tiles;       /* FP32 direct convolution: PK_CONV_TILES_VALU or _MFMA */
+	uint32_t refine_occupancy; /* FP32 refinement wavefronts, percent (1..100) of the most that fit the GPU */
+	uint32_t handover_div;     /* a refinement pass hands its work on once 1/handover_div of its
+	                              wavefronts ran dry (>= 1) */
+	uint32_t claim_pixels;     /* refinement pixels one wavefront claims at once, at most (>= 1) */
+	uint32_t claim_cells;      /* handed-on cells one wavefront claims at once, at most (>= 1) */
+	uint32_t poll_rounds;      /* refinement rounds between two checks of the hand-on condition (>= 1) */
+	double ms_evaluate;        /* device ms of one evaluation with these settings, when measured (else 0) */
+} pk_tuning;
 
 typedef struct pk_batch_info {
 	uint64_t n_refine_entries;  /* pixels that entered sub-pixel refinement */
@@ -253,6 +273,24 @@ int pk_batch_get_image(pk_batch *b, uint32_t i, double *out, size_t out_capacity
 int pk_batch_device_images(const pk_batch *b, void **base, size_t *elem_size);
 int pk_batch_image_offset(const pk_batch *b, uint32_t i, uint64_t *offset);
 int pk_batch_destroy(pk_batch *b);
+
+/* ---- autotuner -------------------------------------------------------------
+ * No counterpart in libprofit (its only tuned state is FFTW wisdom, imported
+ * by init() and exported by finish(), src/library.cpp:96-165).  The tuner
+ * times eager evaluations of the batch over candidate settings, one setting
+ * at a time (convolution tiles, then hand-on divisor, occupancy, claim sizes,
+ * poll interval), keeps the fastest and leaves the images of one more
+ * evaluation with it.  Results are cached per device and workload shape
+ * (precision, gradient flag, model count, pixel count, largest PSF, component
+ * count); pk_init loads the cache from $PROFIT_HOME/hip_tuning (PROFIT_HOME
+ * defaults to $HOME/.profit, as in src/utils.cpp:471-495) and pk_finish
+ * writes it back; pk_clear_cache empties it and removes the file
+ * (profit::clear_cache, src/library.cpp:251-270). */
+int pk_batch_autotune(pk_batch *b);
+int pk_batch_get_tuning(const pk_batch *b, pk_tuning *t);
+/* Replaces the settings (validated; a captured graph is captured again). */
+int pk_batch_set_tuning(pk_batch *b, const pk_tuning *t);
+int pk_clear_cache(void);
 
 /* ---- chi^2 and its gradient ----------------------------------------------
  * No counterpart in libprofit (fits with it use finite differences, e.g. in

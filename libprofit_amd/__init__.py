@@ -13,7 +13,8 @@ def __getattr__(name):
     # Model, Profile, Context, create_convolver are loaded lazily so that
     # `import libprofit_amd` works (and fails loudly on use) without the
     # library.
-    if name in ("Model", "Profile", "Context", "Convolver", "create_convolver", "default_context", "evaluate_spec"):
+    if name in ("Model", "Profile", "Context", "Convolver", "create_convolver", "default_context", "evaluate_spec",
+                "Batch", "init", "finish", "clear_cache"):
         from . import model
         return getattr(model, name)
     raise AttributeError(name)

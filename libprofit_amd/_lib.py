@@ -67,6 +67,12 @@ def load():
         "pk_batch_set_data": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t]),
         "pk_batch_evaluate_grad": (C.c_int, [C.c_void_p]),
         "pk_batch_get_chi2_grad": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p]),
+        "pk_batch_autotune": (C.c_int, [C.c_void_p]),
+        "pk_batch_get_tuning": (C.c_int, [C.c_void_p, P(abi.Tuning)]),
+        "pk_batch_set_tuning": (C.c_int, [C.c_void_p, P(abi.Tuning)]),
+        "pk_clear_cache": (C.c_int, []),
+        "pk_init": (C.c_int, []),
+        "pk_finish": (None, []),
         "pk_qgamma": (dbl, [dbl, dbl]),
         "pk_pgamma": (dbl, [dbl, dbl]),
         "pk_gammafn": (dbl, [dbl]),

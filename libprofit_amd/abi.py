@@ -131,6 +131,26 @@ class BatchInfo(C.Structure):
 
 PK_BATCH_GRAPH = 1
 PK_BATCH_GRAD = 2
+PK_BATCH_AUTOTUNE = 4
+
+PK_CONV_TILES_VALU = 0
+PK_CONV_TILES_MFMA = 1
+
+
+class Tuning(C.Structure):
+    """pk_tuning: launch settings of a batch (autotuner)."""
+    _fields_ = [
+        ("conv_tiles", C.c_uint32),
+        ("refine_occupancy", C.c_uint32),
+        ("handover_div", C.c_uint32),
+        ("claim_pixels", C.c_uint32),
+        ("claim_cells", C.c_uint32),
+        ("poll_rounds", C.c_uint32),
+        ("ms_evaluate", C.c_double),
+    ]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
 
 # gradient slots of a component (pk_batch_get_chi2_grad)
 GRAD_SLOTS = 8

@@ -13,9 +13,13 @@
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
+#include <map>
 #include <mutex>
 #include <string>
 #include <vector>
+
+#include <sys/stat.h>
+#include <unistd.h>
 
 #include "pk_device.h"
 #include "pk_fft.h"
@@ -68,6 +72,7 @@ struct pk_ctx {
 	// device at once (the hand-over rule counts them)
 	uint32_t r32_grid_main = 0, r32_grid_last = 0;
 	uint32_t r32g_grid_main = 0, r32g_grid_last = 0;  // the same for the adjoint variants
+	std::string arch;  // device name and CU count (autotuner cache key)
 };
 
 static int ensure_dev(pk_ctx *ctx, size_t bytes)
@@ -112,19 +117,110 @@ struct Layout {
 	}
 };
 
+namespace {
+
+// ---- autotuner cache -----------------------------------------------------------
+// One text line per tuned workload: "<key>\t<conv> <occupancy> <handover> <claim
+// pixels> <claim cells> <poll> <ms>", in $PROFIT_HOME/hip_tuning.
+std::mutex g_tune_mu;
+std::map<std::string, pk_tuning> g_tune_cache;
+bool g_tune_dirty = false;
+
+// $PROFIT_HOME, else $HOME/.profit (src/utils.cpp:471-495); "" if neither is set.
+std::string profit_home()
+{
+	if (const char *p = std::getenv("PROFIT_HOME")) return p;
+	if (const char *h = std::getenv("HOME")) return std::string(h) + "/.profit";
+	return "";
+}
+
+std::string tune_cache_file()
+{
+	const std::string h = profit_home();
+	return h.empty() ? h : h + "/hip_tuning";
+}
+
+bool tuning_valid(const pk_tuning &t)
+{
+	return t.conv_tiles <= PK_CONV_TILES_MFMA && t.refine_occupancy >= 1 && t.refine_occupancy <= 100 &&
+	       t.handover_div >= 1 && t.claim_pixels >= 1 && t.claim_cells >= 1 && t.poll_rounds >= 1 &&
+	       t.handover_div <= 1024 && t.claim_pixels <= 4096 && t.claim_cells <= 4096 && t.poll_rounds <= 4096;
+}
+
+void tune_cache_load()
+{
+	const std::string f = tune_cache_file();
+	if (f.empty()) return;
+	FILE *fp = std::fopen(f.c_str(), "r");
+	if (!fp) return;
+	char line[1024];
+	while (std::fgets(line, sizeof line, fp)) {
+		char *tab = std::strchr(line, '\t');
+		if (!tab) continue;
+		*tab = 0;
+		pk_tuning t{};
+		if (std::sscanf(tab + 1, "%u %u %u %u %u %u %lf", &t.conv_tiles, &t.refine_occupancy, &t.handover_div,
+		                &t.claim_pixels, &t.claim_cells, &t.poll_rounds, &t.ms_evaluate) == 7 &&
+		    tuning_valid(t))
+			g_tune_cache[line] = t;
+	}
+	std::fclose(fp);
+}
+
+int tune_cache_save()
+{
+	const std::string h = profit_home();
+	if (h.empty()) return PK_OK;
+	(void)::mkdir(h.c_str(), 0755);
+	const std::string f = h + "/hip_tuning", tmp = f + ".tmp";
+	FILE *fp = std::fopen(tmp.c_str(), "w");
+	if (!fp) return set_err(PK_EINVAL, "cannot write %s", tmp.c_str());
+	for (const auto &kv : g_tune_cache) {
+		const pk_tuning &t = kv.second;
+		std::fprintf(fp, "%s\t%u %u %u %u %u %u %.6f\n", kv.first.c_str(), t.conv_tiles, t.refine_occupancy,
+		             t.handover_div, t.claim_pixels, t.claim_cells, t.poll_rounds, t.ms_evaluate);
+	}
+	const bool ok = std::fclose(fp) == 0;
+	if (!ok || std::rename(tmp.c_str(), f.c_str()) != 0) {
+		(void)std::remove(tmp.c_str());
+		return set_err(PK_EINVAL, "cannot write %s", f.c_str());
+	}
+	return PK_OK;
+}
+
+}  // namespace
+
 extern "C" {
 
 const char *pk_version(void) { return "1.9.3"; }
 
 int pk_init(void)
 {
+	{
+		std::lock_guard<std::mutex> lock(g_tune_mu);
+		tune_cache_load();
+	}
 	int n = 0;
 	hipError_t e = hipGetDeviceCount(&n);
 	if (e != hipSuccess) return set_err(PK_EDEVICE, "hipGetDeviceCount failed: %s", hipGetErrorString(e));
 	return PK_OK;
 }
 
-void pk_finish(void) {}
+void pk_finish(void)
+{
+	std::lock_guard<std::mutex> lock(g_tune_mu);
+	if (g_tune_dirty && tune_cache_save() == PK_OK) g_tune_dirty = false;
+}
+
+int pk_clear_cache(void)
+{
+	std::lock_guard<std::mutex> lock(g_tune_mu);
+	g_tune_cache.clear();
+	g_tune_dirty = false;
+	const std::string f = tune_cache_file();
+	if (!f.empty()) (void)std::remove(f.c_str());
+	return PK_OK;
+}
 
 const char *pk_last_error(void) { return g_err.c_str(); }
 
@@ -168,6 +264,9 @@ int pk_ctx_create(int device, int precision, pk_ctx **ctx_out)
 	PK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb_last, refine32_kernel<1, true, true>, 64, 0));
 	ctx->r32g_grid_main = (uint32_t)std::max(1, nb_main * prop.multiProcessorCount);
 	ctx->r32g_grid_last = (uint32_t)std::max(1, nb_last * prop.multiProcessorCount);
+	ctx->arch = std::string(prop.gcnArchName) + "/" + std::to_string(prop.multiProcessorCount);
+	for (auto &ch : ctx->arch)
+		if (ch == ' ' || ch == '\t' || ch == '\n') ch = '_';
 	*ctx_out = ctx;
 	return PK_OK;
 }
@@ -516,8 +615,18 @@ void psf_conv_layout(const double *krn, uint32_t kw, uint32_t kh, T *kt)
 
 template <typename T>
 int launch_conv(hipStream_t s, const PkJob *djobs, uint32_t njobs, uint32_t ntiles, const T *src, const T *kt,
-                const uint8_t *masks, T *dst, uint32_t max_kw, uint32_t max_kh, const uint32_t *tile_job = nullptr)
+                const uint8_t *masks, T *dst, uint32_t max_kw, uint32_t max_kh, const uint32_t *tile_job = nullptr,
+                bool mfma = false)
 {
+	if (mfma && sizeof(T) == 4 && conv_mfma_lds_bytes((int)max_kw, (int)max_kh) <= LDS_LIMIT) {
+		// FP32 on request: the matrix-core form, whole PSF in one launch
+		hipLaunchKernelGGL(conv_mfma_kernel, dim3(ntiles), dim3(256),
+		                   conv_mfma_lds_bytes((int)max_kw, (int)max_kh), s, djobs, njobs,
+		                   reinterpret_cast<const float *>(src), reinterpret_cast<const float *>(kt), masks,
+		                   reinterpret_cast<float *>(dst), tile_job);
+		PK_HIP(hipGetLastError());
+		return PK_OK;
+	}
 	uint32_t rows;
 	PK_TRY(conv_rows_per_launch(max_kw, max_kh, sizeof(T), rows));
 	const size_t lds = conv_lds_bytes(max_kw, rows, sizeof(T));
@@ -650,7 +759,7 @@ int launch_profiles(hipStream_t s, bool fp32, const PkJob *djobs, uint32_t njobs
                     const PkComp *dcomp, const CompLite *dlite, const uint8_t *dmask, TI *img_nc, TI *img_c,
                     RefineEntry *dent, PkCounters *dcnt, unsigned long long *dacc, RCell *dq, uint32_t qcap,
                     uint32_t ref_blocks, uint32_t grid_main, uint32_t grid_last, bool any_comp, hipEvent_t ev_coarse,
-                    const uint32_t *tile_job = nullptr)
+                    const uint32_t *tile_job = nullptr, R32Tune tune = R32_TUNE_DEFAULT)
 {
 	if (fp32) {
 		hipLaunchKernelGGL((coarse_kernel<TI, true>), dim3(ntiles), dim3(256), 0, s, djobs, njobs, dcomp, dmask, img_nc,
@@ -667,12 +776,13 @@ int launch_profiles(hipStream_t s, bool fp32, const PkJob *djobs, uint32_t njobs
 		RCell *q0 = dq, *q1 = dq + qcap;
 		const uint32_t gm = std::min(grid_main, ref_blocks), gl = std::min(grid_last, ref_blocks);
 		hipLaunchKernelGGL((refine32_kernel<0, false>), dim3(gm), dim3(64), 0, s, djobs, dcomp, dlite, dent, dcnt,
-		                   (const RCell *)nullptr, q0, qcap, 0, dacc);
+		                   (const RCell *)nullptr, q0, qcap, 0, dacc, tune);
 		for (int p = 1; p < R32_PASSES - 1; p++)
 			hipLaunchKernelGGL((refine32_kernel<1, false>), dim3(gm), dim3(64), 0, s, djobs, dcomp, dlite, dent, dcnt,
-			                   (const RCell *)((p & 1) ? q0 : q1), (p & 1) ? q1 : q0, qcap, p, dacc);
+			                   (const RCell *)((p & 1) ? q0 : q1), (p & 1) ? q1 : q0, qcap, p, dacc, tune);
 		hipLaunchKernelGGL((refine32_kernel<1, true>), dim3(gl), dim3(64), 0, s, djobs, dcomp, dlite, dent, dcnt,
-		                   (const RCell *)((R32_PASSES & 1) ? q1 : q0), (RCell *)nullptr, qcap, R32_PASSES - 1, dacc);
+		                   (const RCell *)((R32_PASSES & 1) ? q1 : q0), (RCell *)nullptr, qcap, R32_PASSES - 1, dacc,
+		                   tune);
 		hipLaunchKernelGGL(finalize32_kernel<TI>, dim3(1024), dim3(256), 0, s, djobs, dent, dcnt, dacc, img_nc, img_c);
 	} else {
 		hipLaunchKernelGGL((refine_kernel<TI, false>), dim3(ref_blocks), dim3(64), 0, s, djobs, dcomp, dent, dcnt,
@@ -1250,6 +1360,19 @@ double pk_integrate_qagi(pk_integration_func_t f, double a, void *params)
 // ---------------------------------------------------------------------------
 // batched, device-resident evaluation
 // ---------------------------------------------------------------------------
+static pk_tuning default_tuning()
+{
+	pk_tuning t{};
+	t.conv_tiles = PK_CONV_TILES_VALU;
+	t.refine_occupancy = 100;
+	t.handover_div = R32_TUNE_DEFAULT.handover_div;
+	t.claim_pixels = R32_TUNE_DEFAULT.chunk;
+	t.claim_cells = R32_TUNE_DEFAULT.chunk_cells;
+	t.poll_rounds = R32_TUNE_DEFAULT.poll;
+	t.ms_evaluate = 0;
+	return t;
+}
+
 struct pk_batch {
 	pk_ctx *ctx = nullptr;
 	uint32_t flags = 0;
@@ -1280,6 +1403,12 @@ struct pk_batch {
 	hipGraph_t graph_grad = nullptr;
 	hipGraphExec_t exec_grad = nullptr;
 	hipEvent_t ev_grad[2] = {};
+	// launch settings (autotuner)
+	pk_tuning tune = default_tuning();
+	bool tuned = false;
+	uint64_t n_pixels = 0;  // drawing pixels of all models
+	R32Tune r32() const { return R32Tune{tune.handover_div, tune.claim_pixels, tune.claim_cells, tune.poll_rounds}; }
+	uint32_t grid(uint32_t full) const { return std::max(1u, (uint32_t)((uint64_t)full * tune.refine_occupancy / 100)); }
 	template <typename T> T *d(size_t off) const
 	{
 		return reinterpret_cast<T *>(static_cast<char *>(dmem) + off);
@@ -1315,12 +1444,12 @@ int batch_launch(pk_batch *b)
 		PK_TRY(launch_profiles<float>(s, true, djobs, nj, b->ntiles, dcomp, b->d<CompLite>(b->o_lite), dmask, nc, c,
 		                              dent, dcnt,
 		                              b->d<unsigned long long>(b->o_acc), b->d<RCell>(b->o_q), b->qcap, UINT32_MAX,
-		                              b->ctx->r32_grid_main, b->ctx->r32_grid_last,
-		                              b->n_comps > 0, timed ? b->ev[2] : nullptr, tj_coarse));
+		                              b->grid(ctx->r32_grid_main), b->grid(ctx->r32_grid_last),
+		                              b->n_comps > 0, timed ? b->ev[2] : nullptr, tj_coarse, b->r32()));
 		if (timed) PK_HIP(hipEventRecord(b->ev[3], s));
 		if (b->any_conv) {
 			PK_TRY(launch_conv<float>(s, djobs, nj, b->nctiles, c, b->d<float>(b->o_psfT), dmask, b->d<float>(b->o_conv),
-			                          b->max_kw, b->max_kh, tj_conv));
+			                          b->max_kw, b->max_kh, tj_conv, b->tune.conv_tiles == PK_CONV_TILES_MFMA));
 			PK_HIP(hipGetLastError());
 		}
 		if (timed) PK_HIP(hipEventRecord(b->ev[4], s));
@@ -1371,7 +1500,7 @@ int grad_launch_t(pk_batch *b)
 	PK_HIP(hipGetLastError());
 	if (b->any_conv)
 		PK_TRY(launch_conv<T>(s, djobs, nj, b->nctiles, Gd, b->d<T>(b->o_psfF), b->d<uint8_t>(b->o_mask), Gc,
-		                      b->max_kw, b->max_kh, tj_conv));
+		                      b->max_kw, b->max_kh, tj_conv, b->tune.conv_tiles == PK_CONV_TILES_MFMA));
 	const bool fp32 = sizeof(T) == 4;
 	hipLaunchKernelGGL((coarse_grad_kernel<T, fp32>), dim3(b->ntiles), dim3(256), 0, s, djobs, nj,
 	                   (const PkComp *)dcomp, (const uint8_t *)b->d<uint8_t>(b->o_mask), (const T *)Gd,
@@ -1381,17 +1510,19 @@ int grad_launch_t(pk_batch *b)
 	if (fp32) {
 		const CompLite *dlite = b->d<CompLite>(b->o_lite);
 		RCell *q0 = b->d<RCell>(b->o_q), *q1 = q0 + b->qcap;
-		const uint32_t gm = ctx->r32g_grid_main, gl = ctx->r32g_grid_last;
+		const uint32_t gm = b->grid(ctx->r32g_grid_main), gl = b->grid(ctx->r32g_grid_last);
 		const float *Gf = (const float *)Gd, *Gcf = (const float *)Gc;
+		const R32Tune hd = b->r32();
 		hipLaunchKernelGGL((refine32_kernel<0, false, true>), dim3(gm), dim3(64), 0, s, djobs, dcomp, dlite, dent,
-		                   gcnt, (const RCell *)nullptr, q0, b->qcap, 0, (unsigned long long *)nullptr, Gf, Gcf, gacc);
+		                   gcnt, (const RCell *)nullptr, q0, b->qcap, 0, (unsigned long long *)nullptr, hd, Gf, Gcf,
+		                   gacc);
 		for (int p = 1; p < R32_PASSES - 1; p++)
 			hipLaunchKernelGGL((refine32_kernel<1, false, true>), dim3(gm), dim3(64), 0, s, djobs, dcomp, dlite, dent,
 			                   gcnt, (const RCell *)((p & 1) ? q0 : q1), (p & 1) ? q1 : q0, b->qcap, p,
-			                   (unsigned long long *)nullptr, Gf, Gcf, gacc);
+			                   (unsigned long long *)nullptr, hd, Gf, Gcf, gacc);
 		hipLaunchKernelGGL((refine32_kernel<1, true, true>), dim3(gl), dim3(64), 0, s, djobs, dcomp, dlite, dent, gcnt,
 		                   (const RCell *)((R32_PASSES & 1) ? q1 : q0), (RCell *)nullptr, b->qcap, R32_PASSES - 1,
-		                   (unsigned long long *)nullptr, Gf, Gcf, gacc);
+		                   (unsigned long long *)nullptr, hd, Gf, Gcf, gacc);
 	} else {
 		hipLaunchKernelGGL((refine_kernel<double, false, true>), dim3(4096), dim3(64), 0, s, djobs,
 		                   (const PkComp *)dcomp, dent, gcnt, (double *)nullptr, (double *)nullptr,
@@ -1411,6 +1542,119 @@ int grad_launch(pk_batch *b)
 	return b->elem == 4 ? grad_launch_t<float>(b) : grad_launch_t<double>(b);
 }
 
+// ---- autotuner ------------------------------------------------------------------
+std::string tune_key(const pk_batch *b)
+{
+	char k[512];
+	std::snprintf(k, sizeof k, "%s fp%d grad%d models%u pixels%llu psf%ux%u comps%u", b->ctx->arch.c_str(),
+	              b->elem == 4 ? 32 : 64, (b->flags & PK_BATCH_GRAD) ? 1 : 0, b->n_models,
+	              (unsigned long long)b->n_pixels, b->max_kw, b->max_kh, b->n_comps);
+	return k;
+}
+
+void drop_graphs(pk_batch *b)
+{
+	if (b->exec) (void)hipGraphExecDestroy(b->exec);
+	if (b->graph) (void)hipGraphDestroy(b->graph);
+	if (b->exec_grad) (void)hipGraphExecDestroy(b->exec_grad);
+	if (b->graph_grad) (void)hipGraphDestroy(b->graph_grad);
+	b->exec = nullptr;
+	b->graph = nullptr;
+	b->exec_grad = nullptr;
+	b->graph_grad = nullptr;
+}
+
+// One eager evaluation with settings t; stage[] = device ms of set-up,
+// coarse, refinement, convolution, composition, total.  +inf stage times
+// when the refinement queue overflowed with these settings.
+int tune_run(pk_batch *b, const pk_tuning &t, float stage[6])
+{
+	b->tune = t;
+	const uint32_t flags = b->flags;
+	b->flags &= ~(uint32_t)PK_BATCH_GRAPH;  // eager launches record the stage events
+	const int st = batch_launch(b);
+	b->flags = flags;
+	PK_TRY(st);
+	PK_HIP(hipStreamSynchronize(b->ctx->stream));
+	for (int k = 0; k < 5; k++) PK_HIP(hipEventElapsedTime(&stage[k], b->ev[k], b->ev[k + 1]));
+	PK_HIP(hipEventElapsedTime(&stage[5], b->ev[0], b->ev[5]));
+	PkCounters cnt{};
+	PK_HIP(hipMemcpy(&cnt, b->d<void>(b->o_cnt), sizeof(cnt), hipMemcpyDeviceToHost));
+	if (cnt.overflow)
+		for (int k = 0; k < 6; k++) stage[k] = INFINITY;
+	return PK_OK;
+}
+
+// Coordinate search: for each setting in turn, every candidate value is run
+// twice (the faster run counts) with the others at their best so far, timed
+// on the stage that setting acts on.  The last run uses the chosen settings,
+// so the batch holds its images.
+int autotune(pk_batch *b)
+{
+	const bool fp32 = b->elem == 4;
+	float st[6];
+	pk_tuning best = default_tuning();
+	PK_TRY(tune_run(b, best, st));  // first launches load the code objects
+	auto measure = [&](const pk_tuning &t, int stage, float &ms) -> int {
+		float a[6], c[6];
+		PK_TRY(tune_run(b, t, a));
+		PK_TRY(tune_run(b, t, c));
+		ms = std::min(a[stage], c[stage]);
+		return PK_OK;
+	};
+	auto search = [&](uint32_t pk_tuning::*field, std::initializer_list<uint32_t> values, int stage) -> int {
+		float best_ms = INFINITY;
+		uint32_t best_v = best.*field;
+		for (uint32_t v : values) {
+			pk_tuning t = best;
+			t.*field = v;
+			float ms;
+			PK_TRY(measure(t, stage, ms));
+			if (ms < best_ms) {
+				best_ms = ms;
+				best_v = v;
+			}
+		}
+		best.*field = best_v;
+		return PK_OK;
+	};
+	if (fp32 && b->any_conv && conv_mfma_lds_bytes((int)b->max_kw, (int)b->max_kh) <= LDS_LIMIT)
+		PK_TRY(search(&pk_tuning::conv_tiles, {PK_CONV_TILES_VALU, PK_CONV_TILES_MFMA}, 3));
+	if (fp32 && b->n_comps) {
+		PK_TRY(search(&pk_tuning::handover_div, {2, 4, 8, 16}, 2));
+		PK_TRY(search(&pk_tuning::refine_occupancy, {50, 75, 100}, 2));
+		PK_TRY(search(&pk_tuning::claim_pixels, {16, 32, 64, 128}, 2));
+		PK_TRY(search(&pk_tuning::claim_cells, {4, 16, 64}, 2));
+		PK_TRY(search(&pk_tuning::poll_rounds, {4, 16, 64}, 2));
+	}
+	PK_TRY(tune_run(b, best, st));
+	best.ms_evaluate = st[5];
+	b->tune = best;
+	b->tuned = true;
+	drop_graphs(b);
+	std::lock_guard<std::mutex> lock(g_tune_mu);
+	g_tune_cache[tune_key(b)] = best;
+	g_tune_dirty = true;
+	return PK_OK;
+}
+
+// Settings for the first evaluation of a PK_BATCH_AUTOTUNE batch: the cached
+// ones for this device and workload, else a tuning run.
+int ensure_tuned(pk_batch *b)
+{
+	if (b->tuned || !(b->flags & PK_BATCH_AUTOTUNE)) return PK_OK;
+	{
+		std::lock_guard<std::mutex> lock(g_tune_mu);
+		auto it = g_tune_cache.find(tune_key(b));
+		if (it != g_tune_cache.end()) {
+			b->tune = it->second;
+			b->tuned = true;
+			return PK_OK;
+		}
+	}
+	return autotune(b);
+}
+
 }  // namespace
 
 extern "C" {
@@ -1420,10 +1664,7 @@ int pk_batch_destroy(pk_batch *b)
 	if (!b) return PK_OK;
 	(void)hipSetDevice(b->ctx->device);
 	(void)hipStreamSynchronize(b->ctx->stream);
-	if (b->exec) (void)hipGraphExecDestroy(b->exec);
-	if (b->graph) (void)hipGraphDestroy(b->graph);
-	if (b->exec_grad) (void)hipGraphExecDestroy(b->exec_grad);
-	if (b->graph_grad) (void)hipGraphDestroy(b->graph_grad);
+	drop_graphs(b);
 	for (auto &e : b->ev_grad)
 		if (e) (void)hipEventDestroy(e);
 	for (auto &e : b->ev)
@@ -1437,6 +1678,8 @@ int pk_batch_create(pk_ctx *ctx, const pk_model *models, uint32_t n_models, uint
 {
 	if (!ctx || !out || (!models && n_models)) return set_err(PK_EINVAL, "NULL argument");
 	if (n_models == 0) return set_err(PK_EINVAL, "empty batch");
+	if (flags & ~(uint32_t)(PK_BATCH_GRAPH | PK_BATCH_GRAD | PK_BATCH_AUTOTUNE))
+		return set_err(PK_EINVAL, "unknown batch flags 0x%x", flags);
 	std::lock_guard<std::mutex> lock(ctx->mu);
 	PK_HIP(hipSetDevice(ctx->device));
 	const bool fp32 = ctx->precision == PK_FP32;
@@ -1569,6 +1812,7 @@ int pk_batch_create(pk_ctx *ctx, const pk_model *models, uint32_t n_models, uint
 		out_off += (uint64_t)ow * oh;
 	}
 	b->n_comps = (uint32_t)b->setup.size();
+	b->n_pixels = img_off;
 	b->total_out = out_off;
 	{
 		uint32_t rows;
@@ -1659,6 +1903,39 @@ int pk_batch_create(pk_ctx *ctx, const pk_model *models, uint32_t n_models, uint
 	return PK_OK;
 }
 
+int pk_batch_autotune(pk_batch *b)
+{
+	if (!b) return set_err(PK_EINVAL, "batch is NULL");
+	std::lock_guard<std::mutex> lock(b->ctx->mu);
+	PK_HIP(hipSetDevice(b->ctx->device));
+	PK_TRY(autotune(b));
+	b->ran = true;
+	return PK_OK;
+}
+
+int pk_batch_get_tuning(const pk_batch *b, pk_tuning *t)
+{
+	if (!b || !t) return set_err(PK_EINVAL, "NULL argument");
+	*t = b->tune;
+	return PK_OK;
+}
+
+int pk_batch_set_tuning(pk_batch *b, const pk_tuning *t)
+{
+	if (!b || !t) return set_err(PK_EINVAL, "NULL argument");
+	if (!tuning_valid(*t))
+		return set_err(PK_EINVAL, "invalid tuning (conv_tiles %u, occupancy %u, handover %u, claims %u/%u, poll %u)",
+		               t->conv_tiles, t->refine_occupancy, t->handover_div, t->claim_pixels, t->claim_cells,
+		               t->poll_rounds);
+	std::lock_guard<std::mutex> lock(b->ctx->mu);
+	PK_HIP(hipSetDevice(b->ctx->device));
+	PK_HIP(hipStreamSynchronize(b->ctx->stream));
+	drop_graphs(b);
+	b->tune = *t;
+	b->tuned = true;
+	return PK_OK;
+}
+
 int pk_batch_set_data(pk_batch *b, const double *data, const double *sigma, size_t n)
 {
 	if (!b || !data) return set_err(PK_EINVAL, "NULL argument");
@@ -1694,6 +1971,7 @@ int pk_batch_evaluate_grad(pk_batch *b)
 	if (!b->have_data) return set_err(PK_EINVAL, "no data: call pk_batch_set_data first");
 	std::lock_guard<std::mutex> lock(b->ctx->mu);
 	PK_HIP(hipSetDevice(b->ctx->device));
+	PK_TRY(ensure_tuned(b));
 	b->ran = true;
 	b->ran_grad = true;
 	if (!(b->flags & PK_BATCH_GRAPH)) return grad_launch(b);
@@ -1778,6 +2056,7 @@ int pk_batch_evaluate(pk_batch *b)
 	if (!b) return set_err(PK_EINVAL, "batch is NULL");
 	std::lock_guard<std::mutex> lock(b->ctx->mu);
 	PK_HIP(hipSetDevice(b->ctx->device));
+	PK_TRY(ensure_tuned(b));
 	b->ran = true;
 	if (!(b->flags & PK_BATCH_GRAPH)) return batch_launch(b);
 	if (!b->exec) {

@@ -550,6 +550,106 @@ __global__ __launch_bounds__(256) void conv_kernel(const PkJob *__restrict__ job
 }
 
 // ---------------------------------------------------------------------------
+// direct convolution on the matrix cores (FP32 mode)
+//
+// For one PSF row l, a 16 x 16 block of outputs is a matrix product
+//   out[y][x] += sum_j A_l[y][j] B_l[j][x],
+//   A_l[y][j] = tile(y + l, x0 + j)   (a window of the source tile in LDS),
+//   B_l[j][x] = K'(j - x, l)          (the PSF row as a band, 0 off the band),
+// with j over the 16 + kw - 1 source columns the block reads: the direct
+// convolution recast as a tiled contraction for v_mfma_f32_16x16x4_f32
+// (exact FP32 products).  The band costs (16 + kw - 1) / kw times the useful
+// multiply-adds.  The FP32 accumulators of each PSF row are added in FP64, so
+// a row of <= (16 + kw - 1) products is the only FP32 rounding chain, as in
+// conv_kernel.  Same tiles, geometry and mask as conv_kernel.
+// Lane layout of the 16x16x4 FP32 product: A[i][k] from lane i + 16 k,
+// B[k][j] from lane j + 16 k, D[i][j] in lane j + 16 (i / 4), register i % 4.
+// ---------------------------------------------------------------------------
+typedef float floatx4 __attribute__((ext_vector_type(4)));
+
+// odd row stride of the LDS source tile: the 16 rows one operand read touches
+// fall in distinct banks
+__host__ __device__ inline int conv_mfma_stride(int kw) { return (CT_W + kw - 1) | 1; }
+
+__host__ __device__ inline size_t conv_mfma_lds_bytes(int kw, int kh)
+{
+	return sizeof(float) * ((size_t)conv_mfma_stride(kw) * (CT_H + kh - 1) + (size_t)kw * kh);
+}
+
+__global__ __launch_bounds__(256) void conv_mfma_kernel(const PkJob *__restrict__ jobs, uint32_t njobs,
+                                                        const float *__restrict__ src_all,
+                                                        const float *__restrict__ krn_all,
+                                                        const uint8_t *__restrict__ masks,
+                                                        float *__restrict__ dst_all,
+                                                        const uint32_t *__restrict__ tile_job)
+{
+	extern __shared__ unsigned char smem_raw[];
+	float *tile = reinterpret_cast<float *>(smem_raw);
+	const uint32_t jn = find_job(jobs, njobs, blockIdx.x, 1, tile_job);
+	const PkJob &jb = jobs[jn];
+	const uint32_t t = blockIdx.x - jb.ctile_off;
+	const int bx = (int)(t % jb.ctiles_x) * CT_W, by = (int)(t / jb.ctiles_x) * CT_H;
+	const float *src = src_all + jb.img_off;
+	const float *krn = krn_all + jb.psf_off;
+	float *dst = dst_all + jb.conv_off;
+	const int W = (int)jb.dw, H = (int)jb.dh, kw = (int)jb.kw, kh = (int)jb.kh;
+	const int hw = kw / 2, hh = kh / 2;
+	const int ox = bx + jb.csx, oy = by + jb.csy;
+	const int TW = CT_W + kw - 1, TH = CT_H + kh - 1, TS = conv_mfma_stride(kw);
+	float *kt = tile + (size_t)TS * TH;
+	for (int e = threadIdx.x; e < TW * TH; e += blockDim.x) {
+		const int ty = e / TW, tx = e - ty * TW;
+		const int si = ox + tx - hw, sj = oy + ty - hh;
+		float v = 0.f;
+		if (si >= 0 && si < W && sj >= 0 && sj < H) v = src[si + (size_t)sj * W];
+		tile[ty * TS + tx] = v;
+	}
+	for (int e = threadIdx.x; e < kw * kh; e += blockDim.x) kt[e] = krn[e];  // kt[k kh + l] = K'(k, l)
+	__syncthreads();
+
+	const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+	const int ry0 = (w >> 1) * 16, cx0 = (w & 1) * 32;  // this wave: rows ry0.., columns cx0.. (two 16 x 16 blocks)
+	const int lo = lane & 15, kq = lane >> 4;
+	const int J = 16 + kw - 1, nj = (J + 3) / 4;
+	double out[2][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
+	for (int l = 0; l < kh; l++) {
+		floatx4 acc0 = {0.f, 0.f, 0.f, 0.f}, acc1 = {0.f, 0.f, 0.f, 0.f};
+		const float *arow = tile + (size_t)(ry0 + lo + l) * TS + cx0;  // A row i = lo
+		for (int jc = 0; jc < nj; jc++) {
+			const int j = 4 * jc + kq;  // A column / B row of this lane
+			const int d = j - lo;       // B[j][lo] = K'(j - lo, l)
+			float a0 = 0.f, a1 = 0.f, b = 0.f;
+			if (j < J) {
+				a0 = arow[j];
+				a1 = arow[16 + j];
+			}
+			if (d >= 0 && d < kw) b = kt[d * kh + l];
+			acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, b, acc0, 0, 0, 0);
+			acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b, acc1, 0, 0, 0);
+		}
+#pragma unroll
+		for (int r = 0; r < 4; r++) {
+			out[0][r] += (double)acc0[r];
+			out[1][r] += (double)acc1[r];
+		}
+	}
+#pragma unroll
+	for (int s2 = 0; s2 < 2; s2++)
+#pragma unroll
+		for (int r = 0; r < 4; r++) {
+			const int X = bx + cx0 + 16 * s2 + lo, Y = by + ry0 + 4 * kq + r;
+			if (X < (int)jb.cw && Y < (int)jb.ch) {
+				float v = (float)out[s2][r];
+				if (jb.mask_off != NONE) {
+					const int mi = X - jb.cmx, mj = Y - jb.cmy;
+					if (mi < 0 || mj < 0 || mi >= W || mj >= H || !masks[jb.mask_off + mi + (size_t)mj * W]) v = 0.f;
+				}
+				dst[X + (size_t)Y * jb.cw] = v;
+			}
+		}
+}
+
+// ---------------------------------------------------------------------------
 // composition: returned(i,j) = sum over the fs x fs block of
 //   conv(X,Y) + model(X - mox, Y - moy), (X,Y) = crop origin + fine position
 // ---------------------------------------------------------------------------

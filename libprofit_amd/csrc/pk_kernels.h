@@ -51,9 +51,19 @@ struct RefineEntry {
 };
 
 // The FP32-mode refinement runs in R32_PASSES launches: a launch hands what
-// its wavefronts still hold to the next one once a quarter of them have run
-// out of work, so that no long tail of single wavefronts remains.
+// its wavefronts still hold to the next one once a share of them (a quarter by
+// default) have run out of work, so that no long tail of single wavefronts
+// remains.
 constexpr int R32_PASSES = 4;
+
+// Run-time settings of the FP32 refinement launches (chosen per batch by the
+// autotuner; any values >= 1 give the same result, only the speed changes).
+struct R32Tune {
+	uint32_t handover_div;  // hand over once 1/handover_div of a pass's wavefronts stopped
+	uint32_t chunk;         // pixels per claim at most (pass 0)
+	uint32_t chunk_cells;   // handed-over cells per claim at most (later passes)
+	uint32_t poll;          // rounds between two checks of the hand-over condition
+};
 
 struct PkCounters {
 	uint32_t n_entries;    // refinement entries appended by the coarse pass

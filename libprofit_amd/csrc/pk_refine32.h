@@ -14,9 +14,9 @@
 // Load balance: the refinement runs as R32_PASSES launches.  Pass 0 takes
 // pixels from the coarse pass's entry list, pass p > 0 the cells handed over
 // by pass p - 1.  Work is claimed in chunks that shrink as the source runs
-// dry.  Once a quarter of a pass's wavefronts have stopped, the others hand
-// everything they still hold (their LIFO and unclaimed chunk) to the next
-// pass and stop too, so a few deep pixels never keep a lone wavefront busy
+// dry.  Once a set share of a pass's wavefronts (R32Tune::handover_div; a
+// quarter by default) have stopped, the others hand everything they still
+// hold (their LIFO and unclaimed chunk) to the next pass and stop too, so a few deep pixels never keep a lone wavefront busy
 // while the rest of the GPU idles.  The last pass finishes all that is left,
 // with a deeper LIFO.  A LIFO that fills up before that also hands its new
 // cells to the next pass.
@@ -34,11 +34,10 @@ namespace pk {
 // and the Sersic centre pixels (res 8, 10 levels).
 constexpr int R32_CAP_MAIN = 192;
 constexpr int R32_CAP_LAST = 1024;
-constexpr uint32_t WQ_CHUNK = 64;         // pixels per claim at most
-constexpr uint32_t WQ_CHUNK_CELLS = 16;   // handed-over cells per claim at most
 constexpr int R32_TASKS = 16;              // cells per round at most (their records are staged in LDS)
-constexpr int R32_POLL = 16;               // rounds between two checks of the hand-over condition
-constexpr uint32_t R32_HANDOVER_DIV = 4;   // hand over once 1/R32_HANDOVER_DIV of the wavefronts stopped
+// untuned settings: claims of up to 64 pixels / 16 cells, the hand-over test
+// every 16 rounds, hand-over once a quarter of the wavefronts stopped
+constexpr R32Tune R32_TUNE_DEFAULT = {4u, 64u, 16u, 16u};
 
 template <int CAP>
 struct R32Stack {
@@ -258,6 +257,7 @@ __global__ __launch_bounds__(64) void refine32_kernel(const PkJob *__restrict__ 
                                                       PkCounters *__restrict__ cnt, const RCell *__restrict__ qin,
                                                       RCell *__restrict__ qout, uint32_t qcap, int pass,
                                                       unsigned long long *__restrict__ acc,
+                                                      R32Tune tune,
                                                       const float *__restrict__ G_nc = nullptr,
                                                       const float *__restrict__ G_c = nullptr,
                                                       double *__restrict__ gacc = nullptr)
@@ -291,7 +291,7 @@ __global__ __launch_bounds__(64) void refine32_kernel(const PkJob *__restrict__ 
 			if (wave_sum_u32(r2) >= 64u || source_done) break;
 			if (CAP - top < (SOURCE == 0 ? 32 : 1)) break;
 			if (chunk_next >= chunk_end &&
-			    !r32_claim(head, n_source, SOURCE == 0 ? WQ_CHUNK : WQ_CHUNK_CELLS, seen, chunk_next, chunk_end,
+			    !r32_claim(head, n_source, SOURCE == 0 ? tune.chunk : tune.chunk_cells, seen, chunk_next, chunk_end,
 			               lane)) {
 				source_done = true;
 				break;
@@ -304,11 +304,11 @@ __global__ __launch_bounds__(64) void refine32_kernel(const PkJob *__restrict__ 
 			__syncthreads();
 		}
 		if (top == 0) break;
-		if (!LAST && ++poll >= R32_POLL) {
+		if (!LAST && ++poll >= (int)tune.poll) {
 			poll = 0;
 			uint32_t d = 0;
 			if (lane == 0) d = __hip_atomic_load(&cnt->n_done[pass], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-			if (wave_read(d, 0) * R32_HANDOVER_DIV >= gridDim.x) {
+			if (wave_read(d, 0) * tune.handover_div >= gridDim.x) {
 				handover = true;
 				break;
 			}

@@ -65,7 +65,7 @@ bool has_gpu()
 	int n = 0;
 	return pk_device_count(&n) == PK_OK && n > 0;
 }
-void clear_cache() {}
+void clear_cache() { check(pk_clear_cache()); }
 
 GpuEnv::GpuEnv(int device, Precision precision) : device_(device), precision_(precision)
 {

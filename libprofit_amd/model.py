@@ -47,6 +47,25 @@ class Context:
             pass
 
 
+def init():
+    """profit::init() (include/profit/library.h:67-70): checks for a device
+    and loads the autotuner cache."""
+    check(load().pk_init())
+    return True
+
+
+def finish():
+    """profit::finish() (include/profit/library.h:78-80): writes the
+    autotuner cache back."""
+    load().pk_finish()
+
+
+def clear_cache():
+    """profit::clear_cache() (include/profit/library.h, src/library.cpp:251-270):
+    forgets the autotuner cache and removes its file."""
+    check(load().pk_clear_cache())
+
+
 _default_ctx = {}
 
 
@@ -289,14 +308,15 @@ class Batch:
     all images produced by one launch sequence, optionally a hipGraph; with
     grad=True also chi^2 against set data and its gradient."""
 
-    def __init__(self, ctx, specs, graph=True, grad=False):
+    def __init__(self, ctx, specs, graph=True, grad=False, autotune=False):
         self.ctx = ctx
         self.L = load()
         self.specs = list(specs)  # keeps the host buffers alive
         arr = (abi.Model * len(self.specs))(*[s.m for s in self.specs])
         self._arr = arr
         h = C.c_void_p()
-        flags = (abi.PK_BATCH_GRAPH if graph else 0) | (abi.PK_BATCH_GRAD if grad else 0)
+        flags = ((abi.PK_BATCH_GRAPH if graph else 0) | (abi.PK_BATCH_GRAD if grad else 0)
+                 | (abi.PK_BATCH_AUTOTUNE if autotune else 0))
         check(self.L.pk_batch_create(ctx.handle, arr, len(self.specs), flags, C.byref(h)))
         self.handle = h
         self.n = len(self.specs)
@@ -338,6 +358,26 @@ class Batch:
 
     def evaluate(self):
         check(self.L.pk_batch_evaluate(self.handle))
+
+    def autotune(self):
+        """Times candidate launch settings, keeps the fastest (pk_batch_autotune)."""
+        check(self.L.pk_batch_autotune(self.handle))
+        return self.tuning
+
+    @property
+    def tuning(self):
+        t = abi.Tuning()
+        check(self.L.pk_batch_get_tuning(self.handle, C.byref(t)))
+        return t
+
+    @tuning.setter
+    def tuning(self, t):
+        if isinstance(t, dict):
+            cur = self.tuning
+            for k, v in t.items():
+                setattr(cur, k, v)
+            t = cur
+        check(self.L.pk_batch_set_tuning(self.handle, C.byref(t)))
 
     def sync(self):
         info = abi.BatchInfo()

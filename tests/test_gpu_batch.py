@@ -80,3 +80,111 @@ def test_set_components(ctx64, oracle):
         pix, _ = rel_errors(b.image(i), ref)
         assert pix <= TOL_FP64, (i, pix)
     b.close()
+
+
+DEFAULT_TUNING = {"conv_tiles": 0, "refine_occupancy": 100, "handover_div": 4, "claim_pixels": 64,
+                  "claim_cells": 16, "poll_rounds": 16}
+
+
+def images(b, n):
+    return [b.image(i) for i in range(n)]
+
+
+def assert_same_to_fp32_rounding(got, ref):
+    # the MFMA convolution tiles sum the PSF products in another order
+    for i, (g, r) in enumerate(zip(got, ref)):
+        peak = np.max(np.abs(r))
+        assert np.max(np.abs(g - r)) <= 2e-6 * peak, i
+
+
+def test_tuning_settings_keep_the_images(ctx32):
+    """The refinement settings only move work between wavefronts: the
+    fixed-point sums make the images bit-identical.  The MFMA convolution
+    tiles change the FP32 rounding only."""
+    print("CASE tuning settings", flush=True)
+    from libprofit_amd._lib import InvalidParameter
+    specs = small_batch(16, 11)
+    b = Batch(ctx32, specs, graph=True)
+    t0 = b.tuning.as_dict()
+    assert {k: t0[k] for k in DEFAULT_TUNING} == DEFAULT_TUNING
+    b.evaluate()
+    b.sync()
+    ref = images(b, len(specs))
+    for setting in ({"handover_div": 2, "refine_occupancy": 50, "claim_pixels": 16, "claim_cells": 4,
+                     "poll_rounds": 4},
+                    {"handover_div": 16, "refine_occupancy": 75, "claim_pixels": 128, "claim_cells": 64,
+                     "poll_rounds": 64},
+                    {"handover_div": 1, "refine_occupancy": 1}):
+        print("CASE tuning", setting, flush=True)
+        b.tuning = {**DEFAULT_TUNING, **setting}
+        for _rep in range(2):
+            b.evaluate()
+            b.sync()
+            for i, r in enumerate(ref):
+                assert np.array_equal(b.image(i), r), (setting, i)
+    b.tuning = {**DEFAULT_TUNING, "conv_tiles": 1}
+    b.evaluate()
+    b.sync()
+    assert_same_to_fp32_rounding(images(b, len(specs)), ref)
+    for bad in ({"handover_div": 0}, {"refine_occupancy": 101}, {"conv_tiles": 2}, {"claim_pixels": 0}):
+        with pytest.raises(InvalidParameter):
+            b.tuning = {**DEFAULT_TUNING, **bad}
+    b.close()
+
+
+def test_autotune_and_its_cache(ctx32, tmp_path, monkeypatch):
+    print("CASE autotune", flush=True)
+    from libprofit_amd import model as M
+    home = tmp_path / "profit_home"
+    monkeypatch.setenv("PROFIT_HOME", str(home))
+    M.clear_cache()
+    specs = small_batch(16, 13)
+    plain = Batch(ctx32, specs, graph=False)
+    plain.evaluate()
+    plain.sync()
+    ref = images(plain, len(specs))
+    plain.close()
+
+    b = Batch(ctx32, specs, graph=True, autotune=True)
+    b.evaluate()  # tunes first, then captures and replays the graph
+    b.sync()
+    t = b.tuning.as_dict()
+    print("CASE tuned", t, flush=True)
+    assert t["ms_evaluate"] > 0
+    assert t["conv_tiles"] in (0, 1) and 1 <= t["refine_occupancy"] <= 100
+    for _rep in range(2):
+        got = images(b, len(specs))
+        if t["conv_tiles"] == 0:
+            for i, r in enumerate(ref):
+                assert np.array_equal(got[i], r), i
+        else:
+            assert_same_to_fp32_rounding(got, ref)
+        b.evaluate()
+        b.sync()
+
+    # the next batch of the same workload takes the cached settings
+    b2 = Batch(ctx32, specs, graph=False, autotune=True)
+    b2.evaluate()
+    b2.sync()
+    assert b2.tuning.as_dict() == t
+    b2.close()
+
+    # finish() writes the cache, init() reads it back after clear_cache()
+    M.finish()
+    f = home / "hip_tuning"
+    text = f.read_text()
+    lines = text.splitlines()
+    assert len(lines) == 1 and "fp32" in lines[0] and "models16" in lines[0], text
+    M.clear_cache()
+    assert not f.exists()
+    f.write_text(text)
+    M.init()
+    b3 = Batch(ctx32, specs, graph=False, autotune=True)
+    b3.evaluate()
+    b3.sync()
+    t3 = b3.tuning.as_dict()
+    assert abs(t3.pop("ms_evaluate") - t["ms_evaluate"]) <= 1e-5  # the file keeps 6 decimals
+    assert t3 == {k: v for k, v in t.items() if k != "ms_evaluate"}
+    b3.close()
+    b.close()
+    M.clear_cache()

@@ -150,6 +150,35 @@ def test_graph_replays_match_eager(ctx32, oracle):
     eager.close()
 
 
+def test_tuning_keeps_the_gradient(ctx32, oracle):
+    """Launch settings of the adjoint: the refinement settings reorder only the
+    FP64 atomic sums of the gradient; the MFMA convolution tiles change the
+    FP32 rounding of the PSF correlation."""
+    announce("grad tuning")
+    fm = small_fit()
+    data, sigma = make_data(oracle, fm)
+    b = Batch(ctx32, [fm.spec()], graph=True, grad=True)
+    b.set_data([data], [sigma])
+    b.evaluate_grad()
+    c0, g0 = b.chi2_grad()
+    scale = np.max(np.abs(g0), axis=1, keepdims=True)
+    base = {"conv_tiles": 0, "refine_occupancy": 100, "handover_div": 4, "claim_pixels": 64, "claim_cells": 16,
+            "poll_rounds": 16}
+    for setting, rtol in (({"handover_div": 2, "refine_occupancy": 50, "claim_pixels": 16, "poll_rounds": 4}, 1e-11),
+                          ({"conv_tiles": 1}, 1e-5)):
+        b.tuning = {**base, **setting}
+        b.evaluate_grad()
+        c1, g1 = b.chi2_grad()
+        assert np.allclose(c1, c0, rtol=rtol, atol=0), setting
+        assert np.all(np.abs(g1 - g0) <= rtol * scale), setting
+    b.autotune()
+    b.evaluate_grad()
+    c2, g2 = b.chi2_grad()
+    assert np.allclose(c2, c0, rtol=1e-5, atol=0)
+    assert np.all(np.abs(g2 - g0) <= 1e-5 * scale)
+    b.close()
+
+
 def test_sky_gradient_closed_form(ctx32, oracle):
     """d chi^2 / d bg = sum_p -2 (data_p - model_p) / sigma^2 for an unconvolved sky."""
     announce("sky closed form")
@@ -168,7 +197,11 @@ def test_sky_gradient_closed_form(ctx32, oracle):
 
 
 def test_fd_of_device_forward(ctx32, oracle):
-    """Central differences (relative step 1e-4) of the device FP32 chi^2 itself."""
+    """Central differences of the device FP32 chi^2 itself.  Their precision
+    is bounded by FP32 rounding of the images (eps ~ 6e-8) over the relative
+    change a step makes in them: a relative step of 1e-4 in the sky level moves
+    the pixels by ~1e-5 (FD noise ~0.6 %), so the step is 1e-3 and the
+    tolerance 1 % (the FP64 checks above are the precise ones)."""
     announce("fd of device forward")
     fm = small_fit()
     truth = FitModel(fm.size, [(k, dict(p)) for k, p in zip(fm.kinds, fm.base)], fm.psf)
@@ -178,7 +211,7 @@ def test_fd_of_device_forward(ctx32, oracle):
     chi2, (g,) = device_chi2_grad(ctx32, [fm], [data], [sigma])
     b = Batch(ctx32, [fm.spec()], graph=False, grad=True)
     b.set_data([data], [sigma])
-    h = fd_steps(theta, fm.names())
+    h = fd_steps(theta, fm.names(), rel=1e-3)
     names = fm.names()
     for k in [names.index(n) for n in ("0.xcen", "0.mag", "0.re", "1.nser", "1.axrat", "2.bg")]:
         vals = []
@@ -191,7 +224,7 @@ def test_fd_of_device_forward(ctx32, oracle):
             vals.append(b.chi2_grad()[0][0])
         fd = (vals[0] - vals[1]) / (2 * h[k])
         print("  %-8s adjoint % .8e fd % .8e" % (names[k], g[k], fd), flush=True)
-        assert abs(fd - g[k]) <= 2e-3 * abs(g[k]) + 1e-4 * np.max(np.abs(g)), names[k]
+        assert abs(fd - g[k]) <= 1e-2 * abs(g[k]), names[k]
     b.close()
 
 
