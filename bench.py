@@ -49,13 +49,12 @@ def rel_errors(got, ref, floor=1e-6):
 
 def conv_flops(specs):
     """Algorithmic flops of the direct convolution of one batch evaluation:
-    2 * kw * kh per drawing pixel (drawing = image + PSF padding)."""
+    2 * kw * kh per returned pixel (the crop of the PSF padding keeps only
+    those; the padding pixels are sources, not outputs)."""
     tot = 0
     for s in specs:
         m = s.m
-        dw = m.width + 2 * (m.psf_width // 2)
-        dh = m.height + 2 * (m.psf_height // 2)
-        tot += 2 * m.psf_width * m.psf_height * dw * dh
+        tot += 2 * m.psf_width * m.psf_height * m.width * m.height
     return tot
 
 

@@ -1390,6 +1390,10 @@ struct pk_batch {
 	size_t o_jobs = 0, o_setup = 0, o_comps = 0, o_cnt = 0, o_psfT = 0, o_mask = 0, o_nc = 0, o_c = 0, o_conv = 0,
 	       o_ent = 0, o_acc = 0, o_q = 0, o_out = 0, o_lite = 0;
 	size_t o_tj = 0;  // job of every tile: coarse tiles, then convolution tiles, then composition tiles
+	// adjoint convolution (PK_BATCH_GRAD): jobs with the whole drawing frame as
+	// convolved frame, and the job of each of its tiles
+	size_t o_jobs_adj = 0, o_tja = 0;
+	uint32_t natiles = 0;
 	uint32_t qcap = 0;
 	std::vector<double> fx_scales;
 	uint64_t total_out = 0;
@@ -1494,13 +1498,14 @@ int grad_launch_t(pk_batch *b)
 	const uint32_t nacc = b->n_comps * GA_SLOTS;
 	hipLaunchKernelGGL(reset_grad_kernel, dim3(std::max(1u, std::min(256u, cdiv(std::max(nacc, nj), 256)))),
 	                   dim3(256), 0, s, gcnt, (const PkCounters *)b->d<PkCounters>(b->o_cnt), gacc, nacc, chi2, nj);
-	const uint32_t *tj_coarse = b->d<uint32_t>(b->o_tj), *tj_conv = tj_coarse + b->ntiles;
+	const uint32_t *tj_coarse = b->d<uint32_t>(b->o_tj);
 	hipLaunchKernelGGL(resid_kernel<T>, dim3(b->ntiles), dim3(256), 0, s, djobs, nj, (const T *)b->d<T>(b->o_out),
 	                   (const T *)b->d<T>(b->o_data), (const T *)b->d<T>(b->o_ivar), Gd, chi2, tj_coarse);
 	PK_HIP(hipGetLastError());
 	if (b->any_conv)
-		PK_TRY(launch_conv<T>(s, djobs, nj, b->nctiles, Gd, b->d<T>(b->o_psfF), b->d<uint8_t>(b->o_mask), Gc,
-		                      b->max_kw, b->max_kh, tj_conv, b->tune.conv_tiles == PK_CONV_TILES_MFMA));
+		PK_TRY(launch_conv<T>(s, b->d<PkJob>(b->o_jobs_adj), nj, b->natiles, Gd, b->d<T>(b->o_psfF),
+		                      b->d<uint8_t>(b->o_mask), Gc, b->max_kw, b->max_kh, b->d<uint32_t>(b->o_tja),
+		                      b->tune.conv_tiles == PK_CONV_TILES_MFMA));
 	const bool fp32 = sizeof(T) == 4;
 	hipLaunchKernelGGL((coarse_grad_kernel<T, fp32>), dim3(b->ntiles), dim3(256), 0, s, djobs, nj,
 	                   (const PkComp *)dcomp, (const uint8_t *)b->d<uint8_t>(b->o_mask), (const T *)Gd,
@@ -1620,12 +1625,14 @@ int autotune(pk_batch *b)
 	};
 	if (fp32 && b->any_conv && conv_mfma_lds_bytes((int)b->max_kw, (int)b->max_kh) <= LDS_LIMIT)
 		PK_TRY(search(&pk_tuning::conv_tiles, {PK_CONV_TILES_VALU, PK_CONV_TILES_MFMA}, 3));
-	if (fp32 && b->n_comps) {
-		PK_TRY(search(&pk_tuning::handover_div, {2, 4, 8, 16}, 2));
+	// the refinement settings interact (an earlier hand-over wants larger
+	// claims): two sweeps
+	for (int sweep = 0; fp32 && b->n_comps && sweep < 2; sweep++) {
+		PK_TRY(search(&pk_tuning::handover_div, {4, 8, 16, 32, 64, 128}, 2));
 		PK_TRY(search(&pk_tuning::refine_occupancy, {50, 75, 100}, 2));
-		PK_TRY(search(&pk_tuning::claim_pixels, {16, 32, 64, 128}, 2));
-		PK_TRY(search(&pk_tuning::claim_cells, {4, 16, 64}, 2));
-		PK_TRY(search(&pk_tuning::poll_rounds, {4, 16, 64}, 2));
+		PK_TRY(search(&pk_tuning::claim_pixels, {32, 64, 128, 256, 512}, 2));
+		PK_TRY(search(&pk_tuning::claim_cells, {16, 64, 256}, 2));
+		PK_TRY(search(&pk_tuning::poll_rounds, {4, 16, 64, 256}, 2));
 	}
 	PK_TRY(tune_run(b, best, st));
 	best.ms_evaluate = st[5];
@@ -1768,6 +1775,14 @@ int pk_batch_create(pk_ctx *ctx, const pk_model *models, uint32_t n_models, uint
 		jb.kw = a.conv_required ? m->psf_width : 0;
 		jb.kh = a.conv_required ? m->psf_height : 0;
 		jb.psf_off = psf_all.size();
+		// crop of the PSF padding (src/model.cpp:289-306): origin (padx, pady)
+		// of the drawing frame
+		const bool cropped = a.padx > 0 && a.pady > 0;
+		const int pcx = cropped ? (int)a.padx : 0, pcy = cropped ? (int)a.pady : 0;
+		jb.crw = a.dw - 2 * (uint32_t)pcx;
+		jb.crh = a.dh - 2 * (uint32_t)pcy;
+		jb.conv_off = img_off;
+		jb.ctile_off = b->nctiles;
 		if (a.conv_required) {
 			const size_t pn = (size_t)m->psf_width * m->psf_height;
 			double sum = 0;
@@ -1776,20 +1791,26 @@ int pk_batch_create(pk_ctx *ctx, const pk_model *models, uint32_t n_models, uint
 			b->max_kw = std::max(b->max_kw, m->psf_width);
 			b->max_kh = std::max(b->max_kh, m->psf_height);
 			b->any_conv = true;
-			jb.ctile_off = b->nctiles;
-			jb.ctiles_x = cdiv(a.dw, CT_W);
-			b->nctiles += jb.ctiles_x * cdiv(a.dh, CT_H);
+			// only the pixels the crop keeps are convolved: the convolved frame
+			// is the crop region, its pixel (X, Y) the same-size convolution at
+			// drawing pixel (X + pcx, Y + pcy)
+			jb.cw = jb.crw;
+			jb.ch = jb.crh;
+			jb.csx = pcx;
+			jb.csy = pcy;
+			jb.mox = -pcx;
+			jb.moy = -pcy;
+			jb.cox = 0;
+			jb.coy = 0;
+			jb.ctiles_x = cdiv(jb.cw, CT_W);
+			b->nctiles += jb.ctiles_x * cdiv(jb.ch, CT_H);
 		} else {
-			jb.ctile_off = b->nctiles;
 			jb.ctiles_x = 0;
+			jb.cw = a.dw;
+			jb.ch = a.dh;
+			jb.cox = pcx;
+			jb.coy = pcy;
 		}
-		jb.conv_off = img_off;
-		jb.cw = a.dw;
-		jb.ch = a.dh;
-		jb.cox = (a.padx > 0 && a.pady > 0) ? (int)a.padx : 0;
-		jb.coy = (a.padx > 0 && a.pady > 0) ? (int)a.pady : 0;
-		jb.crw = a.dw - 2 * (uint32_t)jb.cox;
-		jb.crh = a.dh - 2 * (uint32_t)jb.coy;
 		jb.fs = 1;
 		uint32_t ow = jb.crw, oh = jb.crh;
 		if (fs > 1 && !m->return_finesampled) {
@@ -1831,6 +1852,27 @@ int pk_batch_create(pk_ctx *ctx, const pk_model *models, uint32_t n_models, uint
 		std::fill_n(tj.begin() + b->ntiles + jb.ctile_off, nc, jn);
 		std::fill_n(tj.begin() + b->ntiles + b->nctiles + jb.otile_off, no, jn);
 	}
+	// adjoint of the convolution (PK_BATCH_GRAD): the correlation of the
+	// adjoint image with the PSF covers the whole drawing frame
+	std::vector<PkJob> adj;
+	std::vector<uint32_t> tja;
+	if (flags & PK_BATCH_GRAD) {
+		adj = b->jobs;
+		for (uint32_t jn = 0; jn < n_models; jn++) {
+			PkJob &jb = adj[jn];
+			jb.ctile_off = b->natiles;
+			jb.ctiles_x = 0;
+			if (!jb.has_conv) continue;
+			jb.cw = jb.dw;
+			jb.ch = jb.dh;
+			jb.csx = 0;
+			jb.csy = 0;
+			jb.ctiles_x = cdiv(jb.dw, CT_W);
+			const uint32_t n = jb.ctiles_x * cdiv(jb.dh, CT_H);
+			tja.insert(tja.end(), n, jn);
+			b->natiles += n;
+		}
+	}
 	Layout L;
 	b->o_jobs = L.take(sizeof(PkJob) * b->jobs.size());
 	b->o_setup = L.take(sizeof(SetupIn) * std::max<size_t>(1, b->setup.size()));
@@ -1859,6 +1901,8 @@ int pk_batch_create(pk_ctx *ctx, const pk_model *models, uint32_t n_models, uint
 		b->o_gout = L.take(sizeof(double) * PK_GRAD_SLOTS * std::max<size_t>(1, b->setup.size()));
 		b->o_chi2 = L.take(sizeof(double) * n_models);
 		b->o_gcnt = L.take(sizeof(PkCounters));
+		b->o_jobs_adj = L.take(sizeof(PkJob) * adj.size());
+		b->o_tja = L.take(sizeof(uint32_t) * std::max<size_t>(1, tja.size()));
 	}
 	if (hipMalloc(&b->dmem, L.off) != hipSuccess) {
 		(void)hipGetLastError();
@@ -1894,6 +1938,9 @@ int pk_batch_create(pk_ctx *ctx, const pk_model *models, uint32_t n_models, uint
 				}
 		}
 		PK_HIP(hipMemcpy(b->d<void>(b->o_psfF), f.data(), f.size(), hipMemcpyHostToDevice));
+		PK_HIP(hipMemcpy(b->d<void>(b->o_jobs_adj), adj.data(), sizeof(PkJob) * adj.size(), hipMemcpyHostToDevice));
+		if (!tja.empty())
+			PK_HIP(hipMemcpy(b->d<void>(b->o_tja), tja.data(), sizeof(uint32_t) * tja.size(), hipMemcpyHostToDevice));
 		for (auto &e : b->ev_grad)
 			if (hipEventCreate(&e) != hipSuccess) return fail(set_err(PK_EDEVICE, "hipEventCreate failed"));
 	}

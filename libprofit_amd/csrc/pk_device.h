@@ -741,7 +741,8 @@ __global__ __launch_bounds__(256) void resid_kernel(const PkJob *__restrict__ jo
 		const uint32_t j = j0 + r;
 		if (!(i < jb.dw && j < jb.dh)) continue;
 		double g = 0;
-		const int fx = (int)i - jb.cox, fy = (int)j - jb.coy;
+		// drawing pixel i is pixel i + mox of the convolved frame, whose crop starts at cox
+		const int fx = (int)i + jb.mox - jb.cox, fy = (int)j + jb.moy - jb.coy;
 		if (fx >= 0 && fy >= 0 && fx < (int)jb.crw && fy < (int)jb.crh) {
 			const uint32_t oi = (uint32_t)fx / jb.fs, oj = (uint32_t)fy / jb.fs;
 			const size_t p = jb.out_off + oi + (size_t)oj * jb.ow;
