@@ -226,6 +226,8 @@ def main():
     ap.add_argument("--grad-sample", type=int, default=8, help="fits timed on the plain FP64 FD gradient")
     ap.add_argument("--no-grad", action="store_true")
     ap.add_argument("--no-tune", action="store_true", help="default launch settings instead of the autotuner's")
+    ap.add_argument("--tuning", default=None, metavar="JSON",
+                    help="fixed launch settings (pk_tuning fields) instead of the autotuner's")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run a plain device copy of BYTES bytes 3 times (calibrates the HBM byte counters)")
     args = ap.parse_args()
@@ -257,7 +259,10 @@ def main():
 
     # ---- batched FP32 path (graph) --------------------------------------------------
     t0 = time.time()
-    batch = Batch(ctx32, specs, graph=True, autotune=not args.no_tune)
+    fixed = json.loads(args.tuning) if args.tuning else None
+    batch = Batch(ctx32, specs, graph=True, autotune=not (args.no_tune or fixed))
+    if fixed:
+        batch.tuning = fixed
     log("batch created in %.2fs" % (time.time() - t0))
     t0 = time.time()
     batch.evaluate()  # the autotuner runs here, before the graph is captured

@@ -21,9 +21,11 @@
 // with a deeper LIFO.  A LIFO that fills up before that also hands its new
 // cells to the next pass.
 //
-// Latency: lane-crossing sums and scans use DPP (no LDS round trips); the
-// 128-byte CompLite record of a cell is staged in LDS and kept there while the
-// following rounds work on the same component.
+// Latency: lane-crossing scans and whole-wavefront sums use DPP; a round
+// holding several cells sums each cell's sub-cells with LDS integer atomics
+// (one slot per cell) and adds the slots to the pixels.  The 128-byte CompLite
+// record of a cell is staged in LDS and kept there while the following rounds
+// work on the same component.
 #pragma once
 
 namespace pk {
@@ -39,10 +41,16 @@ constexpr int R32_TASKS = 16;              // cells per round at most (their rec
 // every 16 rounds, hand-over once a quarter of the wavefronts stopped
 constexpr R32Tune R32_TUNE_DEFAULT = {4u, 64u, 16u, 16u};
 
+template <int N>
+struct IntTag {
+	static constexpr int value = N;
+};
+
 template <int CAP>
 struct R32Stack {
 	CompLite lite[R32_TASKS];
 	uint32_t lite_g[R32_TASKS];   // component whose record lite[t] holds
+	unsigned long long csum[R32_TASKS];  // fixed-point sum of task t in the current round (0 between rounds)
 	RCell cell[CAP];
 };
 
@@ -92,26 +100,6 @@ __device__ inline unsigned long long wave_sum_u64(unsigned long long v)
 	v += dpp0_u64<DPP_ROW_BCAST31, 0xc>(v);
 	const uint32_t lo = wave_read((uint32_t)v, 63), hi = wave_read((uint32_t)(v >> 32), 63);
 	return ((unsigned long long)hi << 32) | lo;
-}
-
-// Inclusive prefix sum within segments of consecutive lanes; `sbeg` is the
-// first lane of the calling lane's segment.
-__device__ inline unsigned long long wave_seg_scan_u64(unsigned long long v, uint32_t lane, uint32_t sbeg)
-{
-	unsigned long long t;
-	t = dpp0_u64<DPP_ROW_SHR1>(v);
-	if (lane >= sbeg + 1) v += t;
-	t = dpp0_u64<DPP_ROW_SHR2>(v);
-	if (lane >= sbeg + 2) v += t;
-	t = dpp0_u64<DPP_ROW_SHR4>(v);
-	if (lane >= sbeg + 4) v += t;
-	t = dpp0_u64<DPP_ROW_SHR8>(v);
-	if (lane >= sbeg + 8) v += t;
-	t = dpp0_u64<DPP_ROW_BCAST15, 0xa>(v);  // source: last lane of the previous row
-	if (sbeg < (lane & ~15u)) v += t;
-	t = dpp0_u64<DPP_ROW_BCAST31, 0xc>(v);  // source: lane 31
-	if (sbeg <= 31u) v += t;
-	return v;
 }
 
 // cell metadata: level | res << 8 | max recursions << 16 | convolved << 24
@@ -265,7 +253,10 @@ __global__ __launch_bounds__(64) void refine32_kernel(const PkJob *__restrict__ 
 	constexpr int CAP = LAST ? R32_CAP_LAST : R32_CAP_MAIN;
 	__shared__ R32Stack<CAP> st;
 	const uint32_t lane = lane_id();
-	if (lane < (uint32_t)R32_TASKS) st.lite_g[lane] = ~0u;
+	if (lane < (uint32_t)R32_TASKS) {
+		st.lite_g[lane] = ~0u;
+		st.csum[lane] = 0;
+	}
 	int top = 0;
 	bool source_done = false, handover = false;
 	const uint32_t n_cells_in = SOURCE == 0 ? 0u : min(cnt->n_cells[pass - 1], qcap);
@@ -281,14 +272,16 @@ __global__ __launch_bounds__(64) void refine32_kernel(const PkJob *__restrict__ 
 	int poll = 0;
 	__syncthreads();
 	for (;;) {
-		// ---- refill while the cells on the LIFO cannot fill the lanes ----
+		// ---- refill while the cells on the LIFO cannot fill the lanes (two
+		// sub-cells per lane) ----
 		for (;;) {
+			if (top >= 128) break;  // at least one sub-cell per cell
 			uint32_t r2 = 0;
 			if ((int)lane < top) {
 				const uint32_t res = cm_res(st.cell[top - 1 - (int)lane].meta);
 				r2 = res * res;
 			}
-			if (wave_sum_u32(r2) >= 64u || source_done) break;
+			if (wave_sum_u32(r2) >= 128u || source_done) break;
 			if (CAP - top < (SOURCE == 0 ? 32 : 1)) break;
 			if (chunk_next >= chunk_end &&
 			    !r32_claim(head, n_source, SOURCE == 0 ? tune.chunk : tune.chunk_cells, seen, chunk_next, chunk_end,
@@ -315,41 +308,78 @@ __global__ __launch_bounds__(64) void refine32_kernel(const PkJob *__restrict__ 
 		}
 
 		// ---- cells on top of the LIFO -> lanes ----
+		// A group takes cells from the top while their sub-cells fit the lanes:
+		// 64 sub-cells, one per lane, or -- when the group then holds more than
+		// 64 and every resolution is even -- 128, two adjacent sub-cells
+		// (si, sj) and (si, sj + 1) per lane, which then always belong to the
+		// same cell (even sub-cell index, even resolution: sj is even).  The
+		// per-group and per-round work is then shared by two sub-cells.
 		uint32_t r2 = 0;
 		if ((int)lane < top && lane < (uint32_t)R32_TASKS) {
 			const uint32_t res = cm_res(st.cell[top - 1 - (int)lane].meta);
 			r2 = res * res;
 		}
 		const uint32_t incl = wave_incl_scan_u32(r2);
-		const uint64_t fit = __ballot((int)lane < top && lane < (uint32_t)R32_TASKS && incl <= 64u);
-		const int ntask = fit ? __popcll(fit) : 1;
-		const uint32_t lanes_needed = wave_read(incl, ntask - 1);
-		const int rounds = (int)((lanes_needed + 63u) / 64u);
-		// task of this lane: the first whose inclusive end exceeds the lane
-		int t_local = 0;
-		uint32_t before = 0, seg_end1 = 0;
-		for (int t = 0; t < ntask; t++) {
-			const uint32_t it = wave_read(incl, t);
-			if (it <= lane) {
-				t_local = t + 1;
-				before = it;
-			} else if (seg_end1 == 0) {
-				seg_end1 = it;
+		const uint32_t r2_0 = wave_read(r2, 0);
+		const uint64_t fit128 = __ballot((int)lane < top && lane < (uint32_t)R32_TASKS && incl <= 128u);
+		const int n128 = fit128 ? __popcll(fit128) : 0;
+		bool pair;
+		int ntask;
+		uint32_t n_sub;  // sub-cells of the group
+		if (n128 == 0) {
+			// the top cell alone has more than 128 sub-cells
+			ntask = 1;
+			n_sub = r2_0;
+			pair = (r2_0 & 1u) == 0;
+		} else {
+			const uint32_t in128 = wave_read(incl, n128 - 1);
+			pair = in128 > 64u && __ballot((int)lane < n128 && (r2 & 1u)) == 0;
+			if (pair) {
+				ntask = n128;
+				n_sub = in128;
+			} else {
+				const uint64_t fit = __ballot((int)lane < top && lane < (uint32_t)R32_TASKS && incl <= 64u);
+				ntask = fit ? __popcll(fit) : 1;
+				n_sub = wave_read(incl, ntask - 1);
 			}
 		}
+		const uint32_t per_lane = pair ? 2u : 1u;
+		const int rounds = (int)((n_sub + 64u * per_lane - 1u) / (64u * per_lane));
+		// task of this lane's first sub-cell g0 = per_lane * lane: the first
+		// whose inclusive end exceeds g0
+		const uint32_t g0 = per_lane * lane;
+		int t_local = 0;
+		uint32_t before = 0;
 		if (rounds > 1) {
-			t_local = 0;
-			before = 0;
-		} else if (t_local >= ntask) {
-			t_local = ntask - 1;  // lane beyond the last task: inactive
+			// one cell over several rounds
+		} else if (__ballot((int)lane < ntask && r2 != r2_0) == 0) {
+			// all tasks of one resolution (the usual case): segments of r2_0 sub-cells
+			t_local = (int)((float)g0 * frcp((float)r2_0));
+			if ((uint32_t)t_local * r2_0 > g0) t_local--;
+			else if ((uint32_t)(t_local + 1) * r2_0 <= g0) t_local++;
+			before = (uint32_t)t_local * r2_0;
+			if (t_local >= ntask) t_local = ntask - 1;  // lane beyond the last task: inactive
+		} else {
+			for (int t = 0; t < ntask; t++) {
+				const uint32_t it = wave_read(incl, t);
+				if (it <= g0) {
+					t_local = t + 1;
+					before = it;
+				}
+			}
+			if (t_local >= ntask) t_local = ntask - 1;
 		}
 		const RCell cell = st.cell[top - 1 - t_local];
-		// component records of the tasks -> LDS (kept while unchanged)
+		// component records of the tasks -> LDS (kept while unchanged); lane
+		// t < ntask keeps the accumulator of task t
+		size_t task_target = 0;
 		{
 			uint32_t g = 0;
 			bool miss = false;
 			if ((int)lane < ntask) {
-				g = st.cell[top - 1 - (int)lane].gcomp;
+				const RCell &tc = st.cell[top - 1 - (int)lane];
+				g = tc.gcomp;
+				task_target = 2 * (size_t)tc.entry + cm_conv(tc.meta);
 				miss = st.lite_g[lane] != g;
 			}
 			if (__ballot(miss)) {
@@ -379,109 +409,158 @@ __global__ __launch_bounds__(64) void refine32_kernel(const PkJob *__restrict__ 
 		const bool any_rec = __ballot(recurse) != 0;
 		const int kind0 = (int)wave_read((uint32_t)kind, 0);
 		const bool one_kind = __ballot(kind != kind0) == 0;
-		for (int rd = 0; rd < rounds; rd++) {
-			const uint32_t g = lane + 64u * (uint32_t)rd;
-			const bool active = g < lanes_needed;
-			// sub-cell s = si * res + sj, found without an integer division
-			const int s = (int)(g - before);
-			int si = (int)((float)s * inv_res);
-			int sj = s - si * res;
-			if (sj >= res) {
-				si++;
-				sj -= res;
-			} else if (sj < 0) {
-				si--;
-				sj += res;
-			}
-			const double dx = fma((double)(2 * si + 1), hx, cell.dx0);
-			const double dy = fma((double)(2 * sj + 1), hy, cell.dy0);
-			const double xp = fma(dx, cl.cos_ang, dy * cl.sin_ang);
-			const double yp = fma(dy, cl.cos_ang, -dx * cl.sin_ang) * cl.inv_axrat;
-			const double txp = fabs(xp), typ = fabs(yp) + fabs(ddy);  // accuracy test point
-			float v32 = -1.f, t32 = -1.f;
-			if (one_kind) {
-				// the usual case: one profile kind in all lanes, scalar branch
-				if (any_rec) evaluate_pair32<true>(cl, kind0, xp, yp, txp, typ, v32, t32);
-				else evaluate_pair32<false>(cl, kind0, xp, yp, txp, typ, v32, t32);
-			} else {
-				evaluate_pair32<true>(cl, kind, xp, yp, txp, typ, v32, t32);
-			}
-			bool decided, fail = false;
-			if (recurse) {
-				const float m = fabsf(t32 * frcp(v32) - 1.f);
-				decided = v32 > 0.f && v32 <= 3e38f && m <= 3e38f && fabsf(m - acc_thr) > band;
-				fail = decided && m > acc_thr;
-			} else {
-				decided = v32 >= 0.f && v32 <= 3e38f;
-			}
-			unsigned long long fxv = 0;
-			if (!GRAD && active && decided && !fail && !fixed_point_f32(v32 * wq, &fxv)) bad++;
-			if (__ballot(active && !decided)) {
-				// FP64 value and decision within the guard band (or without an FP32 form)
-				if (active && !decided) {
-					const double val = refine_fp64_decision(comps[cell.gcomp], xp, yp, ddy, recurse, &fail);
-					if (!GRAD && !fail) {
-						const double q = val * ((double)(fr * fr) * cl.wflux);
-						if (q >= 0 && q < 9.2e18) fxv = (unsigned long long)__double2ll_rn(q);
-						else bad++;
-					}
+		auto run_rounds = [&](auto per_lane_tag) {
+			constexpr int P = decltype(per_lane_tag)::value;
+			for (int rd = 0; rd < rounds; rd++) {
+				const uint32_t g = P * lane + 64u * P * (uint32_t)rd;
+				// sub-cell s = si * res + sj, found without an integer division
+				const int s = (int)(g - before);
+				int si = (int)((float)s * inv_res);
+				int sj = s - si * res;
+				if (sj >= res) {
+					si++;
+					sj -= res;
+				} else if (sj < 0) {
+					si--;
+					sj += res;
 				}
-			}
-			fail = fail && active;
-			if (active) nev += recurse ? 2 : 1;
-			if (GRAD) {
-				const bool ok = active && !fail;
-				double ga[GA_SLOTS] = {0, 0, 0, 0, 0, 0, 0, 0};
-				if (ok) {
-					const RefineEntry en = entries[cell.entry];
-					const PkJob &jb = jobs[en.job];
-					const size_t p = jb.img_off + en.pix;
-					const double G = (double)(cm_conv(meta) ? G_c[p] : G_nc[p]);
-					float g[GT_COUNT];
-					const float vg = sersic_grad32(cl, (float)xp, (float)yp, g);
-					grad_accumulate(ga, G * ((double)(fr * fr) * cl.wq / jb.fx_scale) * (double)vg, g);
+				const double dx = fma((double)(2 * si + 1), hx, cell.dx0);
+				double dy[P], xp[P], yp[P];
+				float v32[P], t32[P];
+				bool active[P], fail[P], decided[P];
+#pragma unroll
+				for (int q = 0; q < P; q++) {
+					active[q] = g + (uint32_t)q < n_sub;
+					dy[q] = fma((double)(2 * (sj + q) + 1), hy, cell.dy0);
+					xp[q] = fma(dx, cl.cos_ang, dy[q] * cl.sin_ang);
+					yp[q] = fma(dy[q], cl.cos_ang, -dx * cl.sin_ang) * cl.inv_axrat;
+					v32[q] = -1.f;
+					t32[q] = -1.f;
 				}
-				r32_flush_grad(ok, cell.gcomp, ga, gacc, lane);
-			} else if (ntask == 1) {
-				// exact per-cell sums; the last lane of a cell adds its sum to the
-				// pixel accumulator
-				const unsigned long long v = wave_sum_u64(fxv);
-				if (lane == 0 && v) atomicAdd(&acc[target], v);
-			} else {
-				// several cells: segmented scan over each cell's contiguous lanes
-				const unsigned long long v = wave_seg_scan_u64(fxv, lane, before);
-				if (active && lane == seg_end1 - 1 && v) atomicAdd(&acc[target], v);
-			}
-			// failing sub-cells: onto the LIFO, or to the next pass when it is full
-			const uint64_t ballot = __ballot(fail);
-			if (ballot) {
-				const int n = __popcll(ballot);
-				const int rank = __popcll(ballot & ((1ull << lane) - 1ull));
-				RCell c;
-				c.dx0 = dx - hx;
-				c.dy0 = dy - hy;
-				c.f = fr;
-				c.gcomp = cell.gcomp;
-				c.entry = cell.entry;
-				c.meta = (meta & ~255u) | (uint32_t)(level + 1);
-				if (top + n <= CAP) {
-					if (fail) st.cell[top + rank] = c;
-					top += n;
-				} else if (!LAST) {
-					const int leader = __ffsll((long long)ballot) - 1;
-					uint32_t base = 0;
-					if ((int)lane == leader) base = atomicAdd(&cnt->n_cells[pass], (uint32_t)n);
-					base = wave_read(base, leader);
-					if (fail) {
-						if (base + (uint32_t)rank < qcap) qout[base + rank] = c;
-						else overflow = 1;
+				if (one_kind) {
+					// the usual case: one profile kind in all lanes, scalar branch
+					if (any_rec) {
+#pragma unroll
+						for (int q = 0; q < P; q++)
+							evaluate_pair32<true>(cl, kind0, xp[q], yp[q], fabs(xp[q]), fabs(yp[q]) + fabs(ddy), v32[q],
+							                      t32[q]);
+					} else {
+#pragma unroll
+						for (int q = 0; q < P; q++)
+							evaluate_pair32<false>(cl, kind0, xp[q], yp[q], 0., 0., v32[q], t32[q]);
 					}
 				} else {
-					overflow = 1;
+#pragma unroll
+					for (int q = 0; q < P; q++)
+						evaluate_pair32<true>(cl, kind, xp[q], yp[q], fabs(xp[q]), fabs(yp[q]) + fabs(ddy), v32[q],
+						                      t32[q]);
 				}
+				unsigned long long fxv = 0;  // fixed-point sum of this lane's accepted sub-cells
+				bool any_undecided = false;
+#pragma unroll
+				for (int q = 0; q < P; q++) {
+					fail[q] = false;
+					if (recurse) {
+						const float m = fabsf(t32[q] * frcp(v32[q]) - 1.f);
+						decided[q] = v32[q] > 0.f && v32[q] <= 3e38f && m <= 3e38f && fabsf(m - acc_thr) > band;
+						fail[q] = decided[q] && m > acc_thr;
+					} else {
+						decided[q] = v32[q] >= 0.f && v32[q] <= 3e38f;
+					}
+					unsigned long long f = 0;
+					if (!GRAD && active[q] && decided[q] && !fail[q]) {
+						if (fixed_point_f32(v32[q] * wq, &f)) fxv += f;
+						else bad++;
+					}
+					any_undecided = any_undecided || (active[q] && !decided[q]);
+				}
+				if (__ballot(any_undecided)) {
+					// FP64 value and decision within the guard band (or without an FP32 form)
+#pragma unroll
+					for (int q = 0; q < P; q++) {
+						if (active[q] && !decided[q]) {
+							const double val =
+							    refine_fp64_decision(comps[cell.gcomp], xp[q], yp[q], ddy, recurse, &fail[q]);
+							if (!GRAD && !fail[q]) {
+								const double qv = val * ((double)(fr * fr) * cl.wflux);
+								if (qv >= 0 && qv < 9.2e18) fxv += (unsigned long long)__double2ll_rn(qv);
+								else bad++;
+							}
+						}
+					}
+				}
+#pragma unroll
+				for (int q = 0; q < P; q++) {
+					fail[q] = fail[q] && active[q];
+					if (active[q]) nev += recurse ? 2 : 1;
+				}
+				if (GRAD) {
+					double ga[GA_SLOTS] = {0, 0, 0, 0, 0, 0, 0, 0};
+					bool any_ok = false;
+#pragma unroll
+					for (int q = 0; q < P; q++) {
+						if (active[q] && !fail[q]) {
+							any_ok = true;
+							const RefineEntry en = entries[cell.entry];
+							const PkJob &jb = jobs[en.job];
+							const size_t p = jb.img_off + en.pix;
+							const double G = (double)(cm_conv(meta) ? G_c[p] : G_nc[p]);
+							float gt[GT_COUNT];
+							const float vg = sersic_grad32(cl, (float)xp[q], (float)yp[q], gt);
+							grad_accumulate(ga, G * ((double)(fr * fr) * cl.wq / jb.fx_scale) * (double)vg, gt);
+						}
+					}
+					r32_flush_grad(any_ok, cell.gcomp, ga, gacc, lane);
+				} else if (ntask == 1) {
+					// exact per-cell sum; lane 0 adds it to the pixel accumulator
+					const unsigned long long v = wave_sum_u64(fxv);
+					if (lane == 0 && v) atomicAdd(&acc[target], v);
+				} else {
+					// several cells: exact integer sums per cell in LDS, then one
+					// global atomic per cell (lane t adds task t's sum)
+					if (fxv) atomicAdd(&st.csum[t_local], fxv);
+					__builtin_amdgcn_wave_barrier();
+					if ((int)lane < ntask) {
+						const unsigned long long v = st.csum[lane];
+						st.csum[lane] = 0;
+						if (v) atomicAdd(&acc[task_target], v);
+					}
+				}
+				// failing sub-cells: onto the LIFO, or to the next pass when it is full
+#pragma unroll
+				for (int q = 0; q < P; q++) {
+					const uint64_t ballot = __ballot(fail[q]);
+					if (!ballot) continue;
+					const int n = __popcll(ballot);
+					const int rank = __popcll(ballot & ((1ull << lane) - 1ull));
+					RCell c;
+					c.dx0 = dx - hx;
+					c.dy0 = dy[q] - hy;
+					c.f = fr;
+					c.gcomp = cell.gcomp;
+					c.entry = cell.entry;
+					c.meta = (meta & ~255u) | (uint32_t)(level + 1);
+					if (top + n <= CAP) {
+						if (fail[q]) st.cell[top + rank] = c;
+						top += n;
+					} else if (!LAST) {
+						const int leader = __ffsll((long long)ballot) - 1;
+						uint32_t base = 0;
+						if ((int)lane == leader) base = atomicAdd(&cnt->n_cells[pass], (uint32_t)n);
+						base = wave_read(base, leader);
+						if (fail[q]) {
+							if (base + (uint32_t)rank < qcap) qout[base + rank] = c;
+							else overflow = 1;
+						}
+					} else {
+						overflow = 1;
+					}
+				}
+				__syncthreads();
 			}
-			__syncthreads();
-		}
+		};
+		if (pair) run_rounds(IntTag<2>{});
+		else run_rounds(IntTag<1>{});
 	}
 	if (!LAST && handover) {
 		// everything this wavefront holds goes to the next pass
