@@ -235,7 +235,7 @@ enum {
 /* Launch settings of a batch.  None of them changes which pixels are refined
  * or how; the refinement settings do not change the result at all (its sums
  * are exact fixed-point), the convolution variant changes the FP32 rounding
- * of convolved pixels only.  Defaults: VALU tiles, 100, 4, 64, 16, 16. */
+ * of convolved pixels only.  Defaults: VALU tiles, 100, 4, 64, 16, 16, 0. */
 enum { PK_CONV_TILES_VALU = 0, PK_CONV_TILES_MFMA = 1 };
 typedef struct pk_tuning {
 	uint32_t conv_tiles;       /* FP32 direct convolution: PK_CONV_TILES_VALU or _MFMA */
@@ -245,6 +245,9 @@ typedef struct pk_tuning {
 	uint32_t claim_pixels;     /* refinement pixels one wavefront claims at once, at most (>= 1) */
 	uint32_t claim_cells;      /* handed-on cells one wavefront claims at once, at most (>= 1) */
 	uint32_t poll_rounds;      /* refinement rounds between two checks of the hand-on condition (>= 1) */
+	uint32_t claim_spread;     /* 0: refinement pixels are claimed in the order the pixel pass listed
+	                              them; a power of two n <= 4096: in runs of n taken from 64 places of
+	                              the list in turn (spreads deep galaxy centres over wavefronts) */
 	double ms_evaluate;        /* device ms of one evaluation with these settings, when measured (else 0) */
 } pk_tuning;
 
@@ -279,7 +282,7 @@ int pk_batch_destroy(pk_batch *b);
  * by init() and exported by finish(), src/library.cpp:96-165).  The tuner
  * times eager evaluations of the batch over candidate settings, one setting
  * at a time (convolution tiles, then hand-on divisor, occupancy, claim sizes,
- * poll interval), keeps the fastest and leaves the images of one more
+ * poll interval, claim spread), keeps the fastest and leaves the images of one more
  * evaluation with it.  Results are cached per device and workload shape
  * (precision, gradient flag, model count, pixel count, largest PSF, component
  * count); pk_init loads the cache from $PROFIT_HOME/hip_tuning (PROFIT_HOME

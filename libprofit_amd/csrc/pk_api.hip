@@ -121,7 +121,7 @@ namespace {
 
 // ---- autotuner cache -----------------------------------------------------------
 // One text line per tuned workload: "<key>\t<conv> <occupancy> <handover> <claim
-// pixels> <claim cells> <poll> <ms>", in $PROFIT_HOME/hip_tuning.
+// pixels> <claim cells> <poll> <spread> <ms>", in $PROFIT_HOME/hip_tuning.
 std::mutex g_tune_mu;
 std::map<std::string, pk_tuning> g_tune_cache;
 bool g_tune_dirty = false;
@@ -144,7 +144,8 @@ bool tuning_valid(const pk_tuning &t)
 {
 	return t.conv_tiles <= PK_CONV_TILES_MFMA && t.refine_occupancy >= 1 && t.refine_occupancy <= 100 &&
 	       t.handover_div >= 1 && t.claim_pixels >= 1 && t.claim_cells >= 1 && t.poll_rounds >= 1 &&
-	       t.handover_div <= 1024 && t.claim_pixels <= 4096 && t.claim_cells <= 4096 && t.poll_rounds <= 4096;
+	       t.handover_div <= 1024 && t.claim_pixels <= 4096 && t.claim_cells <= 4096 && t.poll_rounds <= 4096 &&
+	       t.claim_spread <= 4096 && (t.claim_spread & (t.claim_spread - 1)) == 0;
 }
 
 void tune_cache_load()
@@ -159,8 +160,8 @@ void tune_cache_load()
 		if (!tab) continue;
 		*tab = 0;
 		pk_tuning t{};
-		if (std::sscanf(tab + 1, "%u %u %u %u %u %u %lf", &t.conv_tiles, &t.refine_occupancy, &t.handover_div,
-		                &t.claim_pixels, &t.claim_cells, &t.poll_rounds, &t.ms_evaluate) == 7 &&
+		if (std::sscanf(tab + 1, "%u %u %u %u %u %u %u %lf", &t.conv_tiles, &t.refine_occupancy, &t.handover_div,
+		                &t.claim_pixels, &t.claim_cells, &t.poll_rounds, &t.claim_spread, &t.ms_evaluate) == 8 &&
 		    tuning_valid(t))
 			g_tune_cache[line] = t;
 	}
@@ -177,8 +178,8 @@ int tune_cache_save()
 	if (!fp) return set_err(PK_EINVAL, "cannot write %s", tmp.c_str());
 	for (const auto &kv : g_tune_cache) {
 		const pk_tuning &t = kv.second;
-		std::fprintf(fp, "%s\t%u %u %u %u %u %u %.6f\n", kv.first.c_str(), t.conv_tiles, t.refine_occupancy,
-		             t.handover_div, t.claim_pixels, t.claim_cells, t.poll_rounds, t.ms_evaluate);
+		std::fprintf(fp, "%s\t%u %u %u %u %u %u %u %.6f\n", kv.first.c_str(), t.conv_tiles, t.refine_occupancy,
+		             t.handover_div, t.claim_pixels, t.claim_cells, t.poll_rounds, t.claim_spread, t.ms_evaluate);
 	}
 	const bool ok = std::fclose(fp) == 0;
 	if (!ok || std::rename(tmp.c_str(), f.c_str()) != 0) {
@@ -1369,6 +1370,7 @@ static pk_tuning default_tuning()
 	t.claim_pixels = R32_TUNE_DEFAULT.chunk;
 	t.claim_cells = R32_TUNE_DEFAULT.chunk_cells;
 	t.poll_rounds = R32_TUNE_DEFAULT.poll;
+	t.claim_spread = 0;
 	t.ms_evaluate = 0;
 	return t;
 }
@@ -1411,7 +1413,11 @@ struct pk_batch {
 	pk_tuning tune = default_tuning();
 	bool tuned = false;
 	uint64_t n_pixels = 0;  // drawing pixels of all models
-	R32Tune r32() const { return R32Tune{tune.handover_div, tune.claim_pixels, tune.claim_cells, tune.poll_rounds}; }
+	R32Tune r32() const
+	{
+		const uint32_t spread_log2 = tune.claim_spread ? (uint32_t)__builtin_ctz(tune.claim_spread) : 0u;
+		return R32Tune{tune.handover_div, tune.claim_pixels, tune.claim_cells, tune.poll_rounds, spread_log2};
+	}
 	uint32_t grid(uint32_t full) const { return std::max(1u, (uint32_t)((uint64_t)full * tune.refine_occupancy / 100)); }
 	template <typename T> T *d(size_t off) const
 	{
@@ -1633,6 +1639,7 @@ int autotune(pk_batch *b)
 		PK_TRY(search(&pk_tuning::claim_pixels, {32, 64, 128, 256, 512}, 2));
 		PK_TRY(search(&pk_tuning::claim_cells, {16, 64, 256}, 2));
 		PK_TRY(search(&pk_tuning::poll_rounds, {4, 16, 64, 256}, 2));
+		PK_TRY(search(&pk_tuning::claim_spread, {0, 4, 16, 64}, 2));
 	}
 	PK_TRY(tune_run(b, best, st));
 	best.ms_evaluate = st[5];
@@ -1971,9 +1978,10 @@ int pk_batch_set_tuning(pk_batch *b, const pk_tuning *t)
 {
 	if (!b || !t) return set_err(PK_EINVAL, "NULL argument");
 	if (!tuning_valid(*t))
-		return set_err(PK_EINVAL, "invalid tuning (conv_tiles %u, occupancy %u, handover %u, claims %u/%u, poll %u)",
+		return set_err(PK_EINVAL,
+		               "invalid tuning (conv_tiles %u, occupancy %u, handover %u, claims %u/%u, poll %u, spread %u)",
 		               t->conv_tiles, t->refine_occupancy, t->handover_div, t->claim_pixels, t->claim_cells,
-		               t->poll_rounds);
+		               t->poll_rounds, t->claim_spread);
 	std::lock_guard<std::mutex> lock(b->ctx->mu);
 	PK_HIP(hipSetDevice(b->ctx->device));
 	PK_HIP(hipStreamSynchronize(b->ctx->stream));

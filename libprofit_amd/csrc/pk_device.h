@@ -122,6 +122,8 @@ __global__ __launch_bounds__(256) void coarse_kernel(const PkJob *__restrict__ j
 			else acc_nc[r] += v;
 		}
 	}
+	uint64_t ballot[PX_RPT];
+	uint32_t n_need = 0;
 #pragma unroll
 	for (int r = 0; r < PX_RPT; r++) {
 		const uint32_t pix = (j0 + r) * jb.dw + i;
@@ -129,23 +131,26 @@ __global__ __launch_bounds__(256) void coarse_kernel(const PkJob *__restrict__ j
 			if (jb.has_nc) img_nc[jb.img_off + pix] = (TI)acc_nc[r];
 			if (jb.has_conv) img_c[jb.img_off + pix] = (TI)acc_c[r];
 		}
-		// wavefront-aggregated append of the pixels that need refinement
-		const bool need = cmask[r] != 0;
-		const uint64_t ballot = __ballot(need);
-		if (ballot) {
-			const uint32_t n = __popcll(ballot);
-			const uint32_t rank = __popcll(ballot & ((1ull << lane_id()) - 1ull));
-			const uint32_t leader = __ffsll((long long)ballot) - 1;
-			uint32_t base = 0;
-			if (lane_id() == leader) base = atomicAdd(&counters->n_entries, n);
-			base = __shfl(base, (int)leader);
-			if (need) {
-				entries[base + rank] = RefineEntry{jn, pix, cmask[r]};
+		ballot[r] = __ballot(cmask[r] != 0);
+		n_need += __popcll(ballot[r]);
+	}
+	// wavefront-aggregated append of the pixels that need refinement: one
+	// atomic for the wavefront's rows, row r's pixels after those of rows < r
+	if (n_need) {
+		uint32_t base = 0;
+		if (lane_id() == 0) base = atomicAdd(&counters->n_entries, n_need);
+		base = __shfl(base, 0);
+#pragma unroll
+		for (int r = 0; r < PX_RPT; r++) {
+			if (cmask[r]) {
+				const uint32_t e = base + __popcll(ballot[r] & ((1ull << lane_id()) - 1ull));
+				entries[e] = RefineEntry{jn, (j0 + r) * jb.dw + i, cmask[r]};
 				if (acc) {
-					acc[2 * (size_t)(base + rank)] = 0;
-					acc[2 * (size_t)(base + rank) + 1] = 0;
+					acc[2 * (size_t)e] = 0;
+					acc[2 * (size_t)e + 1] = 0;
 				}
 			}
+			base += __popcll(ballot[r]);
 		}
 	}
 }

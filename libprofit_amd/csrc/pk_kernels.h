@@ -63,6 +63,8 @@ struct R32Tune {
 	uint32_t chunk;         // pixels per claim at most (pass 0)
 	uint32_t chunk_cells;   // handed-over cells per claim at most (later passes)
 	uint32_t poll;          // rounds between two checks of the hand-over condition
+	uint32_t spread_log2;   // pass 0 claims runs of 2^spread_log2 entries from 64 places of the entry
+	                        // list in turn (0: in entry order)
 };
 
 struct PkCounters {

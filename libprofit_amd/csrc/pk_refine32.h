@@ -39,7 +39,7 @@ constexpr int R32_CAP_LAST = 1024;
 constexpr int R32_TASKS = 16;              // cells per round at most (their records are staged in LDS)
 // untuned settings: claims of up to 64 pixels / 16 cells, the hand-over test
 // every 16 rounds, hand-over once a quarter of the wavefronts stopped
-constexpr R32Tune R32_TUNE_DEFAULT = {4u, 64u, 16u, 16u};
+constexpr R32Tune R32_TUNE_DEFAULT = {4u, 64u, 16u, 16u, 0u};
 
 template <int N>
 struct IntTag {
@@ -133,22 +133,38 @@ __device__ inline bool fixed_point_f32(float q, unsigned long long *out)
 	return true;
 }
 
-// Expands refinement entries [next, end) into level-0 cells on the LIFO while
-// they fit, one pixel per lane (one cell per refined component).  Needs at
-// least 32 free slots, so that one pixel always fits.
+// Entry of pass-0 claim index i: with spreading (L = spread_log2 > 0) the
+// entry list is cut into runs of 2^L entries, and consecutive runs of claim
+// indices come from 64 places of the list in turn (run b -> (b % 64) * sstride
+// + b / 64), so that the pixels of one deep galaxy centre go to many
+// wavefronts.  Returns an index >= n for claim indices past the list.
+__device__ inline uint32_t r32_entry_of(uint32_t i, uint32_t spread_log2, uint32_t sstride)
+{
+	if (!spread_log2) return i;
+	const uint32_t b = i >> spread_log2, w = i & ((1u << spread_log2) - 1u);
+	return ((((b & 63u) * sstride) + (b >> 6)) << spread_log2) | w;
+}
+
+// Expands the refinement entries of claim indices [next, end) into level-0
+// cells on the LIFO while they fit, one pixel per lane (one cell per refined
+// component).  Needs at least 32 free slots, so that one pixel always fits.
 __device__ inline void r32_expand(const PkJob *__restrict__ jobs, const PkComp *__restrict__ comps,
-                                  const RefineEntry *__restrict__ entries, RCell *cells, int cap, int &top,
-                                  uint32_t &next, uint32_t end, uint32_t lane)
+                                  const RefineEntry *__restrict__ entries, uint32_t n_entries, uint32_t spread_log2,
+                                  uint32_t sstride, RCell *cells, int cap, int &top, uint32_t &next, uint32_t end,
+                                  uint32_t lane)
 {
 	const uint32_t avail = end - next;
 	const uint32_t space = (uint32_t)(cap - top);
-	const bool valid = lane < avail;
+	const uint32_t e = r32_entry_of(next + lane, spread_log2, sstride);
+	const bool valid = lane < avail && e < n_entries;
 	RefineEntry en{0, 0, 0};
-	if (valid) en = entries[next + lane];
+	if (valid) en = entries[e];
 	const uint32_t nb = valid ? (uint32_t)__popc(en.cmask) : 0u;
 	const uint32_t incl = wave_incl_scan_u32(nb);
-	const int take = __popcll(__ballot(valid && incl <= space));
-	if ((int)lane < take) {
+	// claim indices past the list hold no entry (nb = 0): the lanes taken are
+	// a prefix of the available ones
+	const int take = __popcll(__ballot(lane < avail && incl <= space));
+	if ((int)lane < take && valid) {
 		const PkJob &jb = jobs[en.job];
 		const uint32_t i = en.pix % jb.dw, j = en.pix / jb.dw;
 		const double hx = jb.sx / 2., hy = jb.sy / 2.;
@@ -171,7 +187,7 @@ __device__ inline void r32_expand(const PkJob *__restrict__ jobs, const PkComp *
 			cl.dy0 = (y - hy) - cp.ycen;
 			cl.f = 1.f;
 			cl.gcomp = gc;
-			cl.entry = next + lane;
+			cl.entry = e;
 			cl.meta = (res << 8) | (mr << 16) | ((cp.convolve ? 1u : 0u) << 24);
 			cells[pos++] = cl;
 		}
@@ -264,7 +280,10 @@ __global__ __launch_bounds__(64) void refine32_kernel(const PkJob *__restrict__ 
 	// cell (i % 64) * stride + i / 64 -- so that the cells one wavefront handed
 	// over, often parts of the same deep pixel, go to many wavefronts.
 	const uint32_t stride = (n_cells_in + 63u) / 64u;
-	const uint32_t n_source = SOURCE == 0 ? cnt->n_entries : 64u * stride;
+	const uint32_t n_entries = cnt->n_entries;
+	const uint32_t spread = SOURCE == 0 ? tune.spread_log2 : 0u;
+	const uint32_t sstride = ((((n_entries + (1u << spread) - 1u) >> spread) + 63u) / 64u);
+	const uint32_t n_source = SOURCE == 0 ? (spread ? (64u * sstride) << spread : n_entries) : 64u * stride;
 	uint32_t *head = &cnt->claim[pass];
 	uint32_t overflow = 0, bad = 0;
 	unsigned long long nev = 0;
@@ -290,7 +309,8 @@ __global__ __launch_bounds__(64) void refine32_kernel(const PkJob *__restrict__ 
 				break;
 			}
 			if (SOURCE == 0) {
-				r32_expand(jobs, comps, entries, st.cell, CAP, top, chunk_next, chunk_end, lane);
+				r32_expand(jobs, comps, entries, n_entries, spread, sstride, st.cell, CAP, top, chunk_next, chunk_end,
+				           lane);
 			} else {
 				r32_take_cells(qin, n_cells_in, stride, st.cell, CAP, top, chunk_next, chunk_end, lane);
 			}
@@ -578,7 +598,8 @@ __global__ __launch_bounds__(64) void refine32_kernel(const PkJob *__restrict__ 
 			}
 			if (chunk_next >= chunk_end) break;
 			if (SOURCE == 0) {
-				r32_expand(jobs, comps, entries, st.cell, CAP, top, chunk_next, chunk_end, lane);
+				r32_expand(jobs, comps, entries, n_entries, spread, sstride, st.cell, CAP, top, chunk_next, chunk_end,
+				           lane);
 			} else {
 				r32_take_cells(qin, n_cells_in, stride, st.cell, CAP, top, chunk_next, chunk_end, lane);
 			}

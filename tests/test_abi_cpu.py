@@ -120,7 +120,7 @@ def test_clear_cache_removes_the_tuning_file(L, tmp_path, monkeypatch):
     # profit::clear_cache (src/library.cpp:251-270) removes the cache under $PROFIT_HOME
     monkeypatch.setenv("PROFIT_HOME", str(tmp_path))
     f = tmp_path / "hip_tuning"
-    f.write_text("gfx950/256 fp32 grad0 models1 pixels100 psf0x0 comps1\t0 100 4 64 16 16 1.000000\n")
+    f.write_text("gfx950/256 fp32 grad0 models1 pixels100 psf0x0 comps1\t0 100 4 64 16 16 0 1.000000\n")
     assert L.pk_clear_cache() == 0
     assert not f.exists()
     assert L.pk_clear_cache() == 0  # nothing to remove

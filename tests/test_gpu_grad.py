@@ -163,8 +163,9 @@ def test_tuning_keeps_the_gradient(ctx32, oracle):
     c0, g0 = b.chi2_grad()
     scale = np.max(np.abs(g0), axis=1, keepdims=True)
     base = {"conv_tiles": 0, "refine_occupancy": 100, "handover_div": 4, "claim_pixels": 64, "claim_cells": 16,
-            "poll_rounds": 16}
-    for setting, rtol in (({"handover_div": 2, "refine_occupancy": 50, "claim_pixels": 16, "poll_rounds": 4}, 1e-11),
+            "poll_rounds": 16, "claim_spread": 0}
+    for setting, rtol in (({"handover_div": 2, "refine_occupancy": 50, "claim_pixels": 16, "poll_rounds": 4,
+                            "claim_spread": 16}, 1e-11),
                           ({"conv_tiles": 1}, 1e-5)):
         b.tuning = {**base, **setting}
         b.evaluate_grad()
