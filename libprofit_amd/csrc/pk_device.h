@@ -671,6 +671,30 @@ __global__ __launch_bounds__(256) void compose_kernel(const PkJob *__restrict__ 
 	const uint32_t i = (t % jb.otiles_x) * 64 + (threadIdx.x & 63);
 	const uint32_t j0 = (t / jb.otiles_x) * PX_ROWS + (threadIdx.x >> 6) * PX_RPT;
 	if (i >= jb.ow) return;
+	if (jb.fs == 1 && jb.omask_off == NONE) {
+		// no downsampling, no final mask (the batch case): all loads of the
+		// lane's rows are issued before the first sum
+		TI pc[PX_RPT], pn[PX_RPT];
+		const int X = jb.cox + (int)i;
+		const int mx = X - jb.mox;
+		const bool xin = X >= 0 && X < (int)jb.cw && (int)i < (int)jb.crw;
+		const bool mxin = mx >= 0 && mx < (int)jb.dw;
+#pragma unroll
+		for (int r = 0; r < PX_RPT; r++) {
+			const uint32_t j = j0 + r;
+			const int Y = jb.coy + (int)j, my = Y - jb.moy;
+			const bool in = xin && j < jb.oh && (int)j < (int)jb.crh && Y >= 0 && Y < (int)jb.ch;
+			pc[r] = (jb.has_conv && in) ? conv_all[jb.conv_off + X + (size_t)Y * jb.cw] : (TI)0;
+			pn[r] = (jb.has_nc && in && mxin && my >= 0 && my < (int)jb.dh) ? img_nc[jb.img_off + mx + (size_t)my * jb.dw]
+			                                                               : (TI)0;
+		}
+#pragma unroll
+		for (int r = 0; r < PX_RPT; r++) {
+			const uint32_t j = j0 + r;
+			if (j < jb.oh) out_all[jb.out_off + i + (size_t)j * jb.ow] = (TO)((double)pc[r] + (double)pn[r]);
+		}
+		return;
+	}
 #pragma unroll
 	for (int r = 0; r < PX_RPT; r++) {
 		const uint32_t j = j0 + r;
