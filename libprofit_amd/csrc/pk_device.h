@@ -469,6 +469,9 @@ __device__ inline void conv_tile(const PkJob &jb, const T *__restrict__ src, con
 	}
 	__syncthreads();
 	const int lx = threadIdx.x & 63, ly = (threadIdx.x >> 6) * CT_R;
+	// a wavefront whose rows all lie below the image has no output (nothing
+	// after this point waits for the workgroup)
+	if (by + ly >= (int)jb.ch) return;
 	ColSum<T> acc[CT_R];
 	const T *trow = tile + (size_t)ly * TW + lx;
 	for (int k = 0; k < kw; k++) {

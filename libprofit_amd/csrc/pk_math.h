@@ -124,6 +124,7 @@ PK_HD double qgamma(double p, double a)
 	double lg = lgamma(a);
 	const bool upper = p > 0.5;
 	const double q = 1 - p;  // exact for p > 0.5
+	double hprev = INFINITY;
 	for (int it = 0; it < 200; it++) {
 		double f = upper ? q - incgamma(x, a, true) : incgamma(x, a, false) - p;
 		double pdf = exp((a - 1) * log(x) - x - lg);
@@ -137,6 +138,11 @@ PK_HD double qgamma(double p, double a)
 			x = xn;
 			break;
 		}
+		// Halley steps shrink cubically; a step no smaller than half the last
+		// one at the 1e-13 level is rounding noise of P (the device exp/log
+		// can leave it a few ulps above 4e-16 forever): x is converged
+		if (fabs(h) >= 0.5 * fabs(hprev) && fabs(h) <= 1e-13 * x) break;
+		hprev = h;
 		x = xn;
 	}
 	return x;
