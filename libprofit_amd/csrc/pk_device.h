@@ -460,12 +460,35 @@ __device__ inline void conv_tile(const PkJob &jb, const T *__restrict__ src, con
 	const int hw = kw / 2, hh = kh / 2;
 	const int ox = bx + jb.csx, oy = by + jb.csy;  // tile origin in the same-size frame
 	const int TW = CT_W + kw - 1, TH = CT_H + nl - 1;
-	for (int e = threadIdx.x; e < TW * TH; e += blockDim.x) {
-		const int ty = e / TW, tx = e - ty * TW;
-		const int si = ox + tx - hw, sj = oy + ty + l0 - hh;
-		T v = 0;
-		if (si >= 0 && si < W && sj >= 0 && sj < H) v = src[si + (size_t)sj * W];
-		tile[e] = v;
+	// source tile + halo, four loads in flight per thread; the row of element e
+	// from a float reciprocal, corrected by one step (no integer division)
+	const int n_el = TW * TH;
+	const float inv_tw = 1.f / (float)TW;
+	for (int e0 = (int)threadIdx.x; e0 < n_el; e0 += 4 * 256) {
+		T v[4];
+#pragma unroll
+		for (int u = 0; u < 4; u++) {
+			const int e = e0 + u * 256;
+			v[u] = 0;
+			if (e < n_el) {
+				int ty = (int)(((float)e + 0.5f) * inv_tw);
+				int tx = e - ty * TW;
+				if (tx < 0) {
+					ty--;
+					tx += TW;
+				} else if (tx >= TW) {
+					ty++;
+					tx -= TW;
+				}
+				const int si = ox + tx - hw, sj = oy + ty + l0 - hh;
+				if (si >= 0 && si < W && sj >= 0 && sj < H) v[u] = src[si + (size_t)sj * W];
+			}
+		}
+#pragma unroll
+		for (int u = 0; u < 4; u++) {
+			const int e = e0 + u * 256;
+			if (e < n_el) tile[e] = v[u];
+		}
 	}
 	__syncthreads();
 	const int lx = threadIdx.x & 63, ly = (threadIdx.x >> 6) * CT_R;
