@@ -234,11 +234,15 @@ enum {
 
 /* Launch settings of a batch.  None of them changes which pixels are refined
  * or how; the refinement settings do not change the result at all (its sums
- * are exact fixed-point), the convolution variant changes the FP32 rounding
- * of convolved pixels only.  Defaults: VALU tiles, 100, 4, 64, 16, 16, 0. */
-enum { PK_CONV_TILES_VALU = 0, PK_CONV_TILES_MFMA = 1 };
+ * are exact fixed-point), the MFMA convolution tiles change the FP32 rounding
+ * of convolved pixels only.
+ * Defaults: VALU tiles, 100, 4, 64, 16, 16, 0. */
+enum {
+	PK_CONV_TILES_VALU = 0,  /* FP32 FMA tiles */
+	PK_CONV_TILES_MFMA = 1   /* matrix-core tiles (banded PSF rows) */
+};
 typedef struct pk_tuning {
-	uint32_t conv_tiles;       /* FP32 direct convolution: PK_CONV_TILES_VALU or _MFMA */
+	uint32_t conv_tiles;       /* FP32 direct convolution: PK_CONV_TILES_* (FP64 always uses VALU tiles) */
 	uint32_t refine_occupancy; /* FP32 refinement wavefronts, percent (1..100) of the most that fit the GPU */
 	uint32_t handover_div;     /* a refinement pass hands its work on once 1/handover_div of its
 	                              wavefronts ran dry (>= 1) */

@@ -617,9 +617,9 @@ void psf_conv_layout(const double *krn, uint32_t kw, uint32_t kh, T *kt)
 template <typename T>
 int launch_conv(hipStream_t s, const PkJob *djobs, uint32_t njobs, uint32_t ntiles, const T *src, const T *kt,
                 const uint8_t *masks, T *dst, uint32_t max_kw, uint32_t max_kh, const uint32_t *tile_job = nullptr,
-                bool mfma = false)
+                uint32_t tiles = PK_CONV_TILES_VALU)
 {
-	if (mfma && sizeof(T) == 4 && conv_mfma_lds_bytes((int)max_kw, (int)max_kh) <= LDS_LIMIT) {
+	if (tiles == PK_CONV_TILES_MFMA && sizeof(T) == 4 && conv_mfma_lds_bytes((int)max_kw, (int)max_kh) <= LDS_LIMIT) {
 		// FP32 on request: the matrix-core form, whole PSF in one launch
 		hipLaunchKernelGGL(conv_mfma_kernel, dim3(ntiles), dim3(256),
 		                   conv_mfma_lds_bytes((int)max_kw, (int)max_kh), s, djobs, njobs,
@@ -1459,7 +1459,7 @@ int batch_launch(pk_batch *b)
 		if (timed) PK_HIP(hipEventRecord(b->ev[3], s));
 		if (b->any_conv) {
 			PK_TRY(launch_conv<float>(s, djobs, nj, b->nctiles, c, b->d<float>(b->o_psfT), dmask, b->d<float>(b->o_conv),
-			                          b->max_kw, b->max_kh, tj_conv, b->tune.conv_tiles == PK_CONV_TILES_MFMA));
+			                          b->max_kw, b->max_kh, tj_conv, b->tune.conv_tiles));
 			PK_HIP(hipGetLastError());
 		}
 		if (timed) PK_HIP(hipEventRecord(b->ev[4], s));
@@ -1511,7 +1511,7 @@ int grad_launch_t(pk_batch *b)
 	if (b->any_conv)
 		PK_TRY(launch_conv<T>(s, b->d<PkJob>(b->o_jobs_adj), nj, b->natiles, Gd, b->d<T>(b->o_psfF),
 		                      b->d<uint8_t>(b->o_mask), Gc, b->max_kw, b->max_kh, b->d<uint32_t>(b->o_tja),
-		                      b->tune.conv_tiles == PK_CONV_TILES_MFMA));
+		                      b->tune.conv_tiles));
 	const bool fp32 = sizeof(T) == 4;
 	hipLaunchKernelGGL((coarse_grad_kernel<T, fp32>), dim3(b->ntiles), dim3(256), 0, s, djobs, nj,
 	                   (const PkComp *)dcomp, (const uint8_t *)b->d<uint8_t>(b->o_mask), (const T *)Gd,

@@ -556,8 +556,11 @@ __device__ inline bool conv_dispatch(int kw, const PkJob &jb, const T *src, cons
 // Direct "same" convolution of every job (src/convolve.cpp:101-231).  With
 // l0/nl the launch covers PSF rows [l0, l0+nl) only and, if l0 > 0, adds to
 // dst (large PSFs are worked through in row chunks that fit in LDS).
+// The launch bounds' second argument (one workgroup per CU at least) is
+// deliberate: with it hipcc allocates 88 VGPRs for the FP32 kernel (five
+// wavefronts per SIMD) instead of 153, and the kernel ran 4 % faster.
 template <typename T>
-__global__ __launch_bounds__(256) void conv_kernel(const PkJob *__restrict__ jobs, uint32_t njobs,
+__global__ __launch_bounds__(256, 1) void conv_kernel(const PkJob *__restrict__ jobs, uint32_t njobs,
                                                    const T *__restrict__ src_all, const T *__restrict__ krn_all,
                                                    const uint8_t *__restrict__ masks, T *__restrict__ dst_all,
                                                    int l0, int nl, const uint32_t *__restrict__ tile_job)
