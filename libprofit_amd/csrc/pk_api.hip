@@ -1522,18 +1522,23 @@ int grad_launch_t(pk_batch *b)
 		const CompLite *dlite = b->d<CompLite>(b->o_lite);
 		RCell *q0 = b->d<RCell>(b->o_q), *q1 = q0 + b->qcap;
 		const uint32_t gm = b->grid(ctx->r32g_grid_main), gl = b->grid(ctx->r32g_grid_last);
-		const float *Gf = (const float *)Gd, *Gcf = (const float *)Gc;
 		const R32Tune hd = b->r32();
+		// per-entry adjoint weights, in the fixed-point accumulators of the
+		// forward pass (free once it has finished; 2 doubles per entry)
+		double *gent = b->d<double>(b->o_acc);
+		hipLaunchKernelGGL(grad_entry_weights_kernel<T>, dim3(1024), dim3(256), 0, s, djobs, dent,
+		                   (const PkCounters *)b->d<PkCounters>(b->o_cnt), (const T *)Gd, (const T *)Gc, gent);
+		PK_HIP(hipGetLastError());
 		hipLaunchKernelGGL((refine32_kernel<0, false, true>), dim3(gm), dim3(64), 0, s, djobs, dcomp, dlite, dent,
-		                   gcnt, (const RCell *)nullptr, q0, b->qcap, 0, (unsigned long long *)nullptr, hd, Gf, Gcf,
-		                   gacc);
+		                   gcnt, (const RCell *)nullptr, q0, b->qcap, 0, (unsigned long long *)nullptr, hd,
+		                   (const double *)gent, gacc);
 		for (int p = 1; p < R32_PASSES - 1; p++)
 			hipLaunchKernelGGL((refine32_kernel<1, false, true>), dim3(gm), dim3(64), 0, s, djobs, dcomp, dlite, dent,
 			                   gcnt, (const RCell *)((p & 1) ? q0 : q1), (p & 1) ? q1 : q0, b->qcap, p,
-			                   (unsigned long long *)nullptr, hd, Gf, Gcf, gacc);
+			                   (unsigned long long *)nullptr, hd, (const double *)gent, gacc);
 		hipLaunchKernelGGL((refine32_kernel<1, true, true>), dim3(gl), dim3(64), 0, s, djobs, dcomp, dlite, dent, gcnt,
 		                   (const RCell *)((R32_PASSES & 1) ? q1 : q0), (RCell *)nullptr, b->qcap, R32_PASSES - 1,
-		                   (unsigned long long *)nullptr, hd, Gf, Gcf, gacc);
+		                   (unsigned long long *)nullptr, hd, (const double *)gent, gacc);
 	} else {
 		hipLaunchKernelGGL((refine_kernel<double, false, true>), dim3(4096), dim3(64), 0, s, djobs,
 		                   (const PkComp *)dcomp, dent, gcnt, (double *)nullptr, (double *)nullptr,

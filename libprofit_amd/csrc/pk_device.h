@@ -815,6 +815,28 @@ __global__ __launch_bounds__(256) void resid_kernel(const PkJob *__restrict__ jo
 	if (threadIdx.x == 0 && c2[0] != 0) atomicAdd(&chi2[jn], c2[0]);
 }
 
+// Adjoint weights of the refinement entries (FP32 mode): gent[2 e + c] =
+// G_c or G_nc at entry e's pixel, divided by its job's fixed-point scale
+// (the refinement's values carry that scale).
+template <typename T>
+__global__ __launch_bounds__(256) void grad_entry_weights_kernel(const PkJob *__restrict__ jobs,
+                                                                 const RefineEntry *__restrict__ entries,
+                                                                 const PkCounters *__restrict__ cnt,
+                                                                 const T *__restrict__ G_nc,
+                                                                 const T *__restrict__ G_c,
+                                                                 double *__restrict__ gent)
+{
+	const uint32_t n = cnt->n_entries;
+	for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
+		const RefineEntry en = entries[e];
+		const PkJob &jb = jobs[en.job];
+		const size_t p = jb.img_off + en.pix;
+		const double inv = 1.0 / jb.fx_scale;
+		gent[2 * (size_t)e] = (double)G_nc[p] * inv;
+		gent[2 * (size_t)e + 1] = jb.has_conv ? (double)G_c[p] * inv : 0.0;
+	}
+}
+
 // Adjoint of the coarse pass: every pixel a component evaluates directly
 // (src/radial.cpp:316-324) contributes G * flux_scale * f * dL/dθ; the
 // pixels refined by the forward pass are left to the refinement adjoint.

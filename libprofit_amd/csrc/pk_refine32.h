@@ -249,11 +249,28 @@ __device__ inline void r32_flush_grad(bool ok, uint32_t gcomp, double ga[GA_SLOT
 	}
 }
 
+// Adds the per-lane running adjoint sums of component g (wavefront-uniform)
+// to gacc and clears them; g = ~0u: nothing held.
+__device__ inline void r32_flush_run(double run[GA_SLOTS], uint32_t g, double *__restrict__ gacc, uint32_t lane)
+{
+	if (g == ~0u) return;
+#pragma unroll
+	for (int k = 0; k < GA_SLOTS; k++) {
+		const double v = wave_sum(run[k]);
+		if (lane == 0 && v != 0) atomicAdd(&gacc[(size_t)g * GA_SLOTS + k], v);
+		run[k] = 0;
+	}
+}
+
 // SOURCE 0: pixels from the coarse pass (pass 0); 1: cells of the previous pass.
 // LAST: the final pass, which hands nothing over.
 // GRAD: the chi^2 adjoint of the refinement -- the same cells and decisions as
 // the forward pass, but every accepted sub-cell adds G w flux_scale f dL/dθ
-// to gacc (G_nc / G_c: adjoint images) instead of its value to the pixel.
+// to gacc instead of its value to the pixel.  gent[2 e + convolved] holds
+// G / fx_scale of refinement entry e (G: the adjoint image at its pixel).
+// Rounds whose accepted sub-cells all belong to one component add to
+// per-lane running sums, summed over the wavefront only when the component
+// changes.
 template <int SOURCE, bool LAST, bool GRAD = false>
 __global__ __launch_bounds__(64) void refine32_kernel(const PkJob *__restrict__ jobs, const PkComp *__restrict__ comps,
                                                       const CompLite *__restrict__ lite,
@@ -262,8 +279,7 @@ __global__ __launch_bounds__(64) void refine32_kernel(const PkJob *__restrict__ 
                                                       RCell *__restrict__ qout, uint32_t qcap, int pass,
                                                       unsigned long long *__restrict__ acc,
                                                       R32Tune tune,
-                                                      const float *__restrict__ G_nc = nullptr,
-                                                      const float *__restrict__ G_c = nullptr,
+                                                      const double *__restrict__ gent = nullptr,
                                                       double *__restrict__ gacc = nullptr)
 {
 	constexpr int CAP = LAST ? R32_CAP_LAST : R32_CAP_MAIN;
@@ -289,6 +305,8 @@ __global__ __launch_bounds__(64) void refine32_kernel(const PkJob *__restrict__ 
 	unsigned long long nev = 0;
 	uint32_t chunk_next = 0, chunk_end = 0, seen = 0;
 	int poll = 0;
+	double run[GA_SLOTS] = {0, 0, 0, 0, 0, 0, 0, 0};  // GRAD: running sums of component run_g
+	uint32_t run_g = ~0u;
 	__syncthreads();
 	for (;;) {
 		// ---- refill while the cells on the LIFO cannot fill the lanes (two
@@ -423,6 +441,7 @@ __global__ __launch_bounds__(64) void refine32_kernel(const PkJob *__restrict__ 
 		const float band = 1e-4f * (1.f + acc_thr);
 		const double ddy = (-2 * hx * cl.sin_ang + 2 * hy * cl.cos_ang) * cl.inv_axrat;
 		const size_t target = 2 * (size_t)cell.entry + cm_conv(meta);
+		const double gw = GRAD ? gent[target] : 0.0;  // G / fx_scale of the cell's pixel
 		const int kind = cl.fp32 ? (int)cl.kind : -1;
 		// wavefront-uniform facts of this group of rounds (lanes beyond the last
 		// task, which are inactive, share the last task's)
@@ -521,16 +540,26 @@ __global__ __launch_bounds__(64) void refine32_kernel(const PkJob *__restrict__ 
 					for (int q = 0; q < P; q++) {
 						if (active[q] && !fail[q]) {
 							any_ok = true;
-							const RefineEntry en = entries[cell.entry];
-							const PkJob &jb = jobs[en.job];
-							const size_t p = jb.img_off + en.pix;
-							const double G = (double)(cm_conv(meta) ? G_c[p] : G_nc[p]);
 							float gt[GT_COUNT];
 							const float vg = sersic_grad32(cl, (float)xp[q], (float)yp[q], gt);
-							grad_accumulate(ga, G * ((double)(fr * fr) * cl.wq / jb.fx_scale) * (double)vg, gt);
+							grad_accumulate(ga, gw * ((double)(fr * fr) * cl.wq) * (double)vg, gt);
 						}
 					}
-					r32_flush_grad(any_ok, cell.gcomp, ga, gacc, lane);
+					const uint64_t okb = __ballot(any_ok);
+					if (okb) {
+						const uint32_t g0 = wave_read(cell.gcomp, __ffsll((long long)okb) - 1);
+						if (__ballot(any_ok && cell.gcomp != g0) == 0) {
+							// one component: keep adding to the running sums
+							if (g0 != run_g) {
+								r32_flush_run(run, run_g, gacc, lane);
+								run_g = g0;
+							}
+#pragma unroll
+							for (int k = 0; k < GA_SLOTS; k++) run[k] += ga[k];
+						} else {
+							r32_flush_grad(any_ok, cell.gcomp, ga, gacc, lane);
+						}
+					}
 				} else if (ntask == 1) {
 					// exact per-cell sum; lane 0 adds it to the pixel accumulator
 					const unsigned long long v = wave_sum_u64(fxv);
@@ -582,6 +611,7 @@ __global__ __launch_bounds__(64) void refine32_kernel(const PkJob *__restrict__ 
 		if (pair) run_rounds(IntTag<2>{});
 		else run_rounds(IntTag<1>{});
 	}
+	if (GRAD) r32_flush_run(run, run_g, gacc, lane);
 	if (!LAST && handover) {
 		// everything this wavefront holds goes to the next pass
 		for (;;) {
