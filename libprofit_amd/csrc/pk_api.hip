@@ -617,23 +617,27 @@ void psf_conv_layout(const double *krn, uint32_t kw, uint32_t kh, T *kt)
 template <typename T>
 int launch_conv(hipStream_t s, const PkJob *djobs, uint32_t njobs, uint32_t ntiles, const T *src, const T *kt,
                 const uint8_t *masks, T *dst, uint32_t max_kw, uint32_t max_kh, const uint32_t *tile_job = nullptr,
-                uint32_t tiles = PK_CONV_TILES_VALU)
+                uint32_t tiles = PK_CONV_TILES_VALU, T *out_all = nullptr, const T *img_nc = nullptr)
 {
 	if (tiles == PK_CONV_TILES_MFMA && sizeof(T) == 4 && conv_mfma_lds_bytes((int)max_kw, (int)max_kh) <= LDS_LIMIT) {
 		// FP32 on request: the matrix-core form, whole PSF in one launch
 		hipLaunchKernelGGL(conv_mfma_kernel, dim3(ntiles), dim3(256),
 		                   conv_mfma_lds_bytes((int)max_kw, (int)max_kh), s, djobs, njobs,
 		                   reinterpret_cast<const float *>(src), reinterpret_cast<const float *>(kt), masks,
-		                   reinterpret_cast<float *>(dst), tile_job);
+		                   reinterpret_cast<float *>(dst), tile_job, reinterpret_cast<float *>(out_all),
+		                   reinterpret_cast<const float *>(img_nc));
 		PK_HIP(hipGetLastError());
 		return PK_OK;
 	}
 	uint32_t rows;
 	PK_TRY(conv_rows_per_launch(max_kw, max_kh, sizeof(T), rows));
+	if (out_all && rows < max_kh) return set_err(PK_EINVAL, "fused composition needs a single convolution launch");
 	const size_t lds = conv_lds_bytes(max_kw, rows, sizeof(T));
 	for (uint32_t l0 = 0; l0 < max_kh; l0 += rows) {
+		// the returned image is written directly only by a single launch
+		const bool last = l0 + rows >= max_kh;
 		hipLaunchKernelGGL(conv_kernel<T>, dim3(ntiles), dim3(256), lds, s, djobs, njobs, src, kt, masks, dst,
-		                   (int)l0, (int)rows, tile_job);
+		                   (int)l0, (int)rows, tile_job, (l0 == 0 && last) ? out_all : (T *)nullptr, img_nc);
 		PK_HIP(hipGetLastError());
 	}
 	return PK_OK;
@@ -1459,12 +1463,13 @@ int batch_launch(pk_batch *b)
 		if (timed) PK_HIP(hipEventRecord(b->ev[3], s));
 		if (b->any_conv) {
 			PK_TRY(launch_conv<float>(s, djobs, nj, b->nctiles, c, b->d<float>(b->o_psfT), dmask, b->d<float>(b->o_conv),
-			                          b->max_kw, b->max_kh, tj_conv, b->tune.conv_tiles));
+			                          b->max_kw, b->max_kh, tj_conv, b->tune.conv_tiles, b->d<float>(b->o_out), nc));
 			PK_HIP(hipGetLastError());
 		}
 		if (timed) PK_HIP(hipEventRecord(b->ev[4], s));
-		hipLaunchKernelGGL((compose_kernel<float, float>), dim3(b->notiles), dim3(256), 0, s, djobs, nj, nc,
-		                   b->d<float>(b->o_conv), (const uint8_t *)nullptr, b->d<float>(b->o_out), tj_out);
+		if (b->notiles)
+			hipLaunchKernelGGL((compose_kernel<float, float>), dim3(b->notiles), dim3(256), 0, s, djobs, nj, nc,
+			                   b->d<float>(b->o_conv), (const uint8_t *)nullptr, b->d<float>(b->o_out), tj_out);
 	} else {
 		double *nc = b->d<double>(b->o_nc), *c = b->d<double>(b->o_c);
 		PK_TRY(launch_profiles<double>(s, false, djobs, nj, b->ntiles, dcomp, nullptr, dmask, nc, c, dent, dcnt, nullptr,
@@ -1473,12 +1478,14 @@ int batch_launch(pk_batch *b)
 		if (timed) PK_HIP(hipEventRecord(b->ev[3], s));
 		if (b->any_conv) {
 			PK_TRY(launch_conv<double>(s, djobs, nj, b->nctiles, c, b->d<double>(b->o_psfT), dmask,
-			                           b->d<double>(b->o_conv), b->max_kw, b->max_kh, tj_conv));
+			                           b->d<double>(b->o_conv), b->max_kw, b->max_kh, tj_conv, PK_CONV_TILES_VALU,
+			                           b->d<double>(b->o_out), nc));
 			PK_HIP(hipGetLastError());
 		}
 		if (timed) PK_HIP(hipEventRecord(b->ev[4], s));
-		hipLaunchKernelGGL((compose_kernel<double, double>), dim3(b->notiles), dim3(256), 0, s, djobs, nj, nc,
-		                   b->d<double>(b->o_conv), (const uint8_t *)nullptr, b->d<double>(b->o_out), tj_out);
+		if (b->notiles)
+			hipLaunchKernelGGL((compose_kernel<double, double>), dim3(b->notiles), dim3(256), 0, s, djobs, nj, nc,
+			                   b->d<double>(b->o_conv), (const uint8_t *)nullptr, b->d<double>(b->o_out), tj_out);
 	}
 	PK_HIP(hipGetLastError());
 	if (timed) PK_HIP(hipEventRecord(b->ev[5], s));
@@ -1833,9 +1840,9 @@ int pk_batch_create(pk_ctx *ctx, const pk_model *models, uint32_t n_models, uint
 		jb.ow = ow;
 		jb.oh = oh;
 		jb.out_off = out_off;
-		jb.otile_off = b->notiles;
-		jb.otiles_x = cdiv(ow, 64);
-		b->notiles += jb.otiles_x * cdiv(oh, PX_ROWS);
+		// without downsampling the convolved frame is the returned image: the
+		// convolution writes it (composition tiles are set below)
+		jb.fuse_out = (a.conv_required && jb.fs == 1) ? 1 : 0;
 		jb.omask_off = NONE;
 		jb.omask_f = 1;
 		jb.fx_scale = fixed_point_scale(m);
@@ -1848,8 +1855,17 @@ int pk_batch_create(pk_ctx *ctx, const pk_model *models, uint32_t n_models, uint
 	b->n_pixels = img_off;
 	b->total_out = out_off;
 	{
-		uint32_t rows;
+		uint32_t rows = 0;
 		if (b->any_conv && conv_rows_per_launch(b->max_kw, b->max_kh, elem, rows)) return fail(PK_EINVAL);
+		// the convolution writes returned images only when it runs in one
+		// launch; the composition covers every other job
+		const bool one_launch = !b->any_conv || rows >= b->max_kh;
+		for (auto &jb : b->jobs) {
+			if (!one_launch) jb.fuse_out = 0;
+			jb.otile_off = b->notiles;
+			jb.otiles_x = jb.fuse_out ? 0 : cdiv(jb.ow, 64);
+			b->notiles += jb.otiles_x * cdiv(jb.oh, PX_ROWS);
+		}
 	}
 	if (img_off >= (1ull << 32)) return fail(set_err(PK_EINVAL, "batch too large (%llu pixels)", (unsigned long long)img_off));
 
@@ -1897,7 +1913,9 @@ int pk_batch_create(pk_ctx *ctx, const pk_model *models, uint32_t n_models, uint
 	b->o_cnt = L.take(sizeof(PkCounters));
 	b->o_nc = L.take(elem * img_off);
 	b->o_c = L.take(elem * (b->any_conv ? img_off : 1));
-	b->o_conv = L.take(elem * (b->any_conv ? img_off : 1));
+	bool conv_buffer = false;  // some convolved job leaves its composition to compose_kernel
+	for (const auto &jb : b->jobs) conv_buffer = conv_buffer || (jb.has_conv && !jb.fuse_out);
+	b->o_conv = L.take(elem * (conv_buffer ? img_off : 1));
 	b->o_ent = L.take(sizeof(RefineEntry) * img_off);
 	b->o_acc = L.take(fp32 ? 2 * sizeof(unsigned long long) * img_off : 8);
 	b->qcap = fp32 ? R32_QCAP_BATCH : 0;

@@ -450,10 +450,36 @@ __device__ inline float fma_t(float a, float b, float c) { return fmaf(a, b, c);
 // `kt` holds the PSF reversed and transposed: kt[k * kh + l] = krn(kw-1-k, kh-1-l),
 // so the values of one PSF column are contiguous and uniform across the
 // wavefront (scalar loads).  NL > 0 fixes nl (and l0 = 0) at compile time.
+template <typename T>
+struct ConvOut {
+	T *__restrict__ dst;           // convolved image buffer
+	T *__restrict__ out;           // returned images (jobs with fuse_out), or nullptr
+	const T *__restrict__ img_nc;  // unconvolved drawing images added to them
+};
+
+// Stores convolved pixel (X, Y) of job jb: into the convolved image, or -- a
+// job with fuse_out -- as returned pixel (X, Y) with the unconvolved part
+// added exactly as the composition does (sum in FP64, src/model.cpp:289-306).
+template <typename T>
+__device__ inline void conv_store(const PkJob &jb, const ConvOut<T> &o, int X, int Y, T v)
+{
+	if (o.out && jb.fuse_out) {
+		double r = (double)v;
+		if (jb.has_nc) {
+			const int mx = X - jb.mox, my = Y - jb.moy;
+			if (mx >= 0 && my >= 0 && mx < (int)jb.dw && my < (int)jb.dh)
+				r += (double)o.img_nc[jb.img_off + mx + (size_t)my * jb.dw];
+		}
+		o.out[jb.out_off + X + (size_t)Y * jb.ow] = (T)r;
+	} else {
+		o.dst[jb.conv_off + X + (size_t)Y * jb.cw] = v;
+	}
+}
+
 template <typename T, int NL>
 __device__ inline void conv_tile(const PkJob &jb, const T *__restrict__ src, const T *__restrict__ kt, T *tile,
                                  int bx, int by, int l0, int nl_rt, bool accumulate, const uint8_t *__restrict__ masks,
-                                 T *__restrict__ dst)
+                                 const ConvOut<T> &o)
 {
 	const int nl = NL > 0 ? NL : nl_rt;
 	const int W = (int)jb.dw, H = (int)jb.dh, kw = (int)jb.kw, kh = (int)jb.kh;
@@ -529,27 +555,26 @@ __device__ inline void conv_tile(const PkJob &jb, const T *__restrict__ src, con
 	for (int r = 0; r < CT_R; r++) {
 		const int X = bx + lx, Y = by + ly + r;
 		if (X < (int)jb.cw && Y < (int)jb.ch) {
-			const size_t p = X + (size_t)Y * jb.cw;
 			T v = acc[r].get();
-			if (accumulate) v += dst[p];
+			if (accumulate) v += o.dst[jb.conv_off + X + (size_t)Y * jb.cw];
 			if (jb.mask_off != NONE) {
 				const int mi = X - jb.cmx, mj = Y - jb.cmy;
 				if (mi < 0 || mj < 0 || mi >= W || mj >= H || !masks[jb.mask_off + mi + (size_t)mj * W]) v = 0;
 			}
-			dst[p] = v;
+			conv_store(jb, o, X, Y, v);
 		}
 	}
 }
 
 template <typename T, int K>
 __device__ inline bool conv_dispatch(int kw, const PkJob &jb, const T *src, const T *kt, T *tile, int bx, int by,
-                                     const uint8_t *masks, T *dst)
+                                     const uint8_t *masks, const ConvOut<T> &o)
 {
 	if (kw == K) {
-		conv_tile<T, K>(jb, src, kt, tile, bx, by, 0, K, false, masks, dst);
+		conv_tile<T, K>(jb, src, kt, tile, bx, by, 0, K, false, masks, o);
 		return true;
 	}
-	if constexpr (K + 2 <= CT_KMAX) return conv_dispatch<T, K + 2>(kw, jb, src, kt, tile, bx, by, masks, dst);
+	if constexpr (K + 2 <= CT_KMAX) return conv_dispatch<T, K + 2>(kw, jb, src, kt, tile, bx, by, masks, o);
 	return false;
 }
 
@@ -563,7 +588,8 @@ template <typename T>
 __global__ __launch_bounds__(256, 1) void conv_kernel(const PkJob *__restrict__ jobs, uint32_t njobs,
                                                    const T *__restrict__ src_all, const T *__restrict__ krn_all,
                                                    const uint8_t *__restrict__ masks, T *__restrict__ dst_all,
-                                                   int l0, int nl, const uint32_t *__restrict__ tile_job)
+                                                   int l0, int nl, const uint32_t *__restrict__ tile_job,
+                                                   T *__restrict__ out_all, const T *__restrict__ img_nc)
 {
 	extern __shared__ unsigned char smem_raw[];
 	T *tile = reinterpret_cast<T *>(smem_raw);
@@ -573,14 +599,14 @@ __global__ __launch_bounds__(256, 1) void conv_kernel(const PkJob *__restrict__ 
 	const int bx = (int)(t % jb.ctiles_x) * CT_W, by = (int)(t / jb.ctiles_x) * CT_H;
 	const T *src = src_all + jb.img_off;
 	const T *kt = krn_all + jb.psf_off;
-	T *dst = dst_all + jb.conv_off;
+	const ConvOut<T> o{dst_all, out_all, img_nc};
 	const int kw = (int)jb.kw, kh = (int)jb.kh;
 	if (l0 == 0 && nl >= kh && kw == kh && (kw & 1) && kw >= CT_KMIN && kw <= CT_KMAX &&
-	    conv_dispatch<T, CT_KMIN>(kw, jb, src, kt, tile, bx, by, masks, dst))
+	    conv_dispatch<T, CT_KMIN>(kw, jb, src, kt, tile, bx, by, masks, o))
 		return;
 	const int n = nl < kh - l0 ? nl : kh - l0;
 	if (n <= 0) return;
-	conv_tile<T, 0>(jb, src, kt, tile, bx, by, l0, n, l0 > 0, masks, dst);
+	conv_tile<T, 0>(jb, src, kt, tile, bx, by, l0, n, l0 > 0, masks, o);
 }
 
 // ---------------------------------------------------------------------------
@@ -615,7 +641,8 @@ __global__ __launch_bounds__(256) void conv_mfma_kernel(const PkJob *__restrict_
                                                         const float *__restrict__ krn_all,
                                                         const uint8_t *__restrict__ masks,
                                                         float *__restrict__ dst_all,
-                                                        const uint32_t *__restrict__ tile_job)
+                                                        const uint32_t *__restrict__ tile_job,
+                                                        float *__restrict__ out_all, const float *__restrict__ img_nc)
 {
 	extern __shared__ unsigned char smem_raw[];
 	float *tile = reinterpret_cast<float *>(smem_raw);
@@ -625,7 +652,7 @@ __global__ __launch_bounds__(256) void conv_mfma_kernel(const PkJob *__restrict_
 	const int bx = (int)(t % jb.ctiles_x) * CT_W, by = (int)(t / jb.ctiles_x) * CT_H;
 	const float *src = src_all + jb.img_off;
 	const float *krn = krn_all + jb.psf_off;
-	float *dst = dst_all + jb.conv_off;
+	const ConvOut<float> o{dst_all, out_all, img_nc};
 	const int W = (int)jb.dw, H = (int)jb.dh, kw = (int)jb.kw, kh = (int)jb.kh;
 	const int hw = kw / 2, hh = kh / 2;
 	const int ox = bx + jb.csx, oy = by + jb.csy;
@@ -678,7 +705,7 @@ __global__ __launch_bounds__(256) void conv_mfma_kernel(const PkJob *__restrict_
 					const int mi = X - jb.cmx, mj = Y - jb.cmy;
 					if (mi < 0 || mj < 0 || mi >= W || mj >= H || !masks[jb.mask_off + mi + (size_t)mj * W]) v = 0.f;
 				}
-				dst[X + (size_t)Y * jb.cw] = v;
+				conv_store(jb, o, X, Y, v);
 			}
 		}
 }

@@ -29,7 +29,8 @@ struct PkJob {
 	int32_t cmx, cmy;            // calculation mask pixel of convolved (X,Y) is (X-cmx, Y-cmy)
 	int32_t mox, moy;            // position of the drawing image inside the convolved frame
 	uint32_t wrap_y;             // FFT convolver: rows of its circular buffer (0: no wrap)
-	uint32_t pad1_;
+	uint32_t fuse_out;           // batches: the convolution writes the returned image itself
+	                             // (convolved frame = returned image, + the unconvolved part)
 	// composition of the returned image (crop, SUM downsample, final mask)
 	int32_t cox, coy;            // crop origin in the convolved frame
 	uint32_t crw, crh;           // crop dims
