@@ -189,3 +189,34 @@ def test_autotune_and_its_cache(ctx32, tmp_path, monkeypatch):
     b3.close()
     b.close()
     M.clear_cache()
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_batch_mixed_composition(ctx64, ctx32, oracle, prec):
+    """Jobs whose convolution writes the returned image (no downsampling) next
+    to jobs composed afterwards (SUM downsampling, no PSF) in one batch."""
+    print("CASE mixed composition", prec, flush=True)
+    psf = workloads.moffat_psf_image(9, 2.5, 3.0)
+    specs = []
+    for k, (fs, rf, with_psf) in enumerate([(1, True, True), (2, False, True), (2, True, True), (1, True, False),
+                                            (3, False, True), (1, True, True)]):
+        s = 30 + 7 * k
+        comps = [abi.make_component("sersic", xcen=0.45 * s, ycen=0.55 * s, mag=15.0, re=3.0 + k, nser=2.5,
+                                    axrat=0.7, ang=20.0 * k, convolve=with_psf),
+                 abi.make_component("sky", bg=0.5)]
+        specs.append(abi.ModelSpec(s, s, comps, psf=psf if with_psf else None, finesampling=fs,
+                                   return_finesampled=rf))
+    ctx = ctx64 if prec == "fp64" else ctx32
+    tol = TOL_FP64 if prec == "fp64" else TOL_FP32
+    b = Batch(ctx, specs, graph=True)
+    for _rep in range(2):
+        b.evaluate()
+        b.sync()
+        for i, s in enumerate(specs):
+            ref, _ = oracle.evaluate(s)
+            got = b.image(i)
+            assert got.shape == ref.shape, (i, got.shape, ref.shape)
+            pix, flux = rel_errors(got, ref)
+            assert pix <= tol, (i, pix)
+            assert flux <= tol, (i, flux)
+    b.close()
