@@ -457,23 +457,25 @@ struct ConvOut {
 	const T *__restrict__ img_nc;  // unconvolved drawing images added to them
 };
 
+// Unconvolved part of returned pixel (X, Y) of a job with fuse_out (0 when
+// the job has none there, or the pixel lies outside the convolved frame).
+template <typename T>
+__device__ inline T conv_unconvolved(const PkJob &jb, const ConvOut<T> &o, int X, int Y)
+{
+	if (!(o.out && jb.fuse_out && jb.has_nc) || X >= (int)jb.cw || Y >= (int)jb.ch) return (T)0;
+	const int mx = X - jb.mox, my = Y - jb.moy;
+	if (mx < 0 || my < 0 || mx >= (int)jb.dw || my >= (int)jb.dh) return (T)0;
+	return o.img_nc[jb.img_off + mx + (size_t)my * jb.dw];
+}
+
 // Stores convolved pixel (X, Y) of job jb: into the convolved image, or -- a
-// job with fuse_out -- as returned pixel (X, Y) with the unconvolved part
+// job with fuse_out -- as returned pixel (X, Y) with its unconvolved part nc
 // added exactly as the composition does (sum in FP64, src/model.cpp:289-306).
 template <typename T>
-__device__ inline void conv_store(const PkJob &jb, const ConvOut<T> &o, int X, int Y, T v)
+__device__ inline void conv_store(const PkJob &jb, const ConvOut<T> &o, int X, int Y, T v, T nc)
 {
-	if (o.out && jb.fuse_out) {
-		double r = (double)v;
-		if (jb.has_nc) {
-			const int mx = X - jb.mox, my = Y - jb.moy;
-			if (mx >= 0 && my >= 0 && mx < (int)jb.dw && my < (int)jb.dh)
-				r += (double)o.img_nc[jb.img_off + mx + (size_t)my * jb.dw];
-		}
-		o.out[jb.out_off + X + (size_t)Y * jb.ow] = (T)r;
-	} else {
-		o.dst[jb.conv_off + X + (size_t)Y * jb.cw] = v;
-	}
+	if (o.out && jb.fuse_out) o.out[jb.out_off + X + (size_t)Y * jb.ow] = (T)((double)v + (double)nc);
+	else o.dst[jb.conv_off + X + (size_t)Y * jb.cw] = v;
 }
 
 template <typename T, int NL>
@@ -521,6 +523,11 @@ __device__ inline void conv_tile(const PkJob &jb, const T *__restrict__ src, con
 	// a wavefront whose rows all lie below the image has no output (nothing
 	// after this point waits for the workgroup)
 	if (by + ly >= (int)jb.ch) return;
+	// unconvolved parts of the lane's returned pixels, loaded before the sums
+	// so that their latency hides behind them
+	T ncv[CT_R];
+#pragma unroll
+	for (int r = 0; r < CT_R; r++) ncv[r] = conv_unconvolved(jb, o, bx + lx, by + ly + r);
 	ColSum<T> acc[CT_R];
 	const T *trow = tile + (size_t)ly * TW + lx;
 	for (int k = 0; k < kw; k++) {
@@ -561,7 +568,7 @@ __device__ inline void conv_tile(const PkJob &jb, const T *__restrict__ src, con
 				const int mi = X - jb.cmx, mj = Y - jb.cmy;
 				if (mi < 0 || mj < 0 || mi >= W || mj >= H || !masks[jb.mask_off + mi + (size_t)mj * W]) v = 0;
 			}
-			conv_store(jb, o, X, Y, v);
+			conv_store(jb, o, X, Y, v, ncv[r]);
 		}
 	}
 }
@@ -705,7 +712,7 @@ __global__ __launch_bounds__(256) void conv_mfma_kernel(const PkJob *__restrict_
 					const int mi = X - jb.cmx, mj = Y - jb.cmy;
 					if (mi < 0 || mj < 0 || mi >= W || mj >= H || !masks[jb.mask_off + mi + (size_t)mj * W]) v = 0.f;
 				}
-				conv_store(jb, o, X, Y, v);
+				conv_store(jb, o, X, Y, v, conv_unconvolved(jb, o, X, Y));
 			}
 		}
 }
