@@ -634,22 +634,84 @@ __global__ __launch_bounds__(256, 1) void conv_kernel(const PkJob *__restrict__ 
 // ---------------------------------------------------------------------------
 typedef float floatx4 __attribute__((ext_vector_type(4)));
 
-// odd row stride of the LDS source tile: the 16 rows one operand read touches
-// fall in distinct banks
-__host__ __device__ inline int conv_mfma_stride(int kw) { return (CT_W + kw - 1) | 1; }
+// LDS layout of the matrix-core form.  Source tile: (CT_W + kw - 1) columns
+// of source plus CM_PAD zero columns (a block reads up to 3 columns past its
+// band), odd row stride so that the 16 rows one operand read touches fall in
+// distinct banks.  PSF band: row l holds K'(d, l) at d + 16 for
+// d in [-16, kw + 20), zero outside [0, kw), so that B[j][x] = K'(j - x, l) is
+// one read without a bounds test.
+constexpr int CM_PAD = 4;
+__host__ __device__ inline int conv_mfma_stride(int kw) { return (CT_W + kw - 1 + CM_PAD) | 1; }
+__host__ __device__ inline int conv_mfma_band(int kw) { return kw + 36; }
 
 __host__ __device__ inline size_t conv_mfma_lds_bytes(int kw, int kh)
 {
-	return sizeof(float) * ((size_t)conv_mfma_stride(kw) * (CT_H + kh - 1) + (size_t)kw * kh);
+	return sizeof(float) *
+	       ((size_t)conv_mfma_stride(kw) * (CT_H + kh - 1) + (size_t)conv_mfma_band(kw) * (size_t)kh);
 }
 
-__global__ __launch_bounds__(256) void conv_mfma_kernel(const PkJob *__restrict__ jobs, uint32_t njobs,
-                                                        const float *__restrict__ src_all,
-                                                        const float *__restrict__ krn_all,
-                                                        const uint8_t *__restrict__ masks,
-                                                        float *__restrict__ dst_all,
-                                                        const uint32_t *__restrict__ tile_job,
-                                                        float *__restrict__ out_all, const float *__restrict__ img_nc)
+// Products of all PSF rows for this wavefront's two 16 x 16 output blocks
+// (rows ry0.., columns cx0.. and cx0 + 16..).  NJ > 0: 4 * NJ source columns
+// per row known at compile time (fully unrolled, operands read before the
+// products); NJ = 0: run-time count.
+template <int NJ>
+__device__ inline void conv_mfma_rows(const float *tile, const float *band, int TS, int BS, int kh, int nj_rt,
+                                      int lane, int ry0, int cx0, double out[2][4])
+{
+	const int lo = lane & 15, kq = lane >> 4;
+	for (int l = 0; l < kh; l++) {
+		floatx4 acc0 = {0.f, 0.f, 0.f, 0.f}, acc1 = {0.f, 0.f, 0.f, 0.f};
+		const float *arow = tile + (size_t)(ry0 + lo + l) * TS + cx0 + kq;  // A[lo][j], j = 4 jc + kq
+		const float *brow = band + (size_t)l * BS + 16 + kq - lo;           // B[j][lo] = K'(j - lo, l)
+		if (NJ > 0) {
+			float a0[NJ > 0 ? NJ : 1], a1[NJ > 0 ? NJ : 1], b[NJ > 0 ? NJ : 1];
+#pragma unroll
+			for (int jc = 0; jc < NJ; jc++) {
+				a0[jc] = arow[4 * jc];
+				a1[jc] = arow[16 + 4 * jc];
+				b[jc] = brow[4 * jc];
+			}
+#pragma unroll
+			for (int jc = 0; jc < NJ; jc++) {
+				acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(a0[jc], b[jc], acc0, 0, 0, 0);
+				acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(a1[jc], b[jc], acc1, 0, 0, 0);
+			}
+		} else {
+			for (int jc = 0; jc < nj_rt; jc++) {
+				const float bv = brow[4 * jc];
+				acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(arow[4 * jc], bv, acc0, 0, 0, 0);
+				acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(arow[16 + 4 * jc], bv, acc1, 0, 0, 0);
+			}
+		}
+#pragma unroll
+		for (int r = 0; r < 4; r++) {
+			out[0][r] += (double)acc0[r];
+			out[1][r] += (double)acc1[r];
+		}
+	}
+}
+
+// NJ = ceil((16 + kw - 1) / 4) for the odd PSF widths CT_KMIN..CT_KMAX
+template <int K>
+__device__ inline bool conv_mfma_dispatch(int kw, const float *tile, const float *band, int TS, int BS, int kh,
+                                          int lane, int ry0, int cx0, double out[2][4])
+{
+	if (kw == K) {
+		conv_mfma_rows<(16 + K - 1 + 3) / 4>(tile, band, TS, BS, kh, 0, lane, ry0, cx0, out);
+		return true;
+	}
+	if constexpr (K + 2 <= CT_KMAX) return conv_mfma_dispatch<K + 2>(kw, tile, band, TS, BS, kh, lane, ry0, cx0, out);
+	return false;
+}
+
+__global__ __launch_bounds__(256, 1) void conv_mfma_kernel(const PkJob *__restrict__ jobs, uint32_t njobs,
+                                                           const float *__restrict__ src_all,
+                                                           const float *__restrict__ krn_all,
+                                                           const uint8_t *__restrict__ masks,
+                                                           float *__restrict__ dst_all,
+                                                           const uint32_t *__restrict__ tile_job,
+                                                           float *__restrict__ out_all,
+                                                           const float *__restrict__ img_nc)
 {
 	extern __shared__ unsigned char smem_raw[];
 	float *tile = reinterpret_cast<float *>(smem_raw);
@@ -663,44 +725,55 @@ __global__ __launch_bounds__(256) void conv_mfma_kernel(const PkJob *__restrict_
 	const int W = (int)jb.dw, H = (int)jb.dh, kw = (int)jb.kw, kh = (int)jb.kh;
 	const int hw = kw / 2, hh = kh / 2;
 	const int ox = bx + jb.csx, oy = by + jb.csy;
-	const int TW = CT_W + kw - 1, TH = CT_H + kh - 1, TS = conv_mfma_stride(kw);
-	float *kt = tile + (size_t)TS * TH;
-	for (int e = threadIdx.x; e < TW * TH; e += blockDim.x) {
-		const int ty = e / TW, tx = e - ty * TW;
-		const int si = ox + tx - hw, sj = oy + ty - hh;
-		float v = 0.f;
-		if (si >= 0 && si < W && sj >= 0 && sj < H) v = src[si + (size_t)sj * W];
-		tile[ty * TS + tx] = v;
+	const int TW = CT_W + kw - 1, TH = CT_H + kh - 1, TS = conv_mfma_stride(kw), BS = conv_mfma_band(kw);
+	float *band = tile + (size_t)TS * TH;
+	// source tile (zero outside the image and in the pad columns), four loads
+	// in flight per thread; row of element e by float reciprocal + one step
+	{
+		const int n_el = TS * TH;
+		const float inv_ts = 1.f / (float)TS;
+		for (int e0 = (int)threadIdx.x; e0 < n_el; e0 += 4 * 256) {
+			float v[4];
+#pragma unroll
+			for (int u = 0; u < 4; u++) {
+				const int e = e0 + u * 256;
+				v[u] = 0.f;
+				if (e < n_el) {
+					int ty = (int)(((float)e + 0.5f) * inv_ts);
+					int tx = e - ty * TS;
+					if (tx < 0) {
+						ty--;
+						tx += TS;
+					} else if (tx >= TS) {
+						ty++;
+						tx -= TS;
+					}
+					const int si = ox + tx - hw, sj = oy + ty - hh;
+					if (tx < TW && si >= 0 && si < W && sj >= 0 && sj < H) v[u] = src[si + (size_t)sj * W];
+				}
+			}
+#pragma unroll
+			for (int u = 0; u < 4; u++) {
+				const int e = e0 + u * 256;
+				if (e < n_el) tile[e] = v[u];
+			}
+		}
 	}
-	for (int e = threadIdx.x; e < kw * kh; e += blockDim.x) kt[e] = krn[e];  // kt[k kh + l] = K'(k, l)
+	// PSF band: band[l * BS + c] = K'(c - 16, l); kt[k * kh + l] = K'(k, l)
+	for (int e = (int)threadIdx.x; e < kh * BS; e += 256) {
+		const int l = e / BS, d = e - l * BS - 16;
+		band[e] = (d >= 0 && d < kw) ? krn[(size_t)d * kh + l] : 0.f;
+	}
 	__syncthreads();
 
 	const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
 	const int ry0 = (w >> 1) * 16, cx0 = (w & 1) * 32;  // this wave: rows ry0.., columns cx0.. (two 16 x 16 blocks)
+	if (by + ry0 >= (int)jb.ch) return;  // no output rows (nothing below waits for the workgroup)
 	const int lo = lane & 15, kq = lane >> 4;
-	const int J = 16 + kw - 1, nj = (J + 3) / 4;
 	double out[2][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
-	for (int l = 0; l < kh; l++) {
-		floatx4 acc0 = {0.f, 0.f, 0.f, 0.f}, acc1 = {0.f, 0.f, 0.f, 0.f};
-		const float *arow = tile + (size_t)(ry0 + lo + l) * TS + cx0;  // A row i = lo
-		for (int jc = 0; jc < nj; jc++) {
-			const int j = 4 * jc + kq;  // A column / B row of this lane
-			const int d = j - lo;       // B[j][lo] = K'(j - lo, l)
-			float a0 = 0.f, a1 = 0.f, b = 0.f;
-			if (j < J) {
-				a0 = arow[j];
-				a1 = arow[16 + j];
-			}
-			if (d >= 0 && d < kw) b = kt[d * kh + l];
-			acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(a0, b, acc0, 0, 0, 0);
-			acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(a1, b, acc1, 0, 0, 0);
-		}
-#pragma unroll
-		for (int r = 0; r < 4; r++) {
-			out[0][r] += (double)acc0[r];
-			out[1][r] += (double)acc1[r];
-		}
-	}
+	if (!((kw & 1) && kw >= CT_KMIN && kw <= CT_KMAX &&
+	      conv_mfma_dispatch<CT_KMIN>(kw, tile, band, TS, BS, kh, lane, ry0, cx0, out)))
+		conv_mfma_rows<0>(tile, band, TS, BS, kh, (16 + kw - 1 + 3) / 4, lane, ry0, cx0, out);
 #pragma unroll
 	for (int s2 = 0; s2 < 2; s2++)
 #pragma unroll
