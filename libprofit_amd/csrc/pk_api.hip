@@ -617,7 +617,8 @@ void psf_conv_layout(const double *krn, uint32_t kw, uint32_t kh, T *kt)
 template <typename T>
 int launch_conv(hipStream_t s, const PkJob *djobs, uint32_t njobs, uint32_t ntiles, const T *src, const T *kt,
                 const uint8_t *masks, T *dst, uint32_t max_kw, uint32_t max_kh, const uint32_t *tile_job = nullptr,
-                uint32_t tiles = PK_CONV_TILES_VALU, T *out_all = nullptr, const T *img_nc = nullptr)
+                uint32_t tiles = PK_CONV_TILES_VALU, T *out_all = nullptr, const T *img_nc = nullptr,
+                uint32_t tile_base = 0)
 {
 	if (tiles == PK_CONV_TILES_MFMA && sizeof(T) == 4 && conv_mfma_lds_bytes((int)max_kw, (int)max_kh) <= LDS_LIMIT) {
 		// FP32 on request: the matrix-core form, whole PSF in one launch
@@ -625,7 +626,7 @@ int launch_conv(hipStream_t s, const PkJob *djobs, uint32_t njobs, uint32_t ntil
 		                   conv_mfma_lds_bytes((int)max_kw, (int)max_kh), s, djobs, njobs,
 		                   reinterpret_cast<const float *>(src), reinterpret_cast<const float *>(kt), masks,
 		                   reinterpret_cast<float *>(dst), tile_job, reinterpret_cast<float *>(out_all),
-		                   reinterpret_cast<const float *>(img_nc));
+		                   reinterpret_cast<const float *>(img_nc), tile_base);
 		PK_HIP(hipGetLastError());
 		return PK_OK;
 	}
@@ -637,7 +638,7 @@ int launch_conv(hipStream_t s, const PkJob *djobs, uint32_t njobs, uint32_t ntil
 		// the returned image is written directly only by a single launch
 		const bool last = l0 + rows >= max_kh;
 		hipLaunchKernelGGL(conv_kernel<T>, dim3(ntiles), dim3(256), lds, s, djobs, njobs, src, kt, masks, dst,
-		                   (int)l0, (int)rows, tile_job, (l0 == 0 && last) ? out_all : (T *)nullptr, img_nc);
+		                   (int)l0, (int)rows, tile_job, (l0 == 0 && last) ? out_all : (T *)nullptr, img_nc, tile_base);
 		PK_HIP(hipGetLastError());
 	}
 	return PK_OK;
@@ -1396,6 +1397,12 @@ struct pk_batch {
 	size_t o_jobs = 0, o_setup = 0, o_comps = 0, o_cnt = 0, o_psfT = 0, o_mask = 0, o_nc = 0, o_c = 0, o_conv = 0,
 	       o_ent = 0, o_acc = 0, o_q = 0, o_out = 0, o_lite = 0;
 	size_t o_tj = 0;  // job of every tile: coarse tiles, then convolution tiles, then composition tiles
+	// convolution tiles are numbered by PSF-size class, one launch per class
+	// with the LDS its largest PSF needs (smaller PSFs: more workgroups per CU)
+	struct ConvClass {
+		uint32_t start, count, max_kw, max_kh;
+	};
+	std::vector<ConvClass> conv_classes;
 	// adjoint convolution (PK_BATCH_GRAD): jobs with the whole drawing frame as
 	// convolved frame, and the job of each of its tiles
 	size_t o_jobs_adj = 0, o_tja = 0;
@@ -1462,8 +1469,10 @@ int batch_launch(pk_batch *b)
 		                              b->n_comps > 0, timed ? b->ev[2] : nullptr, tj_coarse, b->r32()));
 		if (timed) PK_HIP(hipEventRecord(b->ev[3], s));
 		if (b->any_conv) {
-			PK_TRY(launch_conv<float>(s, djobs, nj, b->nctiles, c, b->d<float>(b->o_psfT), dmask, b->d<float>(b->o_conv),
-			                          b->max_kw, b->max_kh, tj_conv, b->tune.conv_tiles, b->d<float>(b->o_out), nc));
+			for (const auto &cc : b->conv_classes)
+				PK_TRY(launch_conv<float>(s, djobs, nj, cc.count, c, b->d<float>(b->o_psfT), dmask,
+				                          b->d<float>(b->o_conv), cc.max_kw, cc.max_kh, tj_conv, b->tune.conv_tiles,
+				                          b->d<float>(b->o_out), nc, cc.start));
 			PK_HIP(hipGetLastError());
 		}
 		if (timed) PK_HIP(hipEventRecord(b->ev[4], s));
@@ -1477,9 +1486,10 @@ int batch_launch(pk_batch *b)
 		                               0, 4096, 0, 0, b->n_comps > 0, timed ? b->ev[2] : nullptr, tj_coarse));
 		if (timed) PK_HIP(hipEventRecord(b->ev[3], s));
 		if (b->any_conv) {
-			PK_TRY(launch_conv<double>(s, djobs, nj, b->nctiles, c, b->d<double>(b->o_psfT), dmask,
-			                           b->d<double>(b->o_conv), b->max_kw, b->max_kh, tj_conv, PK_CONV_TILES_VALU,
-			                           b->d<double>(b->o_out), nc));
+			for (const auto &cc : b->conv_classes)
+				PK_TRY(launch_conv<double>(s, djobs, nj, cc.count, c, b->d<double>(b->o_psfT), dmask,
+				                           b->d<double>(b->o_conv), cc.max_kw, cc.max_kh, tj_conv, PK_CONV_TILES_VALU,
+				                           b->d<double>(b->o_out), nc, cc.start));
 			PK_HIP(hipGetLastError());
 		}
 		if (timed) PK_HIP(hipEventRecord(b->ev[4], s));
@@ -1801,7 +1811,7 @@ int pk_batch_create(pk_ctx *ctx, const pk_model *models, uint32_t n_models, uint
 		jb.crw = a.dw - 2 * (uint32_t)pcx;
 		jb.crh = a.dh - 2 * (uint32_t)pcy;
 		jb.conv_off = img_off;
-		jb.ctile_off = b->nctiles;
+		jb.ctile_off = 0;  // numbered by PSF-size class below
 		if (a.conv_required) {
 			const size_t pn = (size_t)m->psf_width * m->psf_height;
 			double sum = 0;
@@ -1822,7 +1832,6 @@ int pk_batch_create(pk_ctx *ctx, const pk_model *models, uint32_t n_models, uint
 			jb.cox = 0;
 			jb.coy = 0;
 			jb.ctiles_x = cdiv(jb.cw, CT_W);
-			b->nctiles += jb.ctiles_x * cdiv(jb.ch, CT_H);
 		} else {
 			jb.ctiles_x = 0;
 			jb.cw = a.dw;
@@ -1866,6 +1875,23 @@ int pk_batch_create(pk_ctx *ctx, const pk_model *models, uint32_t n_models, uint
 			jb.otiles_x = jb.fuse_out ? 0 : cdiv(jb.ow, 64);
 			b->notiles += jb.otiles_x * cdiv(jb.oh, PX_ROWS);
 		}
+		// convolution tiles grouped by PSF size (largest side <= 21, 31, 41, more)
+		const uint32_t class_lim[4] = {21, 31, 41, UINT32_MAX};
+		for (uint32_t lim_lo = 0, ci = 0; ci < 4; lim_lo = class_lim[ci], ci++) {
+			pk_batch::ConvClass cc{b->nctiles, 0, 0, 0};
+			for (auto &jb : b->jobs) {
+				const uint32_t side = std::max(jb.kw, jb.kh);
+				if (!jb.has_conv || side <= lim_lo || side > class_lim[ci]) continue;
+				jb.ctile_off = b->nctiles;
+				b->nctiles += jb.ctiles_x * cdiv(jb.ch, CT_H);
+				cc.max_kw = std::max(cc.max_kw, jb.kw);
+				cc.max_kh = std::max(cc.max_kh, jb.kh);
+			}
+			cc.count = b->nctiles - cc.start;
+			if (cc.count) b->conv_classes.push_back(cc);
+		}
+		for (auto &jb : b->jobs)
+			if (!jb.has_conv) jb.ctile_off = b->nctiles;
 	}
 	if (img_off >= (1ull << 32)) return fail(set_err(PK_EINVAL, "batch too large (%llu pixels)", (unsigned long long)img_off));
 
@@ -1874,7 +1900,7 @@ int pk_batch_create(pk_ctx *ctx, const pk_model *models, uint32_t n_models, uint
 	for (uint32_t jn = 0; jn < n_models; jn++) {
 		const PkJob &jb = b->jobs[jn];
 		const uint32_t nt = (jn + 1 < n_models ? b->jobs[jn + 1].tile_off : b->ntiles) - jb.tile_off;
-		const uint32_t nc = (jn + 1 < n_models ? b->jobs[jn + 1].ctile_off : b->nctiles) - jb.ctile_off;
+		const uint32_t nc = jb.has_conv ? jb.ctiles_x * cdiv(jb.ch, CT_H) : 0u;  // (tiles numbered by class)
 		const uint32_t no = (jn + 1 < n_models ? b->jobs[jn + 1].otile_off : b->notiles) - jb.otile_off;
 		std::fill_n(tj.begin() + jb.tile_off, nt, jn);
 		std::fill_n(tj.begin() + b->ntiles + jb.ctile_off, nc, jn);

@@ -596,13 +596,15 @@ __global__ __launch_bounds__(256, 1) void conv_kernel(const PkJob *__restrict__ 
                                                    const T *__restrict__ src_all, const T *__restrict__ krn_all,
                                                    const uint8_t *__restrict__ masks, T *__restrict__ dst_all,
                                                    int l0, int nl, const uint32_t *__restrict__ tile_job,
-                                                   T *__restrict__ out_all, const T *__restrict__ img_nc)
+                                                   T *__restrict__ out_all, const T *__restrict__ img_nc,
+                                                   uint32_t tile_base)
 {
 	extern __shared__ unsigned char smem_raw[];
 	T *tile = reinterpret_cast<T *>(smem_raw);
-	const uint32_t jn = find_job(jobs, njobs, blockIdx.x, 1, tile_job);
+	const uint32_t gt = tile_base + blockIdx.x;  // convolution tile of this workgroup
+	const uint32_t jn = find_job(jobs, njobs, gt, 1, tile_job);
 	const PkJob &jb = jobs[jn];
-	const uint32_t t = blockIdx.x - jb.ctile_off;
+	const uint32_t t = gt - jb.ctile_off;
 	const int bx = (int)(t % jb.ctiles_x) * CT_W, by = (int)(t / jb.ctiles_x) * CT_H;
 	const T *src = src_all + jb.img_off;
 	const T *kt = krn_all + jb.psf_off;
@@ -711,13 +713,14 @@ __global__ __launch_bounds__(256, 1) void conv_mfma_kernel(const PkJob *__restri
                                                            float *__restrict__ dst_all,
                                                            const uint32_t *__restrict__ tile_job,
                                                            float *__restrict__ out_all,
-                                                           const float *__restrict__ img_nc)
+                                                           const float *__restrict__ img_nc, uint32_t tile_base)
 {
 	extern __shared__ unsigned char smem_raw[];
 	float *tile = reinterpret_cast<float *>(smem_raw);
-	const uint32_t jn = find_job(jobs, njobs, blockIdx.x, 1, tile_job);
+	const uint32_t gt = tile_base + blockIdx.x;  // convolution tile of this workgroup
+	const uint32_t jn = find_job(jobs, njobs, gt, 1, tile_job);
 	const PkJob &jb = jobs[jn];
-	const uint32_t t = blockIdx.x - jb.ctile_off;
+	const uint32_t t = gt - jb.ctile_off;
 	const int bx = (int)(t % jb.ctiles_x) * CT_W, by = (int)(t / jb.ctiles_x) * CT_H;
 	const float *src = src_all + jb.img_off;
 	const float *krn = krn_all + jb.psf_off;
