@@ -1875,9 +1875,10 @@ int pk_batch_create(pk_ctx *ctx, const pk_model *models, uint32_t n_models, uint
 			jb.otiles_x = jb.fuse_out ? 0 : cdiv(jb.ow, 64);
 			b->notiles += jb.otiles_x * cdiv(jb.oh, PX_ROWS);
 		}
-		// convolution tiles grouped by PSF size (largest side <= 21, 31, 41, more)
-		const uint32_t class_lim[4] = {21, 31, 41, UINT32_MAX};
-		for (uint32_t lim_lo = 0, ci = 0; ci < 4; lim_lo = class_lim[ci], ci++) {
+		// convolution tiles grouped by PSF size (largest side <= 17, 21, ..., 41, more)
+		const uint32_t class_lim[] = {17, 21, 25, 29, 33, 37, 41, UINT32_MAX};
+		const uint32_t n_class = sizeof(class_lim) / sizeof(class_lim[0]);
+		for (uint32_t lim_lo = 0, ci = 0; ci < n_class; lim_lo = class_lim[ci], ci++) {
 			pk_batch::ConvClass cc{b->nctiles, 0, 0, 0};
 			for (auto &jb : b->jobs) {
 				const uint32_t side = std::max(jb.kw, jb.kh);
