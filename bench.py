@@ -58,14 +58,15 @@ def conv_flops(specs):
     return tot
 
 
-def conv_traffic():
-    """HBM bytes per launch of the convolution kernel on this workload, as
-    measured with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE and recorded in
-    profiles/conv_kernel_galaxy4096_traffic.txt (null if absent)."""
-    path = os.path.join(ROOT, "profiles", "conv_kernel_galaxy4096_traffic.txt")
+def conv_traffic(kernel):
+    """HBM bytes per step (all launches of one evaluation) of the convolution
+    kernel on this workload, as measured with rocprofv3 --pmc FETCH_SIZE /
+    WRITE_SIZE and recorded in profiles/<kernel>_galaxy4096_traffic.txt (null
+    if absent)."""
+    path = os.path.join(ROOT, "profiles", "%s_galaxy4096_traffic.txt" % kernel)
     try:
         for line in open(path):
-            if line.startswith("hbm_bytes_per_launch:"):
+            if line.startswith("hbm_bytes_per_step:"):
                 return int(float(line.split(":", 1)[1]))
     except OSError:
         pass
@@ -305,19 +306,23 @@ def main():
     flops = conv_flops(specs)
     conv_ms = stages["conv"]
     achieved_tflops = flops / (conv_ms * 1e-3) / 1e12
+    mfma = inf.conv_tiles == 1
+    kernel = "conv_mfma_kernel" if mfma else "conv_kernel"
     roofline = {
-        "kernel": ("conv_mfma_kernel (direct PSF convolution, MFMA tiles)" if tuning["conv_tiles"]
-                   else "conv_kernel<float> (direct PSF convolution)"),
-        # 2*kw*kh flop per pixel over ~8 B per pixel: far right of the HBM ridge,
-        # bound by FP32 vector FMA issue
-        "bound": "valu",
+        "kernel": "%s (direct PSF convolution%s, composition fused in; %d launches per step, one per "
+                  "PSF-size class; figures per step)" % (kernel, ", FP32 MFMA tiles" if mfma else ", FP32 FMA",
+                                                         inf.n_conv_launches),
+        # 2*kw*kh flop per returned pixel over ~12 B per pixel: far right of the
+        # HBM ridge, bound by the matrix cores (MFMA tiles) or FP32 FMA issue
+        "bound": "mfma" if mfma else "valu",
         "achieved": round(achieved_tflops, 3),
         "peak": FP32_PEAK_TFLOPS,
         "unit": "TFLOP/s",
         "frac": round(achieved_tflops / FP32_PEAK_TFLOPS, 4),
-        "traffic": conv_traffic(),
+        "traffic": conv_traffic(kernel),
         "algorithmic_flops_per_launch": flops,
         "avg_launch_ms": round(conv_ms, 4),
+        "launches_per_step": int(inf.n_conv_launches),
         "stage_ms": {k: round(v, 4) for k, v in stages.items()},
     }
 

@@ -261,6 +261,8 @@ typedef struct pk_batch_info {
 	/* per-stage device time of the last evaluation (not available in graph mode: 0) */
 	double ms_setup, ms_coarse, ms_refine, ms_conv, ms_compose, ms_total;
 	double ms_grad;             /* adjoint part of the last pk_batch_evaluate_grad */
+	uint32_t n_conv_launches;   /* convolution launches per evaluation (one per PSF-size class) */
+	uint32_t conv_tiles;        /* PK_CONV_TILES_* they used */
 } pk_batch_info;
 
 int pk_batch_create(pk_ctx *ctx, const pk_model *models, uint32_t n_models, uint32_t flags, pk_batch **out);

@@ -126,6 +126,8 @@ class BatchInfo(C.Structure):
         ("ms_compose", C.c_double),
         ("ms_total", C.c_double),
         ("ms_grad", C.c_double),
+        ("n_conv_launches", C.c_uint32),
+        ("conv_tiles", C.c_uint32),
     ]
 
 

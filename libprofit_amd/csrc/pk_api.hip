@@ -2193,6 +2193,8 @@ int pk_batch_sync(pk_batch *b, pk_batch_info *info)
 		std::memset(info, 0, sizeof(*info));
 		info->n_refine_entries = cnt.n_entries;
 		info->n_subcells = cnt.n_subcells;
+		info->n_conv_launches = (uint32_t)b->conv_classes.size();
+		info->conv_tiles = b->elem == 4 ? b->tune.conv_tiles : (uint32_t)PK_CONV_TILES_VALU;
 		if (b->ran && !(b->flags & PK_BATCH_GRAPH)) {
 			float t[6] = {};
 			(void)hipEventElapsedTime(&t[0], b->ev[0], b->ev[1]);
