@@ -220,3 +220,30 @@ def test_batch_mixed_composition(ctx64, ctx32, oracle, prec):
             assert pix <= tol, (i, pix)
             assert flux <= tol, (i, flux)
     b.close()
+
+
+@pytest.mark.parametrize("tiles", [0, 1])
+def test_batch_psf_shapes(ctx32, oracle, tiles):
+    """Even, non-square, tiny and wider-than-specialised PSFs (the run-time
+    column paths of both convolution forms) in PSF-size classes of their own."""
+    print("CASE psf shapes, conv_tiles", tiles, flush=True)
+    specs = []
+    for k, (pw, ph) in enumerate([(3, 3), (8, 8), (5, 9), (51, 51), (15, 15), (12, 7), (1, 5)]):
+        s = 48 + 5 * k
+        psf = workloads.moffat_psf_image(max(pw, ph), 2.5, 3.0)[:ph, :pw]
+        comps = [abi.make_component("sersic", xcen=0.5 * s + 0.3, ycen=0.5 * s - 0.2, mag=16.0, re=4.0, nser=1.5,
+                                    axrat=0.8, ang=15.0 * k, convolve=True),
+                 abi.make_component("sky", bg=0.2)]
+        specs.append(abi.ModelSpec(s, s, comps, psf=psf))
+    b = Batch(ctx32, specs, graph=False)
+    b.tuning = {**DEFAULT_TUNING, "conv_tiles": tiles}
+    b.evaluate()
+    b.sync()
+    for i, s in enumerate(specs):
+        ref, _ = oracle.evaluate(s)
+        got = b.image(i)
+        assert got.shape == ref.shape, (i, got.shape, ref.shape)
+        pix, flux = rel_errors(got, ref)
+        assert pix <= TOL_FP32, (i, pix)
+        assert flux <= TOL_FP32, (i, flux)
+    b.close()
