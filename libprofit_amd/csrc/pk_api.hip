@@ -505,6 +505,47 @@ std::vector<uint8_t> mask_expand(const std::vector<uint8_t> &m, uint32_t w, uint
 	return out;
 }
 
+// Calculation mask of a model in its drawing frame (0/1 per pixel), adjusted
+// as Model::evaluate does (src/model.cpp:230-246, 265-285): upsampled by the
+// finesampling, placed in the PSF padding, expanded by the PSF half-size.
+// `adjusted`: the user's mask is applied again to the returned image.
+int calc_mask(const pk_model *m, const Analysis &a, std::vector<uint8_t> &dmask, bool &adjusted)
+{
+	const uint32_t fs = m->finesampling;
+	adjusted = false;
+	if (m->adjust_mask && a.mask_needs_adjustment) {
+		uint32_t w = m->mask_width, h = m->mask_height;
+		std::vector<uint8_t> cur(m->mask, m->mask + (size_t)w * h);
+		for (auto &v : cur) v = v ? 1 : 0;
+		if (fs > 1) {
+			std::vector<uint8_t> up((size_t)w * fs * h * fs);
+			for (uint32_t r = 0; r < h * fs; r++)
+				for (uint32_t c = 0; c < w * fs; c++) up[c + (size_t)r * w * fs] = cur[c / fs + (size_t)(r / fs) * w];
+			cur.swap(up);
+			w *= fs;
+			h *= fs;
+		}
+		if (a.mask_needs_psf_padding) {
+			std::vector<uint8_t> ext((size_t)a.dw * a.dh, 0);
+			for (uint32_t j = 0; j < h; j++)
+				for (uint32_t i = 0; i < w; i++) ext[(i + a.padx) + (size_t)(j + a.pady) * a.dw] = cur[i + (size_t)j * w];
+			cur.swap(ext);
+			w = a.dw;
+			h = a.dh;
+		}
+		if (a.mask_needs_convolution) cur = mask_expand(cur, w, h, m->psf_width / 2, m->psf_height / 2);
+		dmask.swap(cur);
+		adjusted = true;
+		return PK_OK;
+	}
+	if (!m->adjust_mask && (m->mask_width != a.dw || m->mask_height != a.dh))
+		return set_err(PK_EINVAL, "Mask dimensions != drawing dimensions: [%u, %u] != [%u, %u]", m->mask_width,
+		               m->mask_height, a.dw, a.dh);
+	dmask.assign(m->mask, m->mask + (size_t)a.dw * a.dh);
+	for (auto &v : dmask) v = v ? 1 : 0;
+	return PK_OK;
+}
+
 // FFTConvolver::offset_after_convolution (src/convolve.cpp:270-281)
 void fft_offset(uint32_t sw, uint32_t sh, uint32_t kw, uint32_t kh, int &ox, int &oy)
 {
@@ -822,37 +863,8 @@ int evaluate_impl(pk_ctx *ctx, const pk_model *m, const Analysis &a, double *out
 	std::vector<uint8_t> dmask;
 	bool use_mask = false, adjusted = false;
 	if (has_mask(m)) {
-		if (m->adjust_mask && a.mask_needs_adjustment) {
-			uint32_t w = m->mask_width, h = m->mask_height;
-			std::vector<uint8_t> cur(m->mask, m->mask + (size_t)w * h);
-			for (auto &v : cur) v = v ? 1 : 0;
-			if (fs > 1) {
-				std::vector<uint8_t> up((size_t)w * fs * h * fs);
-				for (uint32_t r = 0; r < h * fs; r++)
-					for (uint32_t c = 0; c < w * fs; c++) up[c + (size_t)r * w * fs] = cur[c / fs + (size_t)(r / fs) * w];
-				cur.swap(up);
-				w *= fs;
-				h *= fs;
-			}
-			if (a.mask_needs_psf_padding) {
-				std::vector<uint8_t> ext((size_t)a.dw * a.dh, 0);
-				for (uint32_t j = 0; j < h; j++)
-					for (uint32_t i = 0; i < w; i++) ext[(i + a.padx) + (size_t)(j + a.pady) * a.dw] = cur[i + (size_t)j * w];
-				cur.swap(ext);
-				w = a.dw;
-				h = a.dh;
-			}
-			if (a.mask_needs_convolution) cur = mask_expand(cur, w, h, m->psf_width / 2, m->psf_height / 2);
-			dmask.swap(cur);
-			use_mask = adjusted = true;
-		} else {
-			if (!m->adjust_mask && (m->mask_width != a.dw || m->mask_height != a.dh))
-				return set_err(PK_EINVAL, "Mask dimensions != drawing dimensions: [%u, %u] != [%u, %u]",
-				               m->mask_width, m->mask_height, a.dw, a.dh);
-			dmask.assign(m->mask, m->mask + (size_t)a.dw * a.dh);
-			for (auto &v : dmask) v = v ? 1 : 0;
-			use_mask = true;
-		}
+		PK_TRY(calc_mask(m, a, dmask, adjusted));
+		use_mask = true;
 	}
 
 	// components: radial + sky go to the coarse/refine kernels, psf profiles
@@ -1478,7 +1490,7 @@ int batch_launch(pk_batch *b)
 		if (timed) PK_HIP(hipEventRecord(b->ev[4], s));
 		if (b->notiles)
 			hipLaunchKernelGGL((compose_kernel<float, float>), dim3(b->notiles), dim3(256), 0, s, djobs, nj, nc,
-			                   b->d<float>(b->o_conv), (const uint8_t *)nullptr, b->d<float>(b->o_out), tj_out);
+			                   b->d<float>(b->o_conv), dmask, b->d<float>(b->o_out), tj_out);
 	} else {
 		double *nc = b->d<double>(b->o_nc), *c = b->d<double>(b->o_c);
 		PK_TRY(launch_profiles<double>(s, false, djobs, nj, b->ntiles, dcomp, nullptr, dmask, nc, c, dent, dcnt, nullptr,
@@ -1495,7 +1507,7 @@ int batch_launch(pk_batch *b)
 		if (timed) PK_HIP(hipEventRecord(b->ev[4], s));
 		if (b->notiles)
 			hipLaunchKernelGGL((compose_kernel<double, double>), dim3(b->notiles), dim3(256), 0, s, djobs, nj, nc,
-			                   b->d<double>(b->o_conv), (const uint8_t *)nullptr, b->d<double>(b->o_out), tj_out);
+			                   b->d<double>(b->o_conv), dmask, b->d<double>(b->o_out), tj_out);
 	}
 	PK_HIP(hipGetLastError());
 	if (timed) PK_HIP(hipEventRecord(b->ev[5], s));
@@ -1743,10 +1755,17 @@ int pk_batch_create(pk_ctx *ctx, const pk_model *models, uint32_t n_models, uint
 		if (fp32 && check_fp32_limits(m)) return fail(set_err(PK_EINVAL, "model %u: %s", mi, g_err.c_str()));
 		if (a.conv_required && m->convolver == PK_CONV_FFT)
 			return fail(set_err(PK_EINVAL, "model %u: the FFT convolver is not supported in a batch", mi));
-		if (has_mask(m))
-			return fail(set_err(PK_EINVAL, "model %u: calculation masks are not supported in a batch", mi));
-		if (a.padx == 0 && a.pady == 0 && a.conv_required && m->psf_width > 1 && m->psf_height > 1)
-			return fail(set_err(PK_EINVAL, "model %u: unsupported padding", mi));
+		// calculation mask in the drawing frame (and the user's mask for the
+		// returned image when it was adjusted), as Model::evaluate
+		std::vector<uint8_t> dmask;
+		bool use_mask = false, adjusted = false;
+		if (has_mask(m)) {
+			if (flags & PK_BATCH_GRAD)
+				return fail(set_err(PK_EINVAL, "model %u: gradients do not support masks", mi));
+			const int ms = calc_mask(m, a, dmask, adjusted);
+			if (ms) return fail(set_err(ms, "model %u: %s", mi, g_err.c_str()));
+			use_mask = true;
+		}
 		if (flags & PK_BATCH_GRAD) {
 			// the adjoint of the PSF convolution is the convolution with the
 			// flipped PSF on the same grid only for odd PSF dimensions
@@ -1768,6 +1787,10 @@ int pk_batch_create(pk_ctx *ctx, const pk_model *models, uint32_t n_models, uint
 		jb.comp_off = (uint32_t)b->setup.size();
 		jb.img_off = img_off;
 		jb.mask_off = NONE;
+		if (use_mask) {
+			jb.mask_off = mask_all.size();
+			mask_all.insert(mask_all.end(), dmask.begin(), dmask.end());
+		}
 		jb.sx = sx;
 		jb.sy = sy;
 		jb.tile_off = b->ntiles;
@@ -1831,6 +1854,8 @@ int pk_batch_create(pk_ctx *ctx, const pk_model *models, uint32_t n_models, uint
 			jb.csy = pcy;
 			jb.mox = -pcx;
 			jb.moy = -pcy;
+			jb.cmx = -pcx;  // calculation-mask pixel of convolved (X, Y): (X + pcx, Y + pcy)
+			jb.cmy = -pcy;
 			jb.cox = 0;
 			jb.coy = 0;
 			jb.ctiles_x = cdiv(jb.cw, CT_W);
@@ -1851,11 +1876,20 @@ int pk_batch_create(pk_ctx *ctx, const pk_model *models, uint32_t n_models, uint
 		jb.ow = ow;
 		jb.oh = oh;
 		jb.out_off = out_off;
-		// without downsampling the convolved frame is the returned image: the
-		// convolution writes it (composition tiles are set below)
-		jb.fuse_out = (a.conv_required && jb.fs == 1) ? 1 : 0;
+		// without downsampling (and without the user's mask to apply again) the
+		// convolved frame is the returned image: the convolution writes it
+		// (composition tiles are set below)
+		jb.fuse_out = (a.conv_required && jb.fs == 1 && !adjusted) ? 1 : 0;
 		jb.omask_off = NONE;
 		jb.omask_f = 1;
+		if (adjusted) {
+			jb.omask_off = mask_all.size();
+			jb.omask_w = m->mask_width;
+			jb.omask_h = m->mask_height;
+			jb.omask_f = (ow == m->mask_width) ? 1 : fs;
+			for (size_t t = 0; t < (size_t)m->mask_width * m->mask_height; t++)
+				mask_all.push_back(m->mask[t] ? 1 : 0);
+		}
 		jb.fx_scale = fixed_point_scale(m);
 		for (uint32_t k = jb.comp_off; k < jb.comp_off + jb.ncomp; k++) b->setup[k].fx_scale = jb.fx_scale;
 		b->jobs.push_back(jb);
@@ -1936,7 +1970,7 @@ int pk_batch_create(pk_ctx *ctx, const pk_model *models, uint32_t n_models, uint
 	b->o_psfT = L.take(elem * std::max<size_t>(1, psf_all.size()));
 	b->o_tj = L.take(sizeof(uint32_t) * std::max<size_t>(1, tj.size()));
 	const size_t in_bytes = L.off;
-	b->o_mask = L.take(1);
+	b->o_mask = L.take(std::max<size_t>(1, mask_all.size()));
 	b->o_comps = L.take(sizeof(PkComp) * std::max<size_t>(1, b->setup.size()));
 	b->o_lite = L.take(sizeof(CompLite) * std::max<size_t>(1, b->setup.size()));
 	b->o_cnt = L.take(sizeof(PkCounters));
@@ -1981,6 +2015,8 @@ int pk_batch_create(pk_ctx *ctx, const pk_model *models, uint32_t n_models, uint
 			                        reinterpret_cast<double *>(stage.data() + b->o_psfT) + jb.psf_off);
 	}
 	PK_HIP(hipMemcpy(b->dmem, stage.data(), in_bytes, hipMemcpyHostToDevice));
+	if (!mask_all.empty())
+		PK_HIP(hipMemcpy(b->d<void>(b->o_mask), mask_all.data(), mask_all.size(), hipMemcpyHostToDevice));
 	if (flags & PK_BATCH_GRAD) {
 		// PSF layout of the adjoint convolution: kt[k * kh + l] = krn(k, l)
 		// (the flipped PSF in the layout conv_kernel reads)

@@ -247,3 +247,47 @@ def test_batch_psf_shapes(ctx32, oracle, tiles):
         assert pix <= TOL_FP32, (i, pix)
         assert flux <= TOL_FP32, (i, flux)
     b.close()
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_batch_masks(ctx64, ctx32, oracle, prec):
+    """Calculation masks in a batch (Model::evaluate's mask handling,
+    src/model.cpp:230-316): adjusted masks needing PSF padding or not,
+    finesampled masks, masks in drawing dimensions, masks without a PSF."""
+    print("CASE batch masks", prec, flush=True)
+    rng = np.random.Generator(np.random.PCG64(21))
+    psf = workloads.moffat_psf_image(9, 2.5, 3.0)
+
+    def model(s, mask, psf_=psf, fs=1, adjust=True, rf=True):
+        comps = [abi.make_component("sersic", xcen=0.48 * s, ycen=0.53 * s, mag=15.0, re=5.0, nser=2.0, axrat=0.6,
+                                    ang=35.0, convolve=psf_ is not None),
+                 abi.make_component("sky", bg=0.3)]
+        return abi.ModelSpec(s, s, comps, psf=psf_, mask=mask, finesampling=fs, adjust_mask=adjust,
+                             return_finesampled=rf)
+
+    specs = []
+    s = 50
+    specs.append(model(s, (rng.uniform(size=(s, s)) > 0.3).astype(np.uint8)))          # bbox at the edges
+    inner = np.zeros((s, s), np.uint8)
+    inner[15:35, 12:38] = 1
+    specs.append(model(s, inner))                                                      # no PSF padding needed
+    specs.append(model(s, (rng.uniform(size=(s, s)) > 0.5).astype(np.uint8), fs=2))    # finesampled
+    specs.append(model(s, (rng.uniform(size=(s, s)) > 0.5).astype(np.uint8), fs=2, rf=False))
+    specs.append(model(s, (rng.uniform(size=(s + 8, s + 8)) > 0.4).astype(np.uint8), adjust=False))  # drawing dims
+    specs.append(model(s, (rng.uniform(size=(s, s)) > 0.2).astype(np.uint8), psf_=None))  # no convolution
+    specs.append(model(s, None))                                                       # no mask
+    ctx = ctx64 if prec == "fp64" else ctx32
+    tol = TOL_FP64 if prec == "fp64" else TOL_FP32
+    b = Batch(ctx, specs, graph=True)
+    for _rep in range(2):
+        b.evaluate()
+        b.sync()
+        for i, sp in enumerate(specs):
+            ref, _ = oracle.evaluate(sp)
+            got = b.image(i)
+            assert got.shape == ref.shape, (i, got.shape, ref.shape)
+            assert np.array_equal(got == 0, ref == 0), i  # masked pixels are exactly 0 in both
+            pix, flux = rel_errors(got, ref)
+            assert pix <= tol, (i, pix)
+            assert flux <= tol, (i, flux)
+    b.close()
