@@ -220,10 +220,10 @@ int pk_convolver_padding(int type, uint32_t src_w, uint32_t src_h, uint32_t krn_
  * in one sequence of launches (optionally captured in a hipGraph), with the
  * per-component set-up (bn, normalisation integrals, adjusted subsampling
  * constants) computed in FP64 on the device.  Supported per model: radial,
- * sky and null profiles, BRUTE/BRUTE_OLD convolution, calculation masks (not
- * with PK_BATCH_GRAD), finesampling; psf profiles and the FFT convolver are
- * rejected.  Output images are FP32 in a PK_FP32 context and FP64 in a
- * PK_FP64 one. */
+ * sky, psf and null profiles (psf profiles and calculation masks not with
+ * PK_BATCH_GRAD), BRUTE/BRUTE_OLD convolution, calculation masks,
+ * finesampling; the FFT convolver is rejected.  Output images are FP32 in a
+ * PK_FP32 context and FP64 in a PK_FP64 one. */
 typedef struct pk_batch pk_batch;
 
 enum {

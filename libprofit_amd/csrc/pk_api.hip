@@ -509,6 +509,38 @@ std::vector<uint8_t> mask_expand(const std::vector<uint8_t> &m, uint32_t w, uint
 // as Model::evaluate does (src/model.cpp:230-246, 265-285): upsampled by the
 // finesampling, placed in the PSF padding, expanded by the PSF half-size.
 // `adjusted`: the user's mask is applied again to the returned image.
+// Work item of a psf profile: PsfProfile::evaluate's geometry
+// (src/psf.cpp:58-103), including its mixed x/y use of the offsets and
+// dimensions; job and psf_off are left to the caller.
+PkPsfItem psf_item(const pk_model *m, const pk_component &pc, const Analysis &a, double sx, double sy)
+{
+	PkPsfItem it{};
+	it.to_conv = pc.convolve ? 1 : 0;
+	const double psx = m->psf_scale_x, psy = m->psf_scale_y;
+	const uint32_t pw = m->psf_width, ph = m->psf_height;
+	const double origin_x = pc.xcen + a.padx * sx - pw * psx / 2;
+	const double end_x = pc.xcen + a.pady * sy + pw * psx / 2;
+	const double origin_y = pc.ycen + a.padx * sx - ph * psy / 2;
+	const double end_y = pc.ycen + a.pady * sy + ph * psy / 2;
+	auto bind = [](double v, uint32_t mx) -> uint32_t {
+		int iv = (int)std::floor(v);
+		if (iv < 0) return 0;
+		return std::min((uint32_t)iv, mx);
+	};
+	it.x0 = bind(origin_x / sx, a.dw - 1);
+	it.y0 = bind(origin_y / sy, a.dh - 1);
+	it.x1 = bind(end_x / sx, a.dw - 1);
+	it.y1 = bind(end_y / sy, a.dh - 1);
+	it.origin_x = origin_x;
+	it.origin_y = origin_y;
+	it.scale = std::pow(10, -0.4 * (pc.mag - m->magzero));
+	it.psx = psx;
+	it.psy = psy;
+	it.pw = pw;
+	it.ph = ph;
+	return it;
+}
+
 int calc_mask(const pk_model *m, const Analysis &a, std::vector<uint8_t> &dmask, bool &adjusted)
 {
 	const uint32_t fs = m->finesampling;
@@ -877,34 +909,8 @@ int evaluate_impl(pk_ctx *ctx, const pk_model *m, const Analysis &a, double *out
 		const pk_component &pc = m->components[c];
 		if (pc.kind == PK_PROFILE_NULL) continue;
 		if (pc.kind == PK_PROFILE_PSF) {
-			// PsfProfile::evaluate geometry (src/psf.cpp:58-103), including
-			// its mixed x/y use of the offsets and dimensions
-			PkPsfItem it{};
-			it.to_conv = pc.convolve ? 1 : 0;
-			const double psx = m->psf_scale_x, psy = m->psf_scale_y;
-			const uint32_t pw = m->psf_width, ph = m->psf_height;
-			const double origin_x = pc.xcen + a.padx * sx - pw * psx / 2;
-			const double end_x = pc.xcen + a.pady * sy + pw * psx / 2;
-			const double origin_y = pc.ycen + a.padx * sx - ph * psy / 2;
-			const double end_y = pc.ycen + a.pady * sy + ph * psy / 2;
-			auto bind = [](double v, uint32_t mx) -> uint32_t {
-				int iv = (int)std::floor(v);
-				if (iv < 0) return 0;
-				return std::min((uint32_t)iv, mx);
-			};
-			it.x0 = bind(origin_x / sx, a.dw - 1);
-			it.y0 = bind(origin_y / sy, a.dh - 1);
-			it.x1 = bind(end_x / sx, a.dw - 1);
-			it.y1 = bind(end_y / sy, a.dh - 1);
-			it.origin_x = origin_x;
-			it.origin_y = origin_y;
-			it.scale = std::pow(10, -0.4 * (pc.mag - m->magzero));
-			it.psx = psx;
-			it.psy = psy;
-			it.pw = pw;
-			it.ph = ph;
+			items.push_back(psf_item(m, pc, a, sx, sy));
 			if (!pc.convolve) any_nc = true;
-			items.push_back(it);
 			continue;
 		}
 		SetupIn in{};
@@ -1415,6 +1421,34 @@ struct pk_batch {
 		uint32_t start, count, max_kw, max_kh;
 	};
 	std::vector<ConvClass> conv_classes;
+	// psf profiles: rebinning items (PsfProfile::evaluate), the model data their
+	// geometry is recomputed from when the components change, and the
+	// normalised FP64 PSFs they read
+	struct PsfSrc {
+		pk_model model;  // scalar fields only
+		Analysis a;
+		double sx, sy;
+		uint32_t comp;  // user component index
+	};
+	std::vector<PkPsfItem> psf_items;  // layer-major: layer k holds the k-th psf profile of each model
+	std::vector<PsfSrc> psf_src;       // same order
+	struct PsfLayer {
+		uint32_t start, count, max_pixels;
+	};
+	std::vector<PsfLayer> psf_layers;
+	size_t o_items = 0, o_psfd = 0;
+	// pixels the largest item of each layer covers (grid width of its launch)
+	void psf_update_layers()
+	{
+		for (auto &ly : psf_layers) {
+			ly.max_pixels = 0;
+			for (uint32_t k = ly.start; k < ly.start + ly.count; k++) {
+				const PkPsfItem &it = psf_items[k];
+				if (it.x1 < it.x0 || it.y1 < it.y0) continue;
+				ly.max_pixels = std::max(ly.max_pixels, (it.x1 - it.x0 + 1) * (it.y1 - it.y0 + 1));
+			}
+		}
+	}
 	// adjoint convolution (PK_BATCH_GRAD): jobs with the whole drawing frame as
 	// convolved frame, and the job of each of its tiles
 	size_t o_jobs_adj = 0, o_tja = 0;
@@ -1450,6 +1484,25 @@ struct pk_batch {
 
 namespace {
 
+// psf profiles of a batch, after the radial profiles: one launch per layer
+// (the k-th psf profile of every model), so that the additions into one
+// model's image keep the model's component order, as a single evaluation.
+template <typename T>
+int launch_psf_layers(pk_batch *b, hipStream_t s, T *nc, T *c)
+{
+	for (const auto &ly : b->psf_layers) {
+		if (!ly.max_pixels) continue;
+		for (uint32_t k0 = 0; k0 < ly.count; k0 += 65535u) {
+			const uint32_t nk = std::min(65535u, ly.count - k0);
+			hipLaunchKernelGGL(psf_kernel<T>, dim3(cdiv(ly.max_pixels, 256), nk), dim3(256), 0, s,
+			                   b->d<PkPsfItem>(b->o_items) + ly.start + k0, b->d<PkJob>(b->o_jobs),
+			                   b->d<double>(b->o_psfd), nc, c);
+		}
+	}
+	PK_HIP(hipGetLastError());
+	return PK_OK;
+}
+
 int batch_launch(pk_batch *b)
 {
 	pk_ctx *ctx = b->ctx;
@@ -1479,6 +1532,7 @@ int batch_launch(pk_batch *b)
 		                              b->d<unsigned long long>(b->o_acc), b->d<RCell>(b->o_q), b->qcap, UINT32_MAX,
 		                              b->grid(ctx->r32_grid_main), b->grid(ctx->r32_grid_last),
 		                              b->n_comps > 0, timed ? b->ev[2] : nullptr, tj_coarse, b->r32()));
+		PK_TRY(launch_psf_layers<float>(b, s, nc, c));
 		if (timed) PK_HIP(hipEventRecord(b->ev[3], s));
 		if (b->any_conv) {
 			for (const auto &cc : b->conv_classes)
@@ -1496,6 +1550,7 @@ int batch_launch(pk_batch *b)
 		PK_TRY(launch_profiles<double>(s, false, djobs, nj, b->ntiles, dcomp, nullptr, dmask, nc, c, dent, dcnt, nullptr,
 		                               nullptr,
 		                               0, 4096, 0, 0, b->n_comps > 0, timed ? b->ev[2] : nullptr, tj_coarse));
+		PK_TRY(launch_psf_layers<double>(b, s, nc, c));
 		if (timed) PK_HIP(hipEventRecord(b->ev[3], s));
 		if (b->any_conv) {
 			for (const auto &cc : b->conv_classes)
@@ -1741,6 +1796,9 @@ int pk_batch_create(pk_ctx *ctx, const pk_model *models, uint32_t n_models, uint
 	b->elem = elem;
 	std::vector<double> psf_all;
 	std::vector<uint8_t> mask_all;
+	std::vector<double> psfd_all;  // normalised PSFs of models with psf profiles
+	uint32_t psfd_model = ~0u;
+	uint64_t psfd_off = 0;
 	uint64_t img_off = 0, out_off = 0;
 	auto fail = [&](int code) {
 		pk_batch_destroy(b);
@@ -1804,8 +1862,34 @@ int pk_batch_create(pk_ctx *ctx, const pk_model *models, uint32_t n_models, uint
 				b->comp_index.push_back(~0u);
 				continue;
 			}
-			if (pc.kind == PK_PROFILE_PSF)
-				return fail(set_err(PK_EINVAL, "model %u: psf profiles are not supported in a batch", mi));
+			if (pc.kind == PK_PROFILE_PSF) {
+				if (psfd_model != mi) {
+					// this model's PSF, normalised (Model::set_psf), once
+					psfd_model = mi;
+					psfd_off = psfd_all.size();
+					const size_t pn = (size_t)m->psf_width * m->psf_height;
+					double sum = 0;
+					for (size_t t = 0; t < pn; t++) sum += m->psf[t];
+					for (size_t t = 0; t < pn; t++) psfd_all.push_back(sum > 0 ? m->psf[t] / sum : m->psf[t]);
+				}
+				PkPsfItem it = psf_item(m, pc, a, sx, sy);
+				it.job = mi;
+				it.psf_off = psfd_off;
+				b->psf_items.push_back(it);
+				pk_batch::PsfSrc src{};
+				src.model = *m;
+				src.model.components = nullptr;
+				src.model.psf = nullptr;
+				src.model.mask = nullptr;
+				src.a = a;
+				src.sx = sx;
+				src.sy = sy;
+				src.comp = (uint32_t)b->comp_kind.size() - 1;
+				b->psf_src.push_back(src);
+				b->comp_index.push_back(~0u);
+				if (!pc.convolve) any_nc = true;
+				continue;
+			}
 			SetupIn in{};
 			in.c = pc;
 			in.magzero = m->magzero;
@@ -1964,11 +2048,42 @@ int pk_batch_create(pk_ctx *ctx, const pk_model *models, uint32_t n_models, uint
 			b->natiles += n;
 		}
 	}
+	// psf profiles in layers (k-th psf profile of each model), items in
+	// layer-major order
+	if (!b->psf_items.empty()) {
+		std::vector<uint32_t> layer(b->psf_items.size()), seen(n_models, 0);
+		uint32_t n_layers = 0;
+		for (size_t k = 0; k < b->psf_items.size(); k++) {
+			layer[k] = seen[b->psf_items[k].job]++;
+			n_layers = std::max(n_layers, layer[k] + 1);
+		}
+		std::vector<size_t> order(b->psf_items.size());
+		for (size_t k = 0; k < order.size(); k++) order[k] = k;
+		std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) { return layer[x] < layer[y]; });
+		std::vector<PkPsfItem> items;
+		std::vector<pk_batch::PsfSrc> src;
+		for (size_t k : order) {
+			items.push_back(b->psf_items[k]);
+			src.push_back(b->psf_src[k]);
+		}
+		b->psf_items.swap(items);
+		b->psf_src.swap(src);
+		b->psf_layers.assign(n_layers, pk_batch::PsfLayer{0, 0, 0});
+		uint32_t start = 0;
+		for (uint32_t l = 0; l < n_layers; l++) {
+			b->psf_layers[l].start = start;
+			for (size_t k = start; k < b->psf_items.size() && layer[order[k]] == l; k++) b->psf_layers[l].count++;
+			start += b->psf_layers[l].count;
+		}
+		b->psf_update_layers();
+	}
 	Layout L;
 	b->o_jobs = L.take(sizeof(PkJob) * b->jobs.size());
 	b->o_setup = L.take(sizeof(SetupIn) * std::max<size_t>(1, b->setup.size()));
 	b->o_psfT = L.take(elem * std::max<size_t>(1, psf_all.size()));
 	b->o_tj = L.take(sizeof(uint32_t) * std::max<size_t>(1, tj.size()));
+	b->o_items = L.take(sizeof(PkPsfItem) * std::max<size_t>(1, b->psf_items.size()));
+	b->o_psfd = L.take(sizeof(double) * std::max<size_t>(1, psfd_all.size()));
 	const size_t in_bytes = L.off;
 	b->o_mask = L.take(std::max<size_t>(1, mask_all.size()));
 	b->o_comps = L.take(sizeof(PkComp) * std::max<size_t>(1, b->setup.size()));
@@ -2005,6 +2120,9 @@ int pk_batch_create(pk_ctx *ctx, const pk_model *models, uint32_t n_models, uint
 	std::memcpy(stage.data() + b->o_jobs, b->jobs.data(), sizeof(PkJob) * b->jobs.size());
 	if (!tj.empty()) std::memcpy(stage.data() + b->o_tj, tj.data(), sizeof(uint32_t) * tj.size());
 	if (!b->setup.empty()) std::memcpy(stage.data() + b->o_setup, b->setup.data(), sizeof(SetupIn) * b->setup.size());
+	if (!b->psf_items.empty())
+		std::memcpy(stage.data() + b->o_items, b->psf_items.data(), sizeof(PkPsfItem) * b->psf_items.size());
+	if (!psfd_all.empty()) std::memcpy(stage.data() + b->o_psfd, psfd_all.data(), sizeof(double) * psfd_all.size());
 	for (const auto &jb : b->jobs) {
 		if (!jb.has_conv) continue;
 		if (fp32)
@@ -2162,8 +2280,16 @@ int pk_batch_set_components(pk_batch *b, const pk_component *comps, uint32_t n)
 {
 	if (!b || !comps) return set_err(PK_EINVAL, "NULL argument");
 	if (n != b->comp_index.size()) return set_err(PK_EINVAL, "expected %zu components, got %u", b->comp_index.size(), n);
-	for (uint32_t i = 0; i < n; i++) {
+	for (uint32_t i = 0; i < n; i++)
 		if (comps[i].kind != b->comp_kind[i]) return set_err(PK_EINVAL, "component %u changed its kind", i);
+	for (const auto &src : b->psf_src) {
+		const pk_component &pc = comps[src.comp];
+		const auto &old = b->psf_items[&src - b->psf_src.data()];
+		if ((pc.convolve != 0) != (old.to_conv != 0))
+			return set_err(PK_EINVAL, "component %u changed its convolve flag", src.comp);
+		PK_TRY(validate_component(pc, true));
+	}
+	for (uint32_t i = 0; i < n; i++) {
 		if (b->comp_index[i] == ~0u) continue;
 		SetupIn &in = b->setup[b->comp_index[i]];
 		if ((comps[i].convolve != 0) != (in.c.convolve != 0))
@@ -2190,6 +2316,28 @@ int pk_batch_set_components(pk_batch *b, const pk_component *comps, uint32_t n)
 	                      hipMemcpyHostToDevice, b->ctx->stream));
 	PK_HIP(hipMemcpyAsync(b->d<void>(b->o_setup), b->setup.data(), sizeof(SetupIn) * b->setup.size(),
 	                      hipMemcpyHostToDevice, b->ctx->stream));
+	if (!b->psf_items.empty()) {
+		// new positions and magnitudes of the psf profiles; a layer whose
+		// largest item changed its size needs new launches
+		std::vector<uint32_t> before;
+		for (const auto &ly : b->psf_layers) before.push_back(ly.max_pixels);
+		for (size_t k = 0; k < b->psf_items.size(); k++) {
+			const auto &src = b->psf_src[k];
+			PkPsfItem it = psf_item(&src.model, comps[src.comp], src.a, src.sx, src.sy);
+			it.job = b->psf_items[k].job;
+			it.psf_off = b->psf_items[k].psf_off;
+			b->psf_items[k] = it;
+		}
+		b->psf_update_layers();
+		for (size_t l = 0; l < before.size(); l++)
+			if (before[l] != b->psf_layers[l].max_pixels) {
+				PK_HIP(hipStreamSynchronize(b->ctx->stream));
+				drop_graphs(b);
+				break;
+			}
+		PK_HIP(hipMemcpyAsync(b->d<void>(b->o_items), b->psf_items.data(), sizeof(PkPsfItem) * b->psf_items.size(),
+		                      hipMemcpyHostToDevice, b->ctx->stream));
+	}
 	PK_HIP(hipStreamSynchronize(b->ctx->stream));
 	return PK_OK;
 }

@@ -291,3 +291,65 @@ def test_batch_masks(ctx64, ctx32, oracle, prec):
             assert pix <= tol, (i, pix)
             assert flux <= tol, (i, flux)
     b.close()
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_batch_psf_profiles(ctx64, ctx32, oracle, prec):
+    """PSF profiles in a batch (PsfProfile::evaluate, src/psf.cpp:58-145):
+    convolved and not, several per model (their additions overlap), a model
+    made of psf profiles only, finesampled models, and new positions through
+    set_components (whose footprints change size at the image edges)."""
+    print("CASE batch psf profiles", prec, flush=True)
+    psf = workloads.moffat_psf_image(9, 2.5, 3.0)
+    psf_even = workloads.moffat_psf_image(8, 2.0, 3.0)
+
+    def comp(kind, convolve=True, **kw):
+        return abi.make_component(kind, convolve=convolve, **kw)
+
+    def spec(s, comps, psf_=psf, fs=1):
+        return abi.ModelSpec(s, s, comps, psf=psf_, finesampling=fs)
+
+    specs = [
+        spec(40, [comp("sersic", xcen=20.3, ycen=19.6, mag=15.0, re=4.0, nser=1.5, axrat=0.7, ang=20.0),
+                  comp("psf", xcen=12.4, ycen=25.1, mag=16.0),
+                  comp("psf", xcen=14.0, ycen=24.0, mag=15.5, convolve=False),
+                  comp("sky", bg=2e-10, convolve=False)]),
+        spec(33, [comp("psf", xcen=16.5, ycen=16.5, mag=14.0),
+                  comp("psf", xcen=17.2, ycen=15.9, mag=14.5),
+                  comp("psf", xcen=1.0, ycen=31.5, mag=15.0, convolve=False)], psf_=psf_even),
+        spec(36, [comp("sersic", xcen=18.0, ycen=18.0, mag=14.0, re=3.0, nser=3.0, axrat=0.5, ang=70.0),
+                  comp("psf", xcen=9.7, ycen=8.2, mag=15.0)], fs=2),
+        spec(30, [comp("sersic", xcen=15.0, ycen=15.2, mag=15.0, re=5.0, nser=1.0, axrat=0.9, ang=0.0)]),
+    ]
+    ctx = ctx64 if prec == "fp64" else ctx32
+    tol = TOL_FP64 if prec == "fp64" else TOL_FP32
+    b = Batch(ctx, specs, graph=True)
+
+    def check(sps):
+        b.evaluate()
+        b.sync()
+        for i, sp in enumerate(sps):
+            ref, _ = oracle.evaluate(sp)
+            got = b.image(i)
+            assert got.shape == ref.shape, (i, got.shape, ref.shape)
+            pix, flux = rel_errors(got, ref)
+            assert pix <= tol, (i, pix)
+            assert flux <= tol, (i, flux)
+
+    check(specs)
+    check(specs)
+    new_specs, comps = [], []
+    for sp in specs:
+        cs = []
+        for k in range(sp.m.n_components):
+            c = abi.Component.from_buffer_copy(sp.m.components[k])
+            if c.kind == abi.PROFILE_KINDS["psf"]:
+                c.xcen = c.xcen * 0.2 + 0.3   # towards the corner: smaller footprints
+                c.ycen += 0.45
+                c.mag -= 0.3
+            cs.append(c)
+        comps += cs
+        new_specs.append(abi.ModelSpec(sp.m.width, sp.m.height, cs, psf=sp._psf, finesampling=sp.m.finesampling))
+    b.set_components(comps)
+    check(new_specs)
+    b.close()
