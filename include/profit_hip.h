@@ -220,10 +220,10 @@ int pk_convolver_padding(int type, uint32_t src_w, uint32_t src_h, uint32_t krn_
  * in one sequence of launches (optionally captured in a hipGraph), with the
  * per-component set-up (bn, normalisation integrals, adjusted subsampling
  * constants) computed in FP64 on the device.  Supported per model: radial,
- * sky, psf and null profiles (psf profiles and calculation masks not with
- * PK_BATCH_GRAD), BRUTE/BRUTE_OLD convolution, calculation masks,
- * finesampling; the FFT convolver is rejected.  Output images are FP32 in a
- * PK_FP32 context and FP64 in a PK_FP64 one. */
+ * sky, psf and null profiles, BRUTE/BRUTE_OLD/FFT convolution, cropped or
+ * not, calculation masks, finesampling.  PK_BATCH_GRAD batches take sersic,
+ * sky and null profiles, no masks and the BRUTE convolvers.  Output images
+ * are FP32 in a PK_FP32 context and FP64 in a PK_FP64 one. */
 typedef struct pk_batch pk_batch;
 
 enum {
@@ -283,6 +283,9 @@ int pk_batch_get_image(pk_batch *b, uint32_t i, double *out, size_t out_capacity
  * size in bytes (4 or 8) and the element offset of image i. */
 int pk_batch_device_images(const pk_batch *b, void **base, size_t *elem_size);
 int pk_batch_image_offset(const pk_batch *b, uint32_t i, uint64_t *offset);
+/* The offset Model::evaluate(Point &offset_out) reports for image i
+ * (src/model.cpp:259-318): non-zero only for an uncropped FFT convolution. */
+int pk_batch_image_origin(const pk_batch *b, uint32_t i, int32_t *x, int32_t *y);
 int pk_batch_destroy(pk_batch *b);
 
 /* ---- autotuner -------------------------------------------------------------

@@ -661,6 +661,69 @@ T *hptr(pk_ctx *ctx, size_t off) { return reinterpret_cast<T *>(static_cast<char
 
 uint32_t cdiv(uint32_t a, uint32_t b) { return (a + b - 1) / b; }
 
+// Frames of Model::evaluate (src/model.cpp:286-318) for a job whose dw, dh
+// are set: the convolver's result frame, the crop of the PSF padding inside
+// it, the returned size after downsampling, and the offset evaluate(Point &)
+// reports.
+int frame_geometry(const pk_model *m, const Analysis &a, PkJob &jb, int &offx, int &offy)
+{
+	const uint32_t fs = m->finesampling;
+	ConvGeom g{a.dw, a.dh, 0, 0, 0, 0, 0, 0};
+	if (a.conv_required)
+		PK_TRY(conv_geometry(m->convolver, a.dw, a.dh, m->psf_width, m->psf_height, m->crop != 0, g));
+	jb.cw = a.conv_required ? g.cw : a.dw;
+	jb.ch = a.conv_required ? g.ch : a.dh;
+	jb.ctiles_x = cdiv(jb.cw, CT_W);
+	jb.csx = g.csx;
+	jb.wrap_y = g.wrap_y;
+	jb.csy = g.csy;
+	jb.cmx = g.mx;
+	jb.cmy = g.my;
+	jb.mox = a.conv_required ? g.offx : 0;
+	jb.moy = a.conv_required ? g.offy : 0;
+	offx = a.conv_required ? g.offx : 0;
+	offy = a.conv_required ? g.offy : 0;
+	int cox = 0, coy = 0;
+	int crw = (int)jb.cw, crh = (int)jb.ch;
+	if (a.padx > 0 && a.pady > 0) {
+		cox = (int)a.padx;
+		coy = (int)a.pady;
+		crw = (int)a.dw - 2 * (int)a.padx;
+		crh = (int)a.dh - 2 * (int)a.pady;
+		if (!m->crop) {
+			int pa[4], pi[4];
+			const int ct = a.conv_required ? m->convolver : PK_CONV_BRUTE;
+			conv_padding(ct, a.dw, a.dh, m->psf_width, m->psf_height, pa);
+			conv_padding(ct, a.dw - 2 * a.padx, a.dh - 2 * a.pady, m->psf_width, m->psf_height, pi);
+			const int odx = pa[0] - pi[0], ody = pa[1] - pi[1], ddx = pa[2] - pi[2], ddy = pa[3] - pi[3];
+			crw = (int)jb.cw - 2 * (int)a.padx - (odx + ddx);
+			crh = (int)jb.ch - 2 * (int)a.pady - (ody + ddy);
+			cox += odx;
+			coy += ody;
+			offx -= odx;
+			offy -= ody;
+		}
+	}
+	if (crw <= 0 || crh <= 0) return set_err(PK_EINVAL, "empty image after cropping");
+	jb.cox = cox;
+	jb.coy = coy;
+	jb.crw = (uint32_t)crw;
+	jb.crh = (uint32_t)crh;
+	jb.fs = 1;
+	uint32_t ow = (uint32_t)crw, oh = (uint32_t)crh;
+	if (fs > 1 && !m->return_finesampled) {
+		jb.fs = fs;
+		ow = cdiv(ow, fs);
+		oh = cdiv(oh, fs);
+		offx /= (int)fs;
+		offy /= (int)fs;
+	}
+	jb.ow = ow;
+	jb.oh = oh;
+	return PK_OK;
+}
+
+
 size_t conv_lds_bytes(uint32_t kw, uint32_t rows, size_t elem)
 {
 	return (size_t)(CT_W + kw - 1) * (CT_H + rows - 1) * elem;
@@ -933,11 +996,6 @@ int evaluate_impl(pk_ctx *ctx, const pk_model *m, const Analysis &a, double *out
 	}
 	if (comps.size() > 32) return set_err(PK_EINVAL, "at most 32 radial/sky profiles per model are supported");
 
-	// convolution geometry
-	ConvGeom g{a.dw, a.dh, 0, 0, 0, 0, 0, 0};
-	if (a.conv_required)
-		PK_TRY(conv_geometry(m->convolver, a.dw, a.dh, m->psf_width, m->psf_height, m->crop != 0, g));
-
 	// composition: crop padding (src/model.cpp:289-306), downsample, mask
 	PkJob jb{};
 	jb.dw = a.dw;
@@ -954,54 +1012,9 @@ int evaluate_impl(pk_ctx *ctx, const pk_model *m, const Analysis &a, double *out
 	jb.kw = m->psf_width;
 	jb.kh = m->psf_height;
 	jb.ctile_off = 0;
-	jb.cw = a.conv_required ? g.cw : a.dw;
-	jb.ch = a.conv_required ? g.ch : a.dh;
-	jb.ctiles_x = cdiv(jb.cw, CT_W);
-	jb.csx = g.csx;
-	jb.wrap_y = g.wrap_y;
-	jb.csy = g.csy;
-	jb.cmx = g.mx;
-	jb.cmy = g.my;
-	jb.mox = a.conv_required ? g.offx : 0;
-	jb.moy = a.conv_required ? g.offy : 0;
-	int offx = a.conv_required ? g.offx : 0, offy = a.conv_required ? g.offy : 0;
-	int cox = 0, coy = 0;
-	int crw = (int)jb.cw, crh = (int)jb.ch;
-	if (a.padx > 0 && a.pady > 0) {
-		cox = (int)a.padx;
-		coy = (int)a.pady;
-		crw = (int)a.dw - 2 * (int)a.padx;
-		crh = (int)a.dh - 2 * (int)a.pady;
-		if (!m->crop) {
-			int pa[4], pi[4];
-			const int ct = a.conv_required ? m->convolver : PK_CONV_BRUTE;
-			conv_padding(ct, a.dw, a.dh, m->psf_width, m->psf_height, pa);
-			conv_padding(ct, a.dw - 2 * a.padx, a.dh - 2 * a.pady, m->psf_width, m->psf_height, pi);
-			const int odx = pa[0] - pi[0], ody = pa[1] - pi[1], ddx = pa[2] - pi[2], ddy = pa[3] - pi[3];
-			crw = (int)jb.cw - 2 * (int)a.padx - (odx + ddx);
-			crh = (int)jb.ch - 2 * (int)a.pady - (ody + ddy);
-			cox += odx;
-			coy += ody;
-			offx -= odx;
-			offy -= ody;
-		}
-	}
-	if (crw <= 0 || crh <= 0) return set_err(PK_EINVAL, "empty image after cropping");
-	jb.cox = cox;
-	jb.coy = coy;
-	jb.crw = (uint32_t)crw;
-	jb.crh = (uint32_t)crh;
-	jb.fs = 1;
-	uint32_t ow = (uint32_t)crw, oh = (uint32_t)crh;
-	if (fs > 1 && !m->return_finesampled) {
-		jb.fs = fs;
-		ow = cdiv(ow, fs);
-		oh = cdiv(oh, fs);
-		offx /= (int)fs;
-		offy /= (int)fs;
-	}
-	jb.ow = ow;
-	jb.oh = oh;
+	int offx = 0, offy = 0;
+	PK_TRY(frame_geometry(m, a, jb, offx, offy));
+	const uint32_t ow = jb.ow, oh = jb.oh;
 	jb.out_off = 0;
 	jb.otile_off = 0;
 	jb.otiles_x = cdiv(ow, 64);
@@ -1437,6 +1450,17 @@ struct pk_batch {
 	};
 	std::vector<PsfLayer> psf_layers;
 	size_t o_items = 0, o_psfd = 0;
+	// models convolved by the FFT convolver: one FFT convolution each, in
+	// turn, through one shared transform buffer
+	struct FftJob {
+		uint32_t job;
+		uint64_t psf_off;  // normalised FP64 PSF in o_psfd
+		double krn_max;
+	};
+	std::vector<FftJob> fft_jobs;
+	std::vector<uint8_t> is_fft;  // per model
+	std::vector<int32_t> origin;  // per model: the offset Model::evaluate(Point &) reports
+	size_t o_fz = 0, o_ftw = 0;
 	// pixels the largest item of each layer covers (grid width of its launch)
 	void psf_update_layers()
 	{
@@ -1503,6 +1527,18 @@ int launch_psf_layers(pk_batch *b, hipStream_t s, T *nc, T *c)
 	return PK_OK;
 }
 
+// FFT convolutions of a batch, one model after the other through the shared
+// transform buffer, into each model's region of the convolution buffer.
+template <typename T>
+int launch_fft_jobs(pk_batch *b, hipStream_t s, const T *c)
+{
+	for (const auto &fj : b->fft_jobs)
+		PK_TRY(launch_fft_conv<T>(s, b->jobs[fj.job], b->d<PkJob>(b->o_jobs) + fj.job, c,
+		                          b->d<double>(b->o_psfd) + fj.psf_off, fj.krn_max, b->d<uint8_t>(b->o_mask),
+		                          b->d<T>(b->o_conv), b->d<double2>(b->o_fz), b->d<double2>(b->o_ftw)));
+	return PK_OK;
+}
+
 int batch_launch(pk_batch *b)
 {
 	pk_ctx *ctx = b->ctx;
@@ -1540,6 +1576,7 @@ int batch_launch(pk_batch *b)
 				                          b->d<float>(b->o_conv), cc.max_kw, cc.max_kh, tj_conv, b->tune.conv_tiles,
 				                          b->d<float>(b->o_out), nc, cc.start));
 			PK_HIP(hipGetLastError());
+			PK_TRY(launch_fft_jobs<float>(b, s, c));
 		}
 		if (timed) PK_HIP(hipEventRecord(b->ev[4], s));
 		if (b->notiles)
@@ -1558,6 +1595,7 @@ int batch_launch(pk_batch *b)
 				                           b->d<double>(b->o_conv), cc.max_kw, cc.max_kh, tj_conv, PK_CONV_TILES_VALU,
 				                           b->d<double>(b->o_out), nc, cc.start));
 			PK_HIP(hipGetLastError());
+			PK_TRY(launch_fft_jobs<double>(b, s, c));
 		}
 		if (timed) PK_HIP(hipEventRecord(b->ev[4], s));
 		if (b->notiles)
@@ -1796,9 +1834,23 @@ int pk_batch_create(pk_ctx *ctx, const pk_model *models, uint32_t n_models, uint
 	b->elem = elem;
 	std::vector<double> psf_all;
 	std::vector<uint8_t> mask_all;
-	std::vector<double> psfd_all;  // normalised PSFs of models with psf profiles
+	// normalised FP64 PSFs (Model::set_psf) of models with psf profiles or
+	// the FFT convolver, one per model
+	std::vector<double> psfd_all;
 	uint32_t psfd_model = ~0u;
 	uint64_t psfd_off = 0;
+	auto model_psfd = [&](uint32_t mi, const pk_model *m) -> uint64_t {
+		if (psfd_model != mi) {
+			psfd_model = mi;
+			psfd_off = psfd_all.size();
+			const size_t pn = (size_t)m->psf_width * m->psf_height;
+			double sum = 0;
+			for (size_t t = 0; t < pn; t++) sum += m->psf[t];
+			for (size_t t = 0; t < pn; t++) psfd_all.push_back(sum > 0 ? m->psf[t] / sum : m->psf[t]);
+		}
+		return psfd_off;
+	};
+	size_t fft_points = 0;  // complex points of the largest FFT convolution
 	uint64_t img_off = 0, out_off = 0;
 	auto fail = [&](int code) {
 		pk_batch_destroy(b);
@@ -1811,8 +1863,9 @@ int pk_batch_create(pk_ctx *ctx, const pk_model *models, uint32_t n_models, uint
 		if (st) return fail(set_err(st, "model %u: %s", mi, g_err.c_str()));
 		if (m->dry_run) return fail(set_err(PK_EINVAL, "model %u: dry_run is not supported in a batch", mi));
 		if (fp32 && check_fp32_limits(m)) return fail(set_err(PK_EINVAL, "model %u: %s", mi, g_err.c_str()));
-		if (a.conv_required && m->convolver == PK_CONV_FFT)
-			return fail(set_err(PK_EINVAL, "model %u: the FFT convolver is not supported in a batch", mi));
+		const bool fft = a.conv_required && m->convolver == PK_CONV_FFT;
+		if (fft && (flags & PK_BATCH_GRAD))
+			return fail(set_err(PK_EINVAL, "model %u: gradients need the BRUTE convolver", mi));
 		// calculation mask in the drawing frame (and the user's mask for the
 		// returned image when it was adjusted), as Model::evaluate
 		std::vector<uint8_t> dmask;
@@ -1863,18 +1916,9 @@ int pk_batch_create(pk_ctx *ctx, const pk_model *models, uint32_t n_models, uint
 				continue;
 			}
 			if (pc.kind == PK_PROFILE_PSF) {
-				if (psfd_model != mi) {
-					// this model's PSF, normalised (Model::set_psf), once
-					psfd_model = mi;
-					psfd_off = psfd_all.size();
-					const size_t pn = (size_t)m->psf_width * m->psf_height;
-					double sum = 0;
-					for (size_t t = 0; t < pn; t++) sum += m->psf[t];
-					for (size_t t = 0; t < pn; t++) psfd_all.push_back(sum > 0 ? m->psf[t] / sum : m->psf[t]);
-				}
 				PkPsfItem it = psf_item(m, pc, a, sx, sy);
 				it.job = mi;
-				it.psf_off = psfd_off;
+				it.psf_off = model_psfd(mi, m);
 				b->psf_items.push_back(it);
 				pk_batch::PsfSrc src{};
 				src.model = *m;
@@ -1921,7 +1965,27 @@ int pk_batch_create(pk_ctx *ctx, const pk_model *models, uint32_t n_models, uint
 		jb.crh = a.dh - 2 * (uint32_t)pcy;
 		jb.conv_off = img_off;
 		jb.ctile_off = 0;  // numbered by PSF-size class below
-		if (a.conv_required) {
+		int32_t ox = 0, oy = 0;
+		if (fft) {
+			// the FFT convolver's frames, as a single evaluation; its result
+			// goes to a region of its own after the drawing-size ones
+			int gx = 0, gy = 0;
+			const int gs = frame_geometry(m, a, jb, gx, gy);
+			if (gs) return fail(set_err(gs, "model %u: %s", mi, g_err.c_str()));
+			FftGeom fg;
+			const int fs_ = fft_geometry(a.dw, a.dh, m->psf_width, m->psf_height, fg);
+			if (fs_) return fail(set_err(fs_, "model %u: %s", mi, g_err.c_str()));
+			fft_points = std::max(fft_points, (size_t)fg.P * fg.Q);
+			ox = gx;
+			oy = gy;
+			jb.ctiles_x = 0;
+			jb.conv_off = (uint64_t)jb.cw * jb.ch;  // size for now; placed after the loop
+			b->any_conv = true;
+			pk_batch::FftJob fj{mi, model_psfd(mi, m), 0};
+			for (size_t t = 0; t < (size_t)m->psf_width * m->psf_height; t++)
+				fj.krn_max = std::max(fj.krn_max, std::fabs(psfd_all[fj.psf_off + t]));
+			b->fft_jobs.push_back(fj);
+		} else if (a.conv_required) {
 			const size_t pn = (size_t)m->psf_width * m->psf_height;
 			double sum = 0;
 			for (size_t t = 0; t < pn; t++) sum += m->psf[t];
@@ -1963,7 +2027,7 @@ int pk_batch_create(pk_ctx *ctx, const pk_model *models, uint32_t n_models, uint
 		// without downsampling (and without the user's mask to apply again) the
 		// convolved frame is the returned image: the convolution writes it
 		// (composition tiles are set below)
-		jb.fuse_out = (a.conv_required && jb.fs == 1 && !adjusted) ? 1 : 0;
+		jb.fuse_out = (a.conv_required && !fft && jb.fs == 1 && !adjusted) ? 1 : 0;
 		jb.omask_off = NONE;
 		jb.omask_f = 1;
 		if (adjusted) {
@@ -1977,8 +2041,19 @@ int pk_batch_create(pk_ctx *ctx, const pk_model *models, uint32_t n_models, uint
 		jb.fx_scale = fixed_point_scale(m);
 		for (uint32_t k = jb.comp_off; k < jb.comp_off + jb.ncomp; k++) b->setup[k].fx_scale = jb.fx_scale;
 		b->jobs.push_back(jb);
+		b->is_fft.push_back(fft ? 1 : 0);
+		b->origin.push_back(ox);
+		b->origin.push_back(oy);
 		img_off += (uint64_t)a.dw * a.dh;
 		out_off += (uint64_t)ow * oh;
+	}
+	// FFT results after the drawing-size region of the convolution buffer
+	uint64_t conv_total = img_off;
+	for (const auto &fj : b->fft_jobs) {
+		PkJob &jb = b->jobs[fj.job];
+		const uint64_t n = jb.conv_off;
+		jb.conv_off = conv_total;
+		conv_total += n;
 	}
 	b->n_comps = (uint32_t)b->setup.size();
 	b->n_pixels = img_off;
@@ -2002,7 +2077,7 @@ int pk_batch_create(pk_ctx *ctx, const pk_model *models, uint32_t n_models, uint
 			pk_batch::ConvClass cc{b->nctiles, 0, 0, 0};
 			for (auto &jb : b->jobs) {
 				const uint32_t side = std::max(jb.kw, jb.kh);
-				if (!jb.has_conv || side <= lim_lo || side > class_lim[ci]) continue;
+				if (!jb.has_conv || b->is_fft[&jb - b->jobs.data()] || side <= lim_lo || side > class_lim[ci]) continue;
 				jb.ctile_off = b->nctiles;
 				b->nctiles += jb.ctiles_x * cdiv(jb.ch, CT_H);
 				cc.max_kw = std::max(cc.max_kw, jb.kw);
@@ -2012,7 +2087,7 @@ int pk_batch_create(pk_ctx *ctx, const pk_model *models, uint32_t n_models, uint
 			if (cc.count) b->conv_classes.push_back(cc);
 		}
 		for (auto &jb : b->jobs)
-			if (!jb.has_conv) jb.ctile_off = b->nctiles;
+			if (!jb.has_conv || b->is_fft[&jb - b->jobs.data()]) jb.ctile_off = b->nctiles;
 	}
 	if (img_off >= (1ull << 32)) return fail(set_err(PK_EINVAL, "batch too large (%llu pixels)", (unsigned long long)img_off));
 
@@ -2093,7 +2168,9 @@ int pk_batch_create(pk_ctx *ctx, const pk_model *models, uint32_t n_models, uint
 	b->o_c = L.take(elem * (b->any_conv ? img_off : 1));
 	bool conv_buffer = false;  // some convolved job leaves its composition to compose_kernel
 	for (const auto &jb : b->jobs) conv_buffer = conv_buffer || (jb.has_conv && !jb.fuse_out);
-	b->o_conv = L.take(elem * (conv_buffer ? img_off : 1));
+	b->o_conv = L.take(elem * (conv_buffer ? conv_total : 1));
+	b->o_fz = L.take(fft_points ? sizeof(double2) * (fft_points + 1) : 1);
+	b->o_ftw = L.take(fft_points ? sizeof(double2) * FFT_TWIDDLES : 1);
 	b->o_ent = L.take(sizeof(RefineEntry) * img_off);
 	b->o_acc = L.take(fp32 ? 2 * sizeof(unsigned long long) * img_off : 8);
 	b->qcap = fp32 ? R32_QCAP_BATCH : 0;
@@ -2124,7 +2201,7 @@ int pk_batch_create(pk_ctx *ctx, const pk_model *models, uint32_t n_models, uint
 		std::memcpy(stage.data() + b->o_items, b->psf_items.data(), sizeof(PkPsfItem) * b->psf_items.size());
 	if (!psfd_all.empty()) std::memcpy(stage.data() + b->o_psfd, psfd_all.data(), sizeof(double) * psfd_all.size());
 	for (const auto &jb : b->jobs) {
-		if (!jb.has_conv) continue;
+		if (!jb.has_conv || b->is_fft[&jb - b->jobs.data()]) continue;
 		if (fp32)
 			psf_conv_layout<float>(psf_all.data() + jb.psf_off, jb.kw, jb.kh,
 			                       reinterpret_cast<float *>(stage.data() + b->o_psfT) + jb.psf_off);
@@ -2135,6 +2212,8 @@ int pk_batch_create(pk_ctx *ctx, const pk_model *models, uint32_t n_models, uint
 	PK_HIP(hipMemcpy(b->dmem, stage.data(), in_bytes, hipMemcpyHostToDevice));
 	if (!mask_all.empty())
 		PK_HIP(hipMemcpy(b->d<void>(b->o_mask), mask_all.data(), mask_all.size(), hipMemcpyHostToDevice));
+	if (fft_points)
+		PK_HIP(hipMemcpy(b->d<void>(b->o_ftw), fft_twiddles(), sizeof(double2) * FFT_TWIDDLES, hipMemcpyHostToDevice));
 	if (flags & PK_BATCH_GRAD) {
 		// PSF layout of the adjoint convolution: kt[k * kh + l] = krn(k, l)
 		// (the flipped PSF in the layout conv_kernel reads)
@@ -2446,6 +2525,14 @@ int pk_batch_image_offset(const pk_batch *b, uint32_t i, uint64_t *offset)
 {
 	if (!b || i >= b->n_models) return set_err(PK_EINVAL, "bad batch or image index");
 	if (offset) *offset = b->jobs[i].out_off;
+	return PK_OK;
+}
+
+int pk_batch_image_origin(const pk_batch *b, uint32_t i, int32_t *x, int32_t *y)
+{
+	if (!b || i >= b->n_models) return set_err(PK_EINVAL, "bad batch or image index");
+	if (x) *x = b->origin[2 * (size_t)i];
+	if (y) *y = b->origin[2 * (size_t)i + 1];
 	return PK_OK;
 }
 

@@ -332,6 +332,13 @@ class Batch:
             out.append(o.value)
         return out
 
+    def origin(self, i):
+        """(x, y) offset of image i as Model.evaluate reports it (non-zero
+        only for an uncropped FFT convolution)."""
+        x, y = C.c_int32(), C.c_int32()
+        check(self.L.pk_batch_image_origin(self.handle, i, C.byref(x), C.byref(y)))
+        return x.value, y.value
+
     def set_data(self, images, sigmas=None):
         """Observed images (one array per model, shaped like the returned
         image) and their sigma (arrays, scalars or None = 1)."""

@@ -353,3 +353,57 @@ def test_batch_psf_profiles(ctx64, ctx32, oracle, prec):
     b.set_components(comps)
     check(new_specs)
     b.close()
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_batch_fft_convolver(ctx64, ctx32, oracle, prec):
+    """Models convolved by the FFT convolver in a batch, next to BRUTE and
+    unconvolved ones: cropped and uncropped results (with the offset
+    Model::evaluate reports, src/model.cpp:286-318), finesampled and masked
+    models, a psf profile, an even PSF. FFT rounding is relative to the peak
+    (~log2(N) eps max|image|), so FP64 is bounded against the peak and per
+    pixel at 1e-8 above 1e-6 of the peak, as for a single evaluation."""
+    print("CASE batch fft convolver", prec, flush=True)
+    rng = np.random.Generator(np.random.PCG64(5))
+    psf, _ = oracle.evaluate(abi.ModelSpec(15, 15, [abi.make_component("moffat", xcen=7.5, ycen=7.5, fwhm=3.0,
+                                                                       con=2.5, mag=0.0)]))
+    psf_even = workloads.moffat_psf_image(10, 2.5, 3.0)
+
+    def sersic(s, **kw):
+        p = dict(xcen=0.47 * s, ycen=0.52 * s, mag=15.0, re=0.1 * s, nser=2.0, axrat=0.6, ang=35.0, convolve=True)
+        p.update(kw)
+        return abi.make_component("sersic", **p)
+
+    sky = abi.make_component("sky", bg=1e-9)
+    specs = [
+        abi.ModelSpec(60, 50, [sersic(55), sky], psf=psf, convolver="fft"),
+        abi.ModelSpec(60, 50, [sersic(55), sky], psf=psf, convolver="fft", crop=False),
+        abi.ModelSpec(40, 40, [sersic(40, nser=1.0), sky], psf=psf, convolver="brute"),
+        abi.ModelSpec(36, 36, [sersic(36, nser=3.0)], psf=psf, convolver="fft", finesampling=2,
+                      return_finesampled=False),
+        abi.ModelSpec(48, 48, [sersic(48), sky], psf=psf, convolver="fft",
+                      mask=(rng.uniform(size=(48, 48)) > 0.3).astype(np.uint8)),
+        abi.ModelSpec(44, 44, [sersic(44, convolve=False), abi.make_component("psf", xcen=20.2, ycen=23.6, mag=15.5,
+                                                                             convolve=True)],
+                      psf=psf_even, convolver="fft"),
+        abi.ModelSpec(32, 32, [sersic(32, convolve=False), sky]),
+    ]
+    ctx = ctx64 if prec == "fp64" else ctx32
+    b = Batch(ctx, specs, graph=True)
+    for _rep in range(2):
+        b.evaluate()
+        b.sync()
+        for i, sp in enumerate(specs):
+            ref, roff = oracle.evaluate(sp)
+            got = b.image(i)
+            assert got.shape == ref.shape, (i, got.shape, ref.shape)
+            assert b.origin(i) == tuple(roff), (i, b.origin(i), roff)
+            pix, flux = rel_errors(got, ref)
+            if prec == "fp32":
+                assert pix <= TOL_FP32 and flux <= TOL_FP32, (i, pix, flux)
+            elif sp.m.convolver == abi.FFT and any(sp.m.components[k].convolve for k in range(sp.m.n_components)):
+                assert np.max(np.abs(got - ref)) <= 1e-14 * np.max(np.abs(ref)), i
+                assert pix <= 1e-8, (i, pix)
+            else:
+                assert pix <= TOL_FP64, (i, pix)
+    b.close()
