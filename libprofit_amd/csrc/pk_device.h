@@ -653,10 +653,11 @@ __host__ __device__ inline size_t conv_mfma_lds_bytes(int kw, int kh)
 }
 
 // Products of all PSF rows for this wavefront's two 16 x 16 output blocks
-// (rows ry0.., columns cx0.. and cx0 + 16..).  NJ > 0: 4 * NJ source columns
-// per row known at compile time (fully unrolled, operands read before the
-// products); NJ = 0: run-time count.
-template <int NJ>
+// (rows ry0.., columns cx0.. and cx0 + 16..; TWO = false: the first only, the
+// second lies right of the image).  NJ > 0: 4 * NJ source columns per row
+// known at compile time (fully unrolled, operands read before the products);
+// NJ = 0: run-time count.
+template <int NJ, bool TWO>
 __device__ inline void conv_mfma_rows(const float *tile, const float *band, int TS, int BS, int kh, int nj_rt,
                                       int lane, int ry0, int cx0, double out[2][4])
 {
@@ -670,19 +671,19 @@ __device__ inline void conv_mfma_rows(const float *tile, const float *band, int 
 #pragma unroll
 			for (int jc = 0; jc < NJ; jc++) {
 				a0[jc] = arow[4 * jc];
-				a1[jc] = arow[16 + 4 * jc];
+				if (TWO) a1[jc] = arow[16 + 4 * jc];
 				b[jc] = brow[4 * jc];
 			}
 #pragma unroll
 			for (int jc = 0; jc < NJ; jc++) {
 				acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(a0[jc], b[jc], acc0, 0, 0, 0);
-				acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(a1[jc], b[jc], acc1, 0, 0, 0);
+				if (TWO) acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(a1[jc], b[jc], acc1, 0, 0, 0);
 			}
 		} else {
 			for (int jc = 0; jc < nj_rt; jc++) {
 				const float bv = brow[4 * jc];
 				acc0 = __builtin_amdgcn_mfma_f32_16x16x4f32(arow[4 * jc], bv, acc0, 0, 0, 0);
-				acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(arow[16 + 4 * jc], bv, acc1, 0, 0, 0);
+				if (TWO) acc1 = __builtin_amdgcn_mfma_f32_16x16x4f32(arow[16 + 4 * jc], bv, acc1, 0, 0, 0);
 			}
 		}
 #pragma unroll
@@ -694,15 +695,16 @@ __device__ inline void conv_mfma_rows(const float *tile, const float *band, int 
 }
 
 // NJ = ceil((16 + kw - 1) / 4) for the odd PSF widths CT_KMIN..CT_KMAX
-template <int K>
+template <int K, bool TWO>
 __device__ inline bool conv_mfma_dispatch(int kw, const float *tile, const float *band, int TS, int BS, int kh,
                                           int lane, int ry0, int cx0, double out[2][4])
 {
 	if (kw == K) {
-		conv_mfma_rows<(16 + K - 1 + 3) / 4>(tile, band, TS, BS, kh, 0, lane, ry0, cx0, out);
+		conv_mfma_rows<(16 + K - 1 + 3) / 4, TWO>(tile, band, TS, BS, kh, 0, lane, ry0, cx0, out);
 		return true;
 	}
-	if constexpr (K + 2 <= CT_KMAX) return conv_mfma_dispatch<K + 2>(kw, tile, band, TS, BS, kh, lane, ry0, cx0, out);
+	if constexpr (K + 2 <= CT_KMAX)
+		return conv_mfma_dispatch<K + 2, TWO>(kw, tile, band, TS, BS, kh, lane, ry0, cx0, out);
 	return false;
 }
 
@@ -771,12 +773,18 @@ __global__ __launch_bounds__(256, 1) void conv_mfma_kernel(const PkJob *__restri
 
 	const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
 	const int ry0 = (w >> 1) * 16, cx0 = (w & 1) * 32;  // this wave: rows ry0.., columns cx0.. (two 16 x 16 blocks)
-	if (by + ry0 >= (int)jb.ch) return;  // no output rows (nothing below waits for the workgroup)
+	// no output rows or columns: nothing below waits for the workgroup
+	if (by + ry0 >= (int)jb.ch || bx + cx0 >= (int)jb.cw) return;
 	const int lo = lane & 15, kq = lane >> 4;
 	double out[2][4] = {{0, 0, 0, 0}, {0, 0, 0, 0}};
-	if (!((kw & 1) && kw >= CT_KMIN && kw <= CT_KMAX &&
-	      conv_mfma_dispatch<CT_KMIN>(kw, tile, band, TS, BS, kh, lane, ry0, cx0, out)))
-		conv_mfma_rows<0>(tile, band, TS, BS, kh, (16 + kw - 1 + 3) / 4, lane, ry0, cx0, out);
+	const bool odd = (kw & 1) && kw >= CT_KMIN && kw <= CT_KMAX;
+	if (bx + cx0 + 16 < (int)jb.cw) {
+		if (!(odd && conv_mfma_dispatch<CT_KMIN, true>(kw, tile, band, TS, BS, kh, lane, ry0, cx0, out)))
+			conv_mfma_rows<0, true>(tile, band, TS, BS, kh, (16 + kw - 1 + 3) / 4, lane, ry0, cx0, out);
+	} else {
+		if (!(odd && conv_mfma_dispatch<CT_KMIN, false>(kw, tile, band, TS, BS, kh, lane, ry0, cx0, out)))
+			conv_mfma_rows<0, false>(tile, band, TS, BS, kh, (16 + kw - 1 + 3) / 4, lane, ry0, cx0, out);
+	}
 #pragma unroll
 	for (int s2 = 0; s2 < 2; s2++)
 #pragma unroll
