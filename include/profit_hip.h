@@ -254,7 +254,8 @@ typedef struct pk_tuning {
 	uint32_t claim_spread;     /* 0: refinement pixels are claimed in the order the pixel pass listed
 	                              them; a power of two n <= 4096: in runs of n taken from 64 places of
 	                              the list in turn (spreads deep galaxy centres over wavefronts) */
-	double ms_evaluate;        /* device ms of one evaluation with these settings, when measured (else 0) */
+	double ms_evaluate;        /* device ms of one evaluation with these settings, when measured (else 0);
+	                              a PK_BATCH_GRAD batch tuned with its data: forward + adjoint */
 } pk_tuning;
 
 typedef struct pk_batch_info {

@@ -1705,12 +1705,17 @@ void drop_graphs(pk_batch *b)
 // One eager evaluation with settings t; stage[] = device ms of set-up,
 // coarse, refinement, convolution, composition, total.  +inf stage times
 // when the refinement queue overflowed with these settings.
+// One eager timed evaluation with settings t: the stage times (set-up, coarse,
+// refinement, convolution, composition, total).  A gradient batch with data
+// runs its adjoint as well, and the total includes it.
 int tune_run(pk_batch *b, const pk_tuning &t, float stage[6])
 {
 	b->tune = t;
 	const uint32_t flags = b->flags;
+	const bool adjoint = (flags & PK_BATCH_GRAD) && b->have_data;
 	b->flags &= ~(uint32_t)PK_BATCH_GRAPH;  // eager launches record the stage events
-	const int st = batch_launch(b);
+	int st = batch_launch(b);
+	if (st == PK_OK && adjoint) st = b->elem == 4 ? grad_launch_t<float>(b) : grad_launch_t<double>(b);
 	b->flags = flags;
 	PK_TRY(st);
 	PK_HIP(hipStreamSynchronize(b->ctx->stream));
@@ -1718,7 +1723,15 @@ int tune_run(pk_batch *b, const pk_tuning &t, float stage[6])
 	PK_HIP(hipEventElapsedTime(&stage[5], b->ev[0], b->ev[5]));
 	PkCounters cnt{};
 	PK_HIP(hipMemcpy(&cnt, b->d<void>(b->o_cnt), sizeof(cnt), hipMemcpyDeviceToHost));
-	if (cnt.overflow)
+	bool overflow = cnt.overflow != 0;
+	if (adjoint) {
+		float ms = 0;
+		PK_HIP(hipEventElapsedTime(&ms, b->ev_grad[0], b->ev_grad[1]));
+		stage[5] += ms;
+		PK_HIP(hipMemcpy(&cnt, b->d<void>(b->o_gcnt), sizeof(cnt), hipMemcpyDeviceToHost));
+		overflow = overflow || cnt.overflow != 0;
+	}
+	if (overflow)
 		for (int k = 0; k < 6; k++) stage[k] = INFINITY;
 	return PK_OK;
 }
@@ -1758,17 +1771,21 @@ int autotune(pk_batch *b)
 		best.*field = best_v;
 		return PK_OK;
 	};
+	// a gradient batch with data is timed on forward + adjoint (the settings
+	// act on both), any other on the stage a setting acts on
+	const bool whole = (b->flags & PK_BATCH_GRAD) && b->have_data;
+	const int s_conv = whole ? 5 : 3, s_ref = whole ? 5 : 2;
 	if (fp32 && b->any_conv && conv_mfma_lds_bytes((int)b->max_kw, (int)b->max_kh) <= LDS_LIMIT)
-		PK_TRY(search(&pk_tuning::conv_tiles, {PK_CONV_TILES_VALU, PK_CONV_TILES_MFMA}, 3));
+		PK_TRY(search(&pk_tuning::conv_tiles, {PK_CONV_TILES_VALU, PK_CONV_TILES_MFMA}, s_conv));
 	// the refinement settings interact (an earlier hand-over wants larger
 	// claims): two sweeps
 	for (int sweep = 0; fp32 && b->n_comps && sweep < 2; sweep++) {
-		PK_TRY(search(&pk_tuning::handover_div, {4, 8, 16, 32, 64, 128}, 2));
-		PK_TRY(search(&pk_tuning::refine_occupancy, {50, 75, 100}, 2));
-		PK_TRY(search(&pk_tuning::claim_pixels, {32, 64, 128, 256, 512}, 2));
-		PK_TRY(search(&pk_tuning::claim_cells, {16, 64, 256}, 2));
-		PK_TRY(search(&pk_tuning::poll_rounds, {4, 16, 64, 256}, 2));
-		PK_TRY(search(&pk_tuning::claim_spread, {0, 4, 16, 64}, 2));
+		PK_TRY(search(&pk_tuning::handover_div, {4, 8, 16, 32, 64, 128}, s_ref));
+		PK_TRY(search(&pk_tuning::refine_occupancy, {50, 75, 100}, s_ref));
+		PK_TRY(search(&pk_tuning::claim_pixels, {32, 64, 128, 256, 512}, s_ref));
+		PK_TRY(search(&pk_tuning::claim_cells, {16, 64, 256}, s_ref));
+		PK_TRY(search(&pk_tuning::poll_rounds, {4, 16, 64, 256}, s_ref));
+		PK_TRY(search(&pk_tuning::claim_spread, {0, 4, 16, 64}, s_ref));
 	}
 	PK_TRY(tune_run(b, best, st));
 	best.ms_evaluate = st[5];
