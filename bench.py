@@ -125,7 +125,10 @@ def grad_bench(args, Batch, Context, evaluate_spec, ctx32, ctx64):
         datas.append(d)
         sigmas.append(sg)
     tb.close()
-    gb = Batch(ctx32, start, graph=True, grad=True, autotune=not args.no_tune)
+    fixed = json.loads(args.fits_tuning) if args.fits_tuning else None
+    gb = Batch(ctx32, start, graph=True, grad=True, autotune=not (args.no_tune or fixed))
+    if fixed:
+        gb.tuning = fixed
     gb.set_data(datas, sigmas)
     for _ in range(args.warmup):
         gb.evaluate_grad()
@@ -229,6 +232,10 @@ def main():
     ap.add_argument("--no-tune", action="store_true", help="default launch settings instead of the autotuner's")
     ap.add_argument("--tuning", default=None, metavar="JSON",
                     help="fixed launch settings (pk_tuning fields) instead of the autotuner's")
+    ap.add_argument("--fits-tuning", default=None, metavar="JSON",
+                    help="fixed launch settings of the chi^2-gradient batch instead of the autotuner's")
+    ap.add_argument("--fits-only", action="store_true",
+                    help="only run the chi^2-gradient workload (prints its figures as one JSON line)")
     ap.add_argument("--copy-probe", type=int, default=0, metavar="BYTES",
                     help="only run a plain device copy of BYTES bytes 3 times (calibrates the HBM byte counters)")
     args = ap.parse_args()
@@ -249,6 +256,12 @@ def main():
 
     from libprofit_amd import workloads
     from libprofit_amd.model import Batch, Context, evaluate_spec
+
+    if args.fits_only:
+        ctx32 = Context(0, "fp32")
+        ctx64 = Context(0, "fp64")
+        print(json.dumps(grad_bench(args, Batch, Context, evaluate_spec, ctx32, ctx64)), flush=True)
+        return
 
     t0 = time.time()
     specs = workloads.galaxy_batch(args.galaxies, args.seed)
