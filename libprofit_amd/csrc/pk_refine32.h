@@ -305,8 +305,9 @@ __global__ __launch_bounds__(64) void refine32_kernel(const PkJob *__restrict__ 
 	unsigned long long nev = 0;
 	uint32_t chunk_next = 0, chunk_end = 0, seen = 0;
 	int poll = 0;
-	double run[GA_SLOTS] = {0, 0, 0, 0, 0, 0, 0, 0};  // GRAD: running sums of component run_g
-	uint32_t run_g = ~0u;
+	// GRAD: per-lane running sums of two components, run_g and run_g2
+	double run[GA_SLOTS] = {0, 0, 0, 0, 0, 0, 0, 0}, run2[GA_SLOTS] = {0, 0, 0, 0, 0, 0, 0, 0};
+	uint32_t run_g = ~0u, run_g2 = ~0u;
 	__syncthreads();
 	for (;;) {
 		// ---- refill while the cells on the LIFO cannot fill the lanes (two
@@ -548,14 +549,29 @@ __global__ __launch_bounds__(64) void refine32_kernel(const PkJob *__restrict__ 
 					const uint64_t okb = __ballot(any_ok);
 					if (okb) {
 						const uint32_t g0 = wave_read(cell.gcomp, __ffsll((long long)okb) - 1);
-						if (__ballot(any_ok && cell.gcomp != g0) == 0) {
-							// one component: keep adding to the running sums
-							if (g0 != run_g) {
-								r32_flush_run(run, run_g, gacc, lane);
-								run_g = g0;
-							}
+						const uint64_t ob = __ballot(any_ok && cell.gcomp != g0);
+						const uint32_t g1 = ob ? wave_read(cell.gcomp, __ffsll((long long)ob) - 1) : ~0u;
+						if (!ob || __ballot(any_ok && cell.gcomp != g0 && cell.gcomp != g1) == 0) {
+							// at most two components: keep adding to the running sums,
+							// after handing on a slot held by a third one
+							auto hold = [&](uint32_t g, uint32_t keep) {
+								if (g == ~0u || g == run_g || g == run_g2) return;
+								if (run_g != keep) {
+									r32_flush_run(run, run_g, gacc, lane);
+									run_g = g;
+								} else {
+									r32_flush_run(run2, run_g2, gacc, lane);
+									run_g2 = g;
+								}
+							};
+							hold(g0, g1);
+							hold(g1, g0);
+							const bool in1 = any_ok && cell.gcomp == run_g, in2 = any_ok && cell.gcomp == run_g2;
 #pragma unroll
-							for (int k = 0; k < GA_SLOTS; k++) run[k] += ga[k];
+							for (int k = 0; k < GA_SLOTS; k++) {
+								run[k] += in1 ? ga[k] : 0.0;
+								run2[k] += in2 ? ga[k] : 0.0;
+							}
 						} else {
 							r32_flush_grad(any_ok, cell.gcomp, ga, gacc, lane);
 						}
@@ -611,7 +627,10 @@ __global__ __launch_bounds__(64) void refine32_kernel(const PkJob *__restrict__ 
 		if (pair) run_rounds(IntTag<2>{});
 		else run_rounds(IntTag<1>{});
 	}
-	if (GRAD) r32_flush_run(run, run_g, gacc, lane);
+	if (GRAD) {
+		r32_flush_run(run, run_g, gacc, lane);
+		r32_flush_run(run2, run_g2, gacc, lane);
+	}
 	if (!LAST && handover) {
 		// everything this wavefront holds goes to the next pass
 		for (;;) {
