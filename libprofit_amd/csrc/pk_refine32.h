@@ -249,6 +249,17 @@ __device__ inline void r32_flush_grad(bool ok, uint32_t gcomp, double ga[GA_SLOT
 	}
 }
 
+// Adds the per-lane running fixed-point sums of pixel accumulator t
+// (wavefront-uniform) to acc and clears them; t = ~0: nothing held.
+__device__ inline void r32_flush_fx(unsigned long long &run, uint64_t t, unsigned long long *__restrict__ acc,
+                                    uint32_t lane)
+{
+	if (t == ~0ull) return;
+	const unsigned long long v = wave_sum_u64(run);
+	if (lane == 0 && v) atomicAdd(&acc[t], v);
+	run = 0;
+}
+
 // Adds the per-lane running adjoint sums of component g (wavefront-uniform)
 // to gacc and clears them; g = ~0u: nothing held.
 __device__ inline void r32_flush_run(double run[GA_SLOTS], uint32_t g, double *__restrict__ gacc, uint32_t lane)
@@ -305,6 +316,10 @@ __global__ __launch_bounds__(64) void refine32_kernel(const PkJob *__restrict__ 
 	unsigned long long nev = 0;
 	uint32_t chunk_next = 0, chunk_end = 0, seen = 0;
 	int poll = 0;
+	// per-lane running fixed-point sum of the pixel accumulator run_t (rounds
+	// of one cell whose pixel is the last one's add to it)
+	unsigned long long run_fx = 0;
+	uint64_t run_t = ~0ull;
 	// GRAD: per-lane running sums of two components, run_g and run_g2
 	double run[GA_SLOTS] = {0, 0, 0, 0, 0, 0, 0, 0}, run2[GA_SLOTS] = {0, 0, 0, 0, 0, 0, 0, 0};
 	uint32_t run_g = ~0u, run_g2 = ~0u;
@@ -577,9 +592,15 @@ __global__ __launch_bounds__(64) void refine32_kernel(const PkJob *__restrict__ 
 						}
 					}
 				} else if (ntask == 1) {
-					// exact per-cell sum; lane 0 adds it to the pixel accumulator
-					const unsigned long long v = wave_sum_u64(fxv);
-					if (lane == 0 && v) atomicAdd(&acc[target], v);
+					// one cell in all lanes: exact per-lane sums, added to the
+					// pixel accumulator (lane 0) when the pixel changes
+					const uint64_t tu =
+					    (uint64_t)wave_read((uint32_t)(target >> 32), 0) << 32 | wave_read((uint32_t)target, 0);
+					if (tu != run_t) {
+						r32_flush_fx(run_fx, run_t, acc, lane);
+						run_t = tu;
+					}
+					run_fx += fxv;
 				} else {
 					// several cells: exact integer sums per cell in LDS, then one
 					// global atomic per cell (lane t adds task t's sum)
@@ -630,6 +651,8 @@ __global__ __launch_bounds__(64) void refine32_kernel(const PkJob *__restrict__ 
 	if (GRAD) {
 		r32_flush_run(run, run_g, gacc, lane);
 		r32_flush_run(run2, run_g2, gacc, lane);
+	} else {
+		r32_flush_fx(run_fx, run_t, acc, lane);
 	}
 	if (!LAST && handover) {
 		// everything this wavefront holds goes to the next pass
