@@ -221,7 +221,8 @@ int pk_convolver_padding(int type, uint32_t src_w, uint32_t src_h, uint32_t krn_
  * per-component set-up (bn, normalisation integrals, adjusted subsampling
  * constants) computed in FP64 on the device.  Supported per model: radial,
  * sky, psf and null profiles, BRUTE/BRUTE_OLD/FFT convolution, cropped or
- * not, calculation masks, finesampling.  PK_BATCH_GRAD batches take sersic,
+ * not, calculation masks, finesampling, dry runs (a zero image of the
+ * drawing dimensions).  PK_BATCH_GRAD batches take sersic,
  * sky and null profiles, no masks and the BRUTE convolvers.  Output images
  * are FP32 in a PK_FP32 context and FP64 in a PK_FP64 one. */
 typedef struct pk_batch pk_batch;

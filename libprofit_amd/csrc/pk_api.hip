@@ -1878,7 +1878,47 @@ int pk_batch_create(pk_ctx *ctx, const pk_model *models, uint32_t n_models, uint
 		Analysis a;
 		int st = analyze(m, a);
 		if (st) return fail(set_err(st, "model %u: %s", mi, g_err.c_str()));
-		if (m->dry_run) return fail(set_err(PK_EINVAL, "model %u: dry_run is not supported in a batch", mi));
+		if (m->dry_run) {
+			// src/model.cpp:263-267: a zero image of the drawing dimensions; the
+			// pixel pass writes the zeros (no components), the composition
+			// returns them
+			for (uint32_t c = 0; c < m->n_components; c++) {
+				b->comp_kind.push_back(m->components[c].kind);
+				b->comp_index.push_back(~0u);
+			}
+			PkJob jb{};
+			jb.dw = a.dw;
+			jb.dh = a.dh;
+			jb.comp_off = (uint32_t)b->setup.size();
+			jb.img_off = img_off;
+			jb.mask_off = NONE;
+			jb.sx = m->scale_x / m->finesampling;
+			jb.sy = m->scale_y / m->finesampling;
+			jb.tile_off = b->ntiles;
+			jb.tiles_x = cdiv(a.dw, 64);
+			b->ntiles += jb.tiles_x * cdiv(a.dh, PX_ROWS);
+			jb.has_nc = 1;
+			jb.psf_off = psf_all.size();
+			jb.conv_off = img_off;
+			jb.cw = a.dw;
+			jb.ch = a.dh;
+			jb.crw = a.dw;
+			jb.crh = a.dh;
+			jb.fs = 1;
+			jb.ow = a.dw;
+			jb.oh = a.dh;
+			jb.out_off = out_off;
+			jb.omask_off = NONE;
+			jb.omask_f = 1;
+			jb.fx_scale = 1.0;
+			b->jobs.push_back(jb);
+			b->is_fft.push_back(0);
+			b->origin.push_back(0);
+			b->origin.push_back(0);
+			img_off += (uint64_t)a.dw * a.dh;
+			out_off += (uint64_t)a.dw * a.dh;
+			continue;
+		}
 		if (fp32 && check_fp32_limits(m)) return fail(set_err(PK_EINVAL, "model %u: %s", mi, g_err.c_str()));
 		const bool fft = a.conv_required && m->convolver == PK_CONV_FFT;
 		if (fft && (flags & PK_BATCH_GRAD))

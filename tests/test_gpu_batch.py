@@ -407,3 +407,35 @@ def test_batch_fft_convolver(ctx64, ctx32, oracle, prec):
             else:
                 assert pix <= TOL_FP64, (i, pix)
     b.close()
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_batch_dry_run(ctx64, ctx32, oracle, prec):
+    """A dry-run model in a batch returns zeros of the drawing dimensions
+    (Model::evaluate, src/model.cpp:263-267), as a single evaluation does, and
+    leaves the other models' images alone."""
+    print("CASE batch dry run", prec, flush=True)
+    from libprofit_amd.model import evaluate_spec
+    psf = workloads.moffat_psf_image(9, 2.5, 3.0)
+    sers = abi.make_component("sersic", xcen=20.2, ycen=19.7, mag=15.0, re=4.0, nser=2.0, axrat=0.7, ang=30.0,
+                              convolve=True)
+    specs = [abi.ModelSpec(40, 40, [sers], psf=psf),
+             abi.ModelSpec(40, 36, [sers], psf=psf, dry_run=True),
+             abi.ModelSpec(30, 30, [abi.make_component("sky", bg=0.5)], dry_run=True, finesampling=2),
+             abi.ModelSpec(40, 40, [sers], psf=psf)]
+    ctx = ctx64 if prec == "fp64" else ctx32
+    tol = TOL_FP64 if prec == "fp64" else TOL_FP32
+    b = Batch(ctx, specs, graph=True)
+    b.evaluate()
+    b.sync()
+    for i, sp in enumerate(specs):
+        got = b.image(i)
+        if sp.m.dry_run:
+            single, _, _ = evaluate_spec(ctx, sp)
+            assert got.shape == single.shape, (i, got.shape, single.shape)
+            assert not np.any(got), i
+        else:
+            ref, _ = oracle.evaluate(sp)
+            pix, flux = rel_errors(got, ref)
+            assert pix <= tol and flux <= tol, (i, pix, flux)
+    b.close()
