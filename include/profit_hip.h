@@ -223,7 +223,7 @@ int pk_convolver_padding(int type, uint32_t src_w, uint32_t src_h, uint32_t krn_
  * sky, psf and null profiles, BRUTE/BRUTE_OLD/FFT convolution, cropped or
  * not, calculation masks, finesampling, dry runs (a zero image of the
  * drawing dimensions).  PK_BATCH_GRAD batches take sersic,
- * sky and null profiles, no masks and the BRUTE convolvers.  Output images
+ * sky and null profiles and the BRUTE convolvers.  Output images
  * are FP32 in a PK_FP32 context and FP64 in a PK_FP64 one. */
 typedef struct pk_batch pk_batch;
 
@@ -319,7 +319,10 @@ int pk_clear_cache(void);
  * rscale_switch, rscale_max and acc) is that of the evaluation at the given
  * parameters and is held fixed: the gradient is that of the piecewise-smooth
  * image with this pattern.  Gradients are available for sersic and sky
- * profiles (null profiles: zeros) with odd PSF dimensions.
+ * profiles (null profiles: zeros) with odd PSF dimensions.  With a
+ * calculation mask, the pixels it leaves out (and the returned pixels the
+ * adjusted mask zeroes) carry no gradient; chi2 still counts them, with
+ * model 0 (give them a large sigma to leave them out of the fit).
  * Gradient slots per component (PK_GRAD_SLOTS doubles, unused slots 0):
  *   sersic: xcen, ycen, mag, re, nser, ang (degrees), axrat, box
  *   sky:    bg */

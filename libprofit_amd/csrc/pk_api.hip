@@ -1628,7 +1628,8 @@ int grad_launch_t(pk_batch *b)
 	                   dim3(256), 0, s, gcnt, (const PkCounters *)b->d<PkCounters>(b->o_cnt), gacc, nacc, chi2, nj);
 	const uint32_t *tj_coarse = b->d<uint32_t>(b->o_tj);
 	hipLaunchKernelGGL(resid_kernel<T>, dim3(b->ntiles), dim3(256), 0, s, djobs, nj, (const T *)b->d<T>(b->o_out),
-	                   (const T *)b->d<T>(b->o_data), (const T *)b->d<T>(b->o_ivar), Gd, chi2, tj_coarse);
+	                   (const T *)b->d<T>(b->o_data), (const T *)b->d<T>(b->o_ivar), Gd, chi2, tj_coarse,
+	                   (const uint8_t *)b->d<uint8_t>(b->o_mask));
 	PK_HIP(hipGetLastError());
 	if (b->any_conv)
 		PK_TRY(launch_conv<T>(s, b->d<PkJob>(b->o_jobs_adj), nj, b->natiles, Gd, b->d<T>(b->o_psfF),
@@ -1928,8 +1929,6 @@ int pk_batch_create(pk_ctx *ctx, const pk_model *models, uint32_t n_models, uint
 		std::vector<uint8_t> dmask;
 		bool use_mask = false, adjusted = false;
 		if (has_mask(m)) {
-			if (flags & PK_BATCH_GRAD)
-				return fail(set_err(PK_EINVAL, "model %u: gradients do not support masks", mi));
 			const int ms = calc_mask(m, a, dmask, adjusted);
 			if (ms) return fail(set_err(ms, "model %u: %s", mi, g_err.c_str()));
 			use_mask = true;
@@ -2169,6 +2168,7 @@ int pk_batch_create(pk_ctx *ctx, const pk_model *models, uint32_t n_models, uint
 			PkJob &jb = adj[jn];
 			jb.ctile_off = b->natiles;
 			jb.ctiles_x = 0;
+			jb.mask_off = NONE;  // the adjoint image is zero at masked pixels already
 			if (!jb.has_conv) continue;
 			jb.cw = jb.dw;
 			jb.ch = jb.dh;

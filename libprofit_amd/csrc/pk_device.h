@@ -904,7 +904,8 @@ template <typename TI>
 __global__ __launch_bounds__(256) void resid_kernel(const PkJob *__restrict__ jobs, uint32_t njobs,
                                                     const TI *__restrict__ out_all, const TI *__restrict__ data,
                                                     const TI *__restrict__ ivar, TI *__restrict__ G,
-                                                    double *__restrict__ chi2, const uint32_t *__restrict__ tile_job)
+                                                    double *__restrict__ chi2, const uint32_t *__restrict__ tile_job,
+                                                    const uint8_t *__restrict__ masks)
 {
 	const uint32_t jn = find_job(jobs, njobs, blockIdx.x, 0, tile_job);
 	const PkJob &jb = jobs[jn];
@@ -926,6 +927,14 @@ __global__ __launch_bounds__(256) void resid_kernel(const PkJob *__restrict__ jo
 			const double iv = (double)ivar[p];
 			g = -2 * rr * iv;
 			if ((uint32_t)fx % jb.fs == 0 && (uint32_t)fy % jb.fs == 0) c2[0] += rr * rr * iv;
+			// a returned pixel the final mask zeroes, or a drawing pixel the
+			// calculation mask leaves out (neither its profiles nor its
+			// convolved value reach the image), carries no gradient
+			if (jb.omask_off != NONE) {
+				const uint32_t mi = oi / jb.omask_f, mj = oj / jb.omask_f;
+				if (mi < jb.omask_w && mj < jb.omask_h && !masks[jb.omask_off + mi + (size_t)mj * jb.omask_w]) g = 0;
+			}
+			if (jb.mask_off != NONE && !masks[jb.mask_off + i + (size_t)j * jb.dw]) g = 0;
 		}
 		G[jb.img_off + i + (size_t)j * jb.dw] = (TI)g;
 	}

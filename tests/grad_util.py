@@ -12,8 +12,9 @@ class FitModel:
     """A model whose components are given as (kind, params) with the
     parameters the gradient covers first (abi.GRAD_PARAMS order)."""
 
-    def __init__(self, size, comps, psf, convolver="brute", magzero=0.0):
+    def __init__(self, size, comps, psf, convolver="brute", magzero=0.0, mask=None):
         self.size = size
+        self.mask = mask
         self.magzero = magzero
         self.kinds = [k for k, _ in comps]
         self.base = [dict(p) for _, p in comps]
@@ -39,7 +40,7 @@ class FitModel:
             extra = {"convolve": True} if k == "sersic" and self.psf is not None else {}
             comps.append(abi.make_component(k, **extra, **p))
         return abi.ModelSpec(self.size, self.size, comps, psf=self.psf, convolver=self.convolver,
-                             magzero=self.magzero)
+                             magzero=self.magzero, mask=self.mask)
 
     def flat_grad(self, grad):
         """Gradient slots (n_components x SLOTS) -> vector in theta order."""

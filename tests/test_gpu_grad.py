@@ -102,6 +102,28 @@ def test_c3_gradient_matches_frozen_fd(ctx64, ctx32, oracle, prec):
     check_close(g, g_fd, prec, fm.names(), fd_err, g_rel)
 
 
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_masked_fit_gradient_matches_frozen_fd(ctx64, ctx32, oracle, prec):
+    """A calculation mask (adjusted as Model::evaluate does: expanded by the
+    PSF, applied again to the returned image): masked pixels carry no
+    gradient, the chi^2 still counts them (model 0 there)."""
+    announce("masked fit grad %s" % prec)
+    fm = small_fit()
+    mask = np.ones((fm.size, fm.size), np.uint8)
+    mask[:, 24:] = 0          # a cut through the galaxy
+    mask[5:12, 3:9] = 0       # and a hole in the outskirts
+    fm.mask = mask
+    truth = FitModel(fm.size, [(k, dict(p)) for k, p in zip(fm.kinds, fm.base)], fm.psf, mask=mask)
+    truth.base[0]["mag"] = 14.9
+    truth.base[1]["re"] = 7.4
+    data, sigma = make_data(oracle, truth)
+    g_fd, fd_err, g_rel, c0 = oracle_fd_grad(oracle, fm, fm.theta(), data, sigma, with_error=True)
+    ctx = ctx64 if prec == "fp64" else ctx32
+    chi2, (g,) = device_chi2_grad(ctx, [fm], [data], [sigma])
+    assert chi2[0] == pytest.approx(c0, rel=1e-12 if prec == "fp64" else 1e-5)
+    check_close(g, g_fd, prec, fm.names(), fd_err, g_rel)
+
+
 def test_batch_of_fits(ctx32, oracle):
     """Several independent fits of different sizes and PSFs in one batch."""
     announce("batch of fits")
