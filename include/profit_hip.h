@@ -319,7 +319,7 @@ int pk_clear_cache(void);
  * rscale_switch, rscale_max and acc) is that of the evaluation at the given
  * parameters and is held fixed: the gradient is that of the piecewise-smooth
  * image with this pattern.  Gradients are available for sersic and sky
- * profiles (null profiles: zeros) with odd PSF dimensions.  With a
+ * profiles (null profiles: zeros), any PSF dimensions.  With a
  * calculation mask, the pixels it leaves out (and the returned pixels the
  * adjusted mask zeroes) carry no gradient; chi2 still counts them, with
  * model 0 (give them a large sigma to leave them out of the fit).

@@ -1934,10 +1934,6 @@ int pk_batch_create(pk_ctx *ctx, const pk_model *models, uint32_t n_models, uint
 			use_mask = true;
 		}
 		if (flags & PK_BATCH_GRAD) {
-			// the adjoint of the PSF convolution is the convolution with the
-			// flipped PSF on the same grid only for odd PSF dimensions
-			if (a.conv_required && (m->psf_width % 2 == 0 || m->psf_height % 2 == 0))
-				return fail(set_err(PK_EINVAL, "model %u: gradients need odd PSF dimensions", mi));
 			for (uint32_t c = 0; c < m->n_components; c++) {
 				const pk_component &pc = m->components[c];
 				if (pc.kind != PK_PROFILE_SERSIC && pc.kind != PK_PROFILE_SKY && pc.kind != PK_PROFILE_NULL)
@@ -2172,8 +2168,11 @@ int pk_batch_create(pk_ctx *ctx, const pk_model *models, uint32_t n_models, uint
 			if (!jb.has_conv) continue;
 			jb.cw = jb.dw;
 			jb.ch = jb.dh;
-			jb.csx = 0;
-			jb.csy = 0;
+			// forward: out(X) = sum_k src(X + k - h) K(kw-1-k), h = kw / 2, so
+			// G_src(x) = sum_k G(x + k - (kw-1-h)) K(k): the unflipped PSF with
+			// the source shifted by 2h - kw + 1 (0 for odd, 1 for even sides)
+			jb.csx = 2 * (int32_t)(jb.kw / 2) - (int32_t)jb.kw + 1;
+			jb.csy = 2 * (int32_t)(jb.kh / 2) - (int32_t)jb.kh + 1;
 			jb.ctiles_x = cdiv(jb.dw, CT_W);
 			const uint32_t n = jb.ctiles_x * cdiv(jb.dh, CT_H);
 			tja.insert(tja.end(), n, jn);

@@ -124,6 +124,29 @@ def test_masked_fit_gradient_matches_frozen_fd(ctx64, ctx32, oracle, prec):
     check_close(g, g_fd, prec, fm.names(), fd_err, g_rel)
 
 
+@pytest.mark.parametrize("shape", [(8, 8), (10, 7)])
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_even_psf_gradient_matches_frozen_fd(ctx64, ctx32, oracle, prec, shape):
+    """Even (and non-square) PSF sides: the adjoint correlation is shifted by
+    one pixel on an even axis (the BRUTE convolution centres a kw-wide PSF at
+    kw / 2)."""
+    announce("even psf grad %s %s" % (prec, shape))
+    fm = small_fit()
+    w, h = shape
+    yy, xx = np.mgrid[0:h, 0:w]
+    psf = 1.0 / (1.0 + ((xx - (w - 1) / 2.0) ** 2 + (yy - (h - 1) / 2.0) ** 2) / 4.0) ** 2.5
+    fm.psf = psf / psf.sum()
+    truth = FitModel(fm.size, [(k, dict(p)) for k, p in zip(fm.kinds, fm.base)], fm.psf)
+    truth.base[0]["xcen"] = 20.6
+    truth.base[1]["re"] = 7.4
+    data, sigma = make_data(oracle, truth)
+    g_fd, fd_err, g_rel, c0 = oracle_fd_grad(oracle, fm, fm.theta(), data, sigma, with_error=True)
+    ctx = ctx64 if prec == "fp64" else ctx32
+    chi2, (g,) = device_chi2_grad(ctx, [fm], [data], [sigma])
+    assert chi2[0] == pytest.approx(c0, rel=1e-12 if prec == "fp64" else 1e-5)
+    check_close(g, g_fd, prec, fm.names(), fd_err, g_rel)
+
+
 def test_batch_of_fits(ctx32, oracle):
     """Several independent fits of different sizes and PSFs in one batch."""
     announce("batch of fits")
