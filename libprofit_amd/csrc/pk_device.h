@@ -1023,6 +1023,13 @@ __global__ __launch_bounds__(256) void coarse_grad_kernel(const PkJob *__restric
 				}
 			}
 		}
+		if (cp.kind == PK_PROFILE_SKY) {
+			// the sky's one slot
+			block_sum_256<1>(&acc[GA_W]);
+			if (threadIdx.x == 0 && acc[GA_W] != 0)
+				atomicAdd(&gacc[(size_t)(jb.comp_off + c) * GA_SLOTS + GA_W], acc[GA_W]);
+			continue;
+		}
 		block_sum_256<GA_SLOTS>(acc);
 		if (threadIdx.x == 0)
 #pragma unroll
