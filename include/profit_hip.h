@@ -343,6 +343,22 @@ int pk_batch_evaluate_grad(pk_batch *b);
  * (components in the order given at creation). */
 int pk_batch_get_chi2_grad(pk_batch *b, double *chi2, double *grad);
 
+/* ---- one-shot forms --------------------------------------------------------
+ * No counterpart in libprofit (a caller loops over Model::evaluate).  Each
+ * call creates a batch, runs it once and destroys it (device allocation per
+ * call); repeated calls on the same models should keep a pk_batch instead. */
+/* Evaluates every model; image i (FP64, returned dims) goes to outs[i], which
+ * holds out_capacity[i] doubles; results[i] (may be NULL) gets its dims and
+ * offset. */
+int pk_evaluate_batch(pk_ctx *ctx, const pk_model *models, uint32_t n_models, double *const *outs,
+                      const size_t *out_capacity, pk_result *results);
+/* chi2[n_models] and grad[n_components * PK_GRAD_SLOTS] of the models against
+ * data and sigma (n_values doubles each, packed like the batch's returned
+ * images; sigma NULL = 1), as pk_batch_set_data + pk_batch_evaluate_grad +
+ * pk_batch_get_chi2_grad. */
+int pk_chi2_grad(pk_ctx *ctx, const pk_model *models, uint32_t n_models, const double *data, const double *sigma,
+                 size_t n_values, double *chi2, double *grad);
+
 /* ---- FP64 special functions of the math back-end ------------------------
  * include/profit/utils.h:52-113 (GSL/Rmath in libprofit, own implementations
  * here; the same code runs on the device during per-component set-up). */

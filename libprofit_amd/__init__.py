@@ -14,7 +14,7 @@ def __getattr__(name):
     # `import libprofit_amd` works (and fails loudly on use) without the
     # library.
     if name in ("Model", "Profile", "Context", "Convolver", "create_convolver", "default_context", "evaluate_spec",
-                "Batch", "init", "finish", "clear_cache"):
+                "Batch", "init", "finish", "clear_cache", "evaluate_batch", "chi2_grad_once"):
         from . import model
         return getattr(model, name)
     raise AttributeError(name)

@@ -2592,4 +2592,49 @@ int pk_batch_image_origin(const pk_batch *b, uint32_t i, int32_t *x, int32_t *y)
 	return PK_OK;
 }
 
+int pk_evaluate_batch(pk_ctx *ctx, const pk_model *models, uint32_t n_models, double *const *outs,
+                      const size_t *out_capacity, pk_result *results)
+{
+	if (!ctx || !models || !outs || !out_capacity) return set_err(PK_EINVAL, "NULL argument");
+	pk_batch *b = nullptr;
+	PK_TRY(pk_batch_create(ctx, models, n_models, 0, &b));
+	int st = pk_batch_evaluate(b);
+	pk_batch_info info{};
+	if (st == PK_OK) st = pk_batch_sync(b, &info);
+	for (uint32_t i = 0; st == PK_OK && i < n_models; i++) {
+		if (!outs[i]) {
+			st = set_err(PK_EINVAL, "output buffer %u is NULL", i);
+			break;
+		}
+		st = pk_batch_get_image(b, i, outs[i], out_capacity[i]);
+		if (st == PK_OK && results) {
+			pk_result &r = results[i];
+			std::memset(&r, 0, sizeof(r));
+			r.width = b->jobs[i].ow;
+			r.height = b->jobs[i].oh;
+			r.offset_x = b->origin[2 * (size_t)i];
+			r.offset_y = b->origin[2 * (size_t)i + 1];
+		}
+	}
+	const std::string err = g_err;
+	pk_batch_destroy(b);
+	if (st != PK_OK) g_err = err;
+	return st;
+}
+
+int pk_chi2_grad(pk_ctx *ctx, const pk_model *models, uint32_t n_models, const double *data, const double *sigma,
+                 size_t n_values, double *chi2, double *grad)
+{
+	if (!ctx || !models || !data) return set_err(PK_EINVAL, "NULL argument");
+	pk_batch *b = nullptr;
+	PK_TRY(pk_batch_create(ctx, models, n_models, PK_BATCH_GRAD, &b));
+	int st = pk_batch_set_data(b, data, sigma, n_values);
+	if (st == PK_OK) st = pk_batch_evaluate_grad(b);
+	if (st == PK_OK) st = pk_batch_get_chi2_grad(b, chi2, grad);
+	const std::string err = g_err;
+	pk_batch_destroy(b);
+	if (st != PK_OK) g_err = err;
+	return st;
+}
+
 }  // extern "C"

@@ -303,6 +303,49 @@ def evaluate_spec(ctx, spec):
     return img, (res.offset_x, res.offset_y), res
 
 
+def evaluate_batch(ctx, specs):
+    """pk_evaluate_batch: every model of `specs` evaluated once (a batch created
+    and destroyed inside the call) -> [(image, offset)]."""
+    L = load()
+    specs = list(specs)
+    arr = (abi.Model * len(specs))(*[s.m for s in specs])
+    outs, caps = [], []
+    for s in specs:
+        m = s.m
+        fs = max(1, m.finesampling)
+        dw = m.width * fs + 2 * (m.psf_width or 0)
+        dh = m.height * fs + 2 * (m.psf_height or 0)
+        mx = max(dw, dh, m.psf_width or 0, m.psf_height or 0)
+        outs.append(np.zeros(max(4 * mx * mx, dw * dh, 1)))
+        caps.append(outs[-1].size)
+    ptrs = (C.c_void_p * len(specs))(*[o.ctypes.data for o in outs])
+    capa = (C.c_size_t * len(specs))(*caps)
+    res = (abi.Result * len(specs))()
+    check(L.pk_evaluate_batch(ctx.handle, arr, len(specs), ptrs, capa, res))
+    return [(o[: r.width * r.height].reshape(r.height, r.width).copy(), (r.offset_x, r.offset_y))
+            for o, r in zip(outs, res)]
+
+
+def chi2_grad_once(ctx, specs, images, sigmas=None):
+    """pk_chi2_grad: chi^2 of every model against its data and the gradient of
+    every component (a batch created and destroyed inside the call) ->
+    (chi2[n_models], grad[n_components, GRAD_SLOTS])."""
+    L = load()
+    specs = list(specs)
+    arr = (abi.Model * len(specs))(*[s.m for s in specs])
+    data = np.ascontiguousarray(np.concatenate([np.asarray(d, dtype=np.float64).ravel() for d in images]))
+    sig = None
+    if sigmas is not None:
+        sig = np.ascontiguousarray(np.concatenate(
+            [np.broadcast_to(np.asarray(sg, dtype=np.float64), np.shape(d)).ravel() for d, sg in zip(images, sigmas)]))
+    ncomp = sum(int(s.m.n_components) for s in specs)
+    chi2 = np.zeros(len(specs))
+    grad = np.zeros((max(1, ncomp), abi.GRAD_SLOTS))
+    check(L.pk_chi2_grad(ctx.handle, arr, len(specs), data.ctypes.data, None if sig is None else sig.ctypes.data,
+                         data.size, chi2.ctypes.data, grad.ctypes.data))
+    return chi2, grad[:ncomp]
+
+
 class Batch:
     """Device-resident batch of models (pk_batch_* of include/profit_hip.h):
     all images produced by one launch sequence, optionally a hipGraph; with

@@ -439,3 +439,41 @@ def test_batch_dry_run(ctx64, ctx32, oracle, prec):
             pix, flux = rel_errors(got, ref)
             assert pix <= tol and flux <= tol, (i, pix, flux)
     b.close()
+
+
+def test_one_shot_entry_points(ctx32, ctx64, oracle):
+    """pk_evaluate_batch and pk_chi2_grad give what a pk_batch of the same
+    models gives (images bit-identical; chi^2 and gradient to the FP64 sums'
+    rounding), including an uncropped FFT model's offset."""
+    print("CASE one-shot entry points", flush=True)
+    from libprofit_amd.model import chi2_grad_once, evaluate_batch
+    psf = workloads.moffat_psf_image(9, 2.5, 3.0)
+    specs = small_batch(5, 17) + [abi.ModelSpec(40, 40, [abi.make_component(
+        "sersic", xcen=20.2, ycen=19.7, mag=15.0, re=4.0, nser=2.0, convolve=True)], psf=psf, convolver="fft",
+        crop=False)]
+    for ctx in (ctx64, ctx32):
+        b = Batch(ctx, specs, graph=False)
+        b.evaluate()
+        b.sync()
+        got = evaluate_batch(ctx, specs)
+        for i, (img, off) in enumerate(got):
+            assert np.array_equal(img, b.image(i)), i
+            assert off == b.origin(i), i
+        b.close()
+    # gradients: bulge + disk + sky fits
+    fits, _true = workloads.fit_batch(3, seed=2)
+    rng = np.random.Generator(np.random.PCG64(4))
+    datas = []
+    for sp in fits:
+        ref, _ = oracle.evaluate(sp)
+        datas.append(ref * (1 + 0.01 * rng.standard_normal(ref.shape)))
+    sig = [np.full(d.shape, 0.01 * np.max(np.abs(d))) for d in datas]
+    gb = Batch(ctx32, fits, graph=False, grad=True)
+    gb.set_data(datas, sig)
+    gb.evaluate_grad()
+    c_ref, g_ref = gb.chi2_grad()
+    gb.close()
+    c1, g1 = chi2_grad_once(ctx32, fits, datas, sig)
+    assert np.allclose(c1, c_ref, rtol=1e-12, atol=0)
+    scale = np.max(np.abs(g_ref), axis=1, keepdims=True)
+    assert np.all(np.abs(g1 - g_ref) <= 1e-11 * np.maximum(scale, 1e-300))
