@@ -200,6 +200,10 @@ int pk_component_constants(const pk_model *m, uint32_t index, pk_constants *out)
  * and statistics. */
 int pk_evaluate(pk_ctx *ctx, const pk_model *m, double *out, size_t out_capacity,
                 pk_result *res);
+/* pk_evaluate with the returned image (FP64) written to device memory of the
+ * context's device (out_dev, out_capacity doubles); returns when it is
+ * there. */
+int pk_evaluate_device(pk_ctx *ctx, const pk_model *m, double *out_dev, size_t out_capacity, pk_result *res);
 
 /* Convolver::convolve(src, krn, mask, crop, offset_out)
  * (include/profit/convolve.h:98-99, src/convolve.cpp:55-79) for a convolver of

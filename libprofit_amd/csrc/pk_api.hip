@@ -937,7 +937,7 @@ int launch_profiles(hipStream_t s, bool fp32, const PkJob *djobs, uint32_t njobs
 // Core of Model::evaluate on the device for one model.
 template <typename TI>
 int evaluate_impl(pk_ctx *ctx, const pk_model *m, const Analysis &a, double *out, size_t out_capacity,
-                  pk_result *res)
+                  pk_result *res, bool device_out = false)
 {
 	const bool fp32 = ctx->precision == PK_FP32;
 	const uint32_t fs = m->finesampling;
@@ -1131,14 +1131,17 @@ int evaluate_impl(pk_ctx *ctx, const pk_model *m, const Analysis &a, double *out
 	// download result + counters
 	const size_t out_bytes = sizeof(double) * (size_t)ow * oh;
 	const size_t ho_cnt = (out_bytes + 255) & ~(size_t)255;
-	PK_HIP(hipMemcpyAsync(ctx->hmem, dptr<void>(ctx, o_out), out_bytes, hipMemcpyDeviceToHost, s));
+	if (device_out)
+		PK_HIP(hipMemcpyAsync(out, dptr<void>(ctx, o_out), out_bytes, hipMemcpyDeviceToDevice, s));
+	else
+		PK_HIP(hipMemcpyAsync(ctx->hmem, dptr<void>(ctx, o_out), out_bytes, hipMemcpyDeviceToHost, s));
 	PK_HIP(hipMemcpyAsync(hptr<void>(ctx, ho_cnt), dcnt, sizeof(PkCounters), hipMemcpyDeviceToHost, s));
 	PK_HIP(hipStreamSynchronize(s));
 	const PkCounters cnt = *hptr<PkCounters>(ctx, ho_cnt);
 	if (cnt.overflow)
 		return set_err(PK_ERANGE, "sub-pixel refinement work queue capacity exceeded");
 	if (cnt.n_bad) return set_err(PK_ERANGE, "sub-pixel refinement produced %u non-finite values", cnt.n_bad);
-	std::memcpy(out, ctx->hmem, out_bytes);
+	if (!device_out) std::memcpy(out, ctx->hmem, out_bytes);
 	if (res) {
 		res->width = ow;
 		res->height = oh;
@@ -1242,6 +1245,35 @@ int pk_evaluate(pk_ctx *ctx, const pk_model *m, double *out, size_t out_capacity
 		return evaluate_impl<float>(ctx, m, a, out, out_capacity, res);
 	}
 	return evaluate_impl<double>(ctx, m, a, out, out_capacity, res);
+}
+
+int pk_evaluate_device(pk_ctx *ctx, const pk_model *m, double *out_dev, size_t out_capacity, pk_result *res)
+{
+	if (!ctx) return set_err(PK_EINVAL, "context is NULL");
+	if (!m) return set_err(PK_EINVAL, "model is NULL");
+	if (!out_dev) return set_err(PK_EINVAL, "output buffer is NULL");
+	std::lock_guard<std::mutex> lock(ctx->mu);
+	PK_HIP(hipSetDevice(ctx->device));
+	Analysis a;
+	PK_TRY(analyze(m, a));
+	if (m->dry_run) {
+		// src/model.cpp:263-267
+		const size_t n = (size_t)a.dw * a.dh;
+		if (n > out_capacity) return set_err(PK_ERANGE, "output buffer holds %zu doubles, %zu needed", out_capacity, n);
+		PK_HIP(hipMemsetAsync(out_dev, 0, n * sizeof(double), ctx->stream));
+		PK_HIP(hipStreamSynchronize(ctx->stream));
+		if (res) {
+			std::memset(res, 0, sizeof(*res));
+			res->width = a.dw;
+			res->height = a.dh;
+		}
+		return PK_OK;
+	}
+	if (ctx->precision == PK_FP32) {
+		PK_TRY(check_fp32_limits(m));
+		return evaluate_impl<float>(ctx, m, a, out_dev, out_capacity, res, true);
+	}
+	return evaluate_impl<double>(ctx, m, a, out_dev, out_capacity, res, true);
 }
 
 int pk_convolver_padding(int type, uint32_t src_w, uint32_t src_h, uint32_t krn_w, uint32_t krn_h, int32_t *lo_x,

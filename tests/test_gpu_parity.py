@@ -228,3 +228,39 @@ def test_c2_fft_model(ctx64, ctx32, oracle):
         pix, flux = rel_errors(got, ref)
         print("c2 fft", ctx.precision, pix, flux)
         assert pix <= tol and flux <= tol
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_evaluate_into_device_memory(ctx64, ctx32, oracle, prec):
+    """pk_evaluate_device writes the same image as pk_evaluate, into device
+    memory (from hipMalloc of the HIP runtime the library uses)."""
+    announce("evaluate_device %s" % prec)
+    import ctypes as C
+    from libprofit_amd._lib import check, load
+    ctx = ctx64 if prec == "fp64" else ctx32
+    psf, _ = oracle.evaluate(moffat_psf(15, 3.0, 2.5))
+    specs = [abi.ModelSpec(96, 80, [sersic(xcen=47.1, ycen=41.4, re=7, nser=2, axrat=0.6, ang=10, convolve=True),
+                                    comp("sky", bg=1e-9)], psf=psf),
+             abi.ModelSpec(60, 50, [sersic(xcen=30.1, ycen=24.4, re=5, nser=1.5, convolve=True)], psf=psf,
+                           convolver="fft", crop=False),
+             abi.ModelSpec(30, 20, [sersic(xcen=15, ycen=10)], dry_run=True)]
+    L = load()
+    hip = C.CDLL("libamdhip64.so.7")  # the runtime the library is linked against (already loaded)
+    for sp in specs:
+        host, off, _ = evaluate_spec(ctx, sp)
+        cap = host.size + 7
+        dev = C.c_void_p()
+        assert hip.hipMalloc(C.byref(dev), C.c_size_t(8 * cap)) == 0
+        try:
+            fill = np.full(cap, -1.0)
+            assert hip.hipMemcpy(dev, fill.ctypes.data_as(C.c_void_p), C.c_size_t(8 * cap), 1) == 0  # H2D
+            res = abi.Result()
+            check(L.pk_evaluate_device(ctx.handle, sp.ref, dev, cap, C.byref(res)))
+            got = np.zeros(cap)
+            assert hip.hipMemcpy(got.ctypes.data_as(C.c_void_p), dev, C.c_size_t(8 * cap), 2) == 0  # D2H
+        finally:
+            hip.hipFree(dev)
+        assert (res.width, res.height) == (host.shape[1], host.shape[0])
+        assert (res.offset_x, res.offset_y) == off
+        assert np.array_equal(got[: host.size].reshape(host.shape), host)
+        assert np.all(got[host.size:] == -1.0)  # nothing written past the image
