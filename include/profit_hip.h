@@ -227,7 +227,9 @@ int pk_convolver_padding(int type, uint32_t src_w, uint32_t src_h, uint32_t krn_
  * sky, psf and null profiles, BRUTE/BRUTE_OLD/FFT convolution, cropped or
  * not, calculation masks, finesampling, dry runs (a zero image of the
  * drawing dimensions).  PK_BATCH_GRAD batches take sersic,
- * sky and null profiles and the BRUTE convolvers.  Output images
+ * sky and null profiles, the BRUTE convolvers, and the FFT convolver when it
+ * crops and wraps no rows (drawing height + PSF height - 1 <= 2 max(drawing
+ * width, PSF width)).  Output images
  * are FP32 in a PK_FP32 context and FP64 in a PK_FP64 one. */
 typedef struct pk_batch pk_batch;
 

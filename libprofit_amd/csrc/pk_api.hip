@@ -1454,6 +1454,7 @@ struct pk_batch {
 	std::vector<int32_t> comp_kind;
 	uint32_t ntiles = 0, nctiles = 0, notiles = 0;
 	uint32_t max_kw = 0, max_kh = 0;
+	uint32_t adj_kw = 0, adj_kh = 0;  // largest PSF of the adjoint correlations (FFT jobs included)
 	bool any_conv = false;
 	// device memory
 	void *dmem = nullptr;
@@ -1665,7 +1666,7 @@ int grad_launch_t(pk_batch *b)
 	PK_HIP(hipGetLastError());
 	if (b->any_conv)
 		PK_TRY(launch_conv<T>(s, b->d<PkJob>(b->o_jobs_adj), nj, b->natiles, Gd, b->d<T>(b->o_psfF),
-		                      b->d<uint8_t>(b->o_mask), Gc, b->max_kw, b->max_kh, b->d<uint32_t>(b->o_tja),
+		                      b->d<uint8_t>(b->o_mask), Gc, b->adj_kw, b->adj_kh, b->d<uint32_t>(b->o_tja),
 		                      b->tune.conv_tiles));
 	const bool fp32 = sizeof(T) == 4;
 	hipLaunchKernelGGL((coarse_grad_kernel<T, fp32>), dim3(b->ntiles), dim3(256), 0, s, djobs, nj,
@@ -1954,8 +1955,14 @@ int pk_batch_create(pk_ctx *ctx, const pk_model *models, uint32_t n_models, uint
 		}
 		if (fp32 && check_fp32_limits(m)) return fail(set_err(PK_EINVAL, "model %u: %s", mi, g_err.c_str()));
 		const bool fft = a.conv_required && m->convolver == PK_CONV_FFT;
-		if (fft && (flags & PK_BATCH_GRAD))
-			return fail(set_err(PK_EINVAL, "model %u: gradients need the BRUTE convolver", mi));
+		// the FFT convolver's cropped result without wrapped rows is the
+		// same-size linear convolution, whose adjoint is the direct correlation
+		if (fft && (flags & PK_BATCH_GRAD) &&
+		    !(m->crop && a.dh + m->psf_height - 1 <= 2 * std::max(a.dw, m->psf_width)))
+			return fail(set_err(PK_EINVAL,
+			                    "model %u: gradients with the FFT convolver need crop and a drawing height up to "
+			                    "twice its width",
+			                    mi));
 		// calculation mask in the drawing frame (and the user's mask for the
 		// returned image when it was adjusted), as Model::evaluate
 		std::vector<uint8_t> dmask;
@@ -2065,6 +2072,16 @@ int pk_batch_create(pk_ctx *ctx, const pk_model *models, uint32_t n_models, uint
 			jb.ctiles_x = 0;
 			jb.conv_off = (uint64_t)jb.cw * jb.ch;  // size for now; placed after the loop
 			b->any_conv = true;
+			if (flags & PK_BATCH_GRAD) {
+				// the adjoint correlates with the PSF directly
+				jb.psf_off = psf_all.size();
+				const size_t pn = (size_t)m->psf_width * m->psf_height;
+				double sum = 0;
+				for (size_t t = 0; t < pn; t++) sum += m->psf[t];
+				for (size_t t = 0; t < pn; t++) psf_all.push_back(sum > 0 ? m->psf[t] / sum : m->psf[t]);
+				b->adj_kw = std::max(b->adj_kw, m->psf_width);
+				b->adj_kh = std::max(b->adj_kh, m->psf_height);
+			}
 			pk_batch::FftJob fj{mi, model_psfd(mi, m), 0};
 			for (size_t t = 0; t < (size_t)m->psf_width * m->psf_height; t++)
 				fj.krn_max = std::max(fj.krn_max, std::fabs(psfd_all[fj.psf_off + t]));
@@ -2076,6 +2093,8 @@ int pk_batch_create(pk_ctx *ctx, const pk_model *models, uint32_t n_models, uint
 			for (size_t t = 0; t < pn; t++) psf_all.push_back(sum > 0 ? m->psf[t] / sum : m->psf[t]);
 			b->max_kw = std::max(b->max_kw, m->psf_width);
 			b->max_kh = std::max(b->max_kh, m->psf_height);
+			b->adj_kw = std::max(b->adj_kw, m->psf_width);
+			b->adj_kh = std::max(b->adj_kh, m->psf_height);
 			b->any_conv = true;
 			// only the pixels the crop keeps are convolved: the convolved frame
 			// is the crop region, its pixel (X, Y) the same-size convolution at
@@ -2197,6 +2216,8 @@ int pk_batch_create(pk_ctx *ctx, const pk_model *models, uint32_t n_models, uint
 			jb.ctile_off = b->natiles;
 			jb.ctiles_x = 0;
 			jb.mask_off = NONE;  // the adjoint image is zero at masked pixels already
+			jb.conv_off = jb.img_off;  // the correlation lands in the drawing-frame layout
+			jb.fuse_out = 0;
 			if (!jb.has_conv) continue;
 			jb.cw = jb.dw;
 			jb.ch = jb.dh;

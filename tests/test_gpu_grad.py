@@ -147,6 +147,26 @@ def test_even_psf_gradient_matches_frozen_fd(ctx64, ctx32, oracle, prec, shape):
     check_close(g, g_fd, prec, fm.names(), fd_err, g_rel)
 
 
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_fft_convolved_fit_gradient(ctx64, ctx32, oracle, prec):
+    """The FFT convolver's cropped result is the same-size linear convolution
+    (up to the FFT's rounding, relative to the peak), so its adjoint is the
+    direct correlation: against frozen-pattern differences of the oracle's
+    FFT model."""
+    announce("fft fit grad %s" % prec)
+    fm = small_fit()
+    fm.convolver = "fft"
+    truth = FitModel(fm.size, [(k, dict(p)) for k, p in zip(fm.kinds, fm.base)], fm.psf, convolver="fft")
+    truth.base[0]["mag"] = 14.9
+    truth.base[1]["re"] = 7.4
+    data, sigma = make_data(oracle, truth)
+    g_fd, fd_err, g_rel, c0 = oracle_fd_grad(oracle, fm, fm.theta(), data, sigma, with_error=True)
+    ctx = ctx64 if prec == "fp64" else ctx32
+    chi2, (g,) = device_chi2_grad(ctx, [fm], [data], [sigma])
+    assert chi2[0] == pytest.approx(c0, rel=1e-9 if prec == "fp64" else 1e-5)
+    check_close(g, g_fd, prec, fm.names(), fd_err, g_rel)
+
+
 def test_batch_of_fits(ctx32, oracle):
     """Several independent fits of different sizes and PSFs in one batch."""
     announce("batch of fits")
