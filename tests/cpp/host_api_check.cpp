@@ -206,7 +206,7 @@ static void gpu_checks(const std::string &dir)
 	CHECK(init());
 	Point off;
 	{
-		// config 1 (profit-cli defaults): one Sersic, 100x100, FP64
+		// profit-cli defaults: one Sersic, 100x100, FP64
 		Model m(100, 100);
 		auto s = m.add_profile("sersic");
 		for (const char *spec : {"xcen=50.3", "ycen=49.7", "mag=15", "re=10", "nser=4", "ang=30", "axrat=0.6", "box=0"})

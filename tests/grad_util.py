@@ -111,7 +111,7 @@ def oracle_fd_grad(oracle, fm, theta, data, sigma, rel=1e-4, with_error=False):
     return g_rel, c0
 
 
-def c3_model(size=256, sky=None):
+def bulge_disk256_model(size=256, sky=None):
     """The 256x256 differentiation workload at a perturbed starting point:
     (FitModel at the start, true parameter vector)."""
     from libprofit_amd import workloads

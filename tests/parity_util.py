@@ -40,18 +40,18 @@ def moffat_psf(size, fwhm, con):
                                            mag=0.0)])
 
 
-def c1a():
+def sersic100_adjusted():
     return abi.ModelSpec(100, 100, [sersic(xcen=50.3, ycen=49.7, mag=15, re=10, nser=4, ang=30, axrat=0.6, box=0)])
 
 
-def c1b():
+def sersic100_fixed():
     return abi.ModelSpec(100, 100, [sersic(xcen=50.3, ycen=49.7, mag=15, re=10, nser=4, ang=30, axrat=0.6, box=0,
                                            adjust=False, rscale_switch=1.1, resolution=9, max_recursions=2,
                                            acc=0.1, rscale_max=0)])
 
 
-def c2_params(seed):
-    """Config-2 draws (numpy PCG64, in the listed order)."""
+def galaxy400_params(seed):
+    """400x400 galaxy draws (numpy PCG64, in this order)."""
     rng = np.random.Generator(np.random.PCG64(seed))
     nser = rng.uniform(0.5, 8)
     re = rng.uniform(2, 50)

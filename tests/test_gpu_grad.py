@@ -8,7 +8,7 @@ import pytest
 
 from libprofit_amd import abi, workloads
 from libprofit_amd.model import Batch
-from grad_util import FitModel, c3_model, chi2_of, fd_steps, oracle_fd_grad
+from grad_util import FitModel, bulge_disk256_model, chi2_of, fd_steps, oracle_fd_grad
 
 pytestmark = pytest.mark.gpu
 
@@ -89,11 +89,11 @@ def test_small_fit_gradient_matches_frozen_fd(ctx64, ctx32, oracle, prec):
 
 
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
-def test_c3_gradient_matches_frozen_fd(ctx64, ctx32, oracle, prec):
+def test_bulge_disk256_gradient_matches_frozen_fd(ctx64, ctx32, oracle, prec):
     """256x256 bulge + disk with a 31x31 Moffat PSF, data = true model + 1 %
     noise, gradient over all 16 component parameters at a perturbed point."""
-    announce("c3 grad %s" % prec)
-    fm, truth = c3_model()
+    announce("bulge_disk256 grad %s" % prec)
+    fm, truth = bulge_disk256_model()
     data, sigma = make_data(oracle, truth)
     g_fd, fd_err, g_rel, c0 = oracle_fd_grad(oracle, fm, fm.theta(), data, sigma, with_error=True)
     ctx = ctx64 if prec == "fp64" else ctx32

@@ -13,7 +13,7 @@ import pytest
 from libprofit_amd import abi
 from libprofit_amd._lib import InvalidParameter
 from libprofit_amd.model import evaluate_spec
-from parity_util import TOL_FP32, TOL_FP64, c1a, c1b, c2_params, comp, moffat_psf, rel_errors, sersic
+from parity_util import TOL_FP32, TOL_FP64, sersic100_adjusted, sersic100_fixed, galaxy400_params, comp, moffat_psf, rel_errors, sersic
 
 pytestmark = pytest.mark.gpu
 
@@ -31,9 +31,9 @@ def run_both(spec, ctx, oracle):
     return got, ref, res
 
 
-def test_c1a_fp64(ctx64, oracle):
-    announce("c1a_fp64")
-    got, ref, res = run_both(c1a(), ctx64, oracle)
+def test_sersic100_adjusted_fp64(ctx64, oracle):
+    announce("sersic100 adjusted fp64")
+    got, ref, res = run_both(sersic100_adjusted(), ctx64, oracle)
     pix, flux = rel_errors(got, ref)
     nref, nsub = oracle.stats()
     assert res.n_refined_pixels == nref
@@ -42,15 +42,15 @@ def test_c1a_fp64(ctx64, oracle):
     assert flux <= TOL_FP64, flux
 
 
-def test_c1b_fp64(ctx64, oracle):
-    announce("c1b_fp64")
-    got, ref, _ = run_both(c1b(), ctx64, oracle)
+def test_sersic100_fixed_fp64(ctx64, oracle):
+    announce("sersic100 fixed fp64")
+    got, ref, _ = run_both(sersic100_fixed(), ctx64, oracle)
     pix, flux = rel_errors(got, ref)
     assert pix <= TOL_FP64, pix
 
 
-def test_c1_fp32(ctx32, oracle):
-    for name, spec in (("c1a", c1a()), ("c1b", c1b())):
+def test_sersic100_fp32(ctx32, oracle):
+    for name, spec in (("sersic100_adjusted", sersic100_adjusted()), ("sersic100_fixed", sersic100_fixed())):
         announce(name + "_fp32")
         got, ref, _ = run_both(spec, ctx32, oracle)
         pix, flux = rel_errors(got, ref)
@@ -155,17 +155,17 @@ def test_convolved_model_fp64(ctx64, oracle, conv):
 
 
 @pytest.mark.parametrize("seed", [0, 1])
-def test_c2_fp64_fp32(ctx64, ctx32, oracle, seed):
-    announce("c2 seed %d" % seed)
+def test_galaxy400_fp64_fp32(ctx64, ctx32, oracle, seed):
+    announce("galaxy400 seed %d" % seed)
     psf, _ = oracle.evaluate(moffat_psf(25, 3.0, 2.5))
-    p = c2_params(seed)
+    p = galaxy400_params(seed)
     spec = abi.ModelSpec(400, 400, [sersic(convolve=True, **p)], psf=psf)
     ref, _ = oracle.evaluate(spec)
     g64, _, _ = evaluate_spec(ctx64, spec)
     g32, _, _ = evaluate_spec(ctx32, spec)
     pix64, flux64 = rel_errors(g64, ref)
     pix32, flux32 = rel_errors(g32, ref)
-    print("c2 seed", seed, "fp64", pix64, flux64, "fp32", pix32, flux32)
+    print("galaxy400 seed", seed, "fp64", pix64, flux64, "fp32", pix32, flux32)
     assert pix64 <= TOL_FP64
     assert pix32 <= TOL_FP32
     assert flux32 <= TOL_FP32
@@ -216,17 +216,17 @@ def test_fft_convolver_matches_direct(ctx64, oracle, sw, sh, kw, kh):
         assert np.max(np.abs(got - ref)) <= 1e-13 * np.max(np.abs(ref))
 
 
-def test_c2_fft_model(ctx64, ctx32, oracle):
+def test_galaxy400_fft_model(ctx64, ctx32, oracle):
     """The 400x400 Sersic + 25x25 Moffat PSF model with the FFT convolver,
     against the oracle's BRUTE model: per pixel above 1e-6 of the peak."""
-    announce("c2 fft")
+    announce("galaxy400 fft")
     psf, _ = oracle.evaluate(moffat_psf(25, 3.0, 2.5))
-    p = c2_params(2)
+    p = galaxy400_params(2)
     ref, _ = oracle.evaluate(abi.ModelSpec(400, 400, [sersic(convolve=True, **p)], psf=psf, convolver="brute"))
     for ctx, tol in ((ctx64, 1e-8), (ctx32, TOL_FP32)):
         got, _, _ = evaluate_spec(ctx, abi.ModelSpec(400, 400, [sersic(convolve=True, **p)], psf=psf, convolver="fft"))
         pix, flux = rel_errors(got, ref)
-        print("c2 fft", ctx.precision, pix, flux)
+        print("galaxy400 fft", ctx.precision, pix, flux)
         assert pix <= tol and flux <= tol
 
 

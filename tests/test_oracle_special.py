@@ -146,9 +146,10 @@ def test_r_box_golden(oracle):
         assert relerr(sb["ie"] / s0["ie"], rb) < 1e-13, box
 
 
-def test_adjust_values_config1(oracle):
-    """Auto-adjusted subsampling constants of config 1 (src/radial.cpp:159-186,
-    src/sersic.cpp:257-281), computed independently from the golden bn."""
+def test_adjust_values_sersic100(oracle):
+    """Auto-adjusted subsampling constants of the 100x100 Sersic (n = 4,
+    re = 10; src/radial.cpp:159-186, src/sersic.cpp:257-281), computed
+    independently from the golden bn."""
     from libprofit_amd import abi
     spec = abi.ModelSpec(100, 100, [abi.make_component("sersic", xcen=50.3, ycen=49.7, mag=15, re=10, nser=4, ang=30,
                                                        axrat=0.6, box=0)])
