@@ -1771,7 +1771,7 @@ int tune_run(pk_batch *b, const pk_tuning &t, float stage[6])
 }
 
 // Coordinate search: for each setting in turn, every candidate value is run
-// three times (the fastest run counts) with the others at their best so far, timed
+// five times (the fastest run counts) with the others at their best so far, timed
 // on the stage that setting acts on.  The last run uses the chosen settings,
 // so the batch holds its images.
 int autotune(pk_batch *b)
@@ -1782,7 +1782,7 @@ int autotune(pk_batch *b)
 	PK_TRY(tune_run(b, best, st));  // first launches load the code objects
 	auto measure = [&](const pk_tuning &t, int stage, float &ms) -> int {
 		ms = INFINITY;
-		for (int rep = 0; rep < 3; rep++) {
+		for (int rep = 0; rep < 5; rep++) {
 			float a[6];
 			PK_TRY(tune_run(b, t, a));
 			ms = std::min(ms, a[stage]);
