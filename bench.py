@@ -194,7 +194,7 @@ def grad_bench(args, Batch, Context, evaluate_spec, ctx32, ctx64):
                                      axrat=c.axrat, box=c.box), r2)
         scomps.append(workloads.abi.make_component("sersic", convolve=True, **p))
     sspec = workloads.abi.ModelSpec(256, 256, scomps, psf=psf)
-    one = Batch(ctx32, [sspec], graph=True, grad=True)
+    one = Batch(ctx32, [sspec], graph=True, grad=True, autotune=not args.no_tune)
     one.set_data([d], [sg])
     for _ in range(args.warmup):
         one.evaluate_grad()
