@@ -1126,6 +1126,7 @@ __global__ void reset_counters_kernel(PkCounters *c)
 struct WaveLanes {
 	__device__ int lane() const { return (int)__lane_id(); }
 	__device__ int count() const { return 64; }
+	__device__ double pick(double v, int l) const { return __shfl(v, l); }
 	__device__ double sum(double v) const
 	{
 		for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);

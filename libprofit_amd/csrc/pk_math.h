@@ -195,6 +195,7 @@ struct SerialLanes {
 	PK_HD int lane() const { return 0; }
 	PK_HD int count() const { return 1; }
 	PK_HD double sum(double v) const { return v; }
+	PK_HD double pick(double v, int) const { return v; }
 };
 
 // tanh-sinh on [a, b]: x = a + L q(t) / b - L q(t) with q = e/(1+e),

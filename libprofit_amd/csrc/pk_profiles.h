@@ -552,6 +552,20 @@ PK_HD double moffat_fluxfrac(double rscale, double con, double fraction)
 	return rscale * sqrt(pow(1 - fraction, 1 / (1 - con)) - 1);
 }
 
+// qgamma(p[k], a) for k = 0, 1, 2: one per lane where lanes run side by side,
+// else one after the other (the same values either way).
+template <typename LN>
+PK_HD void qgamma3(double a, const double p[3], double q[3], const LN &ln)
+{
+	if (ln.count() >= 3) {
+		const int l = ln.lane();
+		const double v = qgamma(p[l == 0 ? 0 : l == 1 ? 1 : 2], a);
+		for (int k = 0; k < 3; k++) q[k] = ln.pick(v, k);
+	} else {
+		for (int k = 0; k < 3; k++) q[k] = qgamma(p[k], a);
+	}
+}
+
 template <typename LN = SerialLanes>
 PK_HD void setup_component(const SetupIn &in, PkComp &o, const LN &ln = LN())
 {
@@ -581,11 +595,16 @@ PK_HD void setup_component(const SetupIn &in, PkComp &o, const LN &ln = LN())
 
 	const double *p = c.p;
 	double rscale = 0, lum = 0, bn = 0;
+	double sq[3] = {0, 0, 0};  // Sersic quantiles: bn and those of rscale_switch, rscale_max
 	o.boxy = c.box != 0;
 	switch (c.kind) {
 	case PK_PROFILE_SERSIC: {
 		double re = p[PK_SERSIC_RE], n = p[PK_SERSIC_NSER];
-		bn = qgamma(0.5, 2 * n);  // src/sersic.cpp:297
+		// bn (src/sersic.cpp:297) and the quantiles of the flux fractions of
+		// the adjusted rscale_switch and rscale_max (src/sersic.cpp:257-260)
+		const double fr[3] = {0.5, 1. - n * n / 2e3, 0.9999};
+		qgamma3(2 * n, fr, sq, ln);
+		bn = sq[0];
 		rscale = re;
 		double inv = n;
 		if (o.boxy) inv *= (c.box + 2);
@@ -666,7 +685,7 @@ PK_HD void setup_component(const SetupIn &in, PkComp &o, const LN &ln = LN())
 		double rsw = 0;
 		switch (c.kind) {
 		case PK_PROFILE_SERSIC:
-			rsw = ceil(sersic_fluxfrac(o.q[0], o.q[1], bn, 1. - o.q[1] * o.q[1] / 2e3));
+			rsw = ceil(o.q[0] * pow(sq[1] / bn, o.q[1]));  // sersic_fluxfrac(re, n, bn, 1 - n^2 / 2000)
 			rsw = fmax(fmin(rsw, 20.), 2.);
 			rsw = rsw / o.q[0];
 			break;
@@ -689,7 +708,7 @@ PK_HD void setup_component(const SetupIn &in, PkComp &o, const LN &ln = LN())
 			double rmax = 0;
 			switch (c.kind) {
 			case PK_PROFILE_SERSIC:
-				rmax = ceil(fmax(sersic_fluxfrac(o.q[0], o.q[1], bn, 0.9999), 2.) / o.q[0]);
+				rmax = ceil(fmax(o.q[0] * pow(sq[2] / bn, o.q[1]), 2.) / o.q[0]);  // sersic_fluxfrac(.., 0.9999)
 				break;
 			case PK_PROFILE_MOFFAT: rmax = ceil(fmax(moffat_fluxfrac(rscale, o.q[1], 0.9999), 2.) / rscale); break;
 			case PK_PROFILE_FERRER: rmax = 1; break;
