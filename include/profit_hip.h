@@ -376,6 +376,22 @@ typedef double (*pk_integration_func_t)(double x, void *params);
 double pk_integrate_qags(pk_integration_func_t f, double a, double b, void *params);
 double pk_integrate_qagi(pk_integration_func_t f, double a, void *params);
 
+/* Integrator of the luminosity integrals of the king, brokenexp and
+ * coresersic profiles and of pk_integrate_qags / pk_integrate_qagi
+ * (process-wide, read when a model's or batch's set-up inputs are built).
+ *   PK_INTEGRATOR_QUADPACK (default): libprofit's -- QUADPACK qags (21-point
+ *     Gauss-Kronrod) on [a, b] and qagiu (15-point rule on f(a + (1-t)/t)/t^2
+ *     over (0, 1]) with Wynn epsilon extrapolation, epsabs = epsrel = 1e-4,
+ *     at most 100 intervals, as GSL runs them (src/utils.cpp:155-186).  The
+ *     result carries the quadrature error libprofit's has (up to ~1e-4).
+ *   PK_INTEGRATOR_EXACT: double-exponential quadrature (tanh-sinh on [a, b],
+ *     exp-sinh on [a, inf)) to a relative 1e-14: the integral itself.
+ * No counterpart in the reference (it always uses the first). */
+#define PK_INTEGRATOR_QUADPACK 0
+#define PK_INTEGRATOR_EXACT 1
+int pk_set_integrator(int mode);
+int pk_get_integrator(void);
+
 #ifdef __cplusplus
 }
 #endif

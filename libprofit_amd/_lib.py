@@ -84,6 +84,8 @@ def load():
         "pk_beta": (dbl, [dbl, dbl]),
         "pk_integrate_qags": (dbl, [C.c_void_p, dbl, dbl, C.c_void_p]),
         "pk_integrate_qagi": (dbl, [C.c_void_p, dbl, C.c_void_p]),
+        "pk_set_integrator": (C.c_int, [C.c_int]),
+        "pk_get_integrator": (C.c_int, []),
     }
     for name, (res, args) in sig.items():
         f = getattr(L, name)

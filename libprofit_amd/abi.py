@@ -9,6 +9,7 @@ import ctypes as C
 
 PK_OK, PK_EINVAL, PK_EUNKNOWN, PK_EDEVICE, PK_EFFT, PK_ENOMEM, PK_ERANGE = range(7)
 PK_FP64, PK_FP32 = 0, 1
+PK_INTEGRATOR_QUADPACK, PK_INTEGRATOR_EXACT = 0, 1
 
 PROFILE_KINDS = {
     "null": 0,

@@ -14,6 +14,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <atomic>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -31,6 +32,8 @@ using namespace pk;
 // errors
 // ---------------------------------------------------------------------------
 static thread_local std::string g_err;
+// integrator of the luminosity integrals (pk_set_integrator)
+static std::atomic<int> g_integrator{PK_INTEGRATOR_QUADPACK};
 
 static int set_err(int code, const char *fmt, ...)
 {
@@ -67,7 +70,7 @@ struct pk_ctx {
 	size_t dcap = 0;
 	void *hmem = nullptr;  // pinned staging
 	size_t hcap = 0;
-	hipEvent_t ev[4] = {};
+	hipEvent_t ev[5] = {};
 	// grids of the FP32 refinement passes: as many wavefronts as fit on the
 	// device at once (the hand-over rule counts them)
 	uint32_t r32_grid_main = 0, r32_grid_last = 0;
@@ -962,10 +965,10 @@ int evaluate_impl(pk_ctx *ctx, const pk_model *m, const Analysis &a, double *out
 		use_mask = true;
 	}
 
-	// components: radial + sky go to the coarse/refine kernels, psf profiles
-	// become rebinning items, null profiles do nothing
-	std::vector<PkComp> comps;
-	std::vector<CompLite> lites;
+	// components: radial + sky go to the coarse/refine kernels (their FP64
+	// set-up runs on the device, setup_kernel), psf profiles become rebinning
+	// items, null profiles do nothing
+	std::vector<SetupIn> comps;
 	std::vector<PkPsfItem> items;
 	bool any_nc = false;
 	for (uint32_t c = 0; c < m->n_components; c++) {
@@ -988,11 +991,9 @@ int evaluate_impl(pk_ctx *ctx, const pk_model *m, const Analysis &a, double *out
 		in.fp32_mode = fp32 ? 1 : 0;
 		in.fs = fs;
 		in.fx_scale = fixed_point_scale(m);
-		PkComp pcmp;
-		setup_component(in, pcmp);
+		in.integrator = g_integrator.load();
 		if (!pc.convolve) any_nc = true;
-		comps.push_back(pcmp);
-		lites.push_back(make_lite(pcmp, in));
+		comps.push_back(in);
 	}
 	if (comps.size() > 32) return set_err(PK_EINVAL, "at most 32 radial/sky profiles per model are supported");
 
@@ -1032,14 +1033,15 @@ int evaluate_impl(pk_ctx *ctx, const pk_model *m, const Analysis &a, double *out
 	Layout L;
 	const size_t o_job = L.take(sizeof(PkJob));
 	const size_t o_cnt = L.take(sizeof(PkCounters));
-	const size_t o_comp = L.take(sizeof(PkComp) * std::max<size_t>(1, comps.size()));
-	const size_t o_lite = L.take(sizeof(CompLite) * std::max<size_t>(1, lites.size()));
+	const size_t o_setup = L.take(sizeof(SetupIn) * std::max<size_t>(1, comps.size()));
 	const size_t o_items = L.take(sizeof(PkPsfItem) * std::max<size_t>(1, items.size()));
 	const size_t o_psf64 = L.take(sizeof(double) * std::max<size_t>(1, psf.size()));
 	const size_t o_psfT = L.take(sizeof(TI) * std::max<size_t>(1, psf.size()));
 	const size_t o_mask = L.take(use_mask ? npix : 1);
 	const size_t o_omask = L.take(adjusted ? (size_t)m->mask_width * m->mask_height : 1);
 	const size_t in_bytes = L.off;
+	const size_t o_comp = L.take(sizeof(PkComp) * std::max<size_t>(1, comps.size()));
+	const size_t o_lite = L.take(sizeof(CompLite) * std::max<size_t>(1, comps.size()));
 	const size_t o_nc = L.take(sizeof(TI) * npix);
 	const size_t o_c = L.take(sizeof(TI) * (a.conv_required ? npix : 1));
 	const size_t o_conv = L.take(sizeof(TI) * (a.conv_required ? ncpix : 1));
@@ -1066,8 +1068,7 @@ int evaluate_impl(pk_ctx *ctx, const pk_model *m, const Analysis &a, double *out
 	// stage inputs
 	std::memset(ctx->hmem, 0, in_bytes);
 	*hptr<PkJob>(ctx, o_job) = jb;
-	if (!comps.empty()) std::memcpy(hptr<void>(ctx, o_comp), comps.data(), sizeof(PkComp) * comps.size());
-	if (!lites.empty()) std::memcpy(hptr<void>(ctx, o_lite), lites.data(), sizeof(CompLite) * lites.size());
+	if (!comps.empty()) std::memcpy(hptr<void>(ctx, o_setup), comps.data(), sizeof(SetupIn) * comps.size());
 	if (!items.empty()) std::memcpy(hptr<void>(ctx, o_items), items.data(), sizeof(PkPsfItem) * items.size());
 	if (!psf.empty()) {
 		std::memcpy(hptr<void>(ctx, o_psf64), psf.data(), sizeof(double) * psf.size());
@@ -1085,6 +1086,14 @@ int evaluate_impl(pk_ctx *ctx, const pk_model *m, const Analysis &a, double *out
 	if (use_fft)
 		PK_HIP(hipMemcpyAsync(dptr<void>(ctx, o_ftw), fft_twiddles(), sizeof(double2) * FFT_TWIDDLES,
 		                      hipMemcpyHostToDevice, s));
+	PK_HIP(hipEventRecord(ctx->ev[4], s));
+	if (!comps.empty()) {
+		// per-component FP64 set-up: bn root-solve, normalisation, adjusted
+		// sub-pixel parameters
+		hipLaunchKernelGGL(setup_kernel, dim3((uint32_t)comps.size()), dim3(64), 0, s, dptr<SetupIn>(ctx, o_setup),
+		                   dptr<PkComp>(ctx, o_comp), dptr<CompLite>(ctx, o_lite), (uint32_t)comps.size());
+		PK_HIP(hipGetLastError());
+	}
 	PK_HIP(hipEventRecord(ctx->ev[0], s));
 
 	const PkJob *djob = dptr<PkJob>(ctx, o_job);
@@ -1149,12 +1158,13 @@ int evaluate_impl(pk_ctx *ctx, const pk_model *m, const Analysis &a, double *out
 		res->offset_y = offy;
 		res->n_refined_pixels = cnt.n_entries;
 		res->n_subcells = cnt.n_subcells;
-		float t01 = 0, t12 = 0, t23 = 0, t03 = 0;
+		float t40 = 0, t01 = 0, t12 = 0, t23 = 0, t03 = 0;
+		(void)hipEventElapsedTime(&t40, ctx->ev[4], ctx->ev[0]);
 		(void)hipEventElapsedTime(&t01, ctx->ev[0], ctx->ev[1]);
 		(void)hipEventElapsedTime(&t12, ctx->ev[1], ctx->ev[2]);
 		(void)hipEventElapsedTime(&t23, ctx->ev[2], ctx->ev[3]);
-		(void)hipEventElapsedTime(&t03, ctx->ev[0], ctx->ev[3]);
-		res->ms_setup = 0;
+		(void)hipEventElapsedTime(&t03, ctx->ev[4], ctx->ev[3]);
+		res->ms_setup = t40;
 		res->ms_profiles = t01;
 		res->ms_convolution = t12;
 		res->ms_total = t03;
@@ -1197,8 +1207,10 @@ int pk_component_constants(const pk_model *m, uint32_t index, pk_constants *out)
 	in.pad_y = a.pady;
 	in.fs = fs;
 	in.grad = pc.kind == PK_PROFILE_SERSIC;
+	in.integrator = g_integrator.load();
 	PkComp c;
-	setup_component(in, c);
+	pk::QagWork ws;
+	setup_component(in, c, pk::SerialLanes{&ws});
 	out->dbn_dnser = c.dbn;
 	out->dlogflux_dre = c.glnf_re;
 	out->dlogflux_dnser = c.glnf_n;
@@ -1416,13 +1428,27 @@ double pk_beta(double a, double b) { return pk::beta(a, b); }
 
 double pk_integrate_qags(pk_integration_func_t f, double a, double b, void *params)
 {
-	return pk::integrate_adaptive([&](double x) { return f(x, params); }, a, b);
+	pk::QagWork ws;
+	return pk::integrate_adaptive([&](double x) { return f(x, params); }, a, b, g_integrator.load(),
+	                              pk::SerialLanes{&ws});
 }
 
 double pk_integrate_qagi(pk_integration_func_t f, double a, void *params)
 {
-	return pk::integrate_to_inf([&](double x) { return f(x, params); }, a);
+	pk::QagWork ws;
+	return pk::integrate_to_inf([&](double x) { return f(x, params); }, a, g_integrator.load(),
+	                            pk::SerialLanes{&ws});
 }
+
+int pk_set_integrator(int mode)
+{
+	if (mode != PK_INTEGRATOR_QUADPACK && mode != PK_INTEGRATOR_EXACT)
+		return set_err(PK_EINVAL, "integrator %d: expected PK_INTEGRATOR_QUADPACK or PK_INTEGRATOR_EXACT", mode);
+	g_integrator.store(mode);
+	return PK_OK;
+}
+
+int pk_get_integrator(void) { return g_integrator.load(); }
 
 }  // extern "C"
 
@@ -2037,6 +2063,7 @@ int pk_batch_create(pk_ctx *ctx, const pk_model *models, uint32_t n_models, uint
 			in.fp32_mode = fp32 ? 1 : 0;
 			in.fs = fs;
 			in.grad = (flags & PK_BATCH_GRAD) ? 1 : 0;
+			in.integrator = g_integrator.load();
 			b->comp_index.push_back((uint32_t)b->setup.size());
 			b->setup.push_back(in);
 			if (!pc.convolve) any_nc = true;
@@ -2448,9 +2475,14 @@ int pk_batch_get_chi2_grad(pk_batch *b, double *chi2, double *grad)
 	std::lock_guard<std::mutex> lock(b->ctx->mu);
 	PK_HIP(hipSetDevice(b->ctx->device));
 	PK_HIP(hipStreamSynchronize(b->ctx->stream));
-	PkCounters cnt{};
+	// the forward pass's counters first (its refinement pattern is the one
+	// the adjoint follows), then the adjoint's own
+	PkCounters fwd{}, cnt{};
+	PK_HIP(hipMemcpy(&fwd, b->d<void>(b->o_cnt), sizeof(fwd), hipMemcpyDeviceToHost));
 	PK_HIP(hipMemcpy(&cnt, b->d<void>(b->o_gcnt), sizeof(cnt), hipMemcpyDeviceToHost));
-	if (cnt.overflow) return set_err(PK_ERANGE, "sub-pixel refinement work queue capacity exceeded");
+	if (fwd.overflow || cnt.overflow) return set_err(PK_ERANGE, "sub-pixel refinement work queue capacity exceeded");
+	if (fwd.n_bad) return set_err(PK_ERANGE, "sub-pixel refinement produced %u non-finite values", fwd.n_bad);
+	if (cnt.n_bad) return set_err(PK_ERANGE, "the refinement adjoint produced %u non-finite values", cnt.n_bad);
 	if (chi2) PK_HIP(hipMemcpy(chi2, b->d<void>(b->o_chi2), sizeof(double) * b->n_models, hipMemcpyDeviceToHost));
 	if (grad) {
 		std::vector<double> g((size_t)PK_GRAD_SLOTS * std::max<size_t>(1, b->n_comps));
@@ -2477,16 +2509,19 @@ int pk_batch_set_components(pk_batch *b, const pk_component *comps, uint32_t n)
 			return set_err(PK_EINVAL, "component %u changed its convolve flag", src.comp);
 		PK_TRY(validate_component(pc, true));
 	}
+	// every component is validated before any is copied, so a rejected
+	// update leaves the batch as it was
 	for (uint32_t i = 0; i < n; i++) {
 		if (b->comp_index[i] == ~0u) continue;
-		SetupIn &in = b->setup[b->comp_index[i]];
+		const SetupIn &in = b->setup[b->comp_index[i]];
 		if ((comps[i].convolve != 0) != (in.c.convolve != 0))
 			return set_err(PK_EINVAL, "component %u changed its convolve flag", i);
 		PK_TRY(validate_component(comps[i], true));
 		if (b->elem == 4 && !comps[i].adjust && (comps[i].resolution == 0 || comps[i].resolution > 255))
 			return set_err(PK_EINVAL, "resolution %u: the FP32 path supports 1..255", comps[i].resolution);
-		in.c = comps[i];
 	}
+	for (uint32_t i = 0; i < n; i++)
+		if (b->comp_index[i] != ~0u) b->setup[b->comp_index[i]].c = comps[i];
 	// fixed-point scales follow the new magnitudes
 	for (auto &jb : b->jobs) {
 		double f = 0;

@@ -1124,6 +1124,8 @@ __global__ void reset_counters_kernel(PkCounters *c)
 // butterfly leaves rounding-level differences between lanes, so the total
 // is taken from lane 0 for all.
 struct WaveLanes {
+	QagWork *ws;  // the wavefront's QUADPACK interval list (LDS)
+	__device__ QagWork *work() const { return ws; }
 	__device__ int lane() const { return (int)__lane_id(); }
 	__device__ int count() const { return 64; }
 	__device__ double pick(double v, int l) const { return __shfl(v, l); }
@@ -1135,14 +1137,16 @@ struct WaveLanes {
 };
 
 // One wavefront per component: every lane runs the (sequential) set-up, the
-// luminosity quadratures spread their nodes over the lanes, lane 0 writes.
+// luminosity quadratures spread their nodes over the lanes (the QUADPACK
+// interval list in LDS, written alike by every lane), lane 0 writes.
 __global__ __launch_bounds__(64) void setup_kernel(const SetupIn *__restrict__ in, PkComp *__restrict__ out,
                                                    CompLite *__restrict__ lite, uint32_t n)
 {
 	const uint32_t t = blockIdx.x;
 	if (t >= n) return;
+	__shared__ QagWork qws;
 	PkComp c;
-	setup_component(in[t], c, WaveLanes());
+	setup_component(in[t], c, WaveLanes{&qws});
 	if (__lane_id() == 0) {
 		out[t] = c;
 		if (lite) lite[t] = make_lite(c, in[t]);

@@ -536,7 +536,7 @@ struct SetupIn {
 	uint32_t fs;               // finesampling factor (sky bg is divided by fs^2, src/sky.cpp:42-45)
 	double fx_scale;           // fixed-point scale of the FP32-mode refinement of the model
 	int32_t grad;              // also compute the gradient constants
-	int32_t pad_;
+	int32_t integrator;        // luminosity integrals: PK_INTEGRATOR_QUADPACK or PK_INTEGRATOR_EXACT
 };
 
 PK_HD double sersic_fluxfrac(double re, double nser, double bn, double fraction)
@@ -641,7 +641,7 @@ PK_HD void setup_component(const SetupIn &in, PkComp &o, const LN &ln = LN())
 		o.q[2] = p[PK_KING_A];
 		rscale = o.q[1];
 		const PkComp &k = o;
-		lum = 2 * PK_PI * integrate_adaptive([&](double r) { return r * king_r(k, r); }, 0.0, o.q[1], ln);
+		lum = 2 * PK_PI * integrate_adaptive([&](double r) { return r * king_r(k, r); }, 0.0, o.q[1], in.integrator, ln);
 		break;
 	}
 	case PK_PROFILE_BROKENEXP: {
@@ -651,7 +651,8 @@ PK_HD void setup_component(const SetupIn &in, PkComp &o, const LN &ln = LN())
 		o.q[3] = p[PK_BROKENEXP_A];
 		rscale = o.q[0] * 4;
 		const double h1 = o.q[0], h2 = o.q[1], rb = o.q[2], a = o.q[3];
-		lum = 2 * PK_PI * integrate_to_inf([&](double r) { return r * broken_exponential(r, h1, h2, rb, a); }, 0.0, ln);
+		lum = 2 * PK_PI * integrate_to_inf([&](double r) { return r * broken_exponential(r, h1, h2, rb, a); }, 0.0,
+		                                   in.integrator, ln);
 		break;
 	}
 	case PK_PROFILE_CORESERSIC: {
@@ -664,7 +665,7 @@ PK_HD void setup_component(const SetupIn &in, PkComp &o, const LN &ln = LN())
 		o.q[5] = bn;
 		rscale = o.q[0];
 		const PkComp &k = o;
-		lum = 2 * PK_PI * integrate_to_inf([&](double r) { return r * coresersic_r(k, r); }, 0.0, ln);
+		lum = 2 * PK_PI * integrate_to_inf([&](double r) { return r * coresersic_r(k, r); }, 0.0, in.integrator, ln);
 		break;
 	}
 	default: break;

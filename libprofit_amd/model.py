@@ -66,6 +66,22 @@ def clear_cache():
     check(load().pk_clear_cache())
 
 
+INTEGRATORS = {"quadpack": abi.PK_INTEGRATOR_QUADPACK, "exact": abi.PK_INTEGRATOR_EXACT}
+
+
+def set_integrator(mode):
+    """Integrator of the king / brokenexp / coresersic luminosity integrals:
+    "quadpack" (libprofit's qags / qagiu at 1e-4, the default) or "exact"
+    (double-exponential quadrature to 1e-14).  Read when a model's or a
+    batch's set-up inputs are built."""
+    check(load().pk_set_integrator(INTEGRATORS[mode]))
+
+
+def get_integrator():
+    v = load().pk_get_integrator()
+    return {k for k, m in INTEGRATORS.items() if m == v}.pop()
+
+
 _default_ctx = {}
 
 

@@ -58,8 +58,11 @@ def _profile_params(rng, name, s):
     raise ValueError(name)
 
 
-def galaxy_batch(n=4096, seed=0, smin=32, smax=512):
-    """List of abi.ModelSpec, one stamp per galaxy."""
+def galaxy_batch(n=4096, seed=0, smin=32, smax=512, magzero=30.0, profiles=RADIAL):
+    """List of abi.ModelSpec, one stamp per galaxy.  Magnitudes U[14,20] are
+    counted against `magzero` (30: total fluxes 1e4 .. 2.5e6 against a sky of
+    U[0,10] per pixel, so the galaxies dominate the pixels near their
+    centres).  `profiles` restricts the profile draw (same draw order)."""
     rng = np.random.Generator(np.random.PCG64(seed))
     specs = []
     for _ in range(n):
@@ -67,7 +70,7 @@ def galaxy_batch(n=4096, seed=0, smin=32, smax=512):
         ncomp = int(rng.integers(1, 4))
         comps = []
         for _c in range(ncomp):
-            name = RADIAL[int(rng.integers(0, len(RADIAL)))]
+            name = profiles[int(rng.integers(0, len(profiles)))]
             xc, yc = rng.uniform(0.4 * s, 0.6 * s, size=2)
             mag = rng.uniform(14, 20)
             ang = rng.uniform(0, 180)
@@ -79,7 +82,7 @@ def galaxy_batch(n=4096, seed=0, smin=32, smax=512):
         k = 2 * int(rng.integers(7, 21)) + 1  # odd, 15..41
         fwhm = rng.uniform(2, 4)
         con = rng.uniform(2, 5)
-        specs.append(abi.ModelSpec(s, s, comps, psf=moffat_psf_image(k, fwhm, con)))
+        specs.append(abi.ModelSpec(s, s, comps, psf=moffat_psf_image(k, fwhm, con), magzero=magzero))
     return specs
 
 

@@ -41,6 +41,11 @@ def lib():
         L.or_component_eval_at.restype = C.c_double
         L.or_component_eval_at.argtypes = [C.c_void_p, C.c_uint, C.c_double, C.c_double]
         L.or_set_threads.argtypes = [C.c_int]
+        L.or_set_integrator.argtypes = [C.c_int]
+        L.or_quadpack_test.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
+                                       C.c_uint, C.c_void_p]
+        L.or_quadpack.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_double,
+                                  C.c_double, C.c_uint, C.c_void_p]
         _lib = L
     return _lib
 
@@ -53,6 +58,36 @@ class OracleError(Exception):
 
 def set_threads(n):
     lib().or_set_threads(int(n))
+
+
+INTEGRATORS = {"quadpack": 0, "exact": 1}
+
+
+def set_integrator(mode):
+    """Luminosity integrals by libprofit's QUADPACK qags/qagiu at 1e-4
+    ("quadpack", the default) or by the same algorithm to 1e-13 ("exact")."""
+    lib().or_set_integrator(INTEGRATORS[mode])
+
+
+def quadpack_test(fn, alpha, a, b, epsabs, epsrel, infinite=False, limit=1000):
+    """GSL test-suite integrands f1, f11, f15, f455 through the oracle's
+    qags (finite) / qagiu (infinite): (result, abserr, neval, status)."""
+    out = (C.c_double * 4)()
+    lib().or_quadpack_test(fn, int(infinite), alpha, a, b, epsabs, epsrel, limit, out)
+    return out[0], out[1], int(out[2]), int(out[3])
+
+
+_QFUNC = C.CFUNCTYPE(C.c_double, C.c_double, C.c_void_p)
+
+
+def quadpack(f, a, b=None, epsabs=1e-4, epsrel=1e-4, limit=100):
+    """qags on [a, b] (b given) or qagiu on [a, inf) of a Python function:
+    (result, abserr, neval, status)."""
+    cb = _QFUNC(lambda x, _p: f(x))
+    out = (C.c_double * 4)()
+    lib().or_quadpack(C.cast(cb, C.c_void_p), None, int(b is None), a, 0.0 if b is None else b, epsabs, epsrel,
+                      limit, out)
+    return out[0], out[1], int(out[2]), int(out[3])
 
 
 def gammafn(x):

@@ -60,3 +60,41 @@ def galaxy400_params(seed):
     mag = rng.uniform(12, 20)
     xcen, ycen = rng.uniform(150, 250, size=2)
     return dict(nser=nser, re=re, axrat=axrat, ang=ang, mag=mag, xcen=xcen, ycen=ycen)
+
+
+def without_sky(spec):
+    """The same model without its sky components (the galaxy-only image)."""
+    comps = [abi.Component.from_buffer_copy(spec.m.components[k]) for k in range(spec.m.n_components)]
+    comps = [c for c in comps if c.kind != abi.PROFILE_KINDS["sky"]]
+    m = spec.m
+    return abi.ModelSpec(m.width, m.height, comps, psf=spec._psf, mask=spec._mask, scale=(m.scale_x, m.scale_y),
+                         psf_scale=(m.psf_scale_x, m.psf_scale_y), magzero=m.magzero, finesampling=m.finesampling,
+                         crop=m.crop, return_finesampled=m.return_finesampled, adjust_mask=m.adjust_mask,
+                         dry_run=m.dry_run, convolver=m.convolver, reuse_krn_fft=m.reuse_krn_fft)
+
+
+def galaxy_share(spec, ref, oracle, floor=PEAK_FLOOR):
+    """Largest share of a pixel value that the non-sky profiles contribute,
+    over the pixels a comparison at `floor` of the peak looks at."""
+    ref = np.asarray(ref, dtype=np.float64)
+    peak = np.max(np.abs(ref)) if ref.size else 0.0
+    if peak == 0:
+        return 0.0
+    gal, _ = oracle.evaluate(without_sky(spec))
+    sel = np.abs(ref) > floor * peak
+    return float(np.max(np.abs(gal[sel]) / np.abs(ref[sel])))
+
+
+def check_image(got, spec, ref, oracle, tol, what=""):
+    """Per-pixel and total-flux error of `got` against the oracle image `ref`
+    within `tol`, on a model whose galaxy is observable at that tolerance: on
+    some compared pixel the non-sky profiles make up at least 1e3 * tol of
+    the value, so an image that lost its galaxy (or its convolution or its
+    sub-pixel refinement) fails."""
+    pix, flux = rel_errors(got, ref)
+    assert pix <= tol, (what, "pixel", pix)
+    assert flux <= tol, (what, "flux", flux)
+    if any(spec.m.components[k].kind != abi.PROFILE_KINDS["sky"] for k in range(spec.m.n_components)):
+        share = galaxy_share(spec, ref, oracle)
+        assert share >= 1e3 * tol, (what, "galaxy share of the pixels", share)
+    return pix, flux

@@ -6,7 +6,7 @@ import pytest
 
 from libprofit_amd import abi, workloads
 from libprofit_amd.model import Batch
-from parity_util import TOL_FP32, TOL_FP64, rel_errors
+from parity_util import TOL_FP32, TOL_FP64, check_image, rel_errors
 
 pytestmark = pytest.mark.gpu
 
@@ -26,10 +26,7 @@ def test_batch_matches_oracle(ctx64, ctx32, oracle, prec):
     tol = TOL_FP64 if prec == "fp64" else TOL_FP32
     for i, s in enumerate(specs):
         ref, _ = oracle.evaluate(s)
-        got = b.image(i)
-        pix, flux = rel_errors(got, ref)
-        assert pix <= tol, (i, pix)
-        assert flux <= tol, (i, flux)
+        check_image(b.image(i), s, ref, oracle, tol, i)
     b.close()
 
 
@@ -71,14 +68,13 @@ def test_set_components(ctx64, oracle):
                 c.mag -= 0.5
             cs.append(c)
         comps += cs
-        new_specs.append(abi.ModelSpec(s.m.width, s.m.height, cs, psf=s._psf))
+        new_specs.append(abi.ModelSpec(s.m.width, s.m.height, cs, psf=s._psf, magzero=s.m.magzero))
     b.set_components(comps)
     b.evaluate()
     b.sync()
     for i, s in enumerate(new_specs):
         ref, _ = oracle.evaluate(s)
-        pix, _ = rel_errors(b.image(i), ref)
-        assert pix <= TOL_FP64, (i, pix)
+        check_image(b.image(i), s, ref, oracle, TOL_FP64, i)
     b.close()
 
 
@@ -205,7 +201,7 @@ def test_batch_mixed_composition(ctx64, ctx32, oracle, prec):
                                     axrat=0.7, ang=20.0 * k, convolve=with_psf),
                  abi.make_component("sky", bg=0.5)]
         specs.append(abi.ModelSpec(s, s, comps, psf=psf if with_psf else None, finesampling=fs,
-                                   return_finesampled=rf))
+                                   return_finesampled=rf, magzero=30.0))
     ctx = ctx64 if prec == "fp64" else ctx32
     tol = TOL_FP64 if prec == "fp64" else TOL_FP32
     b = Batch(ctx, specs, graph=True)
@@ -216,25 +212,32 @@ def test_batch_mixed_composition(ctx64, ctx32, oracle, prec):
             ref, _ = oracle.evaluate(s)
             got = b.image(i)
             assert got.shape == ref.shape, (i, got.shape, ref.shape)
-            pix, flux = rel_errors(got, ref)
-            assert pix <= tol, (i, pix)
-            assert flux <= tol, (i, flux)
+            check_image(got, s, ref, oracle, tol, i)
     b.close()
+
+
+# every PSF-size class of the batch convolution (largest side <= 17, 21, ...,
+# 41 and above), odd sides from 15 to 41, and even, non-square, tiny and
+# wide PSFs (the run-time column paths)
+PSF_SHAPES = [(15, 15), (17, 17), (19, 19), (21, 21), (23, 23), (25, 25), (27, 27), (29, 29), (31, 31), (33, 33),
+              (35, 35), (37, 37), (39, 39), (41, 41), (51, 51), (3, 3), (8, 8), (5, 9), (12, 7), (1, 5), (30, 30),
+              (16, 24), (21, 31), (41, 15)]
 
 
 @pytest.mark.parametrize("tiles", [0, 1])
 def test_batch_psf_shapes(ctx32, oracle, tiles):
-    """Even, non-square, tiny and wider-than-specialised PSFs (the run-time
-    column paths of both convolution forms) in PSF-size classes of their own."""
+    """Both convolution forms (VALU tiles, MFMA tiles) against the oracle in
+    FP32 on stamps whose galaxy dominates its pixels, over every PSF-size
+    class and the odd, even and non-square PSF shapes."""
     print("CASE psf shapes, conv_tiles", tiles, flush=True)
     specs = []
-    for k, (pw, ph) in enumerate([(3, 3), (8, 8), (5, 9), (51, 51), (15, 15), (12, 7), (1, 5)]):
+    for k, (pw, ph) in enumerate(PSF_SHAPES):
         s = 48 + 5 * k
-        psf = workloads.moffat_psf_image(max(pw, ph), 2.5, 3.0)[:ph, :pw]
-        comps = [abi.make_component("sersic", xcen=0.5 * s + 0.3, ycen=0.5 * s - 0.2, mag=16.0, re=4.0, nser=1.5,
-                                    axrat=0.8, ang=15.0 * k, convolve=True),
+        psf = workloads.moffat_psf_image(max(pw, ph), 2.0 + 0.1 * k, 2.5 + 0.1 * k)[:ph, :pw]
+        comps = [abi.make_component("sersic", xcen=0.5 * s + 0.3, ycen=0.5 * s - 0.2, mag=16.0, re=3.0 + 0.2 * k,
+                                    nser=1.5 + 0.1 * k, axrat=0.8, ang=15.0 * k, convolve=True),
                  abi.make_component("sky", bg=0.2)]
-        specs.append(abi.ModelSpec(s, s, comps, psf=psf))
+        specs.append(abi.ModelSpec(s, s, comps, psf=psf, magzero=30.0))
     b = Batch(ctx32, specs, graph=False)
     b.tuning = {**DEFAULT_TUNING, "conv_tiles": tiles}
     b.evaluate()
@@ -243,10 +246,30 @@ def test_batch_psf_shapes(ctx32, oracle, tiles):
         ref, _ = oracle.evaluate(s)
         got = b.image(i)
         assert got.shape == ref.shape, (i, got.shape, ref.shape)
-        pix, flux = rel_errors(got, ref)
-        assert pix <= TOL_FP32, (i, pix)
-        assert flux <= TOL_FP32, (i, flux)
+        check_image(got, s, ref, oracle, TOL_FP32, PSF_SHAPES[i])
     b.close()
+
+
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+@pytest.mark.parametrize("profile", workloads.RADIAL)
+def test_batch_each_profile(ctx64, ctx32, oracle, prec, profile):
+    """Galaxies made of one radial profile type only (1-3 components of it
+    with their typical parameter ranges), so the device set-up of that
+    profile (normalisation by closed form or quadrature, adjusted
+    sub-pixel parameters) and its evaluation are compared visibly."""
+    print("CASE one profile type", profile, prec, flush=True)
+    specs = workloads.galaxy_batch(10, 7, smin=24, smax=160, profiles=(profile,))
+    ctx = ctx64 if prec == "fp64" else ctx32
+    tol = TOL_FP64 if prec == "fp64" else TOL_FP32
+    for tiles in ((0,) if prec == "fp64" else (0, 1)):
+        b = Batch(ctx, specs, graph=False)
+        b.tuning = {**DEFAULT_TUNING, "conv_tiles": tiles}
+        b.evaluate()
+        b.sync()
+        for i, s in enumerate(specs):
+            ref, _ = oracle.evaluate(s)
+            check_image(b.image(i), s, ref, oracle, tol, (profile, tiles, i))
+        b.close()
 
 
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
@@ -263,7 +286,7 @@ def test_batch_masks(ctx64, ctx32, oracle, prec):
                                     ang=35.0, convolve=psf_ is not None),
                  abi.make_component("sky", bg=0.3)]
         return abi.ModelSpec(s, s, comps, psf=psf_, mask=mask, finesampling=fs, adjust_mask=adjust,
-                             return_finesampled=rf)
+                             return_finesampled=rf, magzero=30.0)
 
     specs = []
     s = 50
@@ -287,9 +310,7 @@ def test_batch_masks(ctx64, ctx32, oracle, prec):
             got = b.image(i)
             assert got.shape == ref.shape, (i, got.shape, ref.shape)
             assert np.array_equal(got == 0, ref == 0), i  # masked pixels are exactly 0 in both
-            pix, flux = rel_errors(got, ref)
-            assert pix <= tol, (i, pix)
-            assert flux <= tol, (i, flux)
+            check_image(got, sp, ref, oracle, tol, i)
     b.close()
 
 
@@ -307,7 +328,7 @@ def test_batch_psf_profiles(ctx64, ctx32, oracle, prec):
         return abi.make_component(kind, convolve=convolve, **kw)
 
     def spec(s, comps, psf_=psf, fs=1):
-        return abi.ModelSpec(s, s, comps, psf=psf_, finesampling=fs)
+        return abi.ModelSpec(s, s, comps, psf=psf_, finesampling=fs, magzero=30.0)
 
     specs = [
         spec(40, [comp("sersic", xcen=20.3, ycen=19.6, mag=15.0, re=4.0, nser=1.5, axrat=0.7, ang=20.0),
@@ -332,9 +353,7 @@ def test_batch_psf_profiles(ctx64, ctx32, oracle, prec):
             ref, _ = oracle.evaluate(sp)
             got = b.image(i)
             assert got.shape == ref.shape, (i, got.shape, ref.shape)
-            pix, flux = rel_errors(got, ref)
-            assert pix <= tol, (i, pix)
-            assert flux <= tol, (i, flux)
+            check_image(got, sp, ref, oracle, tol, i)
 
     check(specs)
     check(specs)
@@ -349,7 +368,8 @@ def test_batch_psf_profiles(ctx64, ctx32, oracle, prec):
                 c.mag -= 0.3
             cs.append(c)
         comps += cs
-        new_specs.append(abi.ModelSpec(sp.m.width, sp.m.height, cs, psf=sp._psf, finesampling=sp.m.finesampling))
+        new_specs.append(abi.ModelSpec(sp.m.width, sp.m.height, cs, psf=sp._psf, finesampling=sp.m.finesampling,
+                                       magzero=sp.m.magzero))
     b.set_components(comps)
     check(new_specs)
     b.close()
@@ -374,19 +394,20 @@ def test_batch_fft_convolver(ctx64, ctx32, oracle, prec):
         p.update(kw)
         return abi.make_component("sersic", **p)
 
-    sky = abi.make_component("sky", bg=1e-9)
+    sky = abi.make_component("sky", bg=0.5)
+    mz = 30.0
     specs = [
-        abi.ModelSpec(60, 50, [sersic(55), sky], psf=psf, convolver="fft"),
-        abi.ModelSpec(60, 50, [sersic(55), sky], psf=psf, convolver="fft", crop=False),
-        abi.ModelSpec(40, 40, [sersic(40, nser=1.0), sky], psf=psf, convolver="brute"),
+        abi.ModelSpec(60, 50, [sersic(55), sky], psf=psf, convolver="fft", magzero=mz),
+        abi.ModelSpec(60, 50, [sersic(55), sky], psf=psf, convolver="fft", crop=False, magzero=mz),
+        abi.ModelSpec(40, 40, [sersic(40, nser=1.0), sky], psf=psf, convolver="brute", magzero=mz),
         abi.ModelSpec(36, 36, [sersic(36, nser=3.0)], psf=psf, convolver="fft", finesampling=2,
-                      return_finesampled=False),
+                      return_finesampled=False, magzero=mz),
         abi.ModelSpec(48, 48, [sersic(48), sky], psf=psf, convolver="fft",
-                      mask=(rng.uniform(size=(48, 48)) > 0.3).astype(np.uint8)),
+                      mask=(rng.uniform(size=(48, 48)) > 0.3).astype(np.uint8), magzero=mz),
         abi.ModelSpec(44, 44, [sersic(44, convolve=False), abi.make_component("psf", xcen=20.2, ycen=23.6, mag=15.5,
                                                                              convolve=True)],
-                      psf=psf_even, convolver="fft"),
-        abi.ModelSpec(32, 32, [sersic(32, convolve=False), sky]),
+                      psf=psf_even, convolver="fft", magzero=mz),
+        abi.ModelSpec(32, 32, [sersic(32, convolve=False), sky], magzero=mz),
     ]
     ctx = ctx64 if prec == "fp64" else ctx32
     b = Batch(ctx, specs, graph=True)
@@ -400,12 +421,12 @@ def test_batch_fft_convolver(ctx64, ctx32, oracle, prec):
             assert b.origin(i) == tuple(roff), (i, b.origin(i), roff)
             pix, flux = rel_errors(got, ref)
             if prec == "fp32":
-                assert pix <= TOL_FP32 and flux <= TOL_FP32, (i, pix, flux)
+                check_image(got, sp, ref, oracle, TOL_FP32, i)
             elif sp.m.convolver == abi.FFT and any(sp.m.components[k].convolve for k in range(sp.m.n_components)):
                 assert np.max(np.abs(got - ref)) <= 1e-14 * np.max(np.abs(ref)), i
                 assert pix <= 1e-8, (i, pix)
             else:
-                assert pix <= TOL_FP64, (i, pix)
+                check_image(got, sp, ref, oracle, TOL_FP64, i)
     b.close()
 
 
@@ -419,10 +440,10 @@ def test_batch_dry_run(ctx64, ctx32, oracle, prec):
     psf = workloads.moffat_psf_image(9, 2.5, 3.0)
     sers = abi.make_component("sersic", xcen=20.2, ycen=19.7, mag=15.0, re=4.0, nser=2.0, axrat=0.7, ang=30.0,
                               convolve=True)
-    specs = [abi.ModelSpec(40, 40, [sers], psf=psf),
-             abi.ModelSpec(40, 36, [sers], psf=psf, dry_run=True),
+    specs = [abi.ModelSpec(40, 40, [sers], psf=psf, magzero=30.0),
+             abi.ModelSpec(40, 36, [sers], psf=psf, dry_run=True, magzero=30.0),
              abi.ModelSpec(30, 30, [abi.make_component("sky", bg=0.5)], dry_run=True, finesampling=2),
-             abi.ModelSpec(40, 40, [sers], psf=psf)]
+             abi.ModelSpec(40, 40, [sers], psf=psf, magzero=30.0)]
     ctx = ctx64 if prec == "fp64" else ctx32
     tol = TOL_FP64 if prec == "fp64" else TOL_FP32
     b = Batch(ctx, specs, graph=True)
@@ -436,8 +457,7 @@ def test_batch_dry_run(ctx64, ctx32, oracle, prec):
             assert not np.any(got), i
         else:
             ref, _ = oracle.evaluate(sp)
-            pix, flux = rel_errors(got, ref)
-            assert pix <= tol and flux <= tol, (i, pix, flux)
+            check_image(got, sp, ref, oracle, tol, i)
     b.close()
 
 
@@ -450,7 +470,7 @@ def test_one_shot_entry_points(ctx32, ctx64, oracle):
     psf = workloads.moffat_psf_image(9, 2.5, 3.0)
     specs = small_batch(5, 17) + [abi.ModelSpec(40, 40, [abi.make_component(
         "sersic", xcen=20.2, ycen=19.7, mag=15.0, re=4.0, nser=2.0, convolve=True)], psf=psf, convolver="fft",
-        crop=False)]
+        crop=False, magzero=30.0)]
     for ctx in (ctx64, ctx32):
         b = Batch(ctx, specs, graph=False)
         b.evaluate()

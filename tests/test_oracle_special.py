@@ -100,8 +100,20 @@ def test_special_cases(oracle, who):
         assert bt(x, x) == 0.
 
 
-def test_product_integrators():
-    """pk_integrate_qags / pk_integrate_qagi against closed forms."""
+@pytest.fixture
+def exact_integrals(oracle):
+    """Luminosity integrals by the exact integrators (product and oracle)
+    for the duration of a test."""
+    from libprofit_amd import model as M
+    M.set_integrator("exact")
+    oracle.set_integrator("exact")
+    yield
+    M.set_integrator("quadpack")
+    oracle.set_integrator("quadpack")
+
+
+def test_product_integrators(exact_integrals):
+    """pk_integrate_qags / pk_integrate_qagi (exact mode) against closed forms."""
     from libprofit_amd import _lib
     P = product()
     f1 = _lib.INTEGRAND(lambda x, _: x * math.exp(-x))
@@ -118,8 +130,8 @@ def _setup(oracle, name, **kw):
     return oracle.component_setup(spec)
 
 
-def test_lumtot_golden(oracle):
-    """Ie = 10^(-0.4 mag) / (lumtot * axrat / r_box): with mag = 0, axrat = 1,
+def test_lumtot_golden(oracle, exact_integrals):
+    """Exact integrator: Ie = 10^(-0.4 mag) / (lumtot * axrat / r_box): with mag = 0, axrat = 1,
     box = 0 (r_box = 1), 1/Ie is the profile's total luminosity."""
     for re, n, v in GOLD["sersic_lumtot"]:
         s = _setup(oracle, "sersic", re=re, nser=n)
@@ -178,8 +190,9 @@ def _product_constants(name, **kw):
     return out
 
 
-def test_product_lumtot_golden():
-    """The product's set-up (host copy of the device code) against mpmath."""
+def test_product_lumtot_golden(exact_integrals):
+    """The product's set-up (host copy of the device code, exact integrator)
+    against mpmath."""
     for re, n, v in GOLD["sersic_lumtot"]:
         assert relerr(_product_constants("sersic", re=re, nser=n).lumtot, v) < 1e-13, (re, n)
     for rc, rt, a, v in GOLD["king_lumtot"]:
