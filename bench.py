@@ -2,20 +2,30 @@
 
 Workload (one step): 4096 galaxies, square stamps with side ~U{32..512}, each
 with 1-3 components drawn from {sersic, coresersic, moffat, ferrer, king,
-brokenexp} plus a sky level, convolved with a per-galaxy Moffat PSF of odd
-side ~U{15..41}; one batched evaluation (FP64 per-component set-up on the
-device, FP32 profile evaluation with FP64 refinement decisions near the
-threshold, FP32 direct convolution), replayed as a hipGraph.  Inputs are
-resident in device memory before the timed region.
+brokenexp} (magnitudes U[14,20] against magzero 30) plus a sky level
+U[0,10], convolved with a per-galaxy Moffat PSF of odd side ~U{15..41}; one
+batched evaluation (FP64 per-component set-up on the device, FP32 profile
+evaluation with FP64 refinement decisions near the threshold, FP32 direct
+convolution), replayed as a hipGraph.  Inputs are resident in device memory
+before the timed region.
 
 Also reported on the same run:
   - the plain FP64 path (one Model::evaluate per galaxy) on a sample of the
     same galaxies, in images/s, and the speed-up of the batched FP32 path;
-  - the per-pixel (above 1e-6 of the peak) and total-flux relative error of the
-    batched FP32 images against the FP64 ones on that sample;
-  - roofline of the dominant kernel (direct convolution, FP32 VALU) from HIP
-    events on the launch stream;
-  - the CPU baseline: the FP64 oracle (OpenMP) on a bounded sample.
+  - accuracy against the FP64 CPU oracle on a sample spanning every PSF side
+    and the stamp sizes: per-pixel (above 1e-6 of the peak) and total-flux
+    relative error of the batched FP32 and the plain FP64 images; the run
+    fails (exit 3) when the FP32 images miss 1e-5 or a galaxy of the sample
+    does not dominate some pixel of its stamp;
+  - roofline of the dominant kernel (the convolution) and of the pixel pass,
+    the refinement and the refinement adjoint, from HIP events on the launch
+    stream and the kernels' own evaluation counters;
+  - the CPU baseline: the FP64 oracle built -O3 -march=native on the host,
+    OpenMP, on a bounded sample;
+  - the 100x100 Sersic and the 400x400 Sersic + 25x25 PSF models (BRUTE and
+    FFT): plain FP64 GPU, batched FP32 GPU and CPU images/s with their errors;
+  - chi^2-gradient throughput of 1024 bulge+disk+sky fits (128x128, PSF 31)
+    and of the 256x256 bulge+disk fit.
 
 Prints ONE JSON line.  Usage: python bench.py [--steps K] [--warmup W] [--gpus 1]
 """
@@ -33,6 +43,16 @@ sys.path.insert(0, ROOT)
 METRIC = "images/s & chi2-grads/s: batched FP32 vs plain FP64 GPU at <=1e-5 rel flux err"
 FP32_PEAK_TFLOPS = 157.3
 HBM_PEAK_GBS = 8000.0
+# FP32 VALU issue: 256 CUs x 64 lanes x 2.4 GHz lane-slots per second
+VALU_PEAK_TSLOTS = 256 * 64 * 2.4e9 / 1e12
+# algorithmic VALU slots: one FP32 evaluate_at (Sersic general form: 14 FP32
+# operations + 4 transcendentals at quarter rate), the pixel pass's
+# classification of one (pixel, radial component) pair (coordinate transform
+# and radius test), and one evaluate_at of the adjoint (value and partials)
+SLOTS_EVAL = 30
+SLOTS_CLASSIFY = 8
+SLOTS_EVAL_ADJ = 90
+TOL_FP32 = 1e-5
 
 
 def log(*a):
@@ -71,6 +91,110 @@ def conv_traffic(kernel):
     except OSError:
         pass
     return None
+
+
+def accuracy_sample(specs, n, seed=12345):
+    """Indices of a sample of the batch for the plain FP64 timing and the
+    accuracy checks: for every PSF side its smallest and its largest stamp,
+    the rest drawn at random (seeded)."""
+    by_k = {}
+    for i, s in enumerate(specs):
+        by_k.setdefault(int(s.m.psf_width), []).append(i)
+    pick = set()
+    for _k, idx in sorted(by_k.items()):
+        idx = sorted(idx, key=lambda i: (specs[i].m.width, i))
+        pick.update((idx[0], idx[-1]))
+    rng = np.random.Generator(np.random.PCG64(seed))
+    for i in rng.permutation(len(specs)).tolist():
+        if len(pick) >= n:
+            break
+        pick.add(i)
+    return sorted(pick)
+
+
+def host_cpu():
+    """What the CPU baseline ran on: logical CPUs, those this process may
+    use, and the CPU model name."""
+    model = None
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except AttributeError:
+        usable = os.cpu_count()
+    return {"nproc": os.cpu_count(), "usable": usable, "model": model}
+
+
+def galaxy_share(O, spec, ref, L=None):
+    """Largest share of a compared pixel that the non-sky profiles make up."""
+    from libprofit_amd import abi
+    comps = [abi.Component.from_buffer_copy(spec.m.components[k]) for k in range(spec.m.n_components)]
+    comps = [c for c in comps if c.kind != abi.PROFILE_KINDS["sky"]]
+    gal, _ = O.evaluate(abi.ModelSpec(spec.m.width, spec.m.height, comps, psf=spec._psf, magzero=spec.m.magzero,
+                                      convolver=spec.m.convolver), L)
+    sel = np.abs(ref) > 1e-6 * np.max(np.abs(ref))
+    return float(np.max(np.abs(gal[sel]) / np.abs(ref[sel])))
+
+
+def model_rows(args, Batch, evaluate_spec, ctx32, ctx64, O, threads):
+    """The 100x100 Sersic (profit-cli defaults) and the 400x400 Sersic + 25x25
+    Moffat PSF models (64 seeds, BRUTE and FFT): images/s of the plain FP64
+    GPU path (one pk_evaluate per image, host buffers), of the batched FP32
+    path (one batch of all seeds, graph) and of the CPU oracle, with the
+    errors of both GPU paths against the oracle."""
+    from libprofit_amd import abi, workloads
+    rows = {}
+
+    def timed(fn, reps):
+        fn()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        return (time.perf_counter() - t0) / reps
+
+    def errs(got, refs):
+        e = [rel_errors(g, r) for g, r in zip(got, refs)]
+        return {"max_pixel_rel_err": max(x[0] for x in e), "max_flux_rel_err": max(x[1] for x in e)}
+
+    cases = [("sersic100", [abi.ModelSpec(100, 100, [abi.make_component(
+        "sersic", xcen=50.3, ycen=49.7, mag=15, re=10, nser=4, ang=30, axrat=0.6, box=0)])], 200)]
+    for conv in ("brute", "fft"):
+        specs = []
+        for seed in range(args.c2_seeds):
+            sp = workloads.single_sersic_psf(seed)
+            sp.m.convolver = abi.CONVOLVERS[conv]
+            specs.append(sp)
+        cases.append(("sersic400_psf25_" + conv, specs, 1))
+    Lf = O.fast_lib() if O is not None else None
+    for name, specs, reps in cases:
+        row = {"images": len(specs)}
+        # plain FP64 GPU: one pk_evaluate per image
+        plain = []
+        dt = timed(lambda: plain.__setitem__(slice(None), [evaluate_spec(ctx64, s)[0] for s in specs]), reps)
+        row["plain_fp64_gpu_images_per_s"] = round(len(specs) / dt, 2)
+        b = Batch(ctx32, specs, graph=True)
+        b.evaluate()
+        b.sync()
+        dtb = timed(lambda: (b.evaluate(), b.sync()), max(reps, 20))
+        row["batched_fp32_gpu_images_per_s"] = round(len(specs) / dtb, 2)
+        got32 = [b.image(i) for i in range(len(specs))]
+        b.close()
+        if O is not None:
+            refs = [O.evaluate(s)[0] for s in specs]
+            row["plain_fp64_vs_oracle"] = errs(plain, refs)
+            row["batched_fp32_vs_oracle"] = errs(got32, refs)
+            ncpu = min(len(specs), 8)
+            dtc = timed(lambda: [O.evaluate(s, Lf) for s in specs[:ncpu]], 1 if reps == 1 else 20)
+            row["cpu_images_per_s"] = round(ncpu / dtc, 2)
+            row["cpu_threads"] = threads
+        log("%s: %s" % (name, row))
+        rows[name] = row
+    return rows
 
 
 def fd_plain_gradient(ctx64, spec, data, sigma, evaluate_spec):
@@ -151,7 +275,15 @@ def grad_bench(args, Batch, Context, evaluate_spec, ctx32, ctx64):
     eb.sync()
     eb.evaluate_grad()
     inf = eb.sync()
-    out["fits_stage_ms"] = {"forward": round(inf.ms_total, 4), "adjoint": round(inf.ms_grad, 4)}
+    out["fits_stage_ms"] = {"forward": round(inf.ms_total, 4), "adjoint": round(inf.ms_grad, 4),
+                            "adjoint_refine": round(inf.ms_grad_refine, 4)}
+    slots = SLOTS_EVAL_ADJ * inf.n_grad_subcells
+    ach = slots / (inf.ms_grad_refine * 1e-3) / 1e12 if inf.ms_grad_refine > 0 else 0.0
+    out["fits_adjoint_refine_roofline"] = {
+        "kernel": "refine32_kernel<..., GRAD> (4 passes) + grad_finalize_kernel, per step",
+        "bound": "valu", "achieved": round(ach, 3), "peak": round(VALU_PEAK_TSLOTS, 2), "unit": "Tslot/s",
+        "frac": round(ach / VALU_PEAK_TSLOTS, 4), "evaluations": int(inf.n_grad_subcells),
+        "slots_per_evaluation": SLOTS_EVAL_ADJ, "ms": round(inf.ms_grad_refine, 4)}
     eb.close()
     # FP64 adjoint of the same fits: accuracy of the FP32 gradients
     b64 = Batch(ctx64, start, graph=False, grad=True)
@@ -223,7 +355,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--galaxies", type=int, default=4096)
     ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--sample", type=int, default=96, help="galaxies timed on the plain FP64 path")
+    ap.add_argument("--sample", type=int, default=96,
+                    help="galaxies timed on the plain FP64 path and checked against the oracle")
+    ap.add_argument("--c2-seeds", type=int, default=64, help="seeds of the 400x400 Sersic + PSF models")
+    ap.add_argument("--no-models", action="store_true", help="skip the 100x100 and 400x400 model rows")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU baseline")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--fits", type=int, default=1024, help="fits of the chi^2-gradient workload")
@@ -338,10 +473,32 @@ def main():
         "launches_per_step": int(inf.n_conv_launches),
         "stage_ms": {k: round(v, 4) for k, v in stages.items()},
     }
+    # the profile-evaluation kernels, priced in FP32 VALU issue slots from
+    # the evaluation counts of the same workload (pk_batch_work_counts)
+    wc = batch.work_counts()
+    pairs = int(wc.pixel_component_pairs)
+
+    def valu_entry(kernel_name, slots, ms, **extra):
+        ach = slots / (ms * 1e-3) / 1e12 if ms > 0 else 0.0
+        e = {"kernel": kernel_name, "bound": "valu", "achieved": round(ach, 3), "peak": round(VALU_PEAK_TSLOTS, 2),
+             "unit": "Tslot/s", "frac": round(ach / VALU_PEAK_TSLOTS, 4), "ms": round(ms, 4)}
+        e.update(extra)
+        return e
+
+    kernels = {
+        "conv": {k: roofline[k] for k in ("bound", "achieved", "peak", "unit", "frac", "avg_launch_ms")},
+        "coarse": valu_entry("coarse_kernel<float, true> (one launch per step)",
+                             SLOTS_CLASSIFY * pairs + SLOTS_EVAL * wc.coarse_evaluations, stages["coarse"],
+                             pixel_component_pairs=pairs, evaluations=int(wc.coarse_evaluations),
+                             slots_per_pair=SLOTS_CLASSIFY, slots_per_evaluation=SLOTS_EVAL),
+        "refine": valu_entry("refine32_kernel (4 passes) + finalize32_kernel, per step",
+                             SLOTS_EVAL * inf.n_subcells, stages["refine"], evaluations=int(inf.n_subcells),
+                             slots_per_evaluation=SLOTS_EVAL),
+    }
+    log("kernels: %s" % json.dumps(kernels))
 
     # ---- plain FP64 path on a sample of the same galaxies ---------------------------
-    rng = np.random.Generator(np.random.PCG64(12345))
-    sample = sorted(rng.choice(len(specs), size=min(args.sample, len(specs)), replace=False).tolist())
+    sample = accuracy_sample(specs, min(args.sample, len(specs)), seed=12345)
     evaluate_spec(ctx64, specs[sample[0]])  # warm
     t0 = time.perf_counter()
     ref64 = {}
@@ -356,27 +513,46 @@ def main():
     log("plain fp64: %.3f ms/image on the sample -> %.1f images/s at the batch mean size" %
         (plain_s_per_img * 1e3, plain_images_per_s))
 
-    # ---- accuracy of the batched FP32 images against FP64 ---------------------------
-    worst_pix, worst_flux = 0.0, 0.0
-    for i in sample:
-        got = batch.image(i)
-        p, f = rel_errors(got, ref64[i])
-        worst_pix = max(worst_pix, p)
-        worst_flux = max(worst_flux, f)
-    log("fp32 vs fp64 on the sample: max pixel rel err %.3g, max flux rel err %.3g" % (worst_pix, worst_flux))
-
-    # ---- CPU baseline: FP64 oracle with OpenMP on a bounded sample -------------------
-    # galaxies of the same batch in a seeded order until --cpu-seconds of work
-    cpu = None
+    # ---- CPU baseline and accuracy: the FP64 oracle ---------------------------------
+    # The oracle (the checker) draws the sample's reference images; its
+    # -O3 -march=native build on this host is timed on galaxies of the same batch
+    # in a seeded order until --cpu-seconds of work.
+    cpu, accuracy, model_row = None, None, {}
+    accuracy_ok = True
     if not args.no_cpu:
         from oracle import oracle as O
         O.build()
-        threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or min(16, os.cpu_count() or 1)
+        host = host_cpu()
+        threads = int(os.environ.get("OMP_NUM_THREADS", "0")) or host["usable"]
         O.set_threads(threads)
+        Lf = O.fast_lib()
+        Lf.or_set_threads(threads)
+        worst = {"fp32": [0.0, 0.0], "fp64": [0.0, 0.0]}
+        min_share, worst_at = 1.0, None
+        for i in sample:
+            ref, _ = O.evaluate(specs[i])
+            p32, f32 = rel_errors(batch.image(i), ref)
+            p64, f64 = rel_errors(ref64[i], ref)
+            if p32 > worst["fp32"][0]:
+                worst_at = i
+            worst["fp32"] = [max(worst["fp32"][0], p32), max(worst["fp32"][1], f32)]
+            worst["fp64"] = [max(worst["fp64"][0], p64), max(worst["fp64"][1], f64)]
+            min_share = min(min_share, galaxy_share(O, specs[i], ref))
+        accuracy = {
+            "reference": "FP64 CPU oracle (QUADPACK normalisation as libprofit's)",
+            "sample": "%d galaxies: every PSF side's smallest and largest stamp + seeded random" % len(sample),
+            "batched_fp32_vs_oracle": {"max_pixel_rel_err": worst["fp32"][0], "max_flux_rel_err": worst["fp32"][1],
+                                       "worst_galaxy": worst_at},
+            "plain_fp64_vs_oracle": {"max_pixel_rel_err": worst["fp64"][0], "max_flux_rel_err": worst["fp64"][1]},
+            "min_galaxy_share_of_a_pixel": min_share,
+            "pixels_compared": "above 1e-6 of each image's peak",
+        }
+        accuracy_ok = worst["fp32"][0] <= TOL_FP32 and worst["fp32"][1] <= TOL_FP32 and min_share >= 1e3 * TOL_FP32
+        log("accuracy: %s" % json.dumps(accuracy))
         order = np.random.Generator(np.random.PCG64(777)).permutation(len(specs)).tolist()
         done, tpix, t0 = 0, 0, time.perf_counter()
         for i in order:
-            O.evaluate(specs[i])
+            O.evaluate(specs[i], Lf)
             done += 1
             tpix += int(specs[i].m.width) * int(specs[i].m.height)
             if time.perf_counter() - t0 > args.cpu_seconds:
@@ -384,9 +560,15 @@ def main():
         dt = time.perf_counter() - t0
         cpu_ips = done / dt * (tpix / done) / (npix / len(specs))
         cpu = {"value": round(cpu_ips, 3), "unit": "images/s", "cores": threads, "kind": "port",
-               "sample": "%d of the %d galaxies in a seeded order (FP64 oracle, OpenMP, BRUTE), %.1f s; "
-                         "images/s normalised to the batch mean image size" % (done, len(specs), dt)}
+               "sample": "%d of the %d galaxies in a seeded order, %.1f s: FP64 oracle built -O3 -march=native "
+                         "on this host, OpenMP over pixels, BRUTE with the AVX dot-product order; images/s "
+                         "normalised to the batch mean image size" % (done, len(specs), dt),
+               "host": host}
         log("cpu baseline: %s" % cpu)
+        if not args.no_models:
+            model_row = model_rows(args, Batch, evaluate_spec, ctx32, ctx64, O, threads)
+    elif not args.no_models:
+        model_row = model_rows(args, Batch, evaluate_spec, ctx32, ctx64, None, 0)
 
     grad = {}
     if not args.no_grad:
@@ -406,8 +588,9 @@ def main():
         "dtype": "fp32",
         "data": "synthetic",
         "config": {
-            "workload": "galaxy_batch: %d galaxies, stamps U{32..512}, 1-3 radial components + sky, "
-                        "per-galaxy Moffat PSF U{15..41}, BRUTE, FP32 bulk / FP64 set-up" % len(specs),
+            "workload": "galaxy_batch: %d galaxies, stamps U{32..512}, 1-3 radial components (mag U[14,20], "
+                        "magzero 30) + sky U[0,10], per-galaxy Moffat PSF U{15..41}, BRUTE, FP32 bulk / FP64 "
+                        "set-up" % len(specs),
             "galaxies": len(specs),
             "output_pixels": npix,
             "seed": args.seed,
@@ -416,16 +599,21 @@ def main():
         },
         "plain_fp64_images_per_s": round(plain_images_per_s, 2),
         "speedup_vs_plain_fp64": round(images_per_s / plain_images_per_s, 2),
-        "fp32_vs_fp64_max_pixel_rel_err": worst_pix,
-        "fp32_vs_fp64_max_flux_rel_err": worst_flux,
+        "accuracy": accuracy,
         "refine_entries": int(info.n_refine_entries),
-        "refine_subcells": int(info.n_subcells),
+        "refine_evaluations": int(info.n_subcells),
+        "coarse_evaluations": int(wc.coarse_evaluations),
         "roofline": roofline,
+        "kernels": kernels,
         "cpu_baseline": cpu,
+        "models": model_row,
     }
     out.update(grad)
     batch.close()
     print(json.dumps(out), flush=True)
+    if not accuracy_ok:
+        log("FAILED: batched FP32 images beyond %g of the oracle, or a galaxy not visible in its stamp" % TOL_FP32)
+        sys.exit(3)
 
 
 if __name__ == "__main__":

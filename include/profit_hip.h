@@ -273,6 +273,8 @@ typedef struct pk_batch_info {
 	double ms_grad;             /* adjoint part of the last pk_batch_evaluate_grad */
 	uint32_t n_conv_launches;   /* convolution launches per evaluation (one per PSF-size class) */
 	uint32_t conv_tiles;        /* PK_CONV_TILES_* they used */
+	double ms_grad_refine;      /* refinement adjoint of the last pk_batch_evaluate_grad (eager mode) */
+	uint64_t n_grad_subcells;   /* evaluate_at calls (value and partials) of that refinement adjoint */
 } pk_batch_info;
 
 int pk_batch_create(pk_ctx *ctx, const pk_model *models, uint32_t n_models, uint32_t flags, pk_batch **out);
@@ -284,6 +286,16 @@ int pk_batch_evaluate(pk_batch *b);
 /* Waits for the last evaluation; fails with PK_ERANGE if a refinement stack
  * overflowed.  `info` may be NULL. */
 int pk_batch_sync(pk_batch *b, pk_batch_info *info);
+/* Work of one evaluation, for rooflines: (pixel, radial component) pairs the
+ * pixel pass classifies, evaluate_at calls it makes outside the refinement
+ * (counted by a separate classification pass launched here, with the last
+ * evaluation's component constants), and the refinement's evaluate_at calls
+ * and refined pixel/component pairs of the last evaluation.  No counterpart
+ * in the reference. */
+typedef struct pk_work_counts {
+	uint64_t pixel_component_pairs, coarse_evaluations, refine_evaluations, refined_pixels;
+} pk_work_counts;
+int pk_batch_work_counts(pk_batch *b, pk_work_counts *out);
 int pk_batch_image_dims(const pk_batch *b, uint32_t i, uint32_t *width, uint32_t *height);
 /* Copies image i (converted to FP64) to host memory. */
 int pk_batch_get_image(pk_batch *b, uint32_t i, double *out, size_t out_capacity);

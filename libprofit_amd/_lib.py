@@ -60,6 +60,7 @@ def load():
         "pk_batch_evaluate": (C.c_int, [C.c_void_p]),
         "pk_batch_sync": (C.c_int, [C.c_void_p, P(abi.BatchInfo)]),
         "pk_batch_image_dims": (C.c_int, [C.c_void_p, u32, P(u32), P(u32)]),
+        "pk_batch_work_counts": (C.c_int, [C.c_void_p, P(abi.WorkCounts)]),
         "pk_batch_get_image": (C.c_int, [C.c_void_p, u32, C.c_void_p, C.c_size_t]),
         "pk_batch_device_images": (C.c_int, [C.c_void_p, P(C.c_void_p), P(C.c_size_t)]),
         "pk_batch_image_offset": (C.c_int, [C.c_void_p, u32, P(C.c_uint64)]),

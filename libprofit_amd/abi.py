@@ -129,7 +129,15 @@ class BatchInfo(C.Structure):
         ("ms_grad", C.c_double),
         ("n_conv_launches", C.c_uint32),
         ("conv_tiles", C.c_uint32),
+        ("ms_grad_refine", C.c_double),
+        ("n_grad_subcells", C.c_uint64),
     ]
+
+
+class WorkCounts(C.Structure):
+    """pk_work_counts: work of one batch evaluation (rooflines)."""
+    _fields_ = [(n, C.c_uint64) for n in ("pixel_component_pairs", "coarse_evaluations", "refine_evaluations",
+                                          "refined_pixels")]
 
 
 PK_BATCH_GRAPH = 1

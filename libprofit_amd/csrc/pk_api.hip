@@ -908,10 +908,10 @@ int launch_profiles(hipStream_t s, bool fp32, const PkJob *djobs, uint32_t njobs
 {
 	if (fp32) {
 		hipLaunchKernelGGL((coarse_kernel<TI, true>), dim3(ntiles), dim3(256), 0, s, djobs, njobs, dcomp, dmask, img_nc,
-		                   img_c, dent, dcnt, dacc, tile_job);
+		                   img_c, dent, dcnt, dacc, tile_job, (uint32_t *)nullptr);
 	} else {
 		hipLaunchKernelGGL((coarse_kernel<TI, false>), dim3(ntiles), dim3(256), 0, s, djobs, njobs, dcomp, dmask,
-		                   img_nc, img_c, dent, dcnt, (unsigned long long *)nullptr, tile_job);
+		                   img_nc, img_c, dent, dcnt, (unsigned long long *)nullptr, tile_job, (uint32_t *)nullptr);
 	}
 	PK_HIP(hipGetLastError());
 	if (ev_coarse) PK_HIP(hipEventRecord(ev_coarse, s));
@@ -1487,6 +1487,8 @@ struct pk_batch {
 	size_t o_jobs = 0, o_setup = 0, o_comps = 0, o_cnt = 0, o_psfT = 0, o_mask = 0, o_nc = 0, o_c = 0, o_conv = 0,
 	       o_ent = 0, o_acc = 0, o_q = 0, o_out = 0, o_lite = 0;
 	size_t o_tj = 0;  // job of every tile: coarse tiles, then convolution tiles, then composition tiles
+	size_t o_wev = 0; // evaluate_at calls of every coarse-pass wavefront (pk_batch_work_counts)
+	size_t o_wsum = 0;
 	// convolution tiles are numbered by PSF-size class, one launch per class
 	// with the LDS its largest PSF needs (smaller PSFs: more workgroups per CU)
 	struct ConvClass {
@@ -1548,7 +1550,7 @@ struct pk_batch {
 	bool have_data = false, ran_grad = false;
 	hipGraph_t graph_grad = nullptr;
 	hipGraphExec_t exec_grad = nullptr;
-	hipEvent_t ev_grad[2] = {};
+	hipEvent_t ev_grad[3] = {};
 	// launch settings (autotuner)
 	pk_tuning tune = default_tuning();
 	bool tuned = false;
@@ -1711,6 +1713,7 @@ int grad_launch_t(pk_batch *b)
 		hipLaunchKernelGGL(grad_entry_weights_kernel<T>, dim3(1024), dim3(256), 0, s, djobs, dent,
 		                   (const PkCounters *)b->d<PkCounters>(b->o_cnt), (const T *)Gd, (const T *)Gc, gent);
 		PK_HIP(hipGetLastError());
+		if (timed) PK_HIP(hipEventRecord(b->ev_grad[2], s));
 		hipLaunchKernelGGL((refine32_kernel<0, false, true>), dim3(gm), dim3(64), 0, s, djobs, dcomp, dlite, dent,
 		                   gcnt, (const RCell *)nullptr, q0, b->qcap, 0, (unsigned long long *)nullptr, hd,
 		                   (const double *)gent, gacc);
@@ -1722,6 +1725,7 @@ int grad_launch_t(pk_batch *b)
 		                   (const RCell *)((R32_PASSES & 1) ? q1 : q0), (RCell *)nullptr, b->qcap, R32_PASSES - 1,
 		                   (unsigned long long *)nullptr, hd, (const double *)gent, gacc);
 	} else {
+		if (timed) PK_HIP(hipEventRecord(b->ev_grad[2], s));
 		hipLaunchKernelGGL((refine_kernel<double, false, true>), dim3(4096), dim3(64), 0, s, djobs,
 		                   (const PkComp *)dcomp, dent, gcnt, (double *)nullptr, (double *)nullptr,
 		                   (const double *)Gd, (const double *)Gc, gacc);
@@ -2300,6 +2304,8 @@ int pk_batch_create(pk_ctx *ctx, const pk_model *models, uint32_t n_models, uint
 	b->o_comps = L.take(sizeof(PkComp) * std::max<size_t>(1, b->setup.size()));
 	b->o_lite = L.take(sizeof(CompLite) * std::max<size_t>(1, b->setup.size()));
 	b->o_cnt = L.take(sizeof(PkCounters));
+	b->o_wev = L.take(sizeof(uint32_t) * 4 * std::max<size_t>(1, b->ntiles));
+	b->o_wsum = L.take(sizeof(unsigned long long));
 	b->o_nc = L.take(elem * img_off);
 	b->o_c = L.take(elem * (b->any_conv ? img_off : 1));
 	bool conv_buffer = false;  // some convolved job leaves its composition to compose_kernel
@@ -2602,6 +2608,11 @@ int pk_batch_sync(pk_batch *b, pk_batch_info *info)
 		std::memset(info, 0, sizeof(*info));
 		info->n_refine_entries = cnt.n_entries;
 		info->n_subcells = cnt.n_subcells;
+		if (b->ran_grad) {
+			PkCounters g{};
+			PK_HIP(hipMemcpy(&g, b->d<void>(b->o_gcnt), sizeof(g), hipMemcpyDeviceToHost));
+			info->n_grad_subcells = g.n_subcells;
+		}
 		info->n_conv_launches = (uint32_t)b->conv_classes.size();
 		info->conv_tiles = b->elem == 4 ? b->tune.conv_tiles : (uint32_t)PK_CONV_TILES_VALU;
 		if (b->ran && !(b->flags & PK_BATCH_GRAPH)) {
@@ -2619,14 +2630,62 @@ int pk_batch_sync(pk_batch *b, pk_batch_info *info)
 			info->ms_compose = t[4];
 			info->ms_total = t[5];
 			if (b->ran_grad) {
-				float tg = 0;
+				float tg = 0, tr = 0;
 				(void)hipEventElapsedTime(&tg, b->ev_grad[0], b->ev_grad[1]);
+				(void)hipEventElapsedTime(&tr, b->ev_grad[2], b->ev_grad[1]);
 				info->ms_grad = tg;
+				info->ms_grad_refine = tr;
 			}
 		}
 	}
 	if (cnt.overflow) return set_err(PK_ERANGE, "sub-pixel refinement work queue capacity exceeded");
 	if (cnt.n_bad) return set_err(PK_ERANGE, "sub-pixel refinement produced %u non-finite values", cnt.n_bad);
+	return PK_OK;
+}
+
+int pk_batch_work_counts(pk_batch *b, pk_work_counts *out)
+{
+	if (!b || !out) return set_err(PK_EINVAL, "NULL argument");
+	std::memset(out, 0, sizeof(*out));
+	for (const auto &jb : b->jobs) {
+		uint32_t nrad = 0;
+		for (uint32_t k = jb.comp_off; k < jb.comp_off + jb.ncomp; k++)
+			if (b->setup[k].c.kind != PK_PROFILE_SKY) nrad++;
+		out->pixel_component_pairs += (uint64_t)jb.dw * jb.dh * nrad;
+	}
+	if (!b->ntiles) return PK_OK;
+	std::lock_guard<std::mutex> lock(b->ctx->mu);
+	PK_HIP(hipSetDevice(b->ctx->device));
+	hipStream_t s = b->ctx->stream;
+	// the component constants of the last evaluation (set-up ran), the
+	// classification of the pixel pass with its evaluations counted
+	const PkJob *djobs = b->d<PkJob>(b->o_jobs);
+	const uint32_t *tj = b->d<uint32_t>(b->o_tj);
+	uint32_t *wev = b->d<uint32_t>(b->o_wev);
+	if (b->elem == 4)
+		hipLaunchKernelGGL((coarse_kernel<float, true, true>), dim3(b->ntiles), dim3(256), 0, s, djobs, b->n_models,
+		                   (const PkComp *)b->d<PkComp>(b->o_comps), (const uint8_t *)b->d<uint8_t>(b->o_mask),
+		                   (float *)nullptr, (float *)nullptr, (RefineEntry *)nullptr, (PkCounters *)nullptr,
+		                   (unsigned long long *)nullptr, tj, wev);
+	else
+		hipLaunchKernelGGL((coarse_kernel<double, false, true>), dim3(b->ntiles), dim3(256), 0, s, djobs, b->n_models,
+		                   (const PkComp *)b->d<PkComp>(b->o_comps), (const uint8_t *)b->d<uint8_t>(b->o_mask),
+		                   (double *)nullptr, (double *)nullptr, (RefineEntry *)nullptr, (PkCounters *)nullptr,
+		                   (unsigned long long *)nullptr, tj, wev);
+	PK_HIP(hipGetLastError());
+	unsigned long long *dsum = b->d<unsigned long long>(b->o_wsum);
+	PK_HIP(hipMemsetAsync(dsum, 0, sizeof(unsigned long long), s));
+	hipLaunchKernelGGL(sum_wave_evals_kernel, dim3(256), dim3(256), 0, s, (const uint32_t *)wev,
+	                   (uint64_t)4 * b->ntiles, dsum);
+	PK_HIP(hipGetLastError());
+	unsigned long long tot = 0;
+	PK_HIP(hipMemcpyAsync(&tot, dsum, sizeof(tot), hipMemcpyDeviceToHost, s));
+	PK_HIP(hipStreamSynchronize(s));
+	out->coarse_evaluations = tot;
+	PkCounters cnt{};
+	PK_HIP(hipMemcpy(&cnt, b->d<void>(b->o_cnt), sizeof(cnt), hipMemcpyDeviceToHost));
+	out->refine_evaluations = cnt.n_subcells;
+	out->refined_pixels = cnt.n_entries;
 	return PK_OK;
 }
 

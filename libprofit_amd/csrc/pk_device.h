@@ -67,14 +67,18 @@ constexpr int PX_RPT = 4;    // rows per lane
 // ---------------------------------------------------------------------------
 // coarse pass
 // ---------------------------------------------------------------------------
-template <typename TI, bool FP32>
+// COUNT: classification only -- no values, writes or appends; each
+// wavefront stores how many evaluate_at calls its pixels need (the work the
+// product instantiation, COUNT = false, does).
+template <typename TI, bool FP32, bool COUNT = false>
 __global__ __launch_bounds__(256) void coarse_kernel(const PkJob *__restrict__ jobs, uint32_t njobs,
                                                      const PkComp *__restrict__ comps,
                                                      const uint8_t *__restrict__ masks, TI *__restrict__ img_nc,
                                                      TI *__restrict__ img_c, RefineEntry *__restrict__ entries,
                                                      PkCounters *__restrict__ counters,
                                                      unsigned long long *__restrict__ acc,
-                                                     const uint32_t *__restrict__ tile_job)
+                                                     const uint32_t *__restrict__ tile_job,
+                                                     uint32_t *__restrict__ wave_evals)
 {
 	const uint32_t jn = find_job(jobs, njobs, blockIdx.x, 0, tile_job);
 	const PkJob &jb = jobs[jn];
@@ -93,6 +97,7 @@ __global__ __launch_bounds__(256) void coarse_kernel(const PkJob *__restrict__ j
 	}
 	const double hx = jb.sx / 2., hy = jb.sy / 2.;
 	const double x = hx + i * jb.sx;
+	uint32_t nev = 0;  // COUNT: evaluate_at calls of this lane
 	for (uint32_t c = 0; c < jb.ncomp; c++) {
 		const PkComp &cp = comps[jb.comp_off + c];
 #pragma unroll
@@ -108,6 +113,10 @@ __global__ __launch_bounds__(256) void coarse_kernel(const PkJob *__restrict__ j
 				const double r2 = xp * xp + yp * yp;
 				if (cp.rscale_max > 0 && radius_exceeds(r2, cp.r2_max, cp.rscale, cp.rscale_max)) continue;
 				if (cp.rough || radius_exceeds(r2, cp.r2_switch, cp.rscale, cp.rscale_switch)) {
+					if (COUNT) {
+						nev++;
+						continue;
+					}
 					float v32 = -1.f;
 					if (FP32 && cp.fp32) v32 = evaluate_at32(cp, (float)xp, (float)yp, radius2_64(cp, xp, yp));
 					if (v32 >= 0.f && v32 <= 3e38f) v = (double)v32 * cp.unfold;
@@ -121,6 +130,12 @@ __global__ __launch_bounds__(256) void coarse_kernel(const PkJob *__restrict__ j
 			if (cp.convolve) acc_c[r] += v;
 			else acc_nc[r] += v;
 		}
+	}
+	if (COUNT) {
+		// evaluation count of the wavefront, stored in its own slot
+		for (int off = 32; off > 0; off >>= 1) nev += __shfl_xor(nev, off);
+		if (lane_id() == 0) wave_evals[blockIdx.x * 4 + (threadIdx.x >> 6)] = nev;
+		return;
 	}
 	uint64_t ballot[PX_RPT];
 	uint32_t n_need = 0;
@@ -1115,6 +1130,18 @@ __global__ void reset_counters_kernel(PkCounters *c)
 			c->n_done[p] = 0;
 		}
 	}
+}
+
+// Sum of the per-wavefront counts of a COUNT pass into *out (cleared
+// before): grid-stride, one atomic per wavefront
+__global__ __launch_bounds__(256) void sum_wave_evals_kernel(const uint32_t *__restrict__ w, uint64_t n,
+                                                              unsigned long long *__restrict__ out)
+{
+	unsigned long long v = 0;
+	for (uint64_t k = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; k < n; k += (uint64_t)gridDim.x * blockDim.x)
+		v += w[k];
+	for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+	if (lane_id() == 0 && v) atomicAdd(out, v);
 }
 
 // ---------------------------------------------------------------------------

@@ -79,10 +79,10 @@ struct PkCounters {
 	uint32_t n_done[R32_PASSES];   // wavefronts of pass p that have stopped
 };
 
-// A cell waiting to be subdivided (FP32-mode refinement), 32 bytes.
+// A cell waiting to be subdivided (FP32-mode refinement), 32 bytes.  Its size
+// in pixels, resolution^-level, follows from `meta` (cell_size_fp64).
 struct alignas(16) RCell {
 	double dx0, dy0;       // lower corner relative to the component centre
-	float f;               // cell size in pixels: resolution^-level
 	uint32_t gcomp;        // index in the component table
 	uint32_t entry;        // pixel accumulator
 	uint32_t meta;         // level | resolution << 8 | max recursions << 16 | convolved << 24

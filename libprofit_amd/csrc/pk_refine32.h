@@ -102,6 +102,16 @@ __device__ inline unsigned long long wave_sum_u64(unsigned long long v)
 	return ((unsigned long long)hi << 32) | lo;
 }
 
+// Size in pixels of a cell of level `level`, res^-level in FP64 (res^level is
+// an exact integer: one rounding, where libprofit's level-by-level
+// (x1 - x0) / resolution takes one per level)
+__device__ inline double cell_size_fp64(int res, int level)
+{
+	double p = 1.0;
+	for (int l = 0; l < level; l++) p *= (double)res;
+	return 1.0 / p;
+}
+
 // cell metadata: level | res << 8 | max recursions << 16 | convolved << 24
 __device__ inline uint32_t cm_level(uint32_t m) { return m & 255u; }
 __device__ inline uint32_t cm_res(uint32_t m) { return (m >> 8) & 255u; }
@@ -185,7 +195,6 @@ __device__ inline void r32_expand(const PkJob *__restrict__ jobs, const PkComp *
 			RCell cl;
 			cl.dx0 = (x - hx) - cp.xcen;
 			cl.dy0 = (y - hy) - cp.ycen;
-			cl.f = 1.f;
 			cl.gcomp = gc;
 			cl.entry = e;
 			cl.meta = (res << 8) | (mr << 16) | ((cp.convolve ? 1u : 0u) << 24);
@@ -449,9 +458,13 @@ __global__ __launch_bounds__(64) void refine32_kernel(const PkJob *__restrict__ 
 		const uint32_t meta = cell.meta;
 		const int level = (int)cm_level(meta), res = (int)cm_res(meta), mr = (int)cm_maxrec(meta);
 		const float inv_res = 1.f / (float)res;
-		const float fr = cell.f * inv_res;
-		const double hx = (double)fr * cl.hsx, hy = (double)fr * cl.hsy;
-		const float wq = (float)((double)(fr * fr) * cl.wq);
+		// sub-cell size in pixels, in FP64 as libprofit forms the sub-pixel
+		// grid: an FP32 size would move the sub-cell centres by ~1e-8 px, which
+		// next to a cusp (coresersic, f ~ r^-b) changes the innermost values
+		// by ~1e-4
+		const double fr = cell_size_fp64(res, level + 1);
+		const double hx = fr * cl.hsx, hy = fr * cl.hsy;
+		const float wq = (float)(fr * fr * cl.wq);
 		const bool recurse = res > 1 && level < mr;
 		const float acc_thr = (float)cl.acc;
 		const float band = 1e-4f * (1.f + acc_thr);
@@ -537,7 +550,7 @@ __global__ __launch_bounds__(64) void refine32_kernel(const PkJob *__restrict__ 
 							const double val =
 							    refine_fp64_decision(comps[cell.gcomp], xp[q], yp[q], ddy, recurse, &fail[q]);
 							if (!GRAD && !fail[q]) {
-								const double qv = val * ((double)(fr * fr) * cl.wflux);
+								const double qv = val * (fr * fr * cl.wflux);
 								if (qv >= 0 && qv < 9.2e18) fxv += (unsigned long long)__double2ll_rn(qv);
 								else bad++;
 							}
@@ -558,7 +571,7 @@ __global__ __launch_bounds__(64) void refine32_kernel(const PkJob *__restrict__ 
 							any_ok = true;
 							float gt[GT_COUNT];
 							const float vg = sersic_grad32(cl, (float)xp[q], (float)yp[q], gt);
-							grad_accumulate(ga, gw * ((double)(fr * fr) * cl.wq) * (double)vg, gt);
+							grad_accumulate(ga, gw * (fr * fr * cl.wq) * (double)vg, gt);
 						}
 					}
 					const uint64_t okb = __ballot(any_ok);
@@ -622,7 +635,6 @@ __global__ __launch_bounds__(64) void refine32_kernel(const PkJob *__restrict__ 
 					RCell c;
 					c.dx0 = dx - hx;
 					c.dy0 = dy[q] - hy;
-					c.f = fr;
 					c.gcomp = cell.gcomp;
 					c.entry = cell.entry;
 					c.meta = (meta & ~255u) | (uint32_t)(level + 1);

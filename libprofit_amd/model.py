@@ -450,6 +450,12 @@ class Batch:
         check(self.L.pk_batch_sync(self.handle, C.byref(info)))
         return info
 
+    def work_counts(self):
+        """pk_batch_work_counts: work of the last evaluation."""
+        wc = abi.WorkCounts()
+        check(self.L.pk_batch_work_counts(self.handle, C.byref(wc)))
+        return wc
+
     def set_components(self, comps):
         arr = (abi.Component * len(comps))(*comps)
         check(self.L.pk_batch_set_components(self.handle, arr, len(comps)))

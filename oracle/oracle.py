@@ -13,12 +13,24 @@ from libprofit_amd import abi
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 LIB = os.path.join(HERE, "_build", "liboracle.so")
+LIB_FAST = os.path.join(HERE, "_build", "liboracle_fast.so")
 
 _lib = None
+_fast = None
 
 
 def build():
     subprocess.check_call(["make", "-s", "-C", HERE])
+
+
+def fast_lib():
+    """The same oracle built at -O3 -march=native on this host (the CPU
+    baseline); rebuilt on first use."""
+    global _fast
+    if _fast is None:
+        subprocess.check_call(["make", "-s", "-C", HERE, "fast"])
+        _fast = _bind(C.CDLL(LIB_FAST))
+    return _fast
 
 
 def lib():
@@ -26,28 +38,31 @@ def lib():
     if _lib is None:
         if not os.path.exists(LIB):
             build()
-        L = C.CDLL(LIB)
-        L.or_gammafn.restype = C.c_double
-        L.or_gammafn.argtypes = [C.c_double]
-        L.or_beta.restype = C.c_double
-        L.or_beta.argtypes = [C.c_double, C.c_double]
-        L.or_pgamma.restype = C.c_double
-        L.or_pgamma.argtypes = [C.c_double, C.c_double]
-        L.or_qgamma.restype = C.c_double
-        L.or_qgamma.argtypes = [C.c_double, C.c_double]
-        L.or_last_error.restype = C.c_char_p
-        L.or_stat_refined.restype = C.c_uint64
-        L.or_stat_subcells.restype = C.c_uint64
-        L.or_component_eval_at.restype = C.c_double
-        L.or_component_eval_at.argtypes = [C.c_void_p, C.c_uint, C.c_double, C.c_double]
-        L.or_set_threads.argtypes = [C.c_int]
-        L.or_set_integrator.argtypes = [C.c_int]
-        L.or_quadpack_test.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
-                                       C.c_uint, C.c_void_p]
-        L.or_quadpack.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_double,
-                                  C.c_double, C.c_uint, C.c_void_p]
-        _lib = L
+        _lib = _bind(C.CDLL(LIB))
     return _lib
+
+
+def _bind(L):
+    L.or_gammafn.restype = C.c_double
+    L.or_gammafn.argtypes = [C.c_double]
+    L.or_beta.restype = C.c_double
+    L.or_beta.argtypes = [C.c_double, C.c_double]
+    L.or_pgamma.restype = C.c_double
+    L.or_pgamma.argtypes = [C.c_double, C.c_double]
+    L.or_qgamma.restype = C.c_double
+    L.or_qgamma.argtypes = [C.c_double, C.c_double]
+    L.or_last_error.restype = C.c_char_p
+    L.or_stat_refined.restype = C.c_uint64
+    L.or_stat_subcells.restype = C.c_uint64
+    L.or_component_eval_at.restype = C.c_double
+    L.or_component_eval_at.argtypes = [C.c_void_p, C.c_uint, C.c_double, C.c_double]
+    L.or_set_threads.argtypes = [C.c_int]
+    L.or_set_integrator.argtypes = [C.c_int]
+    L.or_quadpack_test.argtypes = [C.c_int, C.c_int, C.c_double, C.c_double, C.c_double, C.c_double, C.c_double,
+                                   C.c_uint, C.c_void_p]
+    L.or_quadpack.argtypes = [C.c_void_p, C.c_void_p, C.c_int, C.c_double, C.c_double, C.c_double,
+                              C.c_double, C.c_uint, C.c_void_p]
+    return L
 
 
 class OracleError(Exception):
@@ -114,18 +129,20 @@ def output_dims(spec):
     return w.value, h.value
 
 
-def evaluate(spec):
-    """Model::evaluate -> (image[h, w] float64, (offset_x, offset_y))."""
+def evaluate(spec, L=None):
+    """Model::evaluate -> (image[h, w] float64, (offset_x, offset_y)).  `L`:
+    another build of the oracle (fast_lib()) instead of the parity one."""
+    L = L or lib()
     m = spec.m
     fs = max(1, m.finesampling)
     cap = 4 * (m.width * fs + (m.psf_width or 0) * 2 + 4) * (m.height * fs + (m.psf_height or 0) * 2 + 4) + 64
     cap = max(cap, 4 * max(m.width * fs, m.psf_width) ** 2 * 4)
     out = np.zeros(cap, dtype=np.float64)
     w, h, ox, oy = C.c_uint(), C.c_uint(), C.c_int(), C.c_int()
-    st = lib().or_evaluate(spec.ref, out.ctypes.data_as(C.c_void_p), C.c_size_t(cap), C.byref(w), C.byref(h),
-                           C.byref(ox), C.byref(oy))
+    st = L.or_evaluate(spec.ref, out.ctypes.data_as(C.c_void_p), C.c_size_t(cap), C.byref(w), C.byref(h),
+                       C.byref(ox), C.byref(oy))
     if st:
-        raise OracleError(st, lib().or_last_error().decode())
+        raise OracleError(st, L.or_last_error().decode())
     return out[: w.value * h.value].reshape(h.value, w.value).copy(), (ox.value, oy.value)
 
 

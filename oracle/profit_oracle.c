@@ -18,8 +18,8 @@
  *     include/profit/null.h;
  *   - Model::evaluate: src/model.cpp:152-360;
  *   - convolvers: src/convolve.cpp:55-312 (BRUTE_OLD, BRUTE with the
- *     dot_sw<4> summation order of include/profit/dot_product.h:62-160, FFT
- *     with an own radix-2 transform in place of FFTW).
+ *     summation order of dot_product<AVX>, include/profit/dot_product.h:
+ *     272-384, FFT with an own radix-2 transform in place of FFTW).
  *
  * Compiled with -ffp-contract=off so that products and sums round like the
  * reference's default x86-64 build (no FMA contraction).
@@ -1374,24 +1374,50 @@ static void or_brute_old(const double *src, unsigned W, unsigned H, const double
 	}
 }
 
-/* dot_sw<4> of include/profit/dot_product.h:62-160 */
-static inline double or_dot4(const double *s, const double *k, size_t n)
+/* dot_product<AVX> (include/profit/dot_product.h:272-384), the product
+ * create_convolver(BRUTE) picks on an AVX host (src/convolve.cpp:527-531):
+ * four lane accumulators; 8-element batches add (s[0:4]k[0:4] +
+ * s[4:8]k[4:8]) into them; a remainder of 4, 2 and 1 element(s) adds into
+ * lanes 0-3, 0-1 and 0; the lanes are then summed as (l2 + l3) + (l0 + l1).
+ * Written with GCC vector types, so an -O3 -march=native build runs it in
+ * vector registers. */
+typedef double or_v4d __attribute__((vector_size(32)));
+
+static inline or_v4d or_ld4(const double *p)
 {
-	double buf = 0;
-	size_t nb = n / 4;
+	or_v4d v;
+	memcpy(&v, p, sizeof(v));
+	return v;
+}
+
+static inline double or_dot_avx(const double *s, const double *k, size_t n)
+{
+	or_v4d acc = {0, 0, 0, 0};
+	size_t nb = n / 8;
 	for (size_t b = 0; b < nb; b++) {
-		const double *a = s + 4 * b, *c = k + 4 * b;
-		double t1 = a[0] * c[0], t2 = a[1] * c[1], t3 = a[2] * c[2], t4 = a[3] * c[3];
-		buf += (t1 + t3) + (t2 + t4);
+		const double *a = s + 8 * b, *c = k + 8 * b;
+		acc = (or_ld4(a) * or_ld4(c) + or_ld4(a + 4) * or_ld4(c + 4)) + acc;
 	}
-	size_t rem = n % 4;
+	size_t rem = n % 8;
 	const double *sr = s + n - rem, *kr = k + n - rem;
-	double r = 0;
-	if (rem == 3) r += (sr[0] * kr[0] + sr[1] * kr[1]) + sr[2] * kr[2];
-	else if (rem == 2) r += sr[0] * kr[0] + sr[1] * kr[1];
-	if (rem == 1) r += sr[0] * kr[0];
-	buf += r;
-	return buf;
+	if (rem >= 4) {
+		acc = or_ld4(sr) * or_ld4(kr) + acc;
+		sr += 4;
+		kr += 4;
+		rem -= 4;
+	}
+	if (rem >= 2) {
+		or_v4d t = {sr[0] * kr[0], sr[1] * kr[1], 0, 0};
+		acc = t + acc;
+		sr += 2;
+		kr += 2;
+		rem -= 2;
+	}
+	if (rem == 1) {
+		or_v4d t = {sr[0] * kr[0], 0, 0, 0};
+		acc = t + acc;
+	}
+	return (acc[2] + acc[3]) + (acc[0] + acc[1]);
 }
 
 /* AssociativeBruteForceConvolver::convolve_impl (src/convolve.cpp:158-231) */
@@ -1421,7 +1447,7 @@ static void or_brute(const double *src, unsigned W, unsigned H, const double *kr
 		ko += kmin + (size_t)lmin * kw;
 		double pixel = 0;
 		for (unsigned l = 0; l < lmax - lmin; l++) {
-			pixel += or_dot4(src + so, ik + ko, kmax - kmin);
+			pixel += or_dot_avx(src + so, ik + ko, kmax - kmin);
 			so += W;
 			ko += kw;
 		}

@@ -67,7 +67,8 @@ def oracle_fd_grad(oracle, fm, theta, data, sigma, rel=1e-4, with_error=False):
 
     With with_error: returns (g, err, g_rel, chi2): g the differences at a
     relative step chosen per parameter (below), err = |g(h) - g(2h)| a bound
-    of their truncation and rounding error, g_rel those at `rel`.  The image
+    of their truncation and rounding error (the larger difference to the
+    steps 2h and h/2), g_rel those at `rel`.  The image
     of a Sersic profile with nser > 1 sampled on a fixed refinement pattern
     has a cusp that moves across sub-cells down to ~1e-9 px when its centre
     moves, so for those positions the differences converge to the
@@ -101,9 +102,12 @@ def oracle_fd_grad(oracle, fm, theta, data, sigma, rel=1e-4, with_error=False):
             cusp = fm.kinds[int(ci)] == "sersic" and pname in ("xcen", "ycen") and \
                 fm.base[int(ci)]["nser"] > 1 if "nser" in fm.base[int(ci)] else False
             r = 1e-9 if cusp else 1e-6
-            h = fd_steps(theta, None, r, r)[k]
+            # relative step, but at least r in absolute terms: below that the
+            # chi^2 differences of a small parameter (box ~ 0.05) are a few
+            # ulps and can come out exactly proportional to the step
+            h = r * max(abs(theta[k]), 1.0)
             g[k] = central(k, h)
-            err[k] = abs(g[k] - central(k, 2 * h))
+            err[k] = max(abs(g[k] - central(k, 2 * h)), abs(g[k] - central(k, 0.5 * h)))
     finally:
         oracle.pattern("off")
     if with_error:

@@ -98,3 +98,21 @@ def check_image(got, spec, ref, oracle, tol, what=""):
         share = galaxy_share(spec, ref, oracle)
         assert share >= 1e3 * tol, (what, "galaxy share of the pixels", share)
     return pix, flux
+
+
+def stratified_sample(specs, n, seed=12345):
+    """Indices of up to n models: for every PSF side its smallest and its
+    largest stamp, the rest drawn at random (seeded)."""
+    by_k = {}
+    for i, s in enumerate(specs):
+        by_k.setdefault(int(s.m.psf_width), []).append(i)
+    pick = set()
+    for _k, idx in sorted(by_k.items()):
+        idx = sorted(idx, key=lambda i: (specs[i].m.width, i))
+        pick.update((idx[0], idx[-1]))
+    rng = np.random.Generator(np.random.PCG64(seed))
+    for i in rng.permutation(len(specs)).tolist():
+        if len(pick) >= n:
+            break
+        pick.add(i)
+    return sorted(pick)

@@ -497,3 +497,44 @@ def test_one_shot_entry_points(ctx32, ctx64, oracle):
     assert np.allclose(c1, c_ref, rtol=1e-12, atol=0)
     scale = np.max(np.abs(g_ref), axis=1, keepdims=True)
     assert np.all(np.abs(g1 - g_ref) <= 1e-11 * np.maximum(scale, 1e-300))
+
+
+# coresersic components whose cusp (f ~ r^-b, b ~ 1.4) sits inside a pixel
+# refined to 10^4 sub-cells: the FP32 refinement once carried cell sizes in
+# FP32, which moved the sub-cell centres ~1e-8 px off the sub-pixel grid and
+# the cusp pixel by up to 5e-4 (drawn from the 4096-galaxy workload)
+CUSPS = [
+    (283, dict(xcen=115.67246823524668, ycen=161.5464776775312, mag=18.380238625301047, ang=58.6569794054999,
+               axrat=0.6097819655056684, re=18.046735457838764, rb=4.722559233079145, nser=4.000359166456292,
+               a=1.8206748251676907, b=1.4844564321178517)),
+    (73, dict(xcen=34.04079217214631, ycen=34.34578195114746, mag=19.760283438898483, ang=87.90286770678021,
+              axrat=0.3904136910368339, re=14.166685941225545, rb=3.948601214478852, nser=3.475679002910674,
+              a=1.1146656669163968, b=1.356464158287992)),
+    (111, dict(xcen=54.744727068819024, ycen=44.46922342158413, mag=19.4157668075907, ang=28.039734483398696,
+               axrat=0.9124329192959093, re=13.26279550549425, rb=4.865262260358035, nser=4.459245400184846,
+               a=2.203062723288399, b=1.4102444686436317)),
+]
+
+
+def test_coresersic_cusp_pixels_fp32(ctx32, oracle):
+    """The pixels holding a coresersic cusp, at the adjusted sub-pixel
+    settings and at fixed resolutions 10 and 12 with 2 and 3 levels, in the
+    FP32 batch and the single FP32 evaluation, against the oracle."""
+    print("CASE coresersic cusps fp32", flush=True)
+    from libprofit_amd.model import evaluate_spec
+    specs = []
+    for size, p in CUSPS:
+        specs.append(abi.ModelSpec(size, size, [abi.make_component("coresersic", **p)], magzero=30.0))
+        for res, mr in ((10, 2), (12, 3)):
+            specs.append(abi.ModelSpec(size, size, [abi.make_component(
+                "coresersic", adjust=False, resolution=res, max_recursions=mr, rscale_switch=1.0, rscale_max=100.0,
+                **p)], magzero=30.0))
+    b = Batch(ctx32, specs, graph=False)
+    b.evaluate()
+    b.sync()
+    for i, s in enumerate(specs):
+        ref, _ = oracle.evaluate(s)
+        check_image(b.image(i), s, ref, oracle, TOL_FP32, ("batch", i))
+        single, _, _ = evaluate_spec(ctx32, s)
+        check_image(single, s, ref, oracle, TOL_FP32, ("single", i))
+    b.close()

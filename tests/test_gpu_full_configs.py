@@ -1,0 +1,126 @@
+"""The batched workloads at their full configured size.
+
+galaxy4096: 4096 galaxies, square stamps with side U{32..512}, 1-3 components
+of the six radial profiles (magzero 30) + sky, per-galaxy Moffat PSF of odd
+side U{15..41}.  The FP64 batch is compared with the oracle (1e-12) and the
+FP32 batch with the oracle (1e-5) on a sample of 72 galaxies covering every
+PSF side's smallest and largest stamp; the FP32 batch is then compared with
+the FP64 batch on all 4096 images (1e-5 per pixel above 1e-6 of the peak and
+in total flux), and every image is checked for finite, non-negative pixels.
+The total flux of an image is not a property of the model by itself: outside
+the refinement radius libprofit samples profiles at pixel centres, which for
+compact profiles does not conserve flux (a ferrer of rout 7.9 px and axrat
+0.24 in this batch draws 0.97 to 1.015 of its nominal flux depending on the
+pixel grid's phase), so the flux is checked against the oracle-pinned FP64
+image.
+
+fits1024: 1024 bulge + disk + sky fits of 128x128 with a 31x31 PSF.  The FP64
+adjoint is compared with frozen-pattern differences of the oracle on a sample
+of fits; the FP32 adjoint is compared with the FP64 one on all 1024 fits.
+The FP32 gradient's error follows from the FP32 image's (~1e-6 of a pixel,
+systematic) times the cancellation among the per-pixel terms of a gradient
+component: a component near its optimum (a small sum of large terms of both
+signs) carries a relative error far above 1e-6, while the error relative to
+the fit's largest gradient component stays below 1e-4 (measured 7.6e-5 at
+most, 1.4e-6 median); the bound enforced is 2e-4 of that component, and
+2e-4 of the gradient's norm."""
+import numpy as np
+import pytest
+
+from libprofit_amd import abi, workloads
+from libprofit_amd.model import Batch
+from grad_util import FitModel, oracle_fd_grad
+from parity_util import TOL_FP32, TOL_FP64, check_image, rel_errors, stratified_sample
+
+pytestmark = pytest.mark.gpu
+
+GRAD_TOL_FP32 = 2e-4
+
+
+def test_galaxy4096_full_size(ctx32, ctx64, oracle):
+    print("CASE galaxy4096 full size", flush=True)
+    oracle.set_threads(16)
+    specs = workloads.galaxy_batch(4096, 0)
+    b32 = Batch(ctx32, specs, graph=False)
+    b32.evaluate()
+    b32.sync()  # raises on a work-queue overflow or non-finite refinement values
+    b64 = Batch(ctx64, specs, graph=False)
+    b64.evaluate()
+    b64.sync()
+    sample = stratified_sample(specs, 72)
+    print("  sample of %d galaxies, PSF sides %s" % (len(sample), sorted({specs[i].m.psf_width for i in sample})),
+          flush=True)
+    for i in sample:
+        ref, _ = oracle.evaluate(specs[i])
+        check_image(b64.image(i), specs[i], ref, oracle, TOL_FP64, ("fp64", i))
+        check_image(b32.image(i), specs[i], ref, oracle, TOL_FP32, ("fp32", i))
+    worst_pix = worst_flux = 0.0
+    for i, s in enumerate(specs):
+        g32, g64 = b32.image(i), b64.image(i)
+        assert g32.shape == (s.m.height, s.m.width)
+        assert np.all(np.isfinite(g32)) and np.all(g32 >= 0), i
+        pix, flux = rel_errors(g32, g64)
+        worst_pix, worst_flux = max(worst_pix, pix), max(worst_flux, flux)
+    print("  all 4096: FP32 vs FP64 max pixel %.3g, max flux %.3g" % (worst_pix, worst_flux), flush=True)
+    assert worst_pix <= TOL_FP32 and worst_flux <= TOL_FP32
+    b32.close()
+    b64.close()
+
+
+def _fit_model(spec):
+    comps = []
+    for c in range(spec.m.n_components):
+        cp = spec.m.components[c]
+        if cp.kind == abi.PROFILE_KINDS["sky"]:
+            comps.append(("sky", dict(bg=cp.p[0])))
+        else:
+            comps.append(("sersic", dict(xcen=cp.xcen, ycen=cp.ycen, mag=cp.mag, re=cp.p[0], nser=cp.p[1],
+                                         ang=cp.ang, axrat=cp.axrat, box=cp.box)))
+    return FitModel(spec.m.width, comps, spec._psf, magzero=spec.m.magzero)
+
+
+def test_fits1024_full_size(ctx32, ctx64, oracle):
+    print("CASE fits1024 full size", flush=True)
+    oracle.set_threads(16)
+    start, true = workloads.fit_batch(1024, seed=0)
+    tb = Batch(ctx64, true, graph=False)
+    tb.evaluate()
+    tb.sync()
+    rng = np.random.Generator(np.random.PCG64(0))
+    datas, sigmas = [], []
+    for i in range(len(true)):
+        d, sg = workloads.noisy(tb.image(i), rng)
+        datas.append(d)
+        sigmas.append(sg)
+    tb.close()
+    res = {}
+    for name, ctx in (("fp32", ctx32), ("fp64", ctx64)):
+        b = Batch(ctx, start, graph=False, grad=True)
+        b.set_data(datas, sigmas)
+        b.evaluate_grad()
+        res[name] = b.chi2_grad()
+        b.close()
+    c32, g32 = res["fp32"]
+    c64, g64 = res["fp64"]
+    n = len(start)
+    g32 = g32.reshape(n, -1)
+    g64 = g64.reshape(n, -1)
+    # FP64 adjoint against the oracle's frozen-pattern differences on a sample
+    for i in (0, 28, 511, 1023):
+        fm = _fit_model(start[i])
+        g_fd, fd_err, _g_rel, c0 = oracle_fd_grad(oracle, fm, fm.theta(), datas[i], sigmas[i], with_error=True)
+        assert c64[i] == pytest.approx(c0, rel=1e-9)
+        adj = fm.flat_grad(g64[i].reshape(-1, abi.GRAD_SLOTS))
+        lim = 1e-7 * np.abs(g_fd) + 2 * fd_err
+        bad = [nm for nm, a, f, l in zip(fm.names(), adj, g_fd, lim) if abs(a - f) > l]
+        assert not bad, (i, bad)
+    # FP32 adjoint against the FP64 one on all fits
+    top = np.abs(g64).max(axis=1)
+    elem = np.abs(g32 - g64).max(axis=1) / top
+    norm = np.linalg.norm(g32 - g64, axis=1) / np.linalg.norm(g64, axis=1)
+    chi = np.abs(c32 - c64) / np.abs(c64)
+    print("  FP32 vs FP64 adjoint over %d fits: largest error / largest component: median %.2e max %.2e; "
+          "norm: max %.2e; chi2: max %.2e" % (n, np.median(elem), elem.max(), norm.max(), chi.max()), flush=True)
+    assert elem.max() <= GRAD_TOL_FP32
+    assert norm.max() <= GRAD_TOL_FP32
+    assert chi.max() <= TOL_FP32
