@@ -291,17 +291,23 @@ def grad_bench(args, Batch, Context, evaluate_spec, ctx32, ctx64):
     b64.evaluate_grad()
     chi2_64, grad_64 = b64.chi2_grad()
     b64.close()
+    # per fit: largest FP32 error over the largest FP64 gradient component,
+    # and the error norm over the gradient norm (the full-size test bounds
+    # both by 2e-4); per-element relative errors are large only for
+    # components near zero (terms that cancel near their optimum)
+    n = len(start)
+    g32, g64 = grad_32.reshape(n, -1), grad_64.reshape(n, -1)
+    top = np.abs(g64).max(axis=1)
+    elem = np.abs(g32 - g64).max(axis=1) / top
+    norm = np.linalg.norm(g32 - g64, axis=1) / np.linalg.norm(g64, axis=1)
     nz = np.abs(grad_64) > 0
     rel = np.abs(grad_32[nz] - grad_64[nz]) / np.abs(grad_64[nz])
-    out["fits_fp32_vs_fp64_grad_rel_err"] = {"median": float(np.median(rel)), "p99": float(np.percentile(rel, 99)),
-                                             "max": float(np.max(rel))}
-    # per fit, relative to the largest gradient component of the same parameter kind
-    nrm = []
-    for f in range(len(start)):
-        a, b = grad_32[3 * f:3 * f + 3], grad_64[3 * f:3 * f + 3]
-        scale = np.maximum(np.abs(b[:2]).max(axis=0), 1e-300)
-        nrm.append(float(np.max(np.abs(a[:2] - b[:2]) / scale)))
-    out["fits_fp32_vs_fp64_grad_max_err_vs_param_scale"] = float(np.max(nrm))
+    out["fits_fp32_vs_fp64_grad"] = {
+        "max_error_over_largest_component": {"median": float(np.median(elem)), "max": float(elem.max())},
+        "error_norm_over_gradient_norm": {"median": float(np.median(norm)), "max": float(norm.max())},
+        "per_element_rel_err": {"median": float(np.median(rel)), "p99": float(np.percentile(rel, 99)),
+                                "max": float(np.max(rel))},
+        "bound": 2e-4}
     out["fits_fp32_vs_fp64_chi2_max_rel_err"] = float(np.max(np.abs(chi2_32 - chi2_64) / np.abs(chi2_64)))
     # plain FP64 path: central differences of pk_evaluate on a sample of fits
     ns = min(args.grad_sample, len(start))
