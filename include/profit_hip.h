@@ -205,6 +205,25 @@ int pk_evaluate(pk_ctx *ctx, const pk_model *m, double *out, size_t out_capacity
  * there. */
 int pk_evaluate_device(pk_ctx *ctx, const pk_model *m, double *out_dev, size_t out_capacity, pk_result *res);
 
+/* pk_evaluate_device that allocates nothing: no device memory, no pinned
+ * host memory (the context's own workspace is not used either).  No
+ * counterpart in the reference: for callers that cannot allocate -- code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc.  Its workspace is a fixed array in device memory of
+ * PK_NOALLOC_SCRATCH_BYTES per device (held from the library's load until the
+ * process ends) and a host staging area of PK_NOALLOC_STAGE_BYTES; a model
+ * needing more fails with PK_ERANGE.  Roughly: 28 bytes per drawing pixel
+ * in FP64 (44 with a PSF: up to about 1700 x 1700 drawing pixels), or 64 MiB
+ * of refinement queues + 48 bytes per drawing pixel in FP32 (about
+ * 1150 x 1150), plus the FFT buffer of an FFT-convolved model, must fit the
+ * former; the mask, PSF and model description the latter.  Calls
+ * are serialised by a process-wide lock; the workspace keeps the last call's
+ * contents.  Same results as pk_evaluate_device. */
+#define PK_NOALLOC_SCRATCH_BYTES (128u << 20)
+#define PK_NOALLOC_STAGE_BYTES (16u << 20)
+int pk_evaluate_device_noalloc(pk_ctx *ctx, const pk_model *m, double *out_dev, size_t out_capacity,
+                               pk_result *res);
+
 /* Convolver::convolve(src, krn, mask, crop, offset_out)
  * (include/profit/convolve.h:98-99, src/convolve.cpp:55-79) for a convolver of
  * type `type`; the kernel is used as given (no normalisation). */

@@ -66,6 +66,7 @@ def load():
         "pk_batch_image_offset": (C.c_int, [C.c_void_p, u32, P(C.c_uint64)]),
         "pk_batch_image_origin": (C.c_int, [C.c_void_p, u32, P(C.c_int32), P(C.c_int32)]),
         "pk_evaluate_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p]),
+        "pk_evaluate_device_noalloc": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p]),
         "pk_evaluate_batch": (C.c_int, [C.c_void_p, C.c_void_p, u32, C.c_void_p, C.c_void_p, C.c_void_p]),
         "pk_chi2_grad": (C.c_int, [C.c_void_p, C.c_void_p, u32, C.c_void_p, C.c_void_p, C.c_size_t, C.c_void_p,
                                    C.c_void_p]),

@@ -76,11 +76,17 @@ struct pk_ctx {
 	uint32_t r32_grid_main = 0, r32_grid_last = 0;
 	uint32_t r32g_grid_main = 0, r32g_grid_last = 0;  // the same for the adjoint variants
 	std::string arch;  // device name and CU count (autotuner cache key)
+	// set while pk_evaluate_device_noalloc runs: dmem / hmem are then the
+	// library's fixed workspace, and nothing may be (re)allocated
+	bool fixed_ws = false;
 };
 
 static int ensure_dev(pk_ctx *ctx, size_t bytes)
 {
 	if (bytes <= ctx->dcap) return PK_OK;
+	if (ctx->fixed_ws)
+		return set_err(PK_ERANGE, "the model needs %zu bytes of device workspace, the allocation-free entry point "
+		                          "has %zu (PK_NOALLOC_SCRATCH_BYTES)", bytes, ctx->dcap);
 	if (ctx->dmem) PK_HIP(hipFree(ctx->dmem));
 	ctx->dmem = nullptr;
 	ctx->dcap = 0;
@@ -96,6 +102,9 @@ static int ensure_dev(pk_ctx *ctx, size_t bytes)
 static int ensure_host(pk_ctx *ctx, size_t bytes)
 {
 	if (bytes <= ctx->hcap) return PK_OK;
+	if (ctx->fixed_ws)
+		return set_err(PK_ERANGE, "the model needs %zu bytes of host staging, the allocation-free entry point has "
+		                          "%zu (PK_NOALLOC_STAGE_BYTES)", bytes, ctx->hcap);
 	if (ctx->hmem) PK_HIP(hipHostFree(ctx->hmem));
 	ctx->hmem = nullptr;
 	ctx->hcap = 0;
@@ -1056,7 +1065,8 @@ int evaluate_impl(pk_ctx *ctx, const pk_model *m, const Analysis &a, double *out
 	const size_t o_fz = L.take(use_fft ? sizeof(double2) * ((size_t)fg.P * fg.Q + 1) : 1);
 	const size_t o_ftw = L.take(use_fft ? sizeof(double2) * FFT_TWIDDLES : 1);
 	PK_TRY(ensure_dev(ctx, L.off));
-	PK_TRY(ensure_host(ctx, std::max(in_bytes, sizeof(double) * (size_t)ow * oh + 256 + sizeof(PkCounters))));
+	PK_TRY(ensure_host(ctx, std::max(in_bytes, (device_out ? 0 : sizeof(double) * (size_t)ow * oh + 256) + 256 +
+	                                               sizeof(PkCounters))));
 
 	jb.mask_off = use_mask ? 0 : NONE;
 	jb.psf_off = 0;
@@ -1139,7 +1149,10 @@ int evaluate_impl(pk_ctx *ctx, const pk_model *m, const Analysis &a, double *out
 
 	// download result + counters
 	const size_t out_bytes = sizeof(double) * (size_t)ow * oh;
-	const size_t ho_cnt = (out_bytes + 255) & ~(size_t)255;
+	// counters after the staged image (host output), or at the start of the
+	// staging area (device output: the input upload before it in stream order
+	// has finished by then)
+	const size_t ho_cnt = device_out ? 0 : (out_bytes + 255) & ~(size_t)255;
 	if (device_out)
 		PK_HIP(hipMemcpyAsync(out, dptr<void>(ctx, o_out), out_bytes, hipMemcpyDeviceToDevice, s));
 	else
@@ -1171,6 +1184,22 @@ int evaluate_impl(pk_ctx *ctx, const pk_model *m, const Analysis &a, double *out
 	}
 	return PK_OK;
 }
+
+// Fills n doubles with v (grid-stride).
+__global__ void fill_kernel(double *p, size_t n, double v)
+{
+	for (size_t k = blockIdx.x * (size_t)blockDim.x + threadIdx.x; k < n; k += (size_t)gridDim.x * blockDim.x) p[k] = v;
+}
+
+// ---- allocation-free single evaluation ---------------------------------------
+// The workspace of pk_evaluate_device_noalloc: one array per device that
+// loads this library (held until the process ends) and a host staging area.
+// Host code uses the address hipGetSymbolAddress reports for the current
+// device, never the array's name.
+static __device__ __align__(256) unsigned char scratch[PK_NOALLOC_SCRATCH_BYTES];
+alignas(256) static unsigned char g_noalloc_stage[PK_NOALLOC_STAGE_BYTES];
+static std::mutex g_noalloc_mu;           // one allocation-free evaluation at a time in the process
+static std::map<int, void *> g_scratch_ptr;  // device -> address of `scratch` on it
 
 }  // namespace
 
@@ -1257,6 +1286,68 @@ int pk_evaluate(pk_ctx *ctx, const pk_model *m, double *out, size_t out_capacity
 		return evaluate_impl<float>(ctx, m, a, out, out_capacity, res);
 	}
 	return evaluate_impl<double>(ctx, m, a, out, out_capacity, res);
+}
+
+int pk_evaluate_device_noalloc(pk_ctx *ctx, const pk_model *m, double *out_dev, size_t out_capacity,
+                               pk_result *res)
+{
+	if (!ctx) return set_err(PK_EINVAL, "context is NULL");
+	if (!m) return set_err(PK_EINVAL, "model is NULL");
+	if (!out_dev) return set_err(PK_EINVAL, "output buffer is NULL");
+	std::lock_guard<std::mutex> glock(g_noalloc_mu);
+	std::lock_guard<std::mutex> lock(ctx->mu);
+	PK_HIP(hipSetDevice(ctx->device));
+	void *ws = nullptr;
+	auto it = g_scratch_ptr.find(ctx->device);
+	if (it == g_scratch_ptr.end()) {
+		// first call on this device: the address of its copy of `scratch`
+		const hipError_t e = hipGetSymbolAddress(&ws, HIP_SYMBOL(scratch));
+		if (e != hipSuccess || !ws)
+			return set_err(PK_EDEVICE, "hipGetSymbolAddress(scratch) failed on device %d: %s", ctx->device,
+			               hipGetErrorString(e));
+		g_scratch_ptr[ctx->device] = ws;
+	} else {
+		ws = it->second;
+	}
+	Analysis a;
+	PK_TRY(analyze(m, a));
+	if (m->dry_run) {
+		// src/model.cpp:263-267 (a fill kernel, not a memset node: capturable)
+		const size_t n = (size_t)a.dw * a.dh;
+		if (n > out_capacity) return set_err(PK_ERANGE, "output buffer holds %zu doubles, %zu needed", out_capacity, n);
+		if (n) hipLaunchKernelGGL(fill_kernel, dim3((uint32_t)std::min<size_t>(cdiv(n, 256), 65535)), dim3(256), 0,
+		                          ctx->stream, out_dev, n, 0.0);
+		PK_HIP(hipGetLastError());
+		PK_HIP(hipStreamSynchronize(ctx->stream));
+		if (res) {
+			std::memset(res, 0, sizeof(*res));
+			res->width = a.dw;
+			res->height = a.dh;
+		}
+		return PK_OK;
+	}
+	if (ctx->precision == PK_FP32) PK_TRY(check_fp32_limits(m));
+	// the context's own workspace is set aside for the call
+	struct Swap {
+		pk_ctx *c;
+		void *dmem, *hmem;
+		size_t dcap, hcap;
+		~Swap()
+		{
+			c->dmem = dmem;
+			c->dcap = dcap;
+			c->hmem = hmem;
+			c->hcap = hcap;
+			c->fixed_ws = false;
+		}
+	} swap{ctx, ctx->dmem, ctx->hmem, ctx->dcap, ctx->hcap};
+	ctx->dmem = ws;
+	ctx->dcap = PK_NOALLOC_SCRATCH_BYTES;
+	ctx->hmem = g_noalloc_stage;
+	ctx->hcap = PK_NOALLOC_STAGE_BYTES;
+	ctx->fixed_ws = true;
+	if (ctx->precision == PK_FP32) return evaluate_impl<float>(ctx, m, a, out_dev, out_capacity, res, true);
+	return evaluate_impl<double>(ctx, m, a, out_dev, out_capacity, res, true);
 }
 
 int pk_evaluate_device(pk_ctx *ctx, const pk_model *m, double *out_dev, size_t out_capacity, pk_result *res)
