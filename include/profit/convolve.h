@@ -18,22 +18,27 @@ enum ConvolverType { BRUTE_OLD = 0, BRUTE, OPENCL, FFT };
 
 enum class effort_t { ESTIMATE = 0, MEASURE, PATIENT, EXHAUSTIVE };
 
-/* Options of create_convolver (libprofit's ConvolverCreationPreferences);
- * omp_threads and effort are accepted and have no effect on the device. */
+/* Options of create_convolver (libprofit's ConvolverCreationPreferences,
+ * include/profit/convolve.h:133-182); omp_threads, effort and
+ * instruction_set are accepted and have no effect on the device.
+ * opencl_env selects the device and precision (a GpuEnv; default: device 0,
+ * FP64). */
 class ConvolverCreationPreferences {
 public:
 	ConvolverCreationPreferences() = default;
 	ConvolverCreationPreferences(Dimensions src_dims, Dimensions krn_dims, unsigned int omp_threads,
-	                             GpuEnvPtr gpu_env, effort_t effort, bool reuse_krn_fft)
-	    : src_dims(src_dims), krn_dims(krn_dims), omp_threads(omp_threads), gpu_env(gpu_env), effort(effort),
-	      reuse_krn_fft(reuse_krn_fft)
+	                             OpenCLEnvPtr opencl_env, effort_t effort, bool reuse_krn_fft,
+	                             simd_instruction_set instruction_set = AUTO)
+	    : src_dims(src_dims), krn_dims(krn_dims), omp_threads(omp_threads), opencl_env(opencl_env), effort(effort),
+	      reuse_krn_fft(reuse_krn_fft), instruction_set(instruction_set)
 	{
 	}
 	Dimensions src_dims, krn_dims;
 	unsigned int omp_threads = 1;
-	GpuEnvPtr gpu_env;
+	OpenCLEnvPtr opencl_env;
 	effort_t effort = effort_t::ESTIMATE;
 	bool reuse_krn_fft = false;
+	simd_instruction_set instruction_set = AUTO;
 };
 
 class Convolver {

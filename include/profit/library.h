@@ -8,6 +8,9 @@
 #include <memory>
 #include <string>
 
+#include "profit/common.h"
+#include "profit/opencl.h"
+
 namespace profit {
 
 std::string version();
@@ -30,15 +33,24 @@ bool has_fftw_with_openmp();
 bool has_opencl();
 bool has_gpu();
 void clear_cache();
+/* Whether the host code was built with support for a SIMD instruction set:
+ * AUTO and NONE only (convolution runs on the device) */
+bool has_simd_instruction_set(simd_instruction_set instruction_set);
+/* The OpenCL version libprofit works against: 0 (no OpenCL) */
+unsigned short opencl_version_major();
+unsigned short opencl_version_minor();
 
-/* The device and arithmetic a Model evaluates with (in the role of
- * libprofit's OpenCLEnvPtr, include/profit/opencl.h:79-168). */
+/* The device and arithmetic a Model evaluates with: the implementation of
+ * libprofit's OpenCLEnv (include/profit/opencl.h:79-168). */
 enum class Precision { FP64 = 0, FP32 = 1 };
 
-class GpuEnv {
+class GpuEnv : public OpenCLEnv {
 public:
 	GpuEnv(int device, Precision precision);
-	~GpuEnv();
+	~GpuEnv() override;
+	cl_ver_t get_version() override { return 0; }
+	std::string get_platform_name() override { return "HIP"; }
+	std::string get_device_name() override;
 	GpuEnv(const GpuEnv &) = delete;
 	GpuEnv &operator=(const GpuEnv &) = delete;
 	int device() const { return device_; }

@@ -67,6 +67,17 @@ public:
 	/* device and precision of the evaluation (default: device 0, FP64) */
 	void set_gpu_env(const GpuEnvPtr &env) { gpu_env = env; }
 	GpuEnvPtr get_gpu_env() const { return gpu_env; }
+	/* the same through libprofit's accelerator handle
+	 * (include/profit/model.h:305-311); invalid_parameter for an
+	 * environment that is not a GpuEnv */
+	void set_opencl_env(const OpenCLEnvPtr &env);
+	OpenCLEnvPtr get_opencl_env() const { return gpu_env; }
+	/* Sub-pixel integrations of the last evaluation per profile name, by
+	 * recursion level (libprofit's Model::get_profile_integrations,
+	 * include/profit/model.h:116, a PROFIT_DEBUG build's): level 0, the
+	 * pixels whose integration started (subsample_pixel calls at level 0);
+	 * deeper levels are not counted on the device. */
+	std::map<std::string, std::map<int, int>> get_profile_integrations() const;
 
 	/* Model::adjust: the calculation mask a Model derives from `mask` to
 	 * keep the flux of the convolution (src/model.cpp:230-257) */
@@ -90,6 +101,7 @@ private:
 	ConvolverPtr convolver;
 	GpuEnvPtr gpu_env;
 	std::vector<ProfilePtr> profiles;
+	std::map<std::string, std::map<int, int>> integrations;
 };
 
 }  // namespace profit

@@ -12,16 +12,41 @@
 #include <memory>
 #include <string>
 
+#include "profit/common.h"
+#include "profit/opencl.h"
 #include "profit_hip.h"
 
 namespace profit {
 
 class Model;
 
-/* Per-profile statistics of the last evaluation (libprofit's ProfileStats). */
-struct ProfileStats {
-	double total_ms = 0;        // device time of the whole evaluation
+/* Statistics of the last evaluation (libprofit's ProfileStats,
+ * include/profit/profile.h:47-73).  The device evaluates all profiles of a
+ * model together: `total` is the model's device time (setup to returned
+ * image), as are refined_pixels and subcells the model's counts. */
+class ProfileStats {
+public:
+	virtual ~ProfileStats() {}
+	nsecs_t total = 0;
+	double total_ms = 0;
 	unsigned long long refined_pixels = 0, subcells = 0;
+};
+
+/* Sub-pixel integration statistics of radial profiles; subsampling.total
+ * holds the model's profile stage (pixel pass + refinement) */
+struct radial_subsampling_stats {
+	nsecs_t pre_subsampling = 0;
+	nsecs_t new_subsampling = 0;
+	nsecs_t inital_transform = 0;
+	OpenCL_times cl_times;
+	nsecs_t final_transform = 0;
+	nsecs_t total = 0;
+};
+
+struct RadialProfileStats : ProfileStats {
+	OpenCL_times cl_times;
+	radial_subsampling_stats subsampling;
+	nsecs_t final_image = 0;
 };
 
 class Profile {

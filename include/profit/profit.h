@@ -5,11 +5,13 @@
 #ifndef PROFIT_AMD_PROFIT_H
 #define PROFIT_AMD_PROFIT_H
 
+#include "profit/common.h"
 #include "profit/convolve.h"
 #include "profit/exceptions.h"
 #include "profit/image.h"
 #include "profit/library.h"
 #include "profit/model.h"
+#include "profit/opencl.h"
 #include "profit/profile.h"
 
 #endif

@@ -166,6 +166,9 @@ int pk_device_count(int *count);
 int pk_ctx_create(int device, int precision, pk_ctx **ctx_out);
 int pk_ctx_destroy(pk_ctx *ctx);
 int pk_ctx_precision(const pk_ctx *ctx);
+/* The context's device: its architecture name and compute-unit count
+ * ("gfx950/256"); in the role of OpenCLEnv::get_device_name. */
+const char *pk_ctx_device_name(const pk_ctx *ctx);
 
 /* Fills a component with libprofit's constructor defaults for `kind`
  * (src/radial.cpp:724-737, src/sersic.cpp:352-360, src/moffat.cpp,
@@ -223,6 +226,13 @@ int pk_evaluate_device(pk_ctx *ctx, const pk_model *m, double *out_dev, size_t o
 #define PK_NOALLOC_STAGE_BYTES (16u << 20)
 int pk_evaluate_device_noalloc(pk_ctx *ctx, const pk_model *m, double *out_dev, size_t out_capacity,
                                pk_result *res);
+
+/* Sub-pixel integrations started by the last pk_evaluate / pk_evaluate_device
+ * / pk_evaluate_device_noalloc on this context, per component of its model
+ * (component order; 0 for sky, psf and null profiles): the pixels whose
+ * subsample_pixel recursion began (level 0 of libprofit's
+ * Model::get_profile_integrations, include/profit/model.h:116). */
+int pk_ctx_last_integrations(const pk_ctx *ctx, uint64_t *counts, uint32_t n_components);
 
 /* Convolver::convolve(src, krn, mask, crop, offset_out)
  * (include/profit/convolve.h:98-99, src/convolve.cpp:55-79) for a convolver of

@@ -72,19 +72,23 @@ def build_host(verbose=False):
 
 
 def build_host_check(verbose=False):
-    """tests/cpp/host_api_check (test program of the C++ host API)."""
-    src = os.path.join(ROOT, "tests", "cpp", "host_api_check.cpp")
+    """The test programs of the C++ host API: tests/cpp/host_api_check and
+    tests/cpp/ref_api_check (a caller written against libprofit's API)."""
     out_dir = os.path.join(ROOT, "tests", "cpp", "_build")
     os.makedirs(out_dir, exist_ok=True)
-    out = os.path.join(out_dir, "host_api_check")
-    cmd = [CXX] + HOST_FLAGS + ["-I", os.path.join(ROOT, "include"), src, "-L", OUT_DIR, "-l:libprofit.so",
-                                "-l:libprofit_hip.so", "-Wl,-rpath," + OUT_DIR, "-Wl,-rpath,$ORIGIN/../../../libprofit_amd/_lib",
-                                "-o", out + ".tmp"]
-    if verbose:
-        print(" ".join(cmd))
-    subprocess.check_call(cmd)
-    os.replace(out + ".tmp", out)
-    return out
+    outs = []
+    for name in ("host_api_check", "ref_api_check"):
+        src = os.path.join(ROOT, "tests", "cpp", name + ".cpp")
+        out = os.path.join(out_dir, name)
+        cmd = [CXX] + HOST_FLAGS + ["-I", os.path.join(ROOT, "include"), src, "-L", OUT_DIR, "-l:libprofit.so",
+                                    "-l:libprofit_hip.so", "-Wl,-rpath," + OUT_DIR,
+                                    "-Wl,-rpath,$ORIGIN/../../../libprofit_amd/_lib", "-o", out + ".tmp"]
+        if verbose:
+            print(" ".join(cmd))
+        subprocess.check_call(cmd)
+        os.replace(out + ".tmp", out)
+        outs.append(out)
+    return outs
 
 
 def build_all(verbose=False):

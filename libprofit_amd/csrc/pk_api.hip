@@ -79,6 +79,8 @@ struct pk_ctx {
 	// set while pk_evaluate_device_noalloc runs: dmem / hmem are then the
 	// library's fixed workspace, and nothing may be (re)allocated
 	bool fixed_ws = false;
+	// refined pixels per model component of the last single evaluation
+	std::vector<uint64_t> last_integrations;
 };
 
 static int ensure_dev(pk_ctx *ctx, size_t bytes)
@@ -295,6 +297,15 @@ int pk_ctx_destroy(pk_ctx *ctx)
 	if (ctx->hmem) (void)hipHostFree(ctx->hmem);
 	(void)hipStreamDestroy(ctx->stream);
 	delete ctx;
+	return PK_OK;
+}
+
+const char *pk_ctx_device_name(const pk_ctx *ctx) { return ctx ? ctx->arch.c_str() : ""; }
+
+int pk_ctx_last_integrations(const pk_ctx *ctx, uint64_t *counts, uint32_t n)
+{
+	if (!ctx || (!counts && n)) return set_err(PK_EINVAL, "NULL argument");
+	for (uint32_t k = 0; k < n; k++) counts[k] = k < ctx->last_integrations.size() ? ctx->last_integrations[k] : 0;
 	return PK_OK;
 }
 
@@ -978,6 +989,7 @@ int evaluate_impl(pk_ctx *ctx, const pk_model *m, const Analysis &a, double *out
 	// set-up runs on the device, setup_kernel), psf profiles become rebinning
 	// items, null profiles do nothing
 	std::vector<SetupIn> comps;
+	std::vector<uint32_t> comp_model_index;  // model component of comps[k]
 	std::vector<PkPsfItem> items;
 	bool any_nc = false;
 	for (uint32_t c = 0; c < m->n_components; c++) {
@@ -1003,6 +1015,7 @@ int evaluate_impl(pk_ctx *ctx, const pk_model *m, const Analysis &a, double *out
 		in.integrator = g_integrator.load();
 		if (!pc.convolve) any_nc = true;
 		comps.push_back(in);
+		comp_model_index.push_back(c);
 	}
 	if (comps.size() > 32) return set_err(PK_EINVAL, "at most 32 radial/sky profiles per model are supported");
 
@@ -1048,6 +1061,7 @@ int evaluate_impl(pk_ctx *ctx, const pk_model *m, const Analysis &a, double *out
 	const size_t o_psfT = L.take(sizeof(TI) * std::max<size_t>(1, psf.size()));
 	const size_t o_mask = L.take(use_mask ? npix : 1);
 	const size_t o_omask = L.take(adjusted ? (size_t)m->mask_width * m->mask_height : 1);
+	const size_t o_hist = L.take(sizeof(unsigned long long) * 32);
 	const size_t in_bytes = L.off;
 	const size_t o_comp = L.take(sizeof(PkComp) * std::max<size_t>(1, comps.size()));
 	const size_t o_lite = L.take(sizeof(CompLite) * std::max<size_t>(1, comps.size()));
@@ -1065,7 +1079,7 @@ int evaluate_impl(pk_ctx *ctx, const pk_model *m, const Analysis &a, double *out
 	const size_t o_fz = L.take(use_fft ? sizeof(double2) * ((size_t)fg.P * fg.Q + 1) : 1);
 	const size_t o_ftw = L.take(use_fft ? sizeof(double2) * FFT_TWIDDLES : 1);
 	PK_TRY(ensure_dev(ctx, L.off));
-	PK_TRY(ensure_host(ctx, std::max(in_bytes, (device_out ? 0 : sizeof(double) * (size_t)ow * oh + 256) + 256 +
+	PK_TRY(ensure_host(ctx, std::max(in_bytes, (device_out ? 0 : sizeof(double) * (size_t)ow * oh + 256) + 512 +
 	                                               sizeof(PkCounters))));
 
 	jb.mask_off = use_mask ? 0 : NONE;
@@ -1120,6 +1134,11 @@ int evaluate_impl(pk_ctx *ctx, const pk_model *m, const Analysis &a, double *out
 	                           dptr<unsigned long long>(ctx, o_acc), dptr<RCell>(ctx, o_q), qcap, nref_blocks,
 	                           ctx->r32_grid_main, ctx->r32_grid_last,
 	                           !comps.empty(), nullptr));
+	if (!comps.empty()) {
+		hipLaunchKernelGGL(integrations_kernel, dim3(64), dim3(256), 0, s, (const RefineEntry *)dent,
+		                   (const PkCounters *)dcnt, dptr<unsigned long long>(ctx, o_hist));
+		PK_HIP(hipGetLastError());
+	}
 	for (size_t k = 0; k < items.size(); k++) {
 		const auto &it = items[k];
 		if (it.x1 < it.x0 || it.y1 < it.y0) continue;
@@ -1158,8 +1177,14 @@ int evaluate_impl(pk_ctx *ctx, const pk_model *m, const Analysis &a, double *out
 	else
 		PK_HIP(hipMemcpyAsync(ctx->hmem, dptr<void>(ctx, o_out), out_bytes, hipMemcpyDeviceToHost, s));
 	PK_HIP(hipMemcpyAsync(hptr<void>(ctx, ho_cnt), dcnt, sizeof(PkCounters), hipMemcpyDeviceToHost, s));
+	const size_t ho_hist = (ho_cnt + sizeof(PkCounters) + 7) & ~(size_t)7;
+	PK_HIP(hipMemcpyAsync(hptr<void>(ctx, ho_hist), dptr<void>(ctx, o_hist), sizeof(unsigned long long) * 32,
+	                      hipMemcpyDeviceToHost, s));
 	PK_HIP(hipStreamSynchronize(s));
 	const PkCounters cnt = *hptr<PkCounters>(ctx, ho_cnt);
+	ctx->last_integrations.assign(m->n_components, 0);
+	for (size_t k = 0; k < comp_model_index.size(); k++)
+		ctx->last_integrations[comp_model_index[k]] = hptr<unsigned long long>(ctx, ho_hist)[k];
 	if (cnt.overflow)
 		return set_err(PK_ERANGE, "sub-pixel refinement work queue capacity exceeded");
 	if (cnt.n_bad) return set_err(PK_ERANGE, "sub-pixel refinement produced %u non-finite values", cnt.n_bad);

@@ -1132,6 +1132,28 @@ __global__ void reset_counters_kernel(PkCounters *c)
 	}
 }
 
+// Refined pixels per component (bits of the entries' component masks):
+// LDS histogram per workgroup, one atomic per component and workgroup
+__global__ __launch_bounds__(256) void integrations_kernel(const RefineEntry *__restrict__ entries,
+                                                            const PkCounters *__restrict__ cnt,
+                                                            unsigned long long *__restrict__ hist)
+{
+	__shared__ unsigned int h[32];
+	if (threadIdx.x < 32) h[threadIdx.x] = 0;
+	__syncthreads();
+	const uint32_t n = cnt->n_entries;
+	for (uint32_t e = blockIdx.x * blockDim.x + threadIdx.x; e < n; e += gridDim.x * blockDim.x) {
+		uint32_t m = entries[e].cmask;
+		while (m) {
+			const int c = __ffs(m) - 1;
+			m &= m - 1;
+			atomicAdd(&h[c], 1u);
+		}
+	}
+	__syncthreads();
+	if (threadIdx.x < 32 && h[threadIdx.x]) atomicAdd(&hist[threadIdx.x], (unsigned long long)h[threadIdx.x]);
+}
+
 // Sum of the per-wavefront counts of a COUNT pass into *out (cleared
 // before): grid-stride, one atomic per wavefront
 __global__ __launch_bounds__(256) void sum_wave_evals_kernel(const uint32_t *__restrict__ w, uint64_t n,

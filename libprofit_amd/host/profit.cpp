@@ -66,6 +66,39 @@ bool has_gpu()
 	return pk_device_count(&n) == PK_OK && n > 0;
 }
 void clear_cache() { check(pk_clear_cache()); }
+bool has_simd_instruction_set(simd_instruction_set instruction_set)
+{
+	return instruction_set == AUTO || instruction_set == NONE;
+}
+unsigned short opencl_version_major() { return 0; }
+unsigned short opencl_version_minor() { return 0; }
+
+std::string GpuEnv::get_device_name() { return pk_ctx_device_name(static_cast<const pk_ctx *>(ctx_)); }
+
+std::map<int, OpenCL_plat_info> get_opencl_info()
+{
+	std::map<int, OpenCL_plat_info> info;
+	int n = 0;
+	if (pk_device_count(&n) != PK_OK || n <= 0) return info;
+	OpenCL_plat_info plat{"HIP", 0, {}};
+	for (int d = 0; d < n; d++) plat.dev_info[d] = OpenCL_dev_info{get_gpu_environment(d)->get_device_name(), 0, true};
+	info[0] = plat;
+	return info;
+}
+
+OpenCLEnvPtr get_opencl_environment(unsigned int platform_idx, unsigned int device_idx, bool use_double, bool)
+{
+	if (platform_idx != 0) throw invalid_parameter("platform " + std::to_string(platform_idx) + " does not exist");
+	return get_gpu_environment((int)device_idx, use_double ? Precision::FP64 : Precision::FP32);
+}
+
+static GpuEnvPtr as_gpu_env(const OpenCLEnvPtr &env)
+{
+	if (!env) return nullptr;
+	auto g = std::dynamic_pointer_cast<GpuEnv>(env);
+	if (!g) throw invalid_parameter("the environment is not a device environment of this library (GpuEnv)");
+	return g;
+}
 
 GpuEnv::GpuEnv(int device, Precision precision) : device_(device), precision_(precision)
 {
@@ -232,7 +265,11 @@ Image &Image::operator&=(const Mask &m)
 // ---------------------------------------------------------------------------
 // profiles
 // ---------------------------------------------------------------------------
-Profile::Profile(const std::string &name_, int kind) : name(name_), stats(std::make_shared<ProfileStats>())
+Profile::Profile(const std::string &name_, int kind)
+    : name(name_),
+      stats(kind == PK_PROFILE_NULL || kind == PK_PROFILE_SKY || kind == PK_PROFILE_PSF
+                ? std::make_shared<ProfileStats>()
+                : std::static_pointer_cast<ProfileStats>(std::make_shared<RadialProfileStats>()))
 {
 	check(pk_component_defaults(kind, &component));
 	pk_component &c = component;
@@ -383,7 +420,7 @@ Convolver::Convolver(ConvolverType type, const ConvolverCreationPreferences &pre
 
 Image Convolver::convolve(const Image &src, const Image &krn, const Mask &mask, bool crop, Point &offset_out)
 {
-	GpuEnvPtr env = prefs_.gpu_env ? prefs_.gpu_env : get_gpu_environment();
+	GpuEnvPtr env = prefs_.opencl_env ? as_gpu_env(prefs_.opencl_env) : get_gpu_environment();
 	const Dimensions s = src.getDimensions(), k = krn.getDimensions();
 	const size_t mx = std::max({s.x, s.y, k.x, k.y});
 	std::vector<double> out(std::max<size_t>(4 * mx * mx, (size_t)s.x * s.y));
@@ -516,13 +553,27 @@ Image Model::evaluate(Point &offset_out)
 	check(pk_evaluate(static_cast<pk_ctx *>(gpu_env->handle()), &m, out.data(), out.size(), &r));
 	out.resize((size_t)r.width * r.height);
 	if (&offset_out != &NO_OFFSET) offset_out = Point((unsigned int)r.offset_x, (unsigned int)r.offset_y);
-	for (auto &p : profiles) {
+	std::vector<uint64_t> integr(profiles.size());
+	check(pk_ctx_last_integrations(static_cast<pk_ctx *>(gpu_env->handle()), integr.data(),
+	                               (uint32_t)integr.size()));
+	integrations.clear();
+	for (size_t k = 0; k < profiles.size(); k++) {
+		auto &p = profiles[k];
+		p->stats->total = (nsecs_t)(r.ms_total * 1e6);
 		p->stats->total_ms = r.ms_total;
 		p->stats->refined_pixels = r.n_refined_pixels;
 		p->stats->subcells = r.n_subcells;
+		if (auto *rs = dynamic_cast<RadialProfileStats *>(p->stats.get())) {
+			rs->subsampling.total = (nsecs_t)(r.ms_profiles * 1e6);
+			integrations[p->get_name()][0] += (int)integr[k];
+		}
 	}
 	return Image(std::move(out), r.width, r.height);
 }
+
+void Model::set_opencl_env(const OpenCLEnvPtr &env) { gpu_env = as_gpu_env(env); }
+
+std::map<std::string, std::map<int, int>> Model::get_profile_integrations() const { return integrations; }
 
 void Model::adjust(Mask &mask, const Dimensions &dims, const Image &psf, unsigned int finesampling)
 {

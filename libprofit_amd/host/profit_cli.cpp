@@ -244,7 +244,7 @@ int parse_and_run(int argc, char *argv[], std::ostream &cout, std::ostream &cerr
 	prefs.src_dims = dims * finesampling;
 	auto env = get_gpu_environment(device, precision);
 	m.set_gpu_env(env);
-	prefs.gpu_env = env;
+	prefs.opencl_env = env;
 
 	using clock = std::chrono::system_clock;
 	auto start = clock::now();

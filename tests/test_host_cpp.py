@@ -119,3 +119,47 @@ def test_cli_matches_oracle(oracle):
     ref, _ = oracle.evaluate(spec)
     pix, _ = rel_errors(got, ref)
     assert pix <= 1e-13, pix  # the text output carries 17 significant digits
+
+
+REF_CHECK = os.path.join(ROOT, "tests", "cpp", "_build", "ref_api_check")
+
+
+@pytest.mark.gpu
+def test_reference_api_caller_matches_oracle(oracle):
+    """tests/cpp/ref_api_check, written against libprofit's public API only
+    (ProfileStats::total, RadialProfileStats, OpenCLEnvPtr via
+    get_opencl_environment / Model::set_opencl_env, the 7-argument
+    ConvolverCreationPreferences, has_simd_instruction_set,
+    opencl_version_*, get_opencl_info, get_profile_integrations, the
+    profit-cli run loop), compiled against this library's headers: its
+    image against the oracle, its level-0 integrations against the oracle's
+    refined pixels of each profile."""
+    _need(REF_CHECK)
+    print("CASE reference api caller", flush=True)
+    d = tempfile.mkdtemp(prefix="pk_ref_")
+    out = os.path.join(d, "img.bin")
+    r = subprocess.run([REF_CHECK, out, "3"], capture_output=True, text=True, timeout=120)
+    print(r.stdout, r.stderr, flush=True)
+    assert r.returncode == 0, r.stderr
+    assert "Ran 3 iterations in" in r.stdout
+    assert "Stats for profile sersic" in r.stdout and "Stats for profile moffat" in r.stdout
+    psf, _ = oracle.evaluate(moffat_psf(15, 3.0, 2.5))
+    s = comp("sersic", xcen=40.3, ycen=34.7, mag=15, re=6, nser=2.5, ang=30, axrat=0.6, box=0.1, convolve=True)
+    m = comp("moffat", xcen=30.2, ycen=41.6, mag=16, fwhm=4.0, con=3.0, convolve=True)
+    spec = abi.ModelSpec(80, 70, [s, m, comp("sky", bg=1e-9)], psf=psf)
+    got, goff = _load(out)
+    ref, roff = oracle.evaluate(spec)
+    assert got.shape == ref.shape and goff == roff
+    pix, flux = rel_errors(got, ref)
+    assert pix <= TOL_FP64 and flux <= TOL_FP64, (pix, flux)
+    # level-0 integrations per profile: the pixels the oracle refines for
+    # that profile alone (same PSF padding)
+    for name, c in (("sersic", s), ("moffat", m)):
+        oracle.evaluate(abi.ModelSpec(80, 70, [c], psf=psf))
+        refined, _ = oracle.stats()
+        line = [ln for ln in r.stdout.splitlines() if ln.startswith("Integrations per recursion level for profile "
+                                                                     + name)]
+        assert line, name
+        lines = r.stdout.splitlines()
+        lv0 = lines[lines.index(line[0]) + 1]
+        assert lv0.strip() == "Level 0: %d integrations" % refined, (name, lv0, refined)
