@@ -493,7 +493,7 @@ def main():
 
     kernels = {
         "conv": {k: roofline[k] for k in ("bound", "achieved", "peak", "unit", "frac", "avg_launch_ms")},
-        "coarse": valu_entry("coarse_kernel<float, true> (one launch per step)",
+        "coarse": valu_entry("coarse32_kernel<float> (one launch per step)",
                              SLOTS_CLASSIFY * pairs + SLOTS_EVAL * wc.coarse_evaluations, stages["coarse"],
                              pixel_component_pairs=pairs, evaluations=int(wc.coarse_evaluations),
                              slots_per_pair=SLOTS_CLASSIFY, slots_per_evaluation=SLOTS_EVAL),

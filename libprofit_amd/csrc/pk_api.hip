@@ -927,10 +927,10 @@ int launch_profiles(hipStream_t s, bool fp32, const PkJob *djobs, uint32_t njobs
                     const uint32_t *tile_job = nullptr, R32Tune tune = R32_TUNE_DEFAULT)
 {
 	if (fp32) {
-		hipLaunchKernelGGL((coarse_kernel<TI, true>), dim3(ntiles), dim3(256), 0, s, djobs, njobs, dcomp, dmask, img_nc,
+		hipLaunchKernelGGL((coarse32_kernel<TI>), dim3(ntiles), dim3(256), 0, s, djobs, njobs, dcomp, dmask, img_nc,
 		                   img_c, dent, dcnt, dacc, tile_job, (uint32_t *)nullptr);
 	} else {
-		hipLaunchKernelGGL((coarse_kernel<TI, false>), dim3(ntiles), dim3(256), 0, s, djobs, njobs, dcomp, dmask,
+		hipLaunchKernelGGL((coarse_kernel<TI>), dim3(ntiles), dim3(256), 0, s, djobs, njobs, dcomp, dmask,
 		                   img_nc, img_c, dent, dcnt, (unsigned long long *)nullptr, tile_job, (uint32_t *)nullptr);
 	}
 	PK_HIP(hipGetLastError());
@@ -2779,12 +2779,12 @@ int pk_batch_work_counts(pk_batch *b, pk_work_counts *out)
 	const uint32_t *tj = b->d<uint32_t>(b->o_tj);
 	uint32_t *wev = b->d<uint32_t>(b->o_wev);
 	if (b->elem == 4)
-		hipLaunchKernelGGL((coarse_kernel<float, true, true>), dim3(b->ntiles), dim3(256), 0, s, djobs, b->n_models,
+		hipLaunchKernelGGL((coarse32_kernel<float, true>), dim3(b->ntiles), dim3(256), 0, s, djobs, b->n_models,
 		                   (const PkComp *)b->d<PkComp>(b->o_comps), (const uint8_t *)b->d<uint8_t>(b->o_mask),
 		                   (float *)nullptr, (float *)nullptr, (RefineEntry *)nullptr, (PkCounters *)nullptr,
 		                   (unsigned long long *)nullptr, tj, wev);
 	else
-		hipLaunchKernelGGL((coarse_kernel<double, false, true>), dim3(b->ntiles), dim3(256), 0, s, djobs, b->n_models,
+		hipLaunchKernelGGL((coarse_kernel<double, true>), dim3(b->ntiles), dim3(256), 0, s, djobs, b->n_models,
 		                   (const PkComp *)b->d<PkComp>(b->o_comps), (const uint8_t *)b->d<uint8_t>(b->o_mask),
 		                   (double *)nullptr, (double *)nullptr, (RefineEntry *)nullptr, (PkCounters *)nullptr,
 		                   (unsigned long long *)nullptr, tj, wev);

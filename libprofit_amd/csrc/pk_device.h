@@ -56,6 +56,11 @@ __device__ inline uint32_t find_job(const PkJob *jobs, uint32_t njobs, uint32_t 
 
 __device__ inline uint32_t lane_id() { return __lane_id(); }
 
+template <int N>
+struct IntTag {
+	static constexpr int value = N;
+};
+
 // ---------------------------------------------------------------------------
 // pixel passes (coarse, residual, coarse adjoint, composition): a 256-lane
 // workgroup covers a 64 x PX_ROWS tile, each lane one column of PX_RPT rows,
@@ -67,10 +72,83 @@ constexpr int PX_RPT = 4;    // rows per lane
 // ---------------------------------------------------------------------------
 // coarse pass
 // ---------------------------------------------------------------------------
+enum { CLS_ZERO = 0, CLS_DIRECT = 1, CLS_REFINE = 2, CLS_FP64 = -1 };
+
+// FP32 classification of one pixel against one component (FP32 mode), from
+// its FP32 profile coordinates and the guard bands of the PixHot block:
+// outside the bands the decision is libprofit's (src/radial.cpp:316-321),
+// within one (or for NaN) CLS_FP64 asks for the FP64 test.
+__device__ inline int classify32(const PixHot &h, float xf, float yf)
+{
+	const float r2 = fmaf(xf, xf, yf * yf);
+	int cls = r2 > h.hi_sw ? CLS_DIRECT : r2 < h.lo_sw ? CLS_REFINE : CLS_FP64;
+	if (!(r2 < h.lo_mx)) cls = CLS_FP64;
+	if (r2 > h.hi_mx) cls = CLS_ZERO;
+	return cls;
+}
+
+// libprofit's classification in FP64 (src/radial.cpp:316-321)
+__device__ inline int classify64(const PkComp &c, double xp, double yp)
+{
+	const double r2 = xp * xp + yp * yp;
+	if (c.rscale_max > 0 && radius_exceeds(r2, c.r2_max, c.rscale, c.rscale_max)) return CLS_ZERO;
+	if (c.rough || radius_exceeds(r2, c.r2_switch, c.rscale, c.rscale_switch)) return CLS_DIRECT;
+	return CLS_REFINE;
+}
+
+// Stores the pixels of one lane (PX_RPT rows) and appends those that need
+// refinement: one atomic for the wavefront's rows, row r's pixels after those
+// of rows < r.
+template <typename TI>
+__device__ inline void coarse_store(const PkJob &jb, uint32_t jn, uint32_t i, uint32_t j0, const bool inside[PX_RPT],
+                                    const double acc_nc[PX_RPT], const double acc_c[PX_RPT],
+                                    const uint32_t cmask[PX_RPT], TI *__restrict__ img_nc, TI *__restrict__ img_c,
+                                    RefineEntry *__restrict__ entries, PkCounters *__restrict__ counters,
+                                    unsigned long long *__restrict__ acc)
+{
+	uint64_t ballot[PX_RPT];
+	uint32_t n_need = 0;
+#pragma unroll
+	for (int r = 0; r < PX_RPT; r++) {
+		const uint32_t pix = (j0 + r) * jb.dw + i;
+		if (inside[r]) {
+			if (jb.has_nc) img_nc[jb.img_off + pix] = (TI)acc_nc[r];
+			if (jb.has_conv) img_c[jb.img_off + pix] = (TI)acc_c[r];
+		}
+		ballot[r] = __ballot(cmask[r] != 0);
+		n_need += __popcll(ballot[r]);
+	}
+	if (n_need) {
+		uint32_t base = 0;
+		if (lane_id() == 0) base = atomicAdd(&counters->n_entries, n_need);
+		base = __shfl(base, 0);
+#pragma unroll
+		for (int r = 0; r < PX_RPT; r++) {
+			if (cmask[r]) {
+				const uint32_t e = base + __popcll(ballot[r] & ((1ull << lane_id()) - 1ull));
+				entries[e] = RefineEntry{jn, (j0 + r) * jb.dw + i, cmask[r]};
+				if (acc) {
+					acc[2 * (size_t)e] = 0;
+					acc[2 * (size_t)e + 1] = 0;
+				}
+			}
+			base += __popcll(ballot[r]);
+		}
+	}
+}
+
+// Evaluation count of the wavefront (COUNT instantiations), stored in its own slot.
+__device__ inline void coarse_store_count(uint32_t nev, uint32_t *__restrict__ wave_evals)
+{
+	for (int off = 32; off > 0; off >>= 1) nev += __shfl_xor(nev, off);
+	if (lane_id() == 0) wave_evals[blockIdx.x * 4 + (threadIdx.x >> 6)] = nev;
+}
+
+// FP64 mode: libprofit's pixel loop (src/radial.cpp:291-344) in FP64.
 // COUNT: classification only -- no values, writes or appends; each
 // wavefront stores how many evaluate_at calls its pixels need (the work the
 // product instantiation, COUNT = false, does).
-template <typename TI, bool FP32, bool COUNT = false>
+template <typename TI, bool COUNT = false>
 __global__ __launch_bounds__(256) void coarse_kernel(const PkJob *__restrict__ jobs, uint32_t njobs,
                                                      const PkComp *__restrict__ comps,
                                                      const uint8_t *__restrict__ masks, TI *__restrict__ img_nc,
@@ -110,64 +188,190 @@ __global__ __launch_bounds__(256) void coarse_kernel(const PkJob *__restrict__ j
 			} else {
 				double xp, yp;
 				to_prof(cp, x, y, xp, yp);
-				const double r2 = xp * xp + yp * yp;
-				if (cp.rscale_max > 0 && radius_exceeds(r2, cp.r2_max, cp.rscale, cp.rscale_max)) continue;
-				if (cp.rough || radius_exceeds(r2, cp.r2_switch, cp.rscale, cp.rscale_switch)) {
-					if (COUNT) {
-						nev++;
-						continue;
-					}
-					float v32 = -1.f;
-					if (FP32 && cp.fp32) v32 = evaluate_at32(cp, (float)xp, (float)yp, radius2_64(cp, xp, yp));
-					if (v32 >= 0.f && v32 <= 3e38f) v = (double)v32 * cp.unfold;
-					else v = evaluate_at(cp, xp, yp);
-					v *= cp.flux_scale;
-				} else {
+				const int cls = classify64(cp, xp, yp);
+				if (cls == CLS_ZERO) continue;
+				if (cls == CLS_REFINE) {
 					cmask[r] |= 1u << c;
 					continue;
 				}
+				if (COUNT) {
+					nev++;
+					continue;
+				}
+				v = evaluate_at(cp, xp, yp) * cp.flux_scale;
 			}
 			if (cp.convolve) acc_c[r] += v;
 			else acc_nc[r] += v;
 		}
 	}
 	if (COUNT) {
-		// evaluation count of the wavefront, stored in its own slot
-		for (int off = 32; off > 0; off >>= 1) nev += __shfl_xor(nev, off);
-		if (lane_id() == 0) wave_evals[blockIdx.x * 4 + (threadIdx.x >> 6)] = nev;
+		coarse_store_count(nev, wave_evals);
 		return;
 	}
-	uint64_t ballot[PX_RPT];
-	uint32_t n_need = 0;
+	coarse_store<TI>(jb, jn, i, j0, inside, acc_nc, acc_c, cmask, img_nc, img_c, entries, counters, acc);
+}
+
+// FP32 profile value of kind K at (x, y) (the FP32 forms of the radial profiles
+// that take FP32 coordinates; king and ferrer take an FP64 r^2).
+template <int K, typename CR>
+__device__ inline float value32(const CR &c, float x, float y)
+{
+	switch (K) {
+	case PK_PROFILE_SERSIC: return sersic32(c, x, y);
+	case PK_PROFILE_MOFFAT: return moffat32(c, x, y);
+	case PK_PROFILE_BROKENEXP: return brokenexp32(c, x, y);
+	case PK_PROFILE_CORESERSIC: return coresersic32(c, x, y);
+	default: return -1.f;
+	}
+}
+
+// FP32 mode. Per component: the PixHot block in scalar registers, the FP32
+// classification of the lane's PX_RPT pixels (FP64 test only inside the
+// guard bands), then the direct values.  Branches are wavefront-uniform
+// wherever possible (a row is evaluated in all lanes when any of its lanes
+// needs the value, the others discard theirs): divergent branches would hold
+// one execution mask per row and condition in scalar registers.
+template <typename TI, bool COUNT = false>
+__global__ __launch_bounds__(256) void coarse32_kernel(const PkJob *__restrict__ jobs, uint32_t njobs,
+                                                       const PkComp *__restrict__ comps,
+                                                       const uint8_t *__restrict__ masks, TI *__restrict__ img_nc,
+                                                       TI *__restrict__ img_c, RefineEntry *__restrict__ entries,
+                                                       PkCounters *__restrict__ counters,
+                                                       unsigned long long *__restrict__ acc,
+                                                       const uint32_t *__restrict__ tile_job,
+                                                       uint32_t *__restrict__ wave_evals)
+{
+	const uint32_t jn = find_job(jobs, njobs, blockIdx.x, 0, tile_job);
+	const PkJob &jb = jobs[jn];
+	const uint32_t t = blockIdx.x - jb.tile_off;
+	const uint32_t i = (t % jb.tiles_x) * 64 + (threadIdx.x & 63);
+	const uint32_t j0 = (t / jb.tiles_x) * PX_ROWS + (threadIdx.x >> 6) * PX_RPT;
+	bool inside[PX_RPT];
+	uint32_t live = 0;  // bit r: row r is drawn and not masked
+	double acc_nc[PX_RPT], acc_c[PX_RPT];
+	uint32_t cmask[PX_RPT];
 #pragma unroll
 	for (int r = 0; r < PX_RPT; r++) {
-		const uint32_t pix = (j0 + r) * jb.dw + i;
-		if (inside[r]) {
-			if (jb.has_nc) img_nc[jb.img_off + pix] = (TI)acc_nc[r];
-			if (jb.has_conv) img_c[jb.img_off + pix] = (TI)acc_c[r];
-		}
-		ballot[r] = __ballot(cmask[r] != 0);
-		n_need += __popcll(ballot[r]);
+		inside[r] = i < jb.dw && j0 + r < jb.dh;
+		if (inside[r] && !(jb.mask_off != NONE && !masks[jb.mask_off + (j0 + r) * jb.dw + i])) live |= 1u << r;
+		acc_nc[r] = acc_c[r] = 0;
+		cmask[r] = 0;
 	}
-	// wavefront-aggregated append of the pixels that need refinement: one
-	// atomic for the wavefront's rows, row r's pixels after those of rows < r
-	if (n_need) {
-		uint32_t base = 0;
-		if (lane_id() == 0) base = atomicAdd(&counters->n_entries, n_need);
-		base = __shfl(base, 0);
+	const double hx = jb.sx / 2., hy = jb.sy / 2.;
+	const double x = hx + i * jb.sx;
+	uint32_t nev = 0;  // COUNT: evaluate_at calls of this lane
+	for (uint32_t c = 0; c < jb.ncomp; c++) {
+		const PkComp &cp = comps[jb.comp_off + c];
+		const PixHot h = cp.hot;
+		const bool conv = (h.flags & PIX_CONVOLVE) != 0;
+		if (h.kind == PK_PROFILE_SKY) {
+#pragma unroll
+			for (int r = 0; r < PX_RPT; r++) {
+				const double v = (live >> r) & 1u ? h.vscale : 0.0;
+				if (conv) acc_c[r] += v;
+				else acc_nc[r] += v;
+			}
+			continue;
+		}
+		// classification: profile coordinates in FP32 from the FP64 offsets
+		// to the centre (src/radial.cpp:47-53)
+		const float dxf = (float)(x - h.xcen);
+		const float dxc = dxf * h.cos32, ndxs = -dxf * h.sin32;
+		int cls[PX_RPT];
+		float xf[PX_RPT], yf[PX_RPT];
+		bool any64 = false;
 #pragma unroll
 		for (int r = 0; r < PX_RPT; r++) {
-			if (cmask[r]) {
-				const uint32_t e = base + __popcll(ballot[r] & ((1ull << lane_id()) - 1ull));
-				entries[e] = RefineEntry{jn, (j0 + r) * jb.dw + i, cmask[r]};
-				if (acc) {
-					acc[2 * (size_t)e] = 0;
-					acc[2 * (size_t)e + 1] = 0;
-				}
+			const float dyf = (float)((hy + (j0 + r) * jb.sy) - h.ycen);
+			xf[r] = fmaf(dyf, h.sin32, dxc);
+			yf[r] = fmaf(dyf, h.cos32, ndxs) * h.iq32;
+			const int k = classify32(h, xf[r], yf[r]);
+			cls[r] = (live >> r) & 1u ? k : (int)CLS_ZERO;
+			any64 = any64 || cls[r] == CLS_FP64;
+		}
+		if (__ballot(any64)) {
+			// within a guard band, or no FP32 classification: libprofit's test
+#pragma unroll
+			for (int r = 0; r < PX_RPT; r++) {
+				if (cls[r] != CLS_FP64) continue;
+				double xp, yp;
+				to_prof(cp, x, hy + (j0 + r) * jb.sy, xp, yp);
+				cls[r] = classify64(cp, xp, yp);
+				xf[r] = (float)xp;
+				yf[r] = (float)yp;
 			}
-			base += __popcll(ballot[r]);
+		}
+		uint32_t drow = 0;  // bit r: some lane of the wavefront evaluates row r directly
+#pragma unroll
+		for (int r = 0; r < PX_RPT; r++) {
+			if (cls[r] == CLS_REFINE) cmask[r] |= 1u << c;
+			if (COUNT) nev += cls[r] == CLS_DIRECT ? 1u : 0u;
+			if (__ballot(cls[r] == CLS_DIRECT)) drow |= 1u << r;
+		}
+		if (COUNT || !drow) continue;
+		// direct values: FP32 forms; kind and rows are wavefront-uniform branches
+		float v32[PX_RPT];
+#pragma unroll
+		for (int r = 0; r < PX_RPT; r++) v32[r] = -1.f;
+		auto rows32 = [&](auto kind_tag) {
+			constexpr int K = decltype(kind_tag)::value;
+#pragma unroll
+			for (int r = 0; r < PX_RPT; r++)
+				if ((drow >> r) & 1u) v32[r] = value32<K>(h, xf[r], yf[r]);
+		};
+		if (h.flags & PIX_FP32) {
+			switch (h.kind) {
+			case PK_PROFILE_SERSIC: rows32(IntTag<PK_PROFILE_SERSIC>{}); break;
+			case PK_PROFILE_MOFFAT: rows32(IntTag<PK_PROFILE_MOFFAT>{}); break;
+			case PK_PROFILE_BROKENEXP: rows32(IntTag<PK_PROFILE_BROKENEXP>{}); break;
+			case PK_PROFILE_CORESERSIC: rows32(IntTag<PK_PROFILE_CORESERSIC>{}); break;
+			case PK_PROFILE_KING:
+			case PK_PROFILE_FERRER:
+				// r^2 in FP64: their values vanish at the truncation radius
+				// through a difference of nearly equal terms
+#pragma unroll
+				for (int r = 0; r < PX_RPT; r++) {
+					if (!((drow >> r) & 1u)) continue;
+					const double dx = x - h.xcen, dy = (hy + (j0 + r) * jb.sy) - h.ycen;
+					const double xp = dx * cp.cos_ang + dy * cp.sin_ang;
+					const double yp = (dy * cp.cos_ang - dx * cp.sin_ang) * cp.inv_axrat;
+					const double r2 = radius2_64_kf(cp, xp, yp);
+					v32[r] = h.kind == PK_PROFILE_KING ? king32(cp, r2) : ferrer32(cp, r2);
+				}
+				break;
+			default: break;
+			}
+		}
+		double v[PX_RPT];
+		bool any_bad = false;
+#pragma unroll
+		for (int r = 0; r < PX_RPT; r++) {
+			const bool direct = cls[r] == CLS_DIRECT;
+			const bool ok = v32[r] >= 0.f && v32[r] <= 3e38f;
+			v[r] = direct && ok ? (double)v32[r] * h.vscale : 0.0;
+			any_bad = any_bad || (direct && !ok);
+		}
+		if (__ballot(any_bad)) {
+			// no FP32 form, or its value out of range: FP64
+#pragma unroll
+			for (int r = 0; r < PX_RPT; r++) {
+				if (cls[r] != CLS_DIRECT || (v32[r] >= 0.f && v32[r] <= 3e38f)) continue;
+				double xp, yp;
+				to_prof(cp, x, hy + (j0 + r) * jb.sy, xp, yp);
+				v[r] = evaluate_at(cp, xp, yp) * cp.flux_scale;
+			}
+		}
+#pragma unroll
+		for (int r = 0; r < PX_RPT; r++) {
+			if (conv) acc_c[r] += v[r];
+			else acc_nc[r] += v[r];
 		}
 	}
+	if (COUNT) {
+		coarse_store_count(nev, wave_evals);
+		return;
+	}
+	coarse_store<TI>(jb, jn, i, j0, inside, acc_nc, acc_c, cmask, img_nc, img_c, entries, counters, acc);
 }
 
 // ---------------------------------------------------------------------------

@@ -19,6 +19,28 @@ namespace pk {
 
 enum SersicVariant { SV_GENERAL = 0, SV_HALF, SV_ONE, SV_TWO, SV_THREE, SV_FOUR, SV_EIGHT, SV_SIXTEEN };
 
+// What the FP32 pixel pass reads of a component, in one 96-byte block (two
+// scalar loads): the FP32 classification constants, the FP32 profile
+// constants and the FP64 centre.
+//
+// The classification compares the FP32 r^2 with guard bands [lo, hi] around
+// r2_switch and r2_max (set by fill_pix_hot): r^2 > hi, r^2 < lo decide, a
+// value within a band (or NaN) goes to the FP64 test.
+enum PixFlags : int32_t {
+	PIX_FP32 = 1,     // FP32 profile form
+	PIX_CONVOLVE = 2  // added to the image that is convolved
+};
+
+struct alignas(16) PixHot {
+	double xcen, ycen;
+	double vscale;                 // unfold * flux_scale (sky: the pixel value)
+	float cos32, sin32, iq32;      // cos_ang, sin_ang, inv_axrat
+	float lo_sw, hi_sw;            // band of r2_switch: above hi direct, below lo refined
+	float lo_mx, hi_mx;            // band of r2_max: above hi zero
+	float f[8];                    // PkComp::f
+	int32_t kind, boxy, flags;     // flags: PixFlags
+};
+
 struct PkComp {
 	int32_t kind;       // PK_PROFILE_*
 	int32_t convolve;
@@ -47,6 +69,7 @@ struct PkComp {
 	// squared radii of the pixel classification: (rscale_max rscale)^2 and
 	// (rscale_switch rscale)^2
 	double r2_max, r2_switch;
+	PixHot hot;
 };
 
 // libprofit's test sqrt(x^2 + y^2) / rscale > limit (src/radial.cpp:316-321),
@@ -254,15 +277,22 @@ __device__ inline float log2_r32(const CR &c, float x, float y)
 	return flog2(m) + l;
 }
 
+// Squared boxy radius (|x|^b2 + |y|^b2)^(2/b2) in FP64.  Kept out of line:
+// its three FP64 pow calls, inlined at every call site, would raise the
+// register count (and code size) of every kernel that can reach it.
+PK_HD __attribute__((noinline)) double boxy_radius2_64(double x, double y, double b2)
+{
+	const double r = pow(pow(fabs(x), b2) + pow(fabs(y), b2), 1. / b2);
+	return r * r;
+}
+
 // FP64 squared (boxy) radius, for the FP32 king and ferrer evaluation whose
 // value near the truncation radius depends on rt^2 - r^2.
 template <typename CR>
 PK_HD double radius2_64_kf(const CR &c, double x, double y)
 {
 	if (!c.boxy) return x * x + y * y;
-	const double b2 = kb2(c);
-	const double r = pow(pow(fabs(x), b2) + pow(fabs(y), b2), 1. / b2);
-	return r * r;
+	return boxy_radius2_64(x, y, kb2(c));
 }
 
 // radius2_64_kf for king and ferrer, 0 for the other kinds
@@ -567,7 +597,7 @@ PK_HD void qgamma3(double a, const double p[3], double q[3], const LN &ln)
 }
 
 template <typename LN = SerialLanes>
-PK_HD void setup_component(const SetupIn &in, PkComp &o, const LN &ln = LN())
+PK_HD void setup_component_core(const SetupIn &in, PkComp &o, const LN &ln = LN())
 {
 	const pk_component &c = in.c;
 	o = PkComp{};
@@ -828,6 +858,61 @@ PK_HD void setup_component(const SetupIn &in, PkComp &o, const LN &ln = LN())
 		default: break;
 		}
 	}
+}
+
+PK_HD void fill_pix_hot(PkComp &o)
+{
+	PixHot &h = o.hot;
+	h = PixHot{};
+	h.kind = o.kind;
+	h.boxy = o.boxy;
+	h.xcen = o.xcen;
+	h.ycen = o.ycen;
+	h.vscale = o.kind == PK_PROFILE_SKY ? o.q[0] : o.unfold * o.flux_scale;
+	for (int k = 0; k < 8; k++) h.f[k] = o.f[k];
+	h.flags = (o.convolve ? PIX_CONVOLVE : 0) | (o.fp32 ? PIX_FP32 : 0);
+	h.cos32 = (float)o.cos_ang;
+	h.sin32 = (float)o.sin_ang;
+	h.iq32 = (float)o.inv_axrat;
+	// Guard bands. With x', y' formed in FP32 from the FP32-rounded offsets
+	// to the centre (u = 2^-24 per operand and operation), the FP32 r^2
+	// differs from the exact one by at most 31 u M r^2, M = max(q^-2, q^2)
+	// (q = axrat).  A band of relative half-width c = 3e-5 M (16 times that,
+	// and far wider than the FP64 rounding of libprofit's own test) leaves
+	// every pixel outside it on the side of the limit its exact radius is on.
+	const double inf = (double)INFINITY;
+	const double iq2 = o.inv_axrat * o.inv_axrat;
+	const double cband = 3e-5 * fmax(iq2, 1 / iq2);
+	auto fp32_range = [](double l) { return l == 0 || (l >= 1e-20 && l <= 1e20); };
+	const bool has_max = o.rscale_max > 0;
+	const bool cls32 = o.fp32 && fp32_range(o.r2_switch) && (!has_max || fp32_range(o.r2_max)) &&
+	                   o.inv_axrat >= 1e-4 && o.inv_axrat <= 1e4;
+	if (!cls32) {
+		// every pixel to the FP64 test
+		h.lo_sw = -INFINITY;
+		h.hi_sw = INFINITY;
+		h.lo_mx = h.hi_mx = INFINITY;
+		return;
+	}
+	if (o.rough) {
+		h.lo_sw = h.hi_sw = -INFINITY;  // never refined
+	} else {
+		h.lo_sw = (float)(o.r2_switch * (1 - cband) - 1e-30);
+		h.hi_sw = (float)(o.r2_switch * (1 + cband) + 1e-30);
+	}
+	if (has_max) {
+		h.lo_mx = (float)(o.r2_max * (1 - cband) - 1e-30);
+		h.hi_mx = (float)(o.r2_max * (1 + cband) + 1e-30);
+	} else {
+		h.lo_mx = h.hi_mx = (float)inf;
+	}
+}
+
+template <typename LN = SerialLanes>
+PK_HD void setup_component(const SetupIn &in, PkComp &o, const LN &ln = LN())
+{
+	setup_component_core(in, o, ln);
+	fill_pix_hot(o);
 }
 
 PK_HD CompLite make_lite(const PkComp &c, const SetupIn &in)

@@ -41,11 +41,6 @@ constexpr int R32_TASKS = 16;              // cells per round at most (their rec
 // every 16 rounds, hand-over once a quarter of the wavefronts stopped
 constexpr R32Tune R32_TUNE_DEFAULT = {4u, 64u, 16u, 16u, 0u};
 
-template <int N>
-struct IntTag {
-	static constexpr int value = N;
-};
-
 template <int CAP>
 struct R32Stack {
 	CompLite lite[R32_TASKS];
