@@ -36,6 +36,7 @@ namespace pk {
 // and the Sersic centre pixels (res 8, 10 levels).
 constexpr int R32_CAP_MAIN = 192;
 constexpr int R32_CAP_LAST = 1024;
+constexpr uint32_t R32_CELL_RUN = 4;       // log2 of the runs in which handed-over cells are claimed
 constexpr int R32_TASKS = 16;              // cells per round at most (their records are staged in LDS)
 // untuned settings: claims of up to 64 pixels / 16 cells, the hand-over test
 // every 16 rounds, hand-over once a quarter of the wavefronts stopped
@@ -169,17 +170,30 @@ __device__ inline void r32_expand(const PkJob *__restrict__ jobs, const PkComp *
 	// claim indices past the list hold no entry (nb = 0): the lanes taken are
 	// a prefix of the available ones
 	const int take = __popcll(__ballot(lane < avail && incl <= space));
-	if ((int)lane < take && valid) {
+	// cells in component-major order -- every lane's cell of its first
+	// refined component, then of its second, ... -- so that runs of cells
+	// on the LIFO share a component (and a resolution)
+	const bool mine = (int)lane < take && valid;
+	double x = 0, y = 0, hx = 0, hy = 0;
+	uint32_t comp_off = 0;
+	if (mine) {
 		const PkJob &jb = jobs[en.job];
 		const uint32_t i = en.pix % jb.dw, j = en.pix / jb.dw;
-		const double hx = jb.sx / 2., hy = jb.sy / 2.;
-		const double x = hx + i * jb.sx, y = hy + j * jb.sy;  // pixel centre (src/radial.cpp:313-314)
-		int pos = top + (int)(incl - nb);
-		uint32_t m = en.cmask;
-		while (m) {
+		hx = jb.sx / 2.;
+		hy = jb.sy / 2.;
+		x = hx + i * jb.sx;  // pixel centre (src/radial.cpp:313-314)
+		y = hy + j * jb.sy;
+		comp_off = jb.comp_off;
+	}
+	uint32_t m = mine ? en.cmask : 0u;
+	int pos = top;
+	for (;;) {
+		const uint64_t b = __ballot(m != 0);
+		if (!b) break;
+		if (m) {
 			const int c = __ffs(m) - 1;
 			m &= m - 1;
-			const uint32_t gc = jb.comp_off + (uint32_t)c;
+			const uint32_t gc = comp_off + (uint32_t)c;
 			const PkComp &cp = comps[gc];
 			uint32_t res = (uint32_t)cp.res, mr = (uint32_t)cp.max_rec;
 			if (cp.center_res && fabs(x - cp.xcen) < cp.win_x && fabs(y - cp.ycen) < cp.win_y) {
@@ -193,10 +207,11 @@ __device__ inline void r32_expand(const PkJob *__restrict__ jobs, const PkComp *
 			cl.gcomp = gc;
 			cl.entry = e;
 			cl.meta = (res << 8) | (mr << 16) | ((cp.convolve ? 1u : 0u) << 24);
-			cells[pos++] = cl;
+			cells[pos + __popcll(b & ((1ull << lane) - 1ull))] = cl;
 		}
+		pos += __popcll(b);
 	}
-	if (take > 0) top += (int)wave_read(incl, take - 1);
+	top = pos;
 	next += (uint32_t)take;
 }
 
@@ -218,14 +233,14 @@ __device__ inline bool r32_claim(uint32_t *head, uint32_t n, uint32_t chunk, uin
 }
 
 // Copies handed-over cells of claim indices [next, end) onto the LIFO, as many
-// as fit (at most 64); claim index i holds cell (i % 64) * stride + i / 64 if
-// that is below n.
+// as fit (at most 64); claim index i holds cell r32_entry_of(i, R32_CELL_RUN,
+// stride) if that is below n.
 __device__ inline void r32_take_cells(const RCell *__restrict__ qin, uint32_t n, uint32_t stride, RCell *cells,
                                       int cap, int &top, uint32_t &next, uint32_t end, uint32_t lane)
 {
 	const uint32_t k = min(min(end - next, (uint32_t)(cap - top)), 64u);
 	const uint32_t i = next + lane;
-	const uint32_t c = (i & 63u) * stride + (i >> 6);
+	const uint32_t c = r32_entry_of(i, R32_CELL_RUN, stride);
 	const bool valid = lane < k && c < n;
 	const uint64_t b = __ballot(valid);
 	if (valid) cells[top + __popcll(b & ((1ull << lane) - 1ull))] = qin[c];
@@ -307,14 +322,17 @@ __global__ __launch_bounds__(64) void refine32_kernel(const PkJob *__restrict__ 
 	int top = 0;
 	bool source_done = false, handover = false;
 	const uint32_t n_cells_in = SOURCE == 0 ? 0u : min(cnt->n_cells[pass - 1], qcap);
-	// Handed-over cells are claimed in a transposed order -- claim index i takes
-	// cell (i % 64) * stride + i / 64 -- so that the cells one wavefront handed
-	// over, often parts of the same deep pixel, go to many wavefronts.
-	const uint32_t stride = (n_cells_in + 63u) / 64u;
+	// Handed-over cells are claimed in runs of 2^R32_CELL_RUN taken from 64
+	// places of the queue in turn, so that the cells one wavefront handed
+	// over, often parts of the same deep pixel, go to many wavefronts, while
+	// each run (cells handed over together: mostly one component and
+	// resolution) stays together on one LIFO.
+	const uint32_t stride = ((((n_cells_in + (1u << R32_CELL_RUN) - 1u) >> R32_CELL_RUN) + 63u) / 64u);
 	const uint32_t n_entries = cnt->n_entries;
 	const uint32_t spread = SOURCE == 0 ? tune.spread_log2 : 0u;
 	const uint32_t sstride = ((((n_entries + (1u << spread) - 1u) >> spread) + 63u) / 64u);
-	const uint32_t n_source = SOURCE == 0 ? (spread ? (64u * sstride) << spread : n_entries) : 64u * stride;
+	const uint32_t n_source =
+	    SOURCE == 0 ? (spread ? (64u * sstride) << spread : n_entries) : (64u * stride) << R32_CELL_RUN;
 	uint32_t *head = &cnt->claim[pass];
 	uint32_t overflow = 0, bad = 0;
 	unsigned long long nev = 0;
@@ -329,16 +347,13 @@ __global__ __launch_bounds__(64) void refine32_kernel(const PkJob *__restrict__ 
 	uint32_t run_g = ~0u, run_g2 = ~0u;
 	__syncthreads();
 	for (;;) {
-		// ---- refill while the cells on the LIFO cannot fill the lanes (two
-		// sub-cells per lane) ----
+		// ---- refill while the cells on the LIFO cannot fill the lanes (one
+		// row of sub-cells per lane) ----
 		for (;;) {
-			if (top >= 128) break;  // at least one sub-cell per cell
-			uint32_t r2 = 0;
-			if ((int)lane < top) {
-				const uint32_t res = cm_res(st.cell[top - 1 - (int)lane].meta);
-				r2 = res * res;
-			}
-			if (wave_sum_u32(r2) >= 128u || source_done) break;
+			if (top >= 64) break;  // at least one row per cell
+			uint32_t rows = 0;
+			if ((int)lane < top) rows = cm_res(st.cell[top - 1 - (int)lane].meta);
+			if (wave_sum_u32(rows) >= 64u || source_done) break;
 			if (CAP - top < (SOURCE == 0 ? 32 : 1)) break;
 			if (chunk_next >= chunk_end &&
 			    !r32_claim(head, n_source, SOURCE == 0 ? tune.chunk : tune.chunk_cells, seen, chunk_next, chunk_end,
@@ -366,67 +381,29 @@ __global__ __launch_bounds__(64) void refine32_kernel(const PkJob *__restrict__ 
 		}
 
 		// ---- cells on top of the LIFO -> lanes ----
-		// A group takes cells from the top while their sub-cells fit the lanes:
-		// 64 sub-cells, one per lane, or -- when the group then holds more than
-		// 64 and every resolution is even -- 128, two adjacent sub-cells
-		// (si, sj) and (si, sj + 1) per lane, which then always belong to the
-		// same cell (even sub-cell index, even resolution: sj is even).  The
-		// per-group and per-round work is then shared by two sub-cells.
-		uint32_t r2 = 0;
-		if ((int)lane < top && lane < (uint32_t)R32_TASKS) {
-			const uint32_t res = cm_res(st.cell[top - 1 - (int)lane].meta);
-			r2 = res * res;
+		// A group is the run of cells from the top that share the top cell's
+		// resolution res: 64 / res of them (at most R32_TASKS).  Each lane
+		// takes one row si of one cell and walks that row's res sub-cells
+		// (si, 0 .. res - 1), so the per-group work (task map, component
+		// record, cell geometry, sums) is shared by res sub-cells.  A cell of
+		// more than 64 rows takes several rounds of 64 rows.
+		const uint32_t res = cm_res(st.cell[top - 1].meta);
+		bool same = false;
+		if ((int)lane < top && lane < (uint32_t)R32_TASKS) same = cm_res(st.cell[top - 1 - (int)lane].meta) == res;
+		const uint64_t not_same = ~__ballot(same);
+		const int run_len = not_same ? __ffsll((long long)not_same) - 1 : 64;  // >= 1: the top cell itself
+		const int ntask = min(run_len, res >= 64u ? 1 : (int)(64u / max(res, 1u)));
+		const int rounds = res > 64u ? (int)((res + 63u) / 64u) : 1;
+		// lane -> task t_local (cell top - 1 - t_local), first row si0
+		int t_local = 0, si0 = (int)lane;
+		if (res <= 64u) {
+			t_local = (int)((float)lane * frcp((float)res));
+			if ((uint32_t)t_local * res > lane) t_local--;
+			else if ((uint32_t)(t_local + 1) * res <= lane) t_local++;
+			si0 = (int)lane - t_local * (int)res;
 		}
-		const uint32_t incl = wave_incl_scan_u32(r2);
-		const uint32_t r2_0 = wave_read(r2, 0);
-		const uint64_t fit128 = __ballot((int)lane < top && lane < (uint32_t)R32_TASKS && incl <= 128u);
-		const int n128 = fit128 ? __popcll(fit128) : 0;
-		bool pair;
-		int ntask;
-		uint32_t n_sub;  // sub-cells of the group
-		if (n128 == 0) {
-			// the top cell alone has more than 128 sub-cells
-			ntask = 1;
-			n_sub = r2_0;
-			pair = (r2_0 & 1u) == 0;
-		} else {
-			const uint32_t in128 = wave_read(incl, n128 - 1);
-			pair = in128 > 64u && __ballot((int)lane < n128 && (r2 & 1u)) == 0;
-			if (pair) {
-				ntask = n128;
-				n_sub = in128;
-			} else {
-				const uint64_t fit = __ballot((int)lane < top && lane < (uint32_t)R32_TASKS && incl <= 64u);
-				ntask = fit ? __popcll(fit) : 1;
-				n_sub = wave_read(incl, ntask - 1);
-			}
-		}
-		const uint32_t per_lane = pair ? 2u : 1u;
-		const int rounds = (int)((n_sub + 64u * per_lane - 1u) / (64u * per_lane));
-		// task of this lane's first sub-cell g0 = per_lane * lane: the first
-		// whose inclusive end exceeds g0
-		const uint32_t g0 = per_lane * lane;
-		int t_local = 0;
-		uint32_t before = 0;
-		if (rounds > 1) {
-			// one cell over several rounds
-		} else if (__ballot((int)lane < ntask && r2 != r2_0) == 0) {
-			// all tasks of one resolution (the usual case): segments of r2_0 sub-cells
-			t_local = (int)((float)g0 * frcp((float)r2_0));
-			if ((uint32_t)t_local * r2_0 > g0) t_local--;
-			else if ((uint32_t)(t_local + 1) * r2_0 <= g0) t_local++;
-			before = (uint32_t)t_local * r2_0;
-			if (t_local >= ntask) t_local = ntask - 1;  // lane beyond the last task: inactive
-		} else {
-			for (int t = 0; t < ntask; t++) {
-				const uint32_t it = wave_read(incl, t);
-				if (it <= g0) {
-					t_local = t + 1;
-					before = it;
-				}
-			}
-			if (t_local >= ntask) t_local = ntask - 1;
-		}
+		const bool lane_task = t_local < ntask;
+		if (!lane_task) t_local = ntask - 1;  // beyond the last task: inactive, mirrors the last task
 		const RCell cell = st.cell[top - 1 - t_local];
 		// component records of the tasks -> LDS (kept while unchanged); lane
 		// t < ntask keeps the accumulator of task t
@@ -449,199 +426,106 @@ __global__ __launch_bounds__(64) void refine32_kernel(const PkJob *__restrict__ 
 			}
 		}
 		top -= ntask;
-		const CompLite &cl = st.lite[t_local];
+		// the task's component record, copied into registers: read inside the
+		// sub-cell loop from LDS it would be reloaded at every sub-cell
+		const CompLite cl = st.lite[t_local];
 		const uint32_t meta = cell.meta;
-		const int level = (int)cm_level(meta), res = (int)cm_res(meta), mr = (int)cm_maxrec(meta);
-		const float inv_res = 1.f / (float)res;
+		const int level = (int)cm_level(meta), mr = (int)cm_maxrec(meta);
 		// sub-cell size in pixels, in FP64 as libprofit forms the sub-pixel
 		// grid: an FP32 size would move the sub-cell centres by ~1e-8 px, which
 		// next to a cusp (coresersic, f ~ r^-b) changes the innermost values
 		// by ~1e-4
-		const double fr = cell_size_fp64(res, level + 1);
+		const double fr = cell_size_fp64((int)res, level + 1);
 		const double hx = fr * cl.hsx, hy = fr * cl.hsy;
 		const float wq = (float)(fr * fr * cl.wq);
-		const bool recurse = res > 1 && level < mr;
+		const bool recurse = res > 1u && level < mr;
 		const float acc_thr = (float)cl.acc;
 		const float band = 1e-4f * (1.f + acc_thr);
 		const double ddy = (-2 * hx * cl.sin_ang + 2 * hy * cl.cos_ang) * cl.inv_axrat;
 		const size_t target = 2 * (size_t)cell.entry + cm_conv(meta);
 		const double gw = GRAD ? gent[target] : 0.0;  // G / fx_scale of the cell's pixel
 		const int kind = cl.fp32 ? (int)cl.kind : -1;
-		// wavefront-uniform facts of this group of rounds (lanes beyond the last
-		// task, which are inactive, share the last task's)
+		// wavefront-uniform facts of the group (lanes beyond the last task
+		// share the last task's)
 		const bool any_rec = __ballot(recurse) != 0;
 		const int kind0 = (int)wave_read((uint32_t)kind, 0);
 		const bool one_kind = __ballot(kind != kind0) == 0;
-		auto run_rounds = [&](auto per_lane_tag) {
-			constexpr int P = decltype(per_lane_tag)::value;
-			for (int rd = 0; rd < rounds; rd++) {
-				const uint32_t g = P * lane + 64u * P * (uint32_t)rd;
-				// sub-cell s = si * res + sj, found without an integer division
-				const int s = (int)(g - before);
-				int si = (int)((float)s * inv_res);
-				int sj = s - si * res;
-				if (sj >= res) {
-					si++;
-					sj -= res;
-				} else if (sj < 0) {
-					si--;
-					sj += res;
-				}
-				const double dx = fma((double)(2 * si + 1), hx, cell.dx0);
-				double dy[P], xp[P], yp[P];
-				float v32[P], t32[P];
-				bool active[P], fail[P], decided[P];
-#pragma unroll
-				for (int q = 0; q < P; q++) {
-					active[q] = g + (uint32_t)q < n_sub;
-					dy[q] = fma((double)(2 * (sj + q) + 1), hy, cell.dy0);
-					xp[q] = fma(dx, cl.cos_ang, dy[q] * cl.sin_ang);
-					yp[q] = fma(dy[q], cl.cos_ang, -dx * cl.sin_ang) * cl.inv_axrat;
-					v32[q] = -1.f;
-					t32[q] = -1.f;
-				}
+		// all tasks add to one pixel accumulator (one cell, or siblings)
+		const uint64_t tu = (uint64_t)wave_read((uint32_t)(task_target >> 32), 0) << 32 |
+		                    wave_read((uint32_t)task_target, 0);
+		const bool one_target = __ballot((int)lane < ntask && task_target != tu) == 0;
+		const uint32_t nev_sub = recurse ? 2u : 1u;
+		for (int rd = 0; rd < rounds; rd++) {
+			const int si = si0 + 64 * rd;
+			const bool active = lane_task && si < (int)res;
+			const double dx = fma((double)(2 * si + 1), hx, cell.dx0);
+			const double dxc = dx * cl.cos_ang, ndxs = -dx * cl.sin_ang;
+			unsigned long long fxv = 0;  // fixed-point sum of this lane's accepted sub-cells
+			double ga[GA_SLOTS] = {0, 0, 0, 0, 0, 0, 0, 0};
+			bool any_ok = false;
+			for (int sj = 0; sj < (int)res; sj++) {
+				const double dy = fma((double)(2 * sj + 1), hy, cell.dy0);
+				const double xp = fma(dy, cl.sin_ang, dxc);
+				const double yp = fma(dy, cl.cos_ang, ndxs) * cl.inv_axrat;
+				float v32 = -1.f, t32 = -1.f;
 				if (one_kind) {
 					// the usual case: one profile kind in all lanes, scalar branch
-					if (any_rec) {
-#pragma unroll
-						for (int q = 0; q < P; q++)
-							evaluate_pair32<true>(cl, kind0, xp[q], yp[q], fabs(xp[q]), fabs(yp[q]) + fabs(ddy), v32[q],
-							                      t32[q]);
-					} else {
-#pragma unroll
-						for (int q = 0; q < P; q++)
-							evaluate_pair32<false>(cl, kind0, xp[q], yp[q], 0., 0., v32[q], t32[q]);
-					}
+					if (any_rec) evaluate_pair32<true>(cl, kind0, xp, yp, fabs(xp), fabs(yp) + fabs(ddy), v32, t32);
+					else evaluate_pair32<false>(cl, kind0, xp, yp, 0., 0., v32, t32);
 				} else {
-#pragma unroll
-					for (int q = 0; q < P; q++)
-						evaluate_pair32<true>(cl, kind, xp[q], yp[q], fabs(xp[q]), fabs(yp[q]) + fabs(ddy), v32[q],
-						                      t32[q]);
+					evaluate_pair32<true>(cl, kind, xp, yp, fabs(xp), fabs(yp) + fabs(ddy), v32, t32);
 				}
-				unsigned long long fxv = 0;  // fixed-point sum of this lane's accepted sub-cells
-				bool any_undecided = false;
-#pragma unroll
-				for (int q = 0; q < P; q++) {
-					fail[q] = false;
-					if (recurse) {
-						const float m = fabsf(t32[q] * frcp(v32[q]) - 1.f);
-						decided[q] = v32[q] > 0.f && v32[q] <= 3e38f && m <= 3e38f && fabsf(m - acc_thr) > band;
-						fail[q] = decided[q] && m > acc_thr;
-					} else {
-						decided[q] = v32[q] >= 0.f && v32[q] <= 3e38f;
-					}
+				bool fail = false, decided;
+				if (recurse) {
+					const float m = fabsf(t32 * frcp(v32) - 1.f);
+					decided = v32 > 0.f && v32 <= 3e38f && m <= 3e38f && fabsf(m - acc_thr) > band;
+					fail = decided && m > acc_thr;
+				} else {
+					decided = v32 >= 0.f && v32 <= 3e38f;
+				}
+				if (!GRAD && active && decided && !fail) {
 					unsigned long long f = 0;
-					if (!GRAD && active[q] && decided[q] && !fail[q]) {
-						if (fixed_point_f32(v32[q] * wq, &f)) fxv += f;
-						else bad++;
-					}
-					any_undecided = any_undecided || (active[q] && !decided[q]);
+					if (fixed_point_f32(v32 * wq, &f)) fxv += f;
+					else bad++;
 				}
-				if (__ballot(any_undecided)) {
+				if (__ballot(active && !decided)) {
 					// FP64 value and decision within the guard band (or without an FP32 form)
-#pragma unroll
-					for (int q = 0; q < P; q++) {
-						if (active[q] && !decided[q]) {
-							const double val =
-							    refine_fp64_decision(comps[cell.gcomp], xp[q], yp[q], ddy, recurse, &fail[q]);
-							if (!GRAD && !fail[q]) {
-								const double qv = val * (fr * fr * cl.wflux);
-								if (qv >= 0 && qv < 9.2e18) fxv += (unsigned long long)__double2ll_rn(qv);
-								else bad++;
-							}
+					if (active && !decided) {
+						const double val = refine_fp64_decision(comps[cell.gcomp], xp, yp, ddy, recurse, &fail);
+						if (!GRAD && !fail) {
+							const double qv = val * (fr * fr * cl.wflux);
+							if (qv >= 0 && qv < 9.2e18) fxv += (unsigned long long)__double2ll_rn(qv);
+							else bad++;
 						}
 					}
 				}
-#pragma unroll
-				for (int q = 0; q < P; q++) {
-					fail[q] = fail[q] && active[q];
-					if (active[q]) nev += recurse ? 2 : 1;
-				}
-				if (GRAD) {
-					double ga[GA_SLOTS] = {0, 0, 0, 0, 0, 0, 0, 0};
-					bool any_ok = false;
-#pragma unroll
-					for (int q = 0; q < P; q++) {
-						if (active[q] && !fail[q]) {
-							any_ok = true;
-							float gt[GT_COUNT];
-							const float vg = sersic_grad32(cl, (float)xp[q], (float)yp[q], gt);
-							grad_accumulate(ga, gw * (fr * fr * cl.wq) * (double)vg, gt);
-						}
-					}
-					const uint64_t okb = __ballot(any_ok);
-					if (okb) {
-						const uint32_t g0 = wave_read(cell.gcomp, __ffsll((long long)okb) - 1);
-						const uint64_t ob = __ballot(any_ok && cell.gcomp != g0);
-						const uint32_t g1 = ob ? wave_read(cell.gcomp, __ffsll((long long)ob) - 1) : ~0u;
-						if (!ob || __ballot(any_ok && cell.gcomp != g0 && cell.gcomp != g1) == 0) {
-							// at most two components: keep adding to the running sums,
-							// after handing on a slot held by a third one
-							auto hold = [&](uint32_t g, uint32_t keep) {
-								if (g == ~0u || g == run_g || g == run_g2) return;
-								if (run_g != keep) {
-									r32_flush_run(run, run_g, gacc, lane);
-									run_g = g;
-								} else {
-									r32_flush_run(run2, run_g2, gacc, lane);
-									run_g2 = g;
-								}
-							};
-							hold(g0, g1);
-							hold(g1, g0);
-							const bool in1 = any_ok && cell.gcomp == run_g, in2 = any_ok && cell.gcomp == run_g2;
-#pragma unroll
-							for (int k = 0; k < GA_SLOTS; k++) {
-								run[k] += in1 ? ga[k] : 0.0;
-								run2[k] += in2 ? ga[k] : 0.0;
-							}
-						} else {
-							r32_flush_grad(any_ok, cell.gcomp, ga, gacc, lane);
-						}
-					}
-				} else if (ntask == 1) {
-					// one cell in all lanes: exact per-lane sums, added to the
-					// pixel accumulator (lane 0) when the pixel changes
-					const uint64_t tu =
-					    (uint64_t)wave_read((uint32_t)(target >> 32), 0) << 32 | wave_read((uint32_t)target, 0);
-					if (tu != run_t) {
-						r32_flush_fx(run_fx, run_t, acc, lane);
-						run_t = tu;
-					}
-					run_fx += fxv;
-				} else {
-					// several cells: exact integer sums per cell in LDS, then one
-					// global atomic per cell (lane t adds task t's sum)
-					if (fxv) atomicAdd(&st.csum[t_local], fxv);
-					__builtin_amdgcn_wave_barrier();
-					if ((int)lane < ntask) {
-						const unsigned long long v = st.csum[lane];
-						st.csum[lane] = 0;
-						if (v) atomicAdd(&acc[task_target], v);
-					}
+				fail = fail && active;
+				if (GRAD && active && !fail) {
+					any_ok = true;
+					float gt[GT_COUNT];
+					const float vg = sersic_grad32(cl, (float)xp, (float)yp, gt);
+					grad_accumulate(ga, gw * (fr * fr * cl.wq) * (double)vg, gt);
 				}
 				// failing sub-cells: onto the LIFO, or to the next pass when it is full
-#pragma unroll
-				for (int q = 0; q < P; q++) {
-					const uint64_t ballot = __ballot(fail[q]);
-					if (!ballot) continue;
+				const uint64_t ballot = __ballot(fail);
+				if (ballot) {
 					const int n = __popcll(ballot);
 					const int rank = __popcll(ballot & ((1ull << lane) - 1ull));
 					RCell c;
 					c.dx0 = dx - hx;
-					c.dy0 = dy[q] - hy;
+					c.dy0 = dy - hy;
 					c.gcomp = cell.gcomp;
 					c.entry = cell.entry;
 					c.meta = (meta & ~255u) | (uint32_t)(level + 1);
 					if (top + n <= CAP) {
-						if (fail[q]) st.cell[top + rank] = c;
+						if (fail) st.cell[top + rank] = c;
 						top += n;
 					} else if (!LAST) {
 						const int leader = __ffsll((long long)ballot) - 1;
 						uint32_t base = 0;
 						if ((int)lane == leader) base = atomicAdd(&cnt->n_cells[pass], (uint32_t)n);
 						base = wave_read(base, leader);
-						if (fail[q]) {
+						if (fail) {
 							if (base + (uint32_t)rank < qcap) qout[base + rank] = c;
 							else overflow = 1;
 						}
@@ -649,11 +533,60 @@ __global__ __launch_bounds__(64) void refine32_kernel(const PkJob *__restrict__ 
 						overflow = 1;
 					}
 				}
-				__syncthreads();
 			}
-		};
-		if (pair) run_rounds(IntTag<2>{});
-		else run_rounds(IntTag<1>{});
+			if (active) nev += (unsigned long long)(nev_sub * res);
+			if (GRAD) {
+				const uint64_t okb = __ballot(any_ok);
+				if (okb) {
+					const uint32_t g0 = wave_read(cell.gcomp, __ffsll((long long)okb) - 1);
+					const uint64_t ob = __ballot(any_ok && cell.gcomp != g0);
+					const uint32_t g1 = ob ? wave_read(cell.gcomp, __ffsll((long long)ob) - 1) : ~0u;
+					if (!ob || __ballot(any_ok && cell.gcomp != g0 && cell.gcomp != g1) == 0) {
+						// at most two components: keep adding to the running sums,
+						// after handing on a slot held by a third one
+						auto hold = [&](uint32_t g, uint32_t keep) {
+							if (g == ~0u || g == run_g || g == run_g2) return;
+							if (run_g != keep) {
+								r32_flush_run(run, run_g, gacc, lane);
+								run_g = g;
+							} else {
+								r32_flush_run(run2, run_g2, gacc, lane);
+								run_g2 = g;
+							}
+						};
+						hold(g0, g1);
+						hold(g1, g0);
+						const bool in1 = any_ok && cell.gcomp == run_g, in2 = any_ok && cell.gcomp == run_g2;
+#pragma unroll
+						for (int k = 0; k < GA_SLOTS; k++) {
+							run[k] += in1 ? ga[k] : 0.0;
+							run2[k] += in2 ? ga[k] : 0.0;
+						}
+					} else {
+						r32_flush_grad(any_ok, cell.gcomp, ga, gacc, lane);
+					}
+				}
+			} else if (one_target) {
+				// one pixel accumulator for all lanes: exact per-lane sums, added
+				// to it (lane 0) when the pixel changes
+				if (tu != run_t) {
+					r32_flush_fx(run_fx, run_t, acc, lane);
+					run_t = tu;
+				}
+				run_fx += fxv;
+			} else {
+				// several pixels: exact integer sums per cell in LDS, then one
+				// global atomic per cell (lane t adds task t's sum)
+				if (fxv) atomicAdd(&st.csum[t_local], fxv);
+				__builtin_amdgcn_wave_barrier();
+				if ((int)lane < ntask) {
+					const unsigned long long v = st.csum[lane];
+					st.csum[lane] = 0;
+					if (v) atomicAdd(&acc[task_target], v);
+				}
+			}
+			__syncthreads();
+		}
 	}
 	if (GRAD) {
 		r32_flush_run(run, run_g, gacc, lane);
