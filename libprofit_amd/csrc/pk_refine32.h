@@ -129,16 +129,6 @@ __device__ __attribute__((noinline)) double refine_fp64_decision(const PkComp &c
 	return v;
 }
 
-// Non-negative value below 2^63 -> 64-bit integer (truncated); false if out of range.
-__device__ inline bool fixed_point_f32(float q, unsigned long long *out)
-{
-	if (!(q >= 0.f && q < 9.2e18f)) return false;
-	const float hi = truncf(q * 0x1p-32f);
-	const float lo = fmaf(hi, -0x1p32f, q);  // exact
-	*out = ((unsigned long long)(uint32_t)hi << 32) | (uint32_t)lo;
-	return true;
-}
-
 // Entry of pass-0 claim index i: with spreading (L = spread_log2 > 0) the
 // entry list is cut into runs of 2^L entries, and consecutive runs of claim
 // indices come from 64 places of the list in turn (run b -> (b % 64) * sstride
@@ -475,18 +465,23 @@ __global__ __launch_bounds__(64) void refine32_kernel(const PkJob *__restrict__ 
 				} else {
 					evaluate_pair32<true>(cl, kind, xp, yp, fabs(xp), fabs(yp) + fabs(ddy), v32, t32);
 				}
-				bool fail = false, decided;
-				if (recurse) {
-					const float m = fabsf(t32 * frcp(v32) - 1.f);
-					decided = v32 > 0.f && v32 <= 3e38f && m <= 3e38f && fabsf(m - acc_thr) > band;
-					fail = decided && m > acc_thr;
-				} else {
-					decided = v32 >= 0.f && v32 <= 3e38f;
-				}
-				if (!GRAD && active && decided && !fail) {
-					unsigned long long f = 0;
-					if (fixed_point_f32(v32 * wq, &f)) fxv += f;
-					else bad++;
+				// accuracy test (src/radial.cpp:86-109) in FP32, decided outside a
+				// guard band around the threshold; written without branches
+				const float m = fabsf(t32 * frcp(v32) - 1.f);
+				const bool dec_rec = (v32 > 0.f) & (v32 <= 3e38f) & (m <= 3e38f) & (fabsf(m - acc_thr) > band);
+				const bool decided = recurse ? dec_rec : (v32 >= 0.f) & (v32 <= 3e38f);
+				bool fail = recurse & dec_rec & (m > acc_thr);
+				if (!GRAD) {
+					// accepted value as a 64-bit fixed-point integer (truncated)
+					const bool take = active & decided & !fail;
+					const float q = v32 * wq;
+					const bool q_ok = (q >= 0.f) & (q < 9.2e18f);
+					const float qc = q_ok ? q : 0.f;
+					const float hi = truncf(qc * 0x1p-32f);
+					const float lo = fmaf(hi, -0x1p32f, qc);  // exact
+					const unsigned long long f = ((unsigned long long)(uint32_t)hi << 32) | (uint32_t)lo;
+					fxv += (take & q_ok) ? f : 0ull;
+					bad += (take & !q_ok) ? 1u : 0u;
 				}
 				if (__ballot(active && !decided)) {
 					// FP64 value and decision within the guard band (or without an FP32 form)
