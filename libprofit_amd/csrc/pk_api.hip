@@ -942,9 +942,14 @@ int launch_profiles(hipStream_t s, bool fp32, const PkJob *djobs, uint32_t njobs
 		const uint32_t gm = std::min(grid_main, ref_blocks), gl = std::min(grid_last, ref_blocks);
 		hipLaunchKernelGGL((refine32_kernel<0, false>), dim3(gm), dim3(64), 0, s, djobs, dcomp, dlite, dent, dcnt,
 		                   (const RCell *)nullptr, q0, qcap, 0, dacc, tune);
-		for (int p = 1; p < R32_PASSES - 1; p++)
+		hipLaunchKernelGGL(r32_resolve_kernel<false>, dim3(R32_RESOLVE_GRID), dim3(256), 0, s, dcomp, dlite, dcnt,
+		                   q0, qcap, 0, dacc, (const double *)nullptr, (double *)nullptr);
+		for (int p = 1; p < R32_PASSES - 1; p++) {
 			hipLaunchKernelGGL((refine32_kernel<1, false>), dim3(gm), dim3(64), 0, s, djobs, dcomp, dlite, dent, dcnt,
 			                   (const RCell *)((p & 1) ? q0 : q1), (p & 1) ? q1 : q0, qcap, p, dacc, tune);
+			hipLaunchKernelGGL(r32_resolve_kernel<false>, dim3(R32_RESOLVE_GRID), dim3(256), 0, s, dcomp, dlite,
+			                   dcnt, (p & 1) ? q1 : q0, qcap, p, dacc, (const double *)nullptr, (double *)nullptr);
+		}
 		hipLaunchKernelGGL((refine32_kernel<1, true>), dim3(gl), dim3(64), 0, s, djobs, dcomp, dlite, dent, dcnt,
 		                   (const RCell *)((R32_PASSES & 1) ? q1 : q0), (RCell *)nullptr, qcap, R32_PASSES - 1, dacc,
 		                   tune);
@@ -1833,10 +1838,16 @@ int grad_launch_t(pk_batch *b)
 		hipLaunchKernelGGL((refine32_kernel<0, false, true>), dim3(gm), dim3(64), 0, s, djobs, dcomp, dlite, dent,
 		                   gcnt, (const RCell *)nullptr, q0, b->qcap, 0, (unsigned long long *)nullptr, hd,
 		                   (const double *)gent, gacc);
-		for (int p = 1; p < R32_PASSES - 1; p++)
+		hipLaunchKernelGGL(r32_resolve_kernel<true>, dim3(R32_RESOLVE_GRID), dim3(256), 0, s, dcomp, dlite, gcnt, q0,
+		                   b->qcap, 0, (unsigned long long *)nullptr, (const double *)gent, gacc);
+		for (int p = 1; p < R32_PASSES - 1; p++) {
 			hipLaunchKernelGGL((refine32_kernel<1, false, true>), dim3(gm), dim3(64), 0, s, djobs, dcomp, dlite, dent,
 			                   gcnt, (const RCell *)((p & 1) ? q0 : q1), (p & 1) ? q1 : q0, b->qcap, p,
 			                   (unsigned long long *)nullptr, hd, (const double *)gent, gacc);
+			hipLaunchKernelGGL(r32_resolve_kernel<true>, dim3(R32_RESOLVE_GRID), dim3(256), 0, s, dcomp, dlite, gcnt,
+			                   (p & 1) ? q1 : q0, b->qcap, p, (unsigned long long *)nullptr, (const double *)gent,
+			                   gacc);
+		}
 		hipLaunchKernelGGL((refine32_kernel<1, true, true>), dim3(gl), dim3(64), 0, s, djobs, dcomp, dlite, dent, gcnt,
 		                   (const RCell *)((R32_PASSES & 1) ? q1 : q0), (RCell *)nullptr, b->qcap, R32_PASSES - 1,
 		                   (unsigned long long *)nullptr, hd, (const double *)gent, gacc);

@@ -98,6 +98,14 @@ struct alignas(128) CompLite {
 	int32_t pad_;
 };
 
+// A CompLite whose box flag is a compile-time constant: the FP32 profile
+// functions read c.boxy, so their round / boxy branches fold away.
+template <bool B>
+struct CompLiteB : CompLite {
+	static constexpr bool boxy = B;
+	PK_HD explicit CompLiteB(const CompLite &c) : CompLite(c) {}
+};
+
 PK_HD bool almost_equals(double x, double y) { return fabs(x - y) < 1e-10; }
 
 // ---------------------------------------------------------------------------

@@ -36,16 +36,21 @@ namespace pk {
 // and the Sersic centre pixels (res 8, 10 levels).
 constexpr int R32_CAP_MAIN = 192;
 constexpr int R32_CAP_LAST = 1024;
+// workgroups of the launches that decide handed-over sub-cells in FP64
+constexpr uint32_t R32_RESOLVE_GRID = 1024;
 constexpr uint32_t R32_CELL_RUN = 4;       // log2 of the runs in which handed-over cells are claimed
-constexpr int R32_TASKS = 16;              // cells per round at most (their records are staged in LDS)
+constexpr int R32_TASKS = 16;              // cells per round at most
+// Flags of RCell::meta in the queues between passes: a sub-cell handed over
+// undecided (its FP64 decision is made between the passes), and a slot whose
+// sub-cell was accepted there (skipped by the next pass).
+constexpr uint32_t R32_PENDING = 1u << 31;
+constexpr uint32_t R32_DEAD = 0xffffffffu;
 // untuned settings: claims of up to 64 pixels / 16 cells, the hand-over test
 // every 16 rounds, hand-over once a quarter of the wavefronts stopped
 constexpr R32Tune R32_TUNE_DEFAULT = {4u, 64u, 16u, 16u, 0u};
 
 template <int CAP>
 struct R32Stack {
-	CompLite lite[R32_TASKS];
-	uint32_t lite_g[R32_TASKS];   // component whose record lite[t] holds
 	unsigned long long csum[R32_TASKS];  // fixed-point sum of task t in the current round (0 between rounds)
 	RCell cell[CAP];
 };
@@ -106,6 +111,14 @@ __device__ inline double cell_size_fp64(int res, int level)
 	double p = 1.0;
 	for (int l = 0; l < level; l++) p *= (double)res;
 	return 1.0 / p;
+}
+
+// Keeps x in vector registers from here on (an empty asm that takes and
+// returns it in a VGPR).
+template <typename T>
+__device__ inline void pin_vgpr(T &x)
+{
+	asm volatile("" : "+v"(x));
 }
 
 // cell metadata: level | res << 8 | max recursions << 16 | convolved << 24
@@ -231,7 +244,8 @@ __device__ inline void r32_take_cells(const RCell *__restrict__ qin, uint32_t n,
 	const uint32_t k = min(min(end - next, (uint32_t)(cap - top)), 64u);
 	const uint32_t i = next + lane;
 	const uint32_t c = r32_entry_of(i, R32_CELL_RUN, stride);
-	const bool valid = lane < k && c < n;
+	// dead slots: sub-cells accepted between the passes
+	const bool valid = lane < k && c < n && qin[c].meta != R32_DEAD;
 	const uint64_t b = __ballot(valid);
 	if (valid) cells[top + __popcll(b & ((1ull << lane) - 1ull))] = qin[c];
 	top += __popcll(b);
@@ -305,10 +319,7 @@ __global__ __launch_bounds__(64) void refine32_kernel(const PkJob *__restrict__ 
 	constexpr int CAP = LAST ? R32_CAP_LAST : R32_CAP_MAIN;
 	__shared__ R32Stack<CAP> st;
 	const uint32_t lane = lane_id();
-	if (lane < (uint32_t)R32_TASKS) {
-		st.lite_g[lane] = ~0u;
-		st.csum[lane] = 0;
-	}
+	if (lane < (uint32_t)R32_TASKS) st.csum[lane] = 0;
 	int top = 0;
 	bool source_done = false, handover = false;
 	const uint32_t n_cells_in = SOURCE == 0 ? 0u : min(cnt->n_cells[pass - 1], qcap);
@@ -372,16 +383,29 @@ __global__ __launch_bounds__(64) void refine32_kernel(const PkJob *__restrict__ 
 
 		// ---- cells on top of the LIFO -> lanes ----
 		// A group is the run of cells from the top that share the top cell's
-		// resolution res: 64 / res of them (at most R32_TASKS).  Each lane
+		// component and metadata (resolution res, recursion depth, convolved
+		// flag; their levels may differ): 64 / res of them at most (and at
+		// most R32_TASKS).  The component record of a group is wavefront-
+		// uniform (scalar loads), so the profile kind and the box flag are
+		// branched on once per group, outside the sub-cell loop.  Each lane
 		// takes one row si of one cell and walks that row's res sub-cells
-		// (si, 0 .. res - 1), so the per-group work (task map, component
-		// record, cell geometry, sums) is shared by res sub-cells.  A cell of
-		// more than 64 rows takes several rounds of 64 rows.
-		const uint32_t res = cm_res(st.cell[top - 1].meta);
+		// (si, 0 .. res - 1), so the per-group work (task map, cell geometry,
+		// sums) is shared by res sub-cells.  A cell of more than 64 rows takes
+		// several rounds of 64 rows.
+		const uint32_t g0 = wave_read(st.cell[top - 1].gcomp, 0);
+		const uint32_t key = wave_read(st.cell[top - 1].meta >> 8, 0);
 		bool same = false;
-		if ((int)lane < top && lane < (uint32_t)R32_TASKS) same = cm_res(st.cell[top - 1 - (int)lane].meta) == res;
+		size_t task_target = 0;  // lane t < ntask: pixel accumulator of task t
+		if ((int)lane < top && lane < (uint32_t)R32_TASKS) {
+			const RCell &tc = st.cell[top - 1 - (int)lane];
+			same = tc.gcomp == g0 && (tc.meta >> 8) == key;
+			task_target = 2 * (size_t)tc.entry + ((key >> 16) & 1u);
+		}
 		const uint64_t not_same = ~__ballot(same);
 		const int run_len = not_same ? __ffsll((long long)not_same) - 1 : 64;  // >= 1: the top cell itself
+		const uint32_t res = key & 255u;
+		const int mr = (int)((key >> 8) & 255u);
+		const uint32_t conv = (key >> 16) & 1u;
 		const int ntask = min(run_len, res >= 64u ? 1 : (int)(64u / max(res, 1u)));
 		const int rounds = res > 64u ? (int)((res + 63u) / 64u) : 1;
 		// lane -> task t_local (cell top - 1 - t_local), first row si0
@@ -395,192 +419,252 @@ __global__ __launch_bounds__(64) void refine32_kernel(const PkJob *__restrict__ 
 		const bool lane_task = t_local < ntask;
 		if (!lane_task) t_local = ntask - 1;  // beyond the last task: inactive, mirrors the last task
 		const RCell cell = st.cell[top - 1 - t_local];
-		// component records of the tasks -> LDS (kept while unchanged); lane
-		// t < ntask keeps the accumulator of task t
-		size_t task_target = 0;
-		{
-			uint32_t g = 0;
-			bool miss = false;
-			if ((int)lane < ntask) {
-				const RCell &tc = st.cell[top - 1 - (int)lane];
-				g = tc.gcomp;
-				task_target = 2 * (size_t)tc.entry + cm_conv(tc.meta);
-				miss = st.lite_g[lane] != g;
-			}
-			if (__ballot(miss)) {
-				if (miss) {
-					st.lite[lane] = lite[g];
-					st.lite_g[lane] = g;
-				}
-				__syncthreads();
-			}
-		}
 		top -= ntask;
-		// the task's component record, copied into registers: read inside the
-		// sub-cell loop from LDS it would be reloaded at every sub-cell
-		const CompLite cl = st.lite[t_local];
-		const uint32_t meta = cell.meta;
-		const int level = (int)cm_level(meta), mr = (int)cm_maxrec(meta);
+		// the group's component record: one scalar load per group
+		const CompLite L = lite[g0];
+		const int level = (int)cm_level(cell.meta);
 		// sub-cell size in pixels, in FP64 as libprofit forms the sub-pixel
 		// grid: an FP32 size would move the sub-cell centres by ~1e-8 px, which
 		// next to a cusp (coresersic, f ~ r^-b) changes the innermost values
 		// by ~1e-4
 		const double fr = cell_size_fp64((int)res, level + 1);
-		const double hx = fr * cl.hsx, hy = fr * cl.hsy;
-		const float wq = (float)(fr * fr * cl.wq);
+		const double hx = fr * L.hsx, hy = fr * L.hsy;
+		const double wrow = fr * fr * L.wq;  // weight of an accepted FP32 value
 		const bool recurse = res > 1u && level < mr;
-		const float acc_thr = (float)cl.acc;
+		const float acc_thr = (float)L.acc;
 		const float band = 1e-4f * (1.f + acc_thr);
-		const double ddy = (-2 * hx * cl.sin_ang + 2 * hy * cl.cos_ang) * cl.inv_axrat;
-		const size_t target = 2 * (size_t)cell.entry + cm_conv(meta);
+		const double ddy = (-2 * hx * L.sin_ang + 2 * hy * L.cos_ang) * L.inv_axrat;
+		const size_t target = 2 * (size_t)cell.entry + conv;
 		const double gw = GRAD ? gent[target] : 0.0;  // G / fx_scale of the cell's pixel
-		const int kind = cl.fp32 ? (int)cl.kind : -1;
-		// wavefront-uniform facts of the group (lanes beyond the last task
-		// share the last task's)
 		const bool any_rec = __ballot(recurse) != 0;
-		const int kind0 = (int)wave_read((uint32_t)kind, 0);
-		const bool one_kind = __ballot(kind != kind0) == 0;
 		// all tasks add to one pixel accumulator (one cell, or siblings)
-		const uint64_t tu = (uint64_t)wave_read((uint32_t)(task_target >> 32), 0) << 32 |
-		                    wave_read((uint32_t)task_target, 0);
-		const bool one_target = __ballot((int)lane < ntask && task_target != tu) == 0;
+		const uint64_t tu = (uint64_t)wave_read((uint32_t)(target >> 32), 0) << 32 | wave_read((uint32_t)target, 0);
+		const bool one_target = __ballot(lane_task && target != tu) == 0;
 		const uint32_t nev_sub = recurse ? 2u : 1u;
-		for (int rd = 0; rd < rounds; rd++) {
-			const int si = si0 + 64 * rd;
-			const bool active = lane_task && si < (int)res;
-			const double dx = fma((double)(2 * si + 1), hx, cell.dx0);
-			const double dxc = dx * cl.cos_ang, ndxs = -dx * cl.sin_ang;
-			unsigned long long fxv = 0;  // fixed-point sum of this lane's accepted sub-cells
-			double ga[GA_SLOTS] = {0, 0, 0, 0, 0, 0, 0, 0};
-			bool any_ok = false;
-			for (int sj = 0; sj < (int)res; sj++) {
-				const double dy = fma((double)(2 * sj + 1), hy, cell.dy0);
-				const double xp = fma(dy, cl.sin_ang, dxc);
-				const double yp = fma(dy, cl.cos_ang, ndxs) * cl.inv_axrat;
-				float v32 = -1.f, t32 = -1.f;
-				if (one_kind) {
-					// the usual case: one profile kind in all lanes, scalar branch
-					if (any_rec) evaluate_pair32<true>(cl, kind0, xp, yp, fabs(xp), fabs(yp) + fabs(ddy), v32, t32);
-					else evaluate_pair32<false>(cl, kind0, xp, yp, 0., 0., v32, t32);
-				} else {
-					evaluate_pair32<true>(cl, kind, xp, yp, fabs(xp), fabs(yp) + fabs(ddy), v32, t32);
-				}
-				// accuracy test (src/radial.cpp:86-109) in FP32, decided outside a
-				// guard band around the threshold; written without branches
-				const float m = fabsf(t32 * frcp(v32) - 1.f);
-				const bool dec_rec = (v32 > 0.f) & (v32 <= 3e38f) & (m <= 3e38f) & (fabsf(m - acc_thr) > band);
-				const bool decided = recurse ? dec_rec : (v32 >= 0.f) & (v32 <= 3e38f);
-				bool fail = recurse & dec_rec & (m > acc_thr);
-				if (!GRAD) {
-					// accepted value as a 64-bit fixed-point integer (truncated)
-					const bool take = active & decided & !fail;
-					const float q = v32 * wq;
-					const bool q_ok = (q >= 0.f) & (q < 9.2e18f);
-					const float qc = q_ok ? q : 0.f;
-					const float hi = truncf(qc * 0x1p-32f);
-					const float lo = fmaf(hi, -0x1p32f, qc);  // exact
-					const unsigned long long f = ((unsigned long long)(uint32_t)hi << 32) | (uint32_t)lo;
-					fxv += (take & q_ok) ? f : 0ull;
-					bad += (take & !q_ok) ? 1u : 0u;
-				}
-				if (__ballot(active && !decided)) {
-					// FP64 value and decision within the guard band (or without an FP32 form)
-					if (active && !decided) {
-						const double val = refine_fp64_decision(comps[cell.gcomp], xp, yp, ddy, recurse, &fail);
-						if (!GRAD && !fail) {
-							const double qv = val * (fr * fr * cl.wflux);
-							if (qv >= 0 && qv < 9.2e18) fxv += (unsigned long long)__double2ll_rn(qv);
-							else bad++;
-						}
-					}
-				}
-				fail = fail && active;
-				if (GRAD && active && !fail) {
-					any_ok = true;
-					float gt[GT_COUNT];
-					const float vg = sersic_grad32(cl, (float)xp, (float)yp, gt);
-					grad_accumulate(ga, gw * (fr * fr * cl.wq) * (double)vg, gt);
-				}
-				// failing sub-cells: onto the LIFO, or to the next pass when it is full
-				const uint64_t ballot = __ballot(fail);
-				if (ballot) {
+		// the rows of the group for profile kind KIND (-1: no FP32 form, every
+		// sub-cell decided in FP64) with the box flag BOXY
+		auto rows = [&](auto kind_c, auto boxy_c) __attribute__((always_inline)) {
+			constexpr int KIND = decltype(kind_c)::value;
+			// the constants the sub-cell loop reads, held in vector registers:
+			// in scalar registers they would not fit beside the kernel's other
+			// uniform state and be reloaded from spill lanes at every sub-cell
+			CompLiteB<decltype(boxy_c)::value> cl(L);
+			pin_vgpr(cl.cos_ang);
+			pin_vgpr(cl.sin_ang);
+			pin_vgpr(cl.inv_axrat);
+#pragma unroll
+			for (int k = 0; k < 8; k++) pin_vgpr(cl.f[k]);
+			if (KIND == PK_PROFILE_KING || KIND == PK_PROFILE_FERRER) {
+				pin_vgpr(cl.q4);
+				pin_vgpr(cl.q5);
+				pin_vgpr(cl.b2);
+			}
+			float acc_v = acc_thr, band_v = band;
+			pin_vgpr(acc_v);
+			pin_vgpr(band_v);
+			for (int rd = 0; rd < rounds; rd++) {
+				const int si = si0 + 64 * rd;
+				const bool active = lane_task && si < (int)res;
+				const double dx = fma((double)(2 * si + 1), hx, cell.dx0);
+				const double dxc = dx * cl.cos_ang, ndxs = -dx * cl.sin_ang;
+				// accepted values of this lane's row: FP32 values summed in FP64 (in
+				// sub-cell order, so the sum does not depend on scheduling) and
+				// turned into fixed point once per row; FP64 values in fixed point
+				double rs = 0;
+				unsigned long long fxv = 0;
+				double ga[GA_SLOTS] = {0, 0, 0, 0, 0, 0, 0, 0};
+				bool any_ok = false;
+				// sub-cells (lanes with `sel`, centre dx, dy) as cells of the next
+				// level: to the queue of the next pass
+				auto push_global = [&](bool sel, uint64_t ballot, double dy, uint32_t flags)
+				                       __attribute__((always_inline)) {
 					const int n = __popcll(ballot);
 					const int rank = __popcll(ballot & ((1ull << lane) - 1ull));
-					RCell c;
-					c.dx0 = dx - hx;
-					c.dy0 = dy - hy;
-					c.gcomp = cell.gcomp;
-					c.entry = cell.entry;
-					c.meta = (meta & ~255u) | (uint32_t)(level + 1);
+					const int leader = __ffsll((long long)ballot) - 1;
+					uint32_t base = 0;
+					if ((int)lane == leader) base = atomicAdd(&cnt->n_cells[pass], (uint32_t)n);
+					base = wave_read(base, leader);
+					if (sel) {
+						RCell c;
+						c.dx0 = dx - hx;
+						c.dy0 = dy - hy;
+						c.gcomp = g0;
+						c.entry = cell.entry;
+						c.meta = (key << 8) | (uint32_t)(level + 1) | flags;
+						if (base + (uint32_t)rank < qcap) qout[base + rank] = c;
+						else overflow = 1;
+					}
+				};
+				// failing sub-cells: onto the LIFO, or to the next pass when it is full
+				auto push_failing = [&](bool fail, double dy) __attribute__((always_inline)) {
+					const uint64_t ballot = __ballot(fail);
+					if (!ballot) return;
+					const int n = __popcll(ballot);
 					if (top + n <= CAP) {
-						if (fail) st.cell[top + rank] = c;
+						if (fail) {
+							RCell c;
+							c.dx0 = dx - hx;
+							c.dy0 = dy - hy;
+							c.gcomp = g0;
+							c.entry = cell.entry;
+							c.meta = (key << 8) | (uint32_t)(level + 1);
+							st.cell[top + __popcll(ballot & ((1ull << lane) - 1ull))] = c;
+						}
 						top += n;
 					} else if (!LAST) {
-						const int leader = __ffsll((long long)ballot) - 1;
-						uint32_t base = 0;
-						if ((int)lane == leader) base = atomicAdd(&cnt->n_cells[pass], (uint32_t)n);
-						base = wave_read(base, leader);
-						if (fail) {
-							if (base + (uint32_t)rank < qcap) qout[base + rank] = c;
-							else overflow = 1;
-						}
+						push_global(fail, ballot, dy, 0u);
 					} else {
 						overflow = 1;
 					}
-				}
-			}
-			if (active) nev += (unsigned long long)(nev_sub * res);
-			if (GRAD) {
-				const uint64_t okb = __ballot(any_ok);
-				if (okb) {
-					const uint32_t g0 = wave_read(cell.gcomp, __ffsll((long long)okb) - 1);
-					const uint64_t ob = __ballot(any_ok && cell.gcomp != g0);
-					const uint32_t g1 = ob ? wave_read(cell.gcomp, __ffsll((long long)ob) - 1) : ~0u;
-					if (!ob || __ballot(any_ok && cell.gcomp != g0 && cell.gcomp != g1) == 0) {
-						// at most two components: keep adding to the running sums,
-						// after handing on a slot held by a third one
-						auto hold = [&](uint32_t g, uint32_t keep) {
-							if (g == ~0u || g == run_g || g == run_g2) return;
-							if (run_g != keep) {
-								r32_flush_run(run, run_g, gacc, lane);
-								run_g = g;
-							} else {
-								r32_flush_run(run2, run_g2, gacc, lane);
-								run_g2 = g;
-							}
-						};
-						hold(g0, g1);
-						hold(g1, g0);
-						const bool in1 = any_ok && cell.gcomp == run_g, in2 = any_ok && cell.gcomp == run_g2;
-#pragma unroll
-						for (int k = 0; k < GA_SLOTS; k++) {
-							run[k] += in1 ? ga[k] : 0.0;
-							run2[k] += in2 ? ga[k] : 0.0;
+				};
+				// the row in chunks of 64 sub-cells.  Sub-cells whose FP32 decision
+				// fell within the guard band go undecided to the next pass's queue
+				// (decided in FP64 between the passes); in the last pass (bit
+				// sj - sj0 of `pend`) they are decided here after the chunk.  No
+				// pass but the last calls the FP64 evaluation, whose call would
+				// raise the kernel's register count by a third.
+				for (int sj0 = 0; sj0 < (int)res; sj0 += 64) {
+					const int sj1 = min((int)res, sj0 + 64);
+					uint64_t pend = 0;
+					for (int sj = sj0; sj < sj1; sj++) {
+						const double dy = fma((double)(2 * sj + 1), hy, cell.dy0);
+						const double xp = fma(dy, cl.sin_ang, dxc);
+						const double yp = fma(dy, cl.cos_ang, ndxs) * cl.inv_axrat;
+						float v32 = -1.f, t32 = -1.f;
+						if (any_rec) evaluate_pair32<true>(cl, KIND, xp, yp, fabs(xp), fabs(yp) + fabs(ddy), v32, t32);
+						else evaluate_pair32<false>(cl, KIND, xp, yp, 0., 0., v32, t32);
+						// accuracy test (src/radial.cpp:86-109) |t / v - 1| > acc, for
+						// v > 0 the sign of e = |t - v| - acc v; decided in FP32 outside a
+						// guard band |e| <= 1e-4 (1 + acc) v; written without branches
+						const float e = fmaf(-acc_v, v32, fabsf(t32 - v32));
+						const bool dec_rec = (v32 > 0.f) & (v32 <= 3e38f) & (t32 <= 3e38f) & (fabsf(e) > band_v * v32);
+						const bool decided = recurse ? dec_rec : (v32 >= 0.f) & (v32 <= 3e38f);
+						const bool fail = active & recurse & dec_rec & (e > 0.f);
+						if (!GRAD) rs += (active & decided & !fail) ? (double)v32 : 0.0;
+						if (LAST) {
+							pend |= (active & !decided) ? 1ull << (sj - sj0) : 0ull;
+						} else {
+							const bool pnd = active & !decided;
+							const uint64_t pb = __ballot(pnd);
+							if (pb) push_global(pnd, pb, dy, R32_PENDING);
 						}
-					} else {
-						r32_flush_grad(any_ok, cell.gcomp, ga, gacc, lane);
+						if (GRAD && active && decided && !fail) {
+							any_ok = true;
+							float gt[GT_COUNT];
+							const float vg = sersic_grad32(cl, (float)xp, (float)yp, gt);
+							grad_accumulate(ga, gw * wrow * (double)vg, gt);
+						}
+						push_failing(fail, dy);
+					}
+					// FP64 value and decision of the sub-cells within the guard band
+					// (or of every sub-cell without an FP32 form), lowest column first
+					while (LAST && __ballot(pend != 0)) {
+						bool fail = false;
+						double dy = 0;
+						if (pend) {
+							const int sj = sj0 + __ffsll((long long)pend) - 1;
+							pend &= pend - 1;
+							dy = fma((double)(2 * sj + 1), hy, cell.dy0);
+							const double xp = fma(dy, cl.sin_ang, dxc);
+							const double yp = fma(dy, cl.cos_ang, ndxs) * cl.inv_axrat;
+							const double val = refine_fp64_decision(comps[g0], xp, yp, ddy, recurse, &fail);
+							if (!fail) {
+								if (GRAD) {
+									any_ok = true;
+									float gt[GT_COUNT];
+									const float vg = sersic_grad32(cl, (float)xp, (float)yp, gt);
+									grad_accumulate(ga, gw * wrow * (double)vg, gt);
+								} else {
+									const double qv = val * (fr * fr * cl.wflux);
+									if (qv >= 0 && qv < 9.2e18) fxv += (unsigned long long)__double2ll_rn(qv);
+									else bad++;
+								}
+							}
+						}
+						push_failing(fail, dy);
 					}
 				}
-			} else if (one_target) {
-				// one pixel accumulator for all lanes: exact per-lane sums, added
-				// to it (lane 0) when the pixel changes
-				if (tu != run_t) {
-					r32_flush_fx(run_fx, run_t, acc, lane);
-					run_t = tu;
+				if (!GRAD) {
+					// the row's FP32 values in fixed point (truncated)
+					const double q = rs * wrow;
+					if (q >= 0 && q < 9.2e18) fxv += (unsigned long long)q;
+					else bad++;
 				}
-				run_fx += fxv;
-			} else {
-				// several pixels: exact integer sums per cell in LDS, then one
-				// global atomic per cell (lane t adds task t's sum)
-				if (fxv) atomicAdd(&st.csum[t_local], fxv);
-				__builtin_amdgcn_wave_barrier();
-				if ((int)lane < ntask) {
-					const unsigned long long v = st.csum[lane];
-					st.csum[lane] = 0;
-					if (v) atomicAdd(&acc[task_target], v);
+				if (active) nev += (unsigned long long)(nev_sub * res);
+				if (GRAD) {
+					const uint64_t okb = __ballot(any_ok);
+					if (okb) {
+						// one component per group: keep adding to the running sums,
+						// after handing on a slot held by another one
+						if (g0 != run_g && g0 != run_g2) {
+							r32_flush_run(run2, run_g2, gacc, lane);
+							run_g2 = run_g;
+#pragma unroll
+							for (int k = 0; k < GA_SLOTS; k++) run2[k] = run[k];
+#pragma unroll
+							for (int k = 0; k < GA_SLOTS; k++) run[k] = 0;
+							run_g = g0;
+						}
+						const bool in1 = g0 == run_g;
+#pragma unroll
+						for (int k = 0; k < GA_SLOTS; k++) {
+							run[k] += (any_ok && in1) ? ga[k] : 0.0;
+							run2[k] += (any_ok && !in1) ? ga[k] : 0.0;
+						}
+					}
+				} else if (one_target) {
+					// one pixel accumulator for all lanes: exact per-lane sums, added
+					// to it (lane 0) when the pixel changes
+					if (tu != run_t) {
+						r32_flush_fx(run_fx, run_t, acc, lane);
+						run_t = tu;
+					}
+					run_fx += fxv;
+				} else {
+					// several pixels: exact integer sums per cell in LDS, then one
+					// global atomic per cell (lane t adds task t's sum)
+					if (fxv) atomicAdd(&st.csum[t_local], fxv);
+					__builtin_amdgcn_wave_barrier();
+					if ((int)lane < ntask) {
+						const unsigned long long v = st.csum[lane];
+						st.csum[lane] = 0;
+						if (v) atomicAdd(&acc[task_target], v);
+					}
 				}
+				__syncthreads();
 			}
-			__syncthreads();
+		};
+		using BoxN = std::false_type;
+		using BoxY = std::true_type;
+		const int kind = L.fp32 ? (int)L.kind : -1;
+		if (GRAD) {
+			// gradient batches hold sersic profiles only
+			if (L.boxy) rows(std::integral_constant<int, PK_PROFILE_SERSIC>{}, BoxY{});
+			else rows(std::integral_constant<int, PK_PROFILE_SERSIC>{}, BoxN{});
+		} else {
+#define PK_R32_KIND(K)                                                        \
+	case K:                                                               \
+		if (L.boxy) rows(std::integral_constant<int, K>{}, BoxY{});     \
+		else rows(std::integral_constant<int, K>{}, BoxN{});            \
+		break;
+			switch (kind) {
+				PK_R32_KIND(PK_PROFILE_SERSIC)
+				PK_R32_KIND(PK_PROFILE_MOFFAT)
+				PK_R32_KIND(PK_PROFILE_CORESERSIC)
+				PK_R32_KIND(PK_PROFILE_BROKENEXP)
+			// king and ferrer: the FP32 forms of round profiles only (their
+			// boxy radius needs FP64 powers, left to the FP64 evaluation)
+			case PK_PROFILE_KING:
+				if (!L.boxy) rows(std::integral_constant<int, PK_PROFILE_KING>{}, BoxN{});
+				else rows(std::integral_constant<int, -1>{}, BoxN{});
+				break;
+			case PK_PROFILE_FERRER:
+				if (!L.boxy) rows(std::integral_constant<int, PK_PROFILE_FERRER>{}, BoxN{});
+				else rows(std::integral_constant<int, -1>{}, BoxN{});
+				break;
+			default: rows(std::integral_constant<int, -1>{}, BoxN{}); break;
+			}
+#undef PK_R32_KIND
 		}
 	}
 	if (GRAD) {
@@ -622,6 +706,64 @@ __global__ __launch_bounds__(64) void refine32_kernel(const PkJob *__restrict__ 
 		if (nbad) atomicAdd(&cnt->n_bad, nbad);
 		if (!LAST) atomicAdd(&cnt->n_done[pass], 1u);
 	}
+}
+
+// Between two passes: decides in FP64 the sub-cells that pass p handed over
+// undecided (R32_PENDING).  An accepted one adds its value to its pixel's
+// accumulator (GRAD: its adjoint terms to its component) and its slot is
+// marked dead; a failing one stays as an ordinary cell of its level for pass
+// p + 1.  The sub-cell's centre is its corner plus half its size.
+template <bool GRAD>
+__global__ __launch_bounds__(256) void r32_resolve_kernel(const PkComp *__restrict__ comps,
+                                                          const CompLite *__restrict__ lite,
+                                                          PkCounters *__restrict__ cnt, RCell *__restrict__ q,
+                                                          uint32_t qcap, int pass,
+                                                          unsigned long long *__restrict__ acc,
+                                                          const double *__restrict__ gent = nullptr,
+                                                          double *__restrict__ gacc = nullptr)
+{
+	const uint32_t n = min(cnt->n_cells[pass], qcap);
+	uint32_t bad = 0;
+	for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+		const uint32_t meta = q[i].meta;
+		if (meta == R32_DEAD || !(meta & R32_PENDING)) continue;
+		const RCell c = q[i];
+		const CompLite &L = lite[c.gcomp];
+		const int lv = (int)cm_level(meta);  // the sub-cell's level as a cell (its parent's + 1)
+		const uint32_t res = cm_res(meta);
+		const double fr = cell_size_fp64((int)res, lv);
+		const double hx = fr * L.hsx, hy = fr * L.hsy;
+		const double dx = c.dx0 + hx, dy = c.dy0 + hy;
+		const double xp = fma(dy, L.sin_ang, dx * L.cos_ang);
+		const double yp = fma(dy, L.cos_ang, -dx * L.sin_ang) * L.inv_axrat;
+		const double ddy = (-2 * hx * L.sin_ang + 2 * hy * L.cos_ang) * L.inv_axrat;
+		const bool recurse = res > 1u && lv - 1 < (int)cm_maxrec(meta);
+		bool fail = false;
+		const double val = refine_fp64_decision(comps[c.gcomp], xp, yp, ddy, recurse, &fail);
+		if (fail) {
+			q[i].meta = meta & ~R32_PENDING;
+			continue;
+		}
+		const size_t target = 2 * (size_t)c.entry + cm_conv(meta);
+		if (GRAD) {
+			float gt[GT_COUNT];
+			const float vg = sersic_grad32(L, (float)xp, (float)yp, gt);
+			double ga[GA_SLOTS] = {0, 0, 0, 0, 0, 0, 0, 0};
+			grad_accumulate(ga, gent[target] * (fr * fr * L.wq) * (double)vg, gt);
+			for (int k = 0; k < GA_SLOTS; k++)
+				if (ga[k] != 0) atomicAdd(&gacc[(size_t)c.gcomp * GA_SLOTS + k], ga[k]);
+		} else {
+			const double qv = val * (fr * fr * L.wflux);
+			if (qv >= 0 && qv < 9.2e18) {
+				const unsigned long long f = (unsigned long long)__double2ll_rn(qv);
+				if (f) atomicAdd(&acc[target], f);
+			} else {
+				bad++;
+			}
+		}
+		q[i].meta = R32_DEAD;
+	}
+	if (bad) atomicAdd(&cnt->n_bad, bad);
 }
 
 // adds the refined contributions to the pixels (one writer per pixel)
