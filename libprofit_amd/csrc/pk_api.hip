@@ -885,8 +885,16 @@ int launch_fft_conv(hipStream_t s, const PkJob &jb, const PkJob *djob, const TI 
 }
 
 // capacity (cells) of each of the two queues between FP32 refinement passes
-constexpr uint32_t R32_QCAP_SINGLE = 1u << 20;
-constexpr uint32_t R32_QCAP_BATCH = 1u << 22;
+// Capacity (cells) of each FP32 refinement queue: the cells of one level
+// handed to the next pass (two, used in turn) and the guard-band list.  A
+// batch sizes them by its radial components (about 2048 cells of one level
+// per component on the galaxy workloads; a full queue leaves cells on the
+// wavefront's LIFO, so a short one costs speed, not results).
+constexpr uint32_t R32_QCAP_SINGLE = 1u << 19;
+inline uint32_t r32_qcap_batch(size_t n_comps)
+{
+	return (uint32_t)std::min<size_t>(std::max<size_t>(n_comps * 2048, 1u << 20), 1u << 26);
+}
 
 // Fixed-point scale of the FP32-mode refinement accumulators of one model:
 // 2^60 over the summed total flux of its radial profiles (no pixel can hold
@@ -937,22 +945,26 @@ int launch_profiles(hipStream_t s, bool fp32, const PkJob *djobs, uint32_t njobs
 	if (ev_coarse) PK_HIP(hipEventRecord(ev_coarse, s));
 	if (!any_comp) return PK_OK;
 	if (fp32) {
-		// pass p reads queue (p - 1) & 1 and hands over into queue p & 1
-		RCell *q0 = dq, *q1 = dq + qcap;
+		// pass p reads queue (p - 1) & 1 and hands over into queue p & 1; pq
+		// holds the guard-band sub-cells of the pass that just ran
+		RCell *q0 = dq, *q1 = dq + qcap, *pq = dq + 2 * (size_t)qcap;
 		const uint32_t gm = std::min(grid_main, ref_blocks), gl = std::min(grid_last, ref_blocks);
-		hipLaunchKernelGGL((refine32_kernel<0, false>), dim3(gm), dim3(64), 0, s, djobs, dcomp, dlite, dent, dcnt,
-		                   (const RCell *)nullptr, q0, qcap, 0, dacc, tune);
-		hipLaunchKernelGGL(r32_resolve_kernel<false>, dim3(R32_RESOLVE_GRID), dim3(256), 0, s, dcomp, dlite, dcnt,
-		                   q0, qcap, 0, dacc, (const double *)nullptr, (double *)nullptr);
-		for (int p = 1; p < R32_PASSES - 1; p++) {
-			hipLaunchKernelGGL((refine32_kernel<1, false>), dim3(gm), dim3(64), 0, s, djobs, dcomp, dlite, dent, dcnt,
-			                   (const RCell *)((p & 1) ? q0 : q1), (p & 1) ? q1 : q0, qcap, p, dacc, tune);
-			hipLaunchKernelGGL(r32_resolve_kernel<false>, dim3(R32_RESOLVE_GRID), dim3(256), 0, s, dcomp, dlite,
-			                   dcnt, (p & 1) ? q1 : q0, qcap, p, dacc, (const double *)nullptr, (double *)nullptr);
+		for (int p = 0; p < R32_PASSES - 1; p++) {
+			RCell *qin = (p & 1) ? q0 : q1, *qout = (p & 1) ? q1 : q0;
+			if (p == 0)
+				hipLaunchKernelGGL((refine32_kernel<0, false>), dim3(gm), dim3(64), 0, s, djobs, dcomp, dlite, dent,
+				                   dcnt, (const RCell *)nullptr, qout, qcap, 0, dacc, tune, pq,
+				                   (const double *)nullptr, (double *)nullptr);
+			else
+				hipLaunchKernelGGL((refine32_kernel<1, false>), dim3(gm), dim3(64), 0, s, djobs, dcomp, dlite, dent,
+				                   dcnt, (const RCell *)qin, qout, qcap, p, dacc, tune, pq, (const double *)nullptr,
+				                   (double *)nullptr);
+			hipLaunchKernelGGL(r32_resolve_kernel<false>, dim3(R32_RESOLVE_GRID), dim3(256), 0, s, dcomp, dlite, dcnt,
+			                   (const RCell *)pq, qout, qcap, p, dacc, (const double *)nullptr, (double *)nullptr);
 		}
 		hipLaunchKernelGGL((refine32_kernel<1, true>), dim3(gl), dim3(64), 0, s, djobs, dcomp, dlite, dent, dcnt,
 		                   (const RCell *)((R32_PASSES & 1) ? q1 : q0), (RCell *)nullptr, qcap, R32_PASSES - 1, dacc,
-		                   tune);
+		                   tune, (RCell *)nullptr, (const double *)nullptr, (double *)nullptr);
 		hipLaunchKernelGGL(finalize32_kernel<TI>, dim3(1024), dim3(256), 0, s, djobs, dent, dcnt, dacc, img_nc, img_c);
 	} else {
 		hipLaunchKernelGGL((refine_kernel<TI, false>), dim3(ref_blocks), dim3(64), 0, s, djobs, dcomp, dent, dcnt,
@@ -1076,7 +1088,7 @@ int evaluate_impl(pk_ctx *ctx, const pk_model *m, const Analysis &a, double *out
 	const size_t o_ent = L.take(sizeof(RefineEntry) * npix);
 	const size_t o_acc = L.take(fp32 ? 2 * sizeof(unsigned long long) * npix : 8);
 	const uint32_t qcap = fp32 ? R32_QCAP_SINGLE : 0;
-	const size_t o_q = L.take(2 * sizeof(RCell) * std::max<uint32_t>(1, qcap));
+	const size_t o_q = L.take(3 * sizeof(RCell) * std::max<uint32_t>(1, qcap));
 	const size_t o_out = L.take(sizeof(double) * (size_t)ow * oh);
 	const bool use_fft = a.conv_required && m->convolver == PK_CONV_FFT;
 	FftGeom fg;
@@ -1835,22 +1847,24 @@ int grad_launch_t(pk_batch *b)
 		                   (const PkCounters *)b->d<PkCounters>(b->o_cnt), (const T *)Gd, (const T *)Gc, gent);
 		PK_HIP(hipGetLastError());
 		if (timed) PK_HIP(hipEventRecord(b->ev_grad[2], s));
-		hipLaunchKernelGGL((refine32_kernel<0, false, true>), dim3(gm), dim3(64), 0, s, djobs, dcomp, dlite, dent,
-		                   gcnt, (const RCell *)nullptr, q0, b->qcap, 0, (unsigned long long *)nullptr, hd,
-		                   (const double *)gent, gacc);
-		hipLaunchKernelGGL(r32_resolve_kernel<true>, dim3(R32_RESOLVE_GRID), dim3(256), 0, s, dcomp, dlite, gcnt, q0,
-		                   b->qcap, 0, (unsigned long long *)nullptr, (const double *)gent, gacc);
-		for (int p = 1; p < R32_PASSES - 1; p++) {
-			hipLaunchKernelGGL((refine32_kernel<1, false, true>), dim3(gm), dim3(64), 0, s, djobs, dcomp, dlite, dent,
-			                   gcnt, (const RCell *)((p & 1) ? q0 : q1), (p & 1) ? q1 : q0, b->qcap, p,
-			                   (unsigned long long *)nullptr, hd, (const double *)gent, gacc);
+		RCell *pq = q0 + 2 * (size_t)b->qcap;
+		for (int p = 0; p < R32_PASSES - 1; p++) {
+			RCell *qin = (p & 1) ? q0 : q1, *qout = (p & 1) ? q1 : q0;
+			if (p == 0)
+				hipLaunchKernelGGL((refine32_kernel<0, false, true>), dim3(gm), dim3(64), 0, s, djobs, dcomp, dlite,
+				                   dent, gcnt, (const RCell *)nullptr, qout, b->qcap, 0,
+				                   (unsigned long long *)nullptr, hd, pq, (const double *)gent, gacc);
+			else
+				hipLaunchKernelGGL((refine32_kernel<1, false, true>), dim3(gm), dim3(64), 0, s, djobs, dcomp, dlite,
+				                   dent, gcnt, (const RCell *)qin, qout, b->qcap, p, (unsigned long long *)nullptr,
+				                   hd, pq, (const double *)gent, gacc);
 			hipLaunchKernelGGL(r32_resolve_kernel<true>, dim3(R32_RESOLVE_GRID), dim3(256), 0, s, dcomp, dlite, gcnt,
-			                   (p & 1) ? q1 : q0, b->qcap, p, (unsigned long long *)nullptr, (const double *)gent,
+			                   (const RCell *)pq, qout, b->qcap, p, (unsigned long long *)nullptr, (const double *)gent,
 			                   gacc);
 		}
 		hipLaunchKernelGGL((refine32_kernel<1, true, true>), dim3(gl), dim3(64), 0, s, djobs, dcomp, dlite, dent, gcnt,
 		                   (const RCell *)((R32_PASSES & 1) ? q1 : q0), (RCell *)nullptr, b->qcap, R32_PASSES - 1,
-		                   (unsigned long long *)nullptr, hd, (const double *)gent, gacc);
+		                   (unsigned long long *)nullptr, hd, (RCell *)nullptr, (const double *)gent, gacc);
 	} else {
 		if (timed) PK_HIP(hipEventRecord(b->ev_grad[2], s));
 		hipLaunchKernelGGL((refine_kernel<double, false, true>), dim3(4096), dim3(64), 0, s, djobs,
@@ -2442,8 +2456,8 @@ int pk_batch_create(pk_ctx *ctx, const pk_model *models, uint32_t n_models, uint
 	b->o_ftw = L.take(fft_points ? sizeof(double2) * FFT_TWIDDLES : 1);
 	b->o_ent = L.take(sizeof(RefineEntry) * img_off);
 	b->o_acc = L.take(fp32 ? 2 * sizeof(unsigned long long) * img_off : 8);
-	b->qcap = fp32 ? R32_QCAP_BATCH : 0;
-	b->o_q = L.take(2 * sizeof(RCell) * std::max<uint32_t>(1, b->qcap));
+	b->qcap = fp32 ? r32_qcap_batch(b->setup.size()) : 0;
+	b->o_q = L.take(3 * sizeof(RCell) * std::max<uint32_t>(1, b->qcap));
 	b->o_out = L.take(elem * out_off);
 	if (flags & PK_BATCH_GRAD) {
 		b->o_psfF = L.take(elem * std::max<size_t>(1, psf_all.size()));

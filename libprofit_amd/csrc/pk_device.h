@@ -1308,6 +1308,7 @@ __global__ void reset_grad_kernel(PkCounters *__restrict__ g, const PkCounters *
 			g->claim[p] = 0;
 			g->n_cells[p] = 0;
 			g->n_done[p] = 0;
+			g->n_pend[p] = 0;
 		}
 	}
 }
@@ -1332,6 +1333,7 @@ __global__ void reset_counters_kernel(PkCounters *c)
 			c->claim[p] = 0;
 			c->n_cells[p] = 0;
 			c->n_done[p] = 0;
+			c->n_pend[p] = 0;
 		}
 	}
 }

@@ -51,11 +51,11 @@ struct RefineEntry {
 	uint32_t job, pix, cmask;
 };
 
-// The FP32-mode refinement runs in R32_PASSES launches: a launch hands what
-// its wavefronts still hold to the next one once a share of them (a quarter by
-// default) have run out of work, so that no long tail of single wavefronts
-// remains.
-constexpr int R32_PASSES = 4;
+// The FP32-mode refinement runs in R32_PASSES launches: one per level of the
+// sub-pixel recursion (the failing sub-cells of a level are the next launch's
+// work, spread evenly over its wavefronts), the last one refining all deeper
+// levels.
+constexpr int R32_PASSES = 7;
 
 // Run-time settings of the FP32 refinement launches (chosen per batch by the
 // autotuner; any values >= 1 give the same result, only the speed changes).
@@ -77,6 +77,7 @@ struct PkCounters {
 	uint32_t claim[R32_PASSES];    // work-queue head of FP32 refinement pass p
 	uint32_t n_cells[R32_PASSES];  // cells pass p hands to pass p + 1
 	uint32_t n_done[R32_PASSES];   // wavefronts of pass p that have stopped
+	uint32_t n_pend[R32_PASSES];   // guard-band sub-cells pass p listed for the FP64 decision
 };
 
 // A cell waiting to be subdivided (FP32-mode refinement), 32 bytes.  Its size

@@ -11,48 +11,52 @@
 // neither the packing of cells into rounds nor the scheduling of wavefronts
 // can change the result.
 //
-// Load balance: the refinement runs as R32_PASSES launches.  Pass 0 takes
-// pixels from the coarse pass's entry list, pass p > 0 the cells handed over
-// by pass p - 1.  Work is claimed in chunks that shrink as the source runs
-// dry.  Once a set share of a pass's wavefronts (R32Tune::handover_div; a
-// quarter by default) have stopped, the others hand everything they still
-// hold (their LIFO and unclaimed chunk) to the next pass and stop too, so a few deep pixels never keep a lone wavefront busy
-// while the rest of the GPU idles.  The last pass finishes all that is left,
-// with a deeper LIFO.  A LIFO that fills up before that also hands its new
-// cells to the next pass.
+// Load balance: the refinement runs as R32_PASSES launches, one level of the
+// sub-pixel recursion per launch but the last.  Pass 0 takes pixels from the
+// coarse pass's entry list; pass p > 0 takes the cells of level p that pass
+// p - 1 produced.  The failing sub-cells of a level-pass go to the next
+// pass's queue (staged in LDS and written 64 at a time, one atomic per 64),
+// so every pass starts from work spread evenly over all wavefronts, whose
+// items cost at most res^2 sub-cells each: no wavefront is left with a deep
+// pixel's whole subtree while the others idle.  Work is claimed in chunks
+// that shrink as the source runs dry.  The last pass refines what remains
+// (levels >= R32_PASSES - 1: the Sersic centre pixels' deep recursion)
+// depth-first on a LIFO.  A queue that fills up keeps the cells on the
+// wavefront's LIFO instead.
+//
+// Guard band: sub-cells whose FP32 accuracy test falls within the guard band
+// of the threshold are decided in FP64.  Outside the last pass they are
+// listed and decided by r32_resolve_kernel between the passes (the FP64
+// evaluation's call would raise the main kernels' register count by a
+// third); a failing one joins the next pass's queue.
 //
 // Latency: lane-crossing scans and whole-wavefront sums use DPP; a round
 // holding several cells sums each cell's sub-cells with LDS integer atomics
-// (one slot per cell) and adds the slots to the pixels.  The 128-byte CompLite
-// record of a cell is staged in LDS and kept there while the following rounds
-// work on the same component.
+// (one slot per cell) and adds the slots to the pixels.
 #pragma once
 
 namespace pk {
 
-// LIFO capacity: small in the handing-over passes (occupancy), large in the
-// last pass, which has nowhere further to go.  Depth-first processing holds at
-// most about res^2 * (max recursions) cells: 1024 covers res 16 with 3 levels
-// and the Sersic centre pixels (res 8, 10 levels).
-constexpr int R32_CAP_MAIN = 192;
+// LIFO capacity: in the level-passes it holds the claimed cells (the
+// children go to the next pass); the last pass refines depth-first, which
+// holds at most about res^2 * (levels left) cells: 1024 covers the Sersic
+// centre pixels (res 8, 10 levels).
+constexpr int R32_CAP_MAIN = 128;
 constexpr int R32_CAP_LAST = 1024;
+// children staged in LDS before they are written to the next pass's queue
+constexpr int R32_OUTB = 128;
 // workgroups of the launches that decide handed-over sub-cells in FP64
 constexpr uint32_t R32_RESOLVE_GRID = 1024;
 constexpr uint32_t R32_CELL_RUN = 4;       // log2 of the runs in which handed-over cells are claimed
 constexpr int R32_TASKS = 16;              // cells per round at most
-// Flags of RCell::meta in the queues between passes: a sub-cell handed over
-// undecided (its FP64 decision is made between the passes), and a slot whose
-// sub-cell was accepted there (skipped by the next pass).
-constexpr uint32_t R32_PENDING = 1u << 31;
-constexpr uint32_t R32_DEAD = 0xffffffffu;
-// untuned settings: claims of up to 64 pixels / 16 cells, the hand-over test
-// every 16 rounds, hand-over once a quarter of the wavefronts stopped
+// untuned settings: claims of up to 64 pixels / 16 cells, in entry order
 constexpr R32Tune R32_TUNE_DEFAULT = {4u, 64u, 16u, 16u, 0u};
 
-template <int CAP>
+template <int CAP, int OUTB>
 struct R32Stack {
 	unsigned long long csum[R32_TASKS];  // fixed-point sum of task t in the current round (0 between rounds)
 	RCell cell[CAP];
+	RCell outb[OUTB];                    // staged children (level-passes)
 };
 
 // ---- wavefront primitives on DPP ----------------------------------------------
@@ -244,8 +248,7 @@ __device__ inline void r32_take_cells(const RCell *__restrict__ qin, uint32_t n,
 	const uint32_t k = min(min(end - next, (uint32_t)(cap - top)), 64u);
 	const uint32_t i = next + lane;
 	const uint32_t c = r32_entry_of(i, R32_CELL_RUN, stride);
-	// dead slots: sub-cells accepted between the passes
-	const bool valid = lane < k && c < n && qin[c].meta != R32_DEAD;
+	const bool valid = lane < k && c < n;
 	const uint64_t b = __ballot(valid);
 	if (valid) cells[top + __popcll(b & ((1ull << lane) - 1ull))] = qin[c];
 	top += __popcll(b);
@@ -296,8 +299,42 @@ __device__ inline void r32_flush_run(double run[GA_SLOTS], uint32_t g, double *_
 	}
 }
 
+// Writes the n staged children to the next pass's queue (one atomic for all of
+// them) and clears the staging buffer.  Children past the queue's capacity go
+// onto the LIFO (refined by this wavefront) while it has room.
+template <int CAP>
+__device__ inline void r32_flush_out(RCell *outb, int &n, RCell *__restrict__ qout, uint32_t qcap, uint32_t *count,
+                                     RCell *cells, int cap, int &top, uint32_t &overflow, uint32_t lane)
+{
+	if (n == 0) return;
+	uint32_t base = 0;
+	if (lane == 0) base = atomicAdd(count, (uint32_t)n);
+	base = wave_read(base, 0);
+	for (int i0 = 0; i0 < n; i0 += 64) {
+		const int i = i0 + (int)lane;
+		const bool in = i < n;
+		const bool fits = in && base + (uint32_t)i < qcap;
+		if (fits) qout[base + i] = outb[i];
+		const bool local = in && !fits;
+		const uint64_t b = __ballot(local);
+		if (b) {
+			const int k = __popcll(b);
+			if (top + k <= cap) {
+				if (local) cells[top + __popcll(b & ((1ull << lane) - 1ull))] = outb[i];
+				top += k;
+			} else {
+				overflow = 1;
+			}
+		}
+	}
+	n = 0;
+	__builtin_amdgcn_wave_barrier();
+}
+
 // SOURCE 0: pixels from the coarse pass (pass 0); 1: cells of the previous pass.
-// LAST: the final pass, which hands nothing over.
+// LAST: the final pass, which refines depth-first and hands nothing over.
+// Level-passes write their children to qout and their guard-band sub-cells
+// to pq (decided by r32_resolve_kernel before the next pass).
 // GRAD: the chi^2 adjoint of the refinement -- the same cells and decisions as
 // the forward pass, but every accepted sub-cell adds G w flux_scale f dL/dθ
 // to gacc instead of its value to the pixel.  gent[2 e + convolved] holds
@@ -312,22 +349,22 @@ __global__ __launch_bounds__(64) void refine32_kernel(const PkJob *__restrict__ 
                                                       PkCounters *__restrict__ cnt, const RCell *__restrict__ qin,
                                                       RCell *__restrict__ qout, uint32_t qcap, int pass,
                                                       unsigned long long *__restrict__ acc,
-                                                      R32Tune tune,
+                                                      R32Tune tune, RCell *__restrict__ pq,
                                                       const double *__restrict__ gent = nullptr,
                                                       double *__restrict__ gacc = nullptr)
 {
 	constexpr int CAP = LAST ? R32_CAP_LAST : R32_CAP_MAIN;
-	__shared__ R32Stack<CAP> st;
+	__shared__ R32Stack<CAP, LAST ? 1 : R32_OUTB> st;
 	const uint32_t lane = lane_id();
 	if (lane < (uint32_t)R32_TASKS) st.csum[lane] = 0;
 	int top = 0;
-	bool source_done = false, handover = false;
+	int nout = 0;  // children staged in st.outb
+	bool source_done = false;
 	const uint32_t n_cells_in = SOURCE == 0 ? 0u : min(cnt->n_cells[pass - 1], qcap);
-	// Handed-over cells are claimed in runs of 2^R32_CELL_RUN taken from 64
-	// places of the queue in turn, so that the cells one wavefront handed
-	// over, often parts of the same deep pixel, go to many wavefronts, while
-	// each run (cells handed over together: mostly one component and
-	// resolution) stays together on one LIFO.
+	// Cells of the previous pass are claimed in runs of 2^R32_CELL_RUN taken
+	// from 64 places of the queue in turn, so that the children of one deep
+	// pixel go to many wavefronts, while each run (written together: mostly
+	// one component and resolution) stays together on one LIFO.
 	const uint32_t stride = ((((n_cells_in + (1u << R32_CELL_RUN) - 1u) >> R32_CELL_RUN) + 63u) / 64u);
 	const uint32_t n_entries = cnt->n_entries;
 	const uint32_t spread = SOURCE == 0 ? tune.spread_log2 : 0u;
@@ -338,7 +375,6 @@ __global__ __launch_bounds__(64) void refine32_kernel(const PkJob *__restrict__ 
 	uint32_t overflow = 0, bad = 0;
 	unsigned long long nev = 0;
 	uint32_t chunk_next = 0, chunk_end = 0, seen = 0;
-	int poll = 0;
 	// per-lane running fixed-point sum of the pixel accumulator run_t (rounds
 	// of one cell whose pixel is the last one's add to it)
 	unsigned long long run_fx = 0;
@@ -370,15 +406,12 @@ __global__ __launch_bounds__(64) void refine32_kernel(const PkJob *__restrict__ 
 			}
 			__syncthreads();
 		}
-		if (top == 0) break;
-		if (!LAST && ++poll >= (int)tune.poll) {
-			poll = 0;
-			uint32_t d = 0;
-			if (lane == 0) d = __hip_atomic_load(&cnt->n_done[pass], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-			if (wave_read(d, 0) * tune.handover_div >= gridDim.x) {
-				handover = true;
-				break;
-			}
+		if (top == 0) {
+			// the source is exhausted: the staged children go to the next pass
+			// (those a full queue leaves on the LIFO are refined here)
+			if (!LAST && nout > 0)
+				r32_flush_out<CAP>(st.outb, nout, qout, qcap, &cnt->n_cells[pass], st.cell, CAP, top, overflow, lane);
+			if (top == 0) break;
 		}
 
 		// ---- cells on top of the LIFO -> lanes ----
@@ -474,55 +507,42 @@ __global__ __launch_bounds__(64) void refine32_kernel(const PkJob *__restrict__ 
 				unsigned long long fxv = 0;
 				double ga[GA_SLOTS] = {0, 0, 0, 0, 0, 0, 0, 0};
 				bool any_ok = false;
-				// sub-cells (lanes with `sel`, centre dx, dy) as cells of the next
-				// level: to the queue of the next pass
-				auto push_global = [&](bool sel, uint64_t ballot, double dy, uint32_t flags)
-				                       __attribute__((always_inline)) {
-					const int n = __popcll(ballot);
-					const int rank = __popcll(ballot & ((1ull << lane) - 1ull));
-					const int leader = __ffsll((long long)ballot) - 1;
-					uint32_t base = 0;
-					if ((int)lane == leader) base = atomicAdd(&cnt->n_cells[pass], (uint32_t)n);
-					base = wave_read(base, leader);
-					if (sel) {
-						RCell c;
-						c.dx0 = dx - hx;
-						c.dy0 = dy - hy;
-						c.gcomp = g0;
-						c.entry = cell.entry;
-						c.meta = (key << 8) | (uint32_t)(level + 1) | flags;
-						if (base + (uint32_t)rank < qcap) qout[base + rank] = c;
-						else overflow = 1;
-					}
+				// the sub-cell of this lane's row at column centre dy as a cell of
+				// the next level
+				auto child = [&](double dy) __attribute__((always_inline)) {
+					RCell c;
+					c.dx0 = dx - hx;
+					c.dy0 = dy - hy;
+					c.gcomp = g0;
+					c.entry = cell.entry;
+					c.meta = (key << 8) | (uint32_t)(level + 1);
+					return c;
 				};
-				// failing sub-cells: onto the LIFO, or to the next pass when it is full
+				// failing sub-cells: staged for the next pass (level-passes), or
+				// onto the LIFO (last pass)
 				auto push_failing = [&](bool fail, double dy) __attribute__((always_inline)) {
 					const uint64_t ballot = __ballot(fail);
 					if (!ballot) return;
 					const int n = __popcll(ballot);
-					if (top + n <= CAP) {
-						if (fail) {
-							RCell c;
-							c.dx0 = dx - hx;
-							c.dy0 = dy - hy;
-							c.gcomp = g0;
-							c.entry = cell.entry;
-							c.meta = (key << 8) | (uint32_t)(level + 1);
-							st.cell[top + __popcll(ballot & ((1ull << lane) - 1ull))] = c;
-						}
+					const int rank = __popcll(ballot & ((1ull << lane) - 1ull));
+					if (!LAST) {
+						// nout < 64 here (flushed at 64), so n <= 64 more fit
+						if (fail) st.outb[nout + rank] = child(dy);
+						nout += n;
+						if (nout >= 64)
+							r32_flush_out<CAP>(st.outb, nout, qout, qcap, &cnt->n_cells[pass], st.cell, CAP, top,
+							                   overflow, lane);
+					} else if (top + n <= CAP) {
+						if (fail) st.cell[top + rank] = child(dy);
 						top += n;
-					} else if (!LAST) {
-						push_global(fail, ballot, dy, 0u);
 					} else {
 						overflow = 1;
 					}
 				};
 				// the row in chunks of 64 sub-cells.  Sub-cells whose FP32 decision
-				// fell within the guard band go undecided to the next pass's queue
-				// (decided in FP64 between the passes); in the last pass (bit
-				// sj - sj0 of `pend`) they are decided here after the chunk.  No
-				// pass but the last calls the FP64 evaluation, whose call would
-				// raise the kernel's register count by a third.
+				// fell within the guard band are listed for r32_resolve_kernel
+				// (level-passes) or, in the last pass (bit sj - sj0 of `pend`),
+				// decided here after the chunk.
 				for (int sj0 = 0; sj0 < (int)res; sj0 += 64) {
 					const int sj1 = min((int)res, sj0 + 64);
 					uint64_t pend = 0;
@@ -546,7 +566,17 @@ __global__ __launch_bounds__(64) void refine32_kernel(const PkJob *__restrict__ 
 						} else {
 							const bool pnd = active & !decided;
 							const uint64_t pb = __ballot(pnd);
-							if (pb) push_global(pnd, pb, dy, R32_PENDING);
+							if (pb) {
+								const int leader = __ffsll((long long)pb) - 1;
+								uint32_t base = 0;
+								if ((int)lane == leader) base = atomicAdd(&cnt->n_pend[pass], (uint32_t)__popcll(pb));
+								base = wave_read(base, leader);
+								const uint32_t slot = base + (uint32_t)__popcll(pb & ((1ull << lane) - 1ull));
+								if (pnd) {
+									if (slot < qcap) pq[slot] = child(dy);
+									else overflow = 1;
+								}
+							}
 						}
 						if (GRAD && active && decided && !fail) {
 							any_ok = true;
@@ -673,30 +703,6 @@ __global__ __launch_bounds__(64) void refine32_kernel(const PkJob *__restrict__ 
 	} else {
 		r32_flush_fx(run_fx, run_t, acc, lane);
 	}
-	if (!LAST && handover) {
-		// everything this wavefront holds goes to the next pass
-		for (;;) {
-			if (top > 0) {
-				uint32_t base = 0;
-				if (lane == 0) base = atomicAdd(&cnt->n_cells[pass], (uint32_t)top);
-				base = wave_read(base, 0);
-				for (int i = (int)lane; i < top; i += 64) {
-					if (base + (uint32_t)i < qcap) qout[base + i] = st.cell[i];
-					else overflow = 1;
-				}
-				top = 0;
-				__syncthreads();
-			}
-			if (chunk_next >= chunk_end) break;
-			if (SOURCE == 0) {
-				r32_expand(jobs, comps, entries, n_entries, spread, sstride, st.cell, CAP, top, chunk_next, chunk_end,
-				           lane);
-			} else {
-				r32_take_cells(qin, n_cells_in, stride, st.cell, CAP, top, chunk_next, chunk_end, lane);
-			}
-			__syncthreads();
-		}
-	}
 	for (int off = 32; off > 0; off >>= 1) nev += __shfl_xor(nev, off);
 	const uint64_t ov = __ballot(overflow != 0);
 	const uint32_t nbad = wave_sum_u32(bad);
@@ -708,26 +714,25 @@ __global__ __launch_bounds__(64) void refine32_kernel(const PkJob *__restrict__ 
 	}
 }
 
-// Between two passes: decides in FP64 the sub-cells that pass p handed over
-// undecided (R32_PENDING).  An accepted one adds its value to its pixel's
-// accumulator (GRAD: its adjoint terms to its component) and its slot is
-// marked dead; a failing one stays as an ordinary cell of its level for pass
-// p + 1.  The sub-cell's centre is its corner plus half its size.
+// Between two passes: decides in FP64 the guard-band sub-cells that pass p
+// listed in pq (as cells of their own level).  An accepted one adds its value
+// to its pixel's accumulator (GRAD: its adjoint terms to its component); a
+// failing one joins the queue of pass p + 1.  The sub-cell's centre is its
+// corner plus half its size.
 template <bool GRAD>
 __global__ __launch_bounds__(256) void r32_resolve_kernel(const PkComp *__restrict__ comps,
                                                           const CompLite *__restrict__ lite,
-                                                          PkCounters *__restrict__ cnt, RCell *__restrict__ q,
-                                                          uint32_t qcap, int pass,
+                                                          PkCounters *__restrict__ cnt, const RCell *__restrict__ pq,
+                                                          RCell *__restrict__ qout, uint32_t qcap, int pass,
                                                           unsigned long long *__restrict__ acc,
                                                           const double *__restrict__ gent = nullptr,
                                                           double *__restrict__ gacc = nullptr)
 {
-	const uint32_t n = min(cnt->n_cells[pass], qcap);
-	uint32_t bad = 0;
+	const uint32_t n = min(cnt->n_pend[pass], qcap);
+	uint32_t bad = 0, overflow = 0;
 	for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-		const uint32_t meta = q[i].meta;
-		if (meta == R32_DEAD || !(meta & R32_PENDING)) continue;
-		const RCell c = q[i];
+		const RCell c = pq[i];
+		const uint32_t meta = c.meta;
 		const CompLite &L = lite[c.gcomp];
 		const int lv = (int)cm_level(meta);  // the sub-cell's level as a cell (its parent's + 1)
 		const uint32_t res = cm_res(meta);
@@ -741,7 +746,9 @@ __global__ __launch_bounds__(256) void r32_resolve_kernel(const PkComp *__restri
 		bool fail = false;
 		const double val = refine_fp64_decision(comps[c.gcomp], xp, yp, ddy, recurse, &fail);
 		if (fail) {
-			q[i].meta = meta & ~R32_PENDING;
+			const uint32_t slot = atomicAdd(&cnt->n_cells[pass], 1u);
+			if (slot < qcap) qout[slot] = c;
+			else overflow = 1;
 			continue;
 		}
 		const size_t target = 2 * (size_t)c.entry + cm_conv(meta);
@@ -761,9 +768,9 @@ __global__ __launch_bounds__(256) void r32_resolve_kernel(const PkComp *__restri
 				bad++;
 			}
 		}
-		q[i].meta = R32_DEAD;
 	}
 	if (bad) atomicAdd(&cnt->n_bad, bad);
+	if (overflow) atomicOr(&cnt->overflow, 1u);
 }
 
 // adds the refined contributions to the pixels (one writer per pixel)
