@@ -282,11 +282,11 @@ enum {
 typedef struct pk_tuning {
 	uint32_t conv_tiles;       /* FP32 direct convolution: PK_CONV_TILES_* (FP64 always uses VALU tiles) */
 	uint32_t refine_occupancy; /* FP32 refinement wavefronts, percent (1..100) of the most that fit the GPU */
-	uint32_t handover_div;     /* a refinement pass hands its work on once 1/handover_div of its
-	                              wavefronts ran dry (>= 1) */
+	uint32_t handover_div;     /* no effect (kept for the layout): each refinement pass refines one
+	                              level and hands all failing sub-cells to the next (>= 1) */
 	uint32_t claim_pixels;     /* refinement pixels one wavefront claims at once, at most (>= 1) */
-	uint32_t claim_cells;      /* handed-on cells one wavefront claims at once, at most (>= 1) */
-	uint32_t poll_rounds;      /* refinement rounds between two checks of the hand-on condition (>= 1) */
+	uint32_t claim_cells;      /* cells of the previous level one wavefront claims at once, at most (>= 1) */
+	uint32_t poll_rounds;      /* no effect (kept for the layout) (>= 1) */
 	uint32_t claim_spread;     /* 0: refinement pixels are claimed in the order the pixel pass listed
 	                              them; a power of two n <= 4096: in runs of n taken from 64 places of
 	                              the list in turn (spreads deep galaxy centres over wavefronts) */

@@ -271,12 +271,16 @@ int pk_ctx_create(int device, int precision, pk_ctx **ctx_out)
 	hipDeviceProp_t prop;
 	PK_HIP(hipGetDeviceProperties(&prop, device));
 	int nb_main = 0, nb_last = 0;
-	PK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb_main, refine32_kernel<0, false>, 64, 0));
-	PK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb_last, refine32_kernel<1, true>, 64, 0));
+	PK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb_main, refine32_kernel<0, false>,
+	                                                    64 * r32_waves_per_group(false, false), 0));
+	PK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb_last, refine32_kernel<1, true>,
+	                                                    64 * r32_waves_per_group(true, false), 0));
 	ctx->r32_grid_main = (uint32_t)std::max(1, nb_main * prop.multiProcessorCount);
 	ctx->r32_grid_last = (uint32_t)std::max(1, nb_last * prop.multiProcessorCount);
-	PK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb_main, refine32_kernel<1, false, true>, 64, 0));
-	PK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb_last, refine32_kernel<1, true, true>, 64, 0));
+	PK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb_main, refine32_kernel<1, false, true>,
+	                                                    64 * r32_waves_per_group(false, true), 0));
+	PK_HIP(hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb_last, refine32_kernel<1, true, true>,
+	                                                    64 * r32_waves_per_group(true, true), 0));
 	ctx->r32g_grid_main = (uint32_t)std::max(1, nb_main * prop.multiProcessorCount);
 	ctx->r32g_grid_last = (uint32_t)std::max(1, nb_last * prop.multiProcessorCount);
 	ctx->arch = std::string(prop.gcnArchName) + "/" + std::to_string(prop.multiProcessorCount);
@@ -952,11 +956,13 @@ int launch_profiles(hipStream_t s, bool fp32, const PkJob *djobs, uint32_t njobs
 		for (int p = 0; p < R32_PASSES - 1; p++) {
 			RCell *qin = (p & 1) ? q0 : q1, *qout = (p & 1) ? q1 : q0;
 			if (p == 0)
-				hipLaunchKernelGGL((refine32_kernel<0, false>), dim3(gm), dim3(64), 0, s, djobs, dcomp, dlite, dent,
+				hipLaunchKernelGGL((refine32_kernel<0, false>), dim3(gm), dim3(64 * r32_waves_per_group(false, false)), 0,
+				                   s, djobs, dcomp, dlite, dent,
 				                   dcnt, (const RCell *)nullptr, qout, qcap, 0, dacc, tune, pq,
 				                   (const double *)nullptr, (double *)nullptr);
 			else
-				hipLaunchKernelGGL((refine32_kernel<1, false>), dim3(gm), dim3(64), 0, s, djobs, dcomp, dlite, dent,
+				hipLaunchKernelGGL((refine32_kernel<1, false>), dim3(gm), dim3(64 * r32_waves_per_group(false, false)), 0,
+				                   s, djobs, dcomp, dlite, dent,
 				                   dcnt, (const RCell *)qin, qout, qcap, p, dacc, tune, pq, (const double *)nullptr,
 				                   (double *)nullptr);
 			hipLaunchKernelGGL(r32_resolve_kernel<false>, dim3(R32_RESOLVE_GRID), dim3(256), 0, s, dcomp, dlite, dcnt,
@@ -1982,14 +1988,11 @@ int autotune(pk_batch *b)
 	const int s_conv = whole ? 5 : 3, s_ref = whole ? 5 : 2;
 	if (fp32 && b->any_conv && conv_mfma_lds_bytes((int)b->max_kw, (int)b->max_kh) <= LDS_LIMIT)
 		PK_TRY(search(&pk_tuning::conv_tiles, {PK_CONV_TILES_VALU, PK_CONV_TILES_MFMA}, s_conv));
-	// the refinement settings interact (an earlier hand-over wants larger
-	// claims): two sweeps
+	// the refinement settings interact (claim sizes and spreading): two sweeps
 	for (int sweep = 0; fp32 && b->n_comps && sweep < 2; sweep++) {
-		PK_TRY(search(&pk_tuning::handover_div, {4, 8, 16, 32, 64, 128}, s_ref));
 		PK_TRY(search(&pk_tuning::refine_occupancy, {50, 75, 100}, s_ref));
 		PK_TRY(search(&pk_tuning::claim_pixels, {32, 64, 128, 256, 512}, s_ref));
-		PK_TRY(search(&pk_tuning::claim_cells, {16, 64, 256}, s_ref));
-		PK_TRY(search(&pk_tuning::poll_rounds, {4, 16, 64, 256}, s_ref));
+		PK_TRY(search(&pk_tuning::claim_cells, {16, 64, 256, 512}, s_ref));
 		PK_TRY(search(&pk_tuning::claim_spread, {0, 4, 16, 64}, s_ref));
 	}
 	PK_TRY(tune_run(b, best, st));

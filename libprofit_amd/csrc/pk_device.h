@@ -1307,7 +1307,6 @@ __global__ void reset_grad_kernel(PkCounters *__restrict__ g, const PkCounters *
 		for (int p = 0; p < R32_PASSES; p++) {
 			g->claim[p] = 0;
 			g->n_cells[p] = 0;
-			g->n_done[p] = 0;
 			g->n_pend[p] = 0;
 		}
 	}
@@ -1332,7 +1331,6 @@ __global__ void reset_counters_kernel(PkCounters *c)
 		for (int p = 0; p < R32_PASSES; p++) {
 			c->claim[p] = 0;
 			c->n_cells[p] = 0;
-			c->n_done[p] = 0;
 			c->n_pend[p] = 0;
 		}
 	}

@@ -76,7 +76,6 @@ struct PkCounters {
 	unsigned long long n_subcells;  // profile evaluations made by the refinement
 	uint32_t claim[R32_PASSES];    // work-queue head of FP32 refinement pass p
 	uint32_t n_cells[R32_PASSES];  // cells pass p hands to pass p + 1
-	uint32_t n_done[R32_PASSES];   // wavefronts of pass p that have stopped
 	uint32_t n_pend[R32_PASSES];   // guard-band sub-cells pass p listed for the FP64 decision
 };
 

@@ -41,16 +41,33 @@ namespace pk {
 // children go to the next pass); the last pass refines depth-first, which
 // holds at most about res^2 * (levels left) cells: 1024 covers the Sersic
 // centre pixels (res 8, 10 levels).
-constexpr int R32_CAP_MAIN = 128;
+constexpr int R32_CAP_MAIN = 96;
 constexpr int R32_CAP_LAST = 1024;
-// children staged in LDS before they are written to the next pass's queue
-constexpr int R32_OUTB = 128;
+// children staged in LDS before they are written to the next pass's queue:
+// written once R32_FLUSH are staged, so one more row (<= 64) always fits
+constexpr int R32_FLUSH = 64;
+constexpr int R32_OUTB = R32_FLUSH + 64;
 // workgroups of the launches that decide handed-over sub-cells in FP64
 constexpr uint32_t R32_RESOLVE_GRID = 1024;
 constexpr uint32_t R32_CELL_RUN = 4;       // log2 of the runs in which handed-over cells are claimed
+constexpr uint32_t R32_LEAST_CLAIM = 16;   // items of the smallest claim (when the tuned chunk is not smaller)
 constexpr int R32_TASKS = 16;              // cells per round at most
-// untuned settings: claims of up to 64 pixels / 16 cells, in entry order
-constexpr R32Tune R32_TUNE_DEFAULT = {4u, 64u, 16u, 16u, 0u};
+
+// Wavefronts per workgroup of a refinement launch.  The wavefronts of a
+// workgroup share nothing (each has its own LIFO), so any number works; one
+// per workgroup measured fastest (two: 10 % slower on the galaxy batch).
+constexpr int r32_waves_per_group(bool, bool) { return 1; }
+
+// Orders this wavefront's LDS accesses (the wavefronts of a workgroup do not
+// wait for each other).
+__device__ inline void wave_sync()
+{
+	__builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+	__builtin_amdgcn_wave_barrier();
+}
+// untuned settings: claims of up to 256 pixels / 256 cells, in entry order
+// (smaller claims contend on the queue counters)
+constexpr R32Tune R32_TUNE_DEFAULT = {4u, 256u, 256u, 16u, 0u};
 
 template <int CAP, int OUTB>
 struct R32Stack {
@@ -223,12 +240,15 @@ __device__ inline void r32_expand(const PkJob *__restrict__ jobs, const PkComp *
 }
 
 // Claims the next chunk of a work queue of n items; the chunk shrinks as the
-// queue runs dry (`seen` is the head this wavefront saw last).
-__device__ inline bool r32_claim(uint32_t *head, uint32_t n, uint32_t chunk, uint32_t &seen, uint32_t &next,
-                                 uint32_t &end, uint32_t lane)
+// queue runs dry (`seen` is the head this wavefront saw at its last claim),
+// down to `least` items: claims wait on a counter that every wavefront of the
+// launch contends for, so the last items must not cost one claim each.
+// (Reading the head's current value before each claim was measured slower.)
+__device__ inline bool r32_claim(uint32_t *head, uint32_t n, uint32_t chunk, uint32_t least, uint32_t nwaves,
+                                 uint32_t &seen, uint32_t &next, uint32_t &end, uint32_t lane)
 {
 	const uint32_t left = n > seen ? n - seen : 0u;
-	const uint32_t want = max(1u, min(chunk, left / (2u * gridDim.x)));
+	const uint32_t want = max(least, min(chunk, left / (2u * nwaves)));
 	uint32_t base = 0;
 	if (lane == 0) base = atomicAdd(head, want);
 	base = wave_read(base, 0);
@@ -301,34 +321,33 @@ __device__ inline void r32_flush_run(double run[GA_SLOTS], uint32_t g, double *_
 
 // Writes the n staged children to the next pass's queue (one atomic for all of
 // them) and clears the staging buffer.  Children past the queue's capacity go
-// onto the LIFO (refined by this wavefront) while it has room.
+// onto the LIFO (refined by this wavefront) while it has room.  Copied as
+// 32-bit words: one register per lane in flight, coalesced stores.
 template <int CAP>
 __device__ inline void r32_flush_out(RCell *outb, int &n, RCell *__restrict__ qout, uint32_t qcap, uint32_t *count,
                                      RCell *cells, int cap, int &top, uint32_t &overflow, uint32_t lane)
 {
+	constexpr int WORDS = (int)(sizeof(RCell) / 4);
 	if (n == 0) return;
 	uint32_t base = 0;
 	if (lane == 0) base = atomicAdd(count, (uint32_t)n);
 	base = wave_read(base, 0);
-	for (int i0 = 0; i0 < n; i0 += 64) {
-		const int i = i0 + (int)lane;
-		const bool in = i < n;
-		const bool fits = in && base + (uint32_t)i < qcap;
-		if (fits) qout[base + i] = outb[i];
-		const bool local = in && !fits;
-		const uint64_t b = __ballot(local);
-		if (b) {
-			const int k = __popcll(b);
-			if (top + k <= cap) {
-				if (local) cells[top + __popcll(b & ((1ull << lane) - 1ull))] = outb[i];
-				top += k;
-			} else {
-				overflow = 1;
-			}
+	const int fit = base >= qcap ? 0 : (int)min((uint32_t)n, qcap - base);  // children that go to the queue
+	const uint32_t *src = reinterpret_cast<const uint32_t *>(outb);
+	uint32_t *dst = reinterpret_cast<uint32_t *>(qout + base);
+	for (int i = (int)lane; i < fit * WORDS; i += 64) dst[i] = src[i];
+	if (fit < n) {
+		const int k = n - fit;
+		if (top + k <= cap) {
+			uint32_t *lifo = reinterpret_cast<uint32_t *>(cells + top);
+			for (int i = (int)lane; i < k * WORDS; i += 64) lifo[i] = src[fit * WORDS + i];
+			top += k;
+		} else {
+			overflow = 1;
 		}
 	}
 	n = 0;
-	__builtin_amdgcn_wave_barrier();
+	wave_sync();
 }
 
 // SOURCE 0: pixels from the coarse pass (pass 0); 1: cells of the previous pass.
@@ -343,7 +362,7 @@ __device__ inline void r32_flush_out(RCell *outb, int &n, RCell *__restrict__ qo
 // per-lane running sums, summed over the wavefront only when the component
 // changes.
 template <int SOURCE, bool LAST, bool GRAD = false>
-__global__ __launch_bounds__(64) void refine32_kernel(const PkJob *__restrict__ jobs, const PkComp *__restrict__ comps,
+__global__ __launch_bounds__(128) void refine32_kernel(const PkJob *__restrict__ jobs, const PkComp *__restrict__ comps,
                                                       const CompLite *__restrict__ lite,
                                                       const RefineEntry *__restrict__ entries,
                                                       PkCounters *__restrict__ cnt, const RCell *__restrict__ qin,
@@ -354,8 +373,14 @@ __global__ __launch_bounds__(64) void refine32_kernel(const PkJob *__restrict__ 
                                                       double *__restrict__ gacc = nullptr)
 {
 	constexpr int CAP = LAST ? R32_CAP_LAST : R32_CAP_MAIN;
-	__shared__ R32Stack<CAP, LAST ? 1 : R32_OUTB> st;
+	constexpr int WPG = r32_waves_per_group(LAST, GRAD);
+	__shared__ R32Stack<CAP, LAST ? 1 : R32_OUTB> stacks[WPG];
+	// (the wavefront's index in its workgroup, as a uniform value)
+	const uint32_t wig = WPG == 1 ? 0u : (uint32_t)__builtin_amdgcn_readfirstlane(threadIdx.x / 64);
+	R32Stack<CAP, LAST ? 1 : R32_OUTB> &st = stacks[wig];
 	const uint32_t lane = lane_id();
+	const uint32_t wid = blockIdx.x * WPG + wig;  // this wavefront among the launch's
+	const uint32_t nwaves = gridDim.x * WPG;
 	if (lane < (uint32_t)R32_TASKS) st.csum[lane] = 0;
 	int top = 0;
 	int nout = 0;  // children staged in st.outb
@@ -375,6 +400,12 @@ __global__ __launch_bounds__(64) void refine32_kernel(const PkJob *__restrict__ 
 	uint32_t overflow = 0, bad = 0;
 	unsigned long long nev = 0;
 	uint32_t chunk_next = 0, chunk_end = 0, seen = 0;
+	const uint32_t chunk_max = SOURCE == 0 ? tune.chunk : tune.chunk_cells;
+	const uint32_t least = min(R32_LEAST_CLAIM, chunk_max);
+	// wavefronts past the work's smallest claims have nothing to claim: they
+	// stop without touching the queue's counter (a pass over a short queue
+	// would otherwise spend its time on that counter)
+	if ((size_t)wid * least >= n_source) source_done = true;
 	// per-lane running fixed-point sum of the pixel accumulator run_t (rounds
 	// of one cell whose pixel is the last one's add to it)
 	unsigned long long run_fx = 0;
@@ -382,7 +413,7 @@ __global__ __launch_bounds__(64) void refine32_kernel(const PkJob *__restrict__ 
 	// GRAD: per-lane running sums of two components, run_g and run_g2
 	double run[GA_SLOTS] = {0, 0, 0, 0, 0, 0, 0, 0}, run2[GA_SLOTS] = {0, 0, 0, 0, 0, 0, 0, 0};
 	uint32_t run_g = ~0u, run_g2 = ~0u;
-	__syncthreads();
+	wave_sync();
 	for (;;) {
 		// ---- refill while the cells on the LIFO cannot fill the lanes (one
 		// row of sub-cells per lane) ----
@@ -393,8 +424,7 @@ __global__ __launch_bounds__(64) void refine32_kernel(const PkJob *__restrict__ 
 			if (wave_sum_u32(rows) >= 64u || source_done) break;
 			if (CAP - top < (SOURCE == 0 ? 32 : 1)) break;
 			if (chunk_next >= chunk_end &&
-			    !r32_claim(head, n_source, SOURCE == 0 ? tune.chunk : tune.chunk_cells, seen, chunk_next, chunk_end,
-			               lane)) {
+			    !r32_claim(head, n_source, chunk_max, least, nwaves, seen, chunk_next, chunk_end, lane)) {
 				source_done = true;
 				break;
 			}
@@ -404,7 +434,7 @@ __global__ __launch_bounds__(64) void refine32_kernel(const PkJob *__restrict__ 
 			} else {
 				r32_take_cells(qin, n_cells_in, stride, st.cell, CAP, top, chunk_next, chunk_end, lane);
 			}
-			__syncthreads();
+			wave_sync();
 		}
 		if (top == 0) {
 			// the source is exhausted: the staged children go to the next pass
@@ -526,10 +556,10 @@ __global__ __launch_bounds__(64) void refine32_kernel(const PkJob *__restrict__ 
 					const int n = __popcll(ballot);
 					const int rank = __popcll(ballot & ((1ull << lane) - 1ull));
 					if (!LAST) {
-						// nout < 64 here (flushed at 64), so n <= 64 more fit
+						// nout < R32_FLUSH here, so n <= 64 more fit
 						if (fail) st.outb[nout + rank] = child(dy);
 						nout += n;
-						if (nout >= 64)
+						if (nout >= R32_FLUSH)
 							r32_flush_out<CAP>(st.outb, nout, qout, qcap, &cnt->n_cells[pass], st.cell, CAP, top,
 							                   overflow, lane);
 					} else if (top + n <= CAP) {
@@ -661,7 +691,7 @@ __global__ __launch_bounds__(64) void refine32_kernel(const PkJob *__restrict__ 
 						if (v) atomicAdd(&acc[task_target], v);
 					}
 				}
-				__syncthreads();
+				wave_sync();
 			}
 		};
 		using BoxN = std::false_type;
@@ -707,10 +737,9 @@ __global__ __launch_bounds__(64) void refine32_kernel(const PkJob *__restrict__ 
 	const uint64_t ov = __ballot(overflow != 0);
 	const uint32_t nbad = wave_sum_u32(bad);
 	if (lane == 0) {
-		atomicAdd(&cnt->n_subcells, nev);
+		if (nev) atomicAdd(&cnt->n_subcells, nev);
 		if (ov) atomicOr(&cnt->overflow, 1u);
 		if (nbad) atomicAdd(&cnt->n_bad, nbad);
-		if (!LAST) atomicAdd(&cnt->n_done[pass], 1u);
 	}
 }
 

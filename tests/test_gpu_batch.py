@@ -78,8 +78,8 @@ def test_set_components(ctx64, oracle):
     b.close()
 
 
-DEFAULT_TUNING = {"conv_tiles": 0, "refine_occupancy": 100, "handover_div": 4, "claim_pixels": 64,
-                  "claim_cells": 16, "poll_rounds": 16, "claim_spread": 0}
+DEFAULT_TUNING = {"conv_tiles": 0, "refine_occupancy": 100, "handover_div": 4, "claim_pixels": 256,
+                  "claim_cells": 256, "poll_rounds": 16, "claim_spread": 0}
 
 
 def images(b, n):
