@@ -275,26 +275,6 @@ __device__ inline void r32_take_cells(const RCell *__restrict__ qin, uint32_t n,
 	next += k;
 }
 
-// Adds the adjoint terms of this round's accepted sub-cells (lanes with `ok`)
-// to gacc[component]: one wavefront sum per distinct component in the round
-// (usually one).
-__device__ inline void r32_flush_grad(bool ok, uint32_t gcomp, double ga[GA_SLOTS], double *__restrict__ gacc,
-                                      uint32_t lane)
-{
-	uint64_t todo = __ballot(ok);
-	while (todo) {
-		const int l0 = __ffsll((long long)todo) - 1;
-		const uint32_t g0 = wave_read(gcomp, l0);
-		const bool mine = ok && gcomp == g0;
-		todo &= ~__ballot(mine);
-#pragma unroll
-		for (int k = 0; k < GA_SLOTS; k++) {
-			const double v = wave_sum(mine ? ga[k] : 0.0);
-			if (lane == 0 && v != 0) atomicAdd(&gacc[(size_t)g0 * GA_SLOTS + k], v);
-		}
-	}
-}
-
 // Adds the per-lane running fixed-point sums of pixel accumulator t
 // (wavefront-uniform) to acc and clears them; t = ~0: nothing held.
 __device__ inline void r32_flush_fx(unsigned long long &run, uint64_t t, unsigned long long *__restrict__ acc,
@@ -410,9 +390,10 @@ __global__ __launch_bounds__(128) void refine32_kernel(const PkJob *__restrict__
 	// of one cell whose pixel is the last one's add to it)
 	unsigned long long run_fx = 0;
 	uint64_t run_t = ~0ull;
-	// GRAD: per-lane running sums of two components, run_g and run_g2
-	double run[GA_SLOTS] = {0, 0, 0, 0, 0, 0, 0, 0}, run2[GA_SLOTS] = {0, 0, 0, 0, 0, 0, 0, 0};
-	uint32_t run_g = ~0u, run_g2 = ~0u;
+	// GRAD: per-lane running sums of component run_g (a group belongs to one
+	// component; the sums are handed on when the component changes)
+	double run[GA_SLOTS] = {0, 0, 0, 0, 0, 0, 0, 0};
+	uint32_t run_g = ~0u;
 	wave_sync();
 	for (;;) {
 		// ---- refill while the cells on the LIFO cannot fill the lanes (one
@@ -504,6 +485,10 @@ __global__ __launch_bounds__(128) void refine32_kernel(const PkJob *__restrict__
 		const uint64_t tu = (uint64_t)wave_read((uint32_t)(target >> 32), 0) << 32 | wave_read((uint32_t)target, 0);
 		const bool one_target = __ballot(lane_task && target != tu) == 0;
 		const uint32_t nev_sub = recurse ? 2u : 1u;
+		if (GRAD && g0 != run_g) {
+			r32_flush_run(run, run_g, gacc, lane);
+			run_g = g0;
+		}
 		// the rows of the group for profile kind KIND (-1: no FP32 form, every
 		// sub-cell decided in FP64) with the box flag BOXY
 		auto rows = [&](auto kind_c, auto boxy_c) __attribute__((always_inline)) {
@@ -535,8 +520,6 @@ __global__ __launch_bounds__(128) void refine32_kernel(const PkJob *__restrict__
 				// turned into fixed point once per row; FP64 values in fixed point
 				double rs = 0;
 				unsigned long long fxv = 0;
-				double ga[GA_SLOTS] = {0, 0, 0, 0, 0, 0, 0, 0};
-				bool any_ok = false;
 				// the sub-cell of this lane's row at column centre dy as a cell of
 				// the next level
 				auto child = [&](double dy) __attribute__((always_inline)) {
@@ -609,10 +592,9 @@ __global__ __launch_bounds__(128) void refine32_kernel(const PkJob *__restrict__
 							}
 						}
 						if (GRAD && active && decided && !fail) {
-							any_ok = true;
 							float gt[GT_COUNT];
 							const float vg = sersic_grad32(cl, (float)xp, (float)yp, gt);
-							grad_accumulate(ga, gw * wrow * (double)vg, gt);
+							grad_accumulate(run, gw * wrow * (double)vg, gt);
 						}
 						push_failing(fail, dy);
 					}
@@ -630,10 +612,9 @@ __global__ __launch_bounds__(128) void refine32_kernel(const PkJob *__restrict__
 							const double val = refine_fp64_decision(comps[g0], xp, yp, ddy, recurse, &fail);
 							if (!fail) {
 								if (GRAD) {
-									any_ok = true;
 									float gt[GT_COUNT];
 									const float vg = sersic_grad32(cl, (float)xp, (float)yp, gt);
-									grad_accumulate(ga, gw * wrow * (double)vg, gt);
+									grad_accumulate(run, gw * wrow * (double)vg, gt);
 								} else {
 									const double qv = val * (fr * fr * cl.wflux);
 									if (qv >= 0 && qv < 9.2e18) fxv += (unsigned long long)__double2ll_rn(qv);
@@ -652,26 +633,7 @@ __global__ __launch_bounds__(128) void refine32_kernel(const PkJob *__restrict__
 				}
 				if (active) nev += (unsigned long long)(nev_sub * res);
 				if (GRAD) {
-					const uint64_t okb = __ballot(any_ok);
-					if (okb) {
-						// one component per group: keep adding to the running sums,
-						// after handing on a slot held by another one
-						if (g0 != run_g && g0 != run_g2) {
-							r32_flush_run(run2, run_g2, gacc, lane);
-							run_g2 = run_g;
-#pragma unroll
-							for (int k = 0; k < GA_SLOTS; k++) run2[k] = run[k];
-#pragma unroll
-							for (int k = 0; k < GA_SLOTS; k++) run[k] = 0;
-							run_g = g0;
-						}
-						const bool in1 = g0 == run_g;
-#pragma unroll
-						for (int k = 0; k < GA_SLOTS; k++) {
-							run[k] += (any_ok && in1) ? ga[k] : 0.0;
-							run2[k] += (any_ok && !in1) ? ga[k] : 0.0;
-						}
-					}
+					// the adjoint terms went to the running sums
 				} else if (one_target) {
 					// one pixel accumulator for all lanes: exact per-lane sums, added
 					// to it (lane 0) when the pixel changes
@@ -729,7 +691,6 @@ __global__ __launch_bounds__(128) void refine32_kernel(const PkJob *__restrict__
 	}
 	if (GRAD) {
 		r32_flush_run(run, run_g, gacc, lane);
-		r32_flush_run(run2, run_g2, gacc, lane);
 	} else {
 		r32_flush_fx(run_fx, run_t, acc, lane);
 	}
