@@ -280,7 +280,8 @@ def grad_bench(args, Batch, Context, evaluate_spec, ctx32, ctx64):
     slots = SLOTS_EVAL_ADJ * inf.n_grad_subcells
     ach = slots / (inf.ms_grad_refine * 1e-3) / 1e12 if inf.ms_grad_refine > 0 else 0.0
     out["fits_adjoint_refine_roofline"] = {
-        "kernel": "refine32_kernel<..., GRAD> (4 passes) + grad_finalize_kernel, per step",
+        "kernel": "refine32_kernel<..., GRAD> (7 level passes) + r32_resolve_kernel<GRAD> + grad_finalize_kernel, "
+                  "per step",
         "bound": "valu", "achieved": round(ach, 3), "peak": round(VALU_PEAK_TSLOTS, 2), "unit": "Tslot/s",
         "frac": round(ach / VALU_PEAK_TSLOTS, 4), "evaluations": int(inf.n_grad_subcells),
         "slots_per_evaluation": SLOTS_EVAL_ADJ, "ms": round(inf.ms_grad_refine, 4)}
@@ -460,14 +461,14 @@ def main():
     flops = conv_flops(specs)
     conv_ms = stages["conv"]
     achieved_tflops = flops / (conv_ms * 1e-3) / 1e12
-    mfma = inf.conv_tiles == 1
-    kernel = "conv_mfma_kernel" if mfma else "conv_kernel"
+    mfma = inf.conv_tiles in (1, 2)
+    kernel, form = {1: ("conv_mfma_kernel", "FP32 MFMA tiles"),
+                    2: ("conv_strip_kernel", "FP32 MFMA strips")}.get(inf.conv_tiles, ("conv_kernel", "FP32 FMA"))
     roofline = {
-        "kernel": "%s (direct PSF convolution%s, composition fused in; %d launches per step, one per "
-                  "PSF-size class; figures per step)" % (kernel, ", FP32 MFMA tiles" if mfma else ", FP32 FMA",
-                                                         inf.n_conv_launches),
+        "kernel": "%s (direct PSF convolution, %s, composition fused in; %d launches per step, one per "
+                  "PSF-size class; figures per step)" % (kernel, form, inf.n_conv_launches),
         # 2*kw*kh flop per returned pixel over ~12 B per pixel: far right of the
-        # HBM ridge, bound by the matrix cores (MFMA tiles) or FP32 FMA issue
+        # HBM ridge, bound by the matrix cores (MFMA forms) or FP32 FMA issue
         "bound": "mfma" if mfma else "valu",
         "achieved": round(achieved_tflops, 3),
         "peak": FP32_PEAK_TFLOPS,
@@ -497,7 +498,7 @@ def main():
                              SLOTS_CLASSIFY * pairs + SLOTS_EVAL * wc.coarse_evaluations, stages["coarse"],
                              pixel_component_pairs=pairs, evaluations=int(wc.coarse_evaluations),
                              slots_per_pair=SLOTS_CLASSIFY, slots_per_evaluation=SLOTS_EVAL),
-        "refine": valu_entry("refine32_kernel (4 passes) + finalize32_kernel, per step",
+        "refine": valu_entry("refine32_kernel (7 level passes) + r32_resolve_kernel + finalize32_kernel, per step",
                              SLOTS_EVAL * inf.n_subcells, stages["refine"], evaluations=int(inf.n_subcells),
                              slots_per_evaluation=SLOTS_EVAL),
     }

@@ -272,12 +272,14 @@ enum {
 
 /* Launch settings of a batch.  None of them changes which pixels are refined
  * or how; the refinement settings do not change the result at all (its sums
- * are exact fixed-point), the MFMA convolution tiles change the FP32 rounding
- * of convolved pixels only.
+ * are exact fixed-point), the matrix-core convolution forms change the FP32
+ * rounding of convolved pixels only.
  * Defaults: VALU tiles, 100, 4, 64, 16, 16, 0. */
 enum {
-	PK_CONV_TILES_VALU = 0,  /* FP32 FMA tiles */
-	PK_CONV_TILES_MFMA = 1   /* matrix-core tiles (banded PSF rows) */
+	PK_CONV_TILES_VALU = 0,   /* FP32 FMA tiles */
+	PK_CONV_TILES_MFMA = 1,   /* matrix-core tiles (banded PSF rows) */
+	PK_CONV_TILES_STRIPS = 2  /* matrix-core strips (PSF rows as the matrix rows; PSF sides <= 48,
+	                             larger ones and the adjoint's correlation use the banded tiles) */
 };
 typedef struct pk_tuning {
 	uint32_t conv_tiles;       /* FP32 direct convolution: PK_CONV_TILES_* (FP64 always uses VALU tiles) */

@@ -146,6 +146,7 @@ PK_BATCH_AUTOTUNE = 4
 
 PK_CONV_TILES_VALU = 0
 PK_CONV_TILES_MFMA = 1
+PK_CONV_TILES_STRIPS = 2
 
 
 class Tuning(C.Structure):

@@ -156,7 +156,7 @@ std::string tune_cache_file()
 
 bool tuning_valid(const pk_tuning &t)
 {
-	return t.conv_tiles <= PK_CONV_TILES_MFMA && t.refine_occupancy >= 1 && t.refine_occupancy <= 100 &&
+	return t.conv_tiles <= PK_CONV_TILES_STRIPS && t.refine_occupancy >= 1 && t.refine_occupancy <= 100 &&
 	       t.handover_div >= 1 && t.claim_pixels >= 1 && t.claim_cells >= 1 && t.poll_rounds >= 1 &&
 	       t.handover_div <= 1024 && t.claim_pixels <= 4096 && t.claim_cells <= 4096 && t.poll_rounds <= 4096 &&
 	       t.claim_spread <= 4096 && (t.claim_spread & (t.claim_spread - 1)) == 0;
@@ -783,8 +783,9 @@ int launch_conv(hipStream_t s, const PkJob *djobs, uint32_t njobs, uint32_t ntil
                 uint32_t tiles = PK_CONV_TILES_VALU, T *out_all = nullptr, const T *img_nc = nullptr,
                 uint32_t tile_base = 0)
 {
-	if (tiles == PK_CONV_TILES_MFMA && sizeof(T) == 4 && conv_mfma_lds_bytes((int)max_kw, (int)max_kh) <= LDS_LIMIT) {
-		// FP32 on request: the matrix-core form, whole PSF in one launch
+	if (tiles != PK_CONV_TILES_VALU && sizeof(T) == 4 && conv_mfma_lds_bytes((int)max_kw, (int)max_kh) <= LDS_LIMIT) {
+		// FP32 on request: the banded matrix-core form, whole PSF in one launch
+		// (also where the strip form was asked for: its tiling is the batch's)
 		hipLaunchKernelGGL(conv_mfma_kernel, dim3(ntiles), dim3(256),
 		                   conv_mfma_lds_bytes((int)max_kw, (int)max_kh), s, djobs, njobs,
 		                   reinterpret_cast<const float *>(src), reinterpret_cast<const float *>(kt), masks,
@@ -805,6 +806,33 @@ int launch_conv(hipStream_t s, const PkJob *djobs, uint32_t njobs, uint32_t ntil
 		PK_HIP(hipGetLastError());
 	}
 	return PK_OK;
+}
+
+// ---- matrix-core strips ---------------------------------------------------------
+// Arguments of one strip-form launch: the jobs of one class share the PSF
+// column steps NJ = ceil(kw / 4) and row blocks C = ceil(kh / 16).
+struct StripLaunch {
+	const PkJob *jobs;
+	uint32_t njobs, count, start;
+	const float *src, *kt;
+	const uint8_t *masks;
+	float *dst;
+	const uint32_t *tile_job;
+	float *out_all;
+	const float *img_nc;
+};
+
+template <int NJ, int C>
+bool launch_strips(hipStream_t s, uint32_t nj, uint32_t c, const StripLaunch &a)
+{
+	if (nj == (uint32_t)NJ && c == (uint32_t)C) {
+		hipLaunchKernelGGL((conv_strip_kernel<NJ, C>), dim3(a.count), dim3(256), 0, s, a.jobs, a.njobs, a.src, a.kt,
+		                   a.masks, a.dst, a.tile_job, a.out_all, a.img_nc, a.start);
+		return true;
+	}
+	if constexpr (C < (ST_KMAX + 15) / 16) return launch_strips<NJ, C + 1>(s, nj, c, a);
+	else if constexpr (NJ < (ST_KMAX + 3) / 4) return launch_strips<NJ + 1, 1>(s, nj, c, a);
+	else return false;
 }
 
 // ---- FFT convolution ---------------------------------------------------------
@@ -1617,7 +1645,7 @@ struct pk_batch {
 	std::vector<SetupIn> setup;  // one per radial/sky component
 	std::vector<uint32_t> comp_index;  // user component index -> setup slot (or ~0 for null)
 	std::vector<int32_t> comp_kind;
-	uint32_t ntiles = 0, nctiles = 0, notiles = 0;
+	uint32_t ntiles = 0, nctiles = 0, notiles = 0, nstiles = 0;
 	uint32_t max_kw = 0, max_kh = 0;
 	uint32_t adj_kw = 0, adj_kh = 0;  // largest PSF of the adjoint correlations (FFT jobs included)
 	bool any_conv = false;
@@ -1625,15 +1653,22 @@ struct pk_batch {
 	void *dmem = nullptr;
 	size_t o_jobs = 0, o_setup = 0, o_comps = 0, o_cnt = 0, o_psfT = 0, o_mask = 0, o_nc = 0, o_c = 0, o_conv = 0,
 	       o_ent = 0, o_acc = 0, o_q = 0, o_out = 0, o_lite = 0;
-	size_t o_tj = 0;  // job of every tile: coarse tiles, then convolution tiles, then composition tiles
+	size_t o_tj = 0;  // job of every tile: coarse, convolution, composition tiles, then convolution strip tiles
 	size_t o_wev = 0; // evaluate_at calls of every coarse-pass wavefront (pk_batch_work_counts)
 	size_t o_wsum = 0;
 	// convolution tiles are numbered by PSF-size class, one launch per class
 	// with the LDS its largest PSF needs (smaller PSFs: more workgroups per CU)
 	struct ConvClass {
 		uint32_t start, count, max_kw, max_kh;
+		bool strips;  // its jobs are in strip-form classes too (PSF sides <= ST_KMAX)
 	};
 	std::vector<ConvClass> conv_classes;
+	// the jobs with PSF sides <= ST_KMAX again, numbered in strip tiles by
+	// strip-form class (PSF column steps nj, row blocks c): one launch each
+	struct StripClass {
+		uint32_t start, count, nj, c;
+	};
+	std::vector<StripClass> strip_classes;
 	// psf profiles: rebinning items (PsfProfile::evaluate), the model data their
 	// geometry is recomputed from when the components change, and the
 	// normalised FP64 PSFs they read
@@ -1752,7 +1787,7 @@ int batch_launch(pk_batch *b)
 	RefineEntry *dent = b->d<RefineEntry>(b->o_ent);
 	const uint32_t nj = b->n_models;
 	const uint32_t *tj_coarse = b->d<uint32_t>(b->o_tj), *tj_conv = tj_coarse + b->ntiles,
-	               *tj_out = tj_conv + b->nctiles;
+	               *tj_out = tj_conv + b->nctiles, *tj_strip = tj_out + b->notiles;
 	if (timed) PK_HIP(hipEventRecord(b->ev[0], s));
 	hipLaunchKernelGGL(reset_counters_kernel, dim3(1), dim3(64), 0, s, dcnt);
 	if (b->n_comps)
@@ -1771,10 +1806,19 @@ int batch_launch(pk_batch *b)
 		PK_TRY(launch_psf_layers<float>(b, s, nc, c));
 		if (timed) PK_HIP(hipEventRecord(b->ev[3], s));
 		if (b->any_conv) {
+			const bool strips = b->tune.conv_tiles == PK_CONV_TILES_STRIPS;
+			if (strips)
+				for (const auto &sc : b->strip_classes) {
+					const StripLaunch a{djobs, nj, sc.count, sc.start, c, b->d<float>(b->o_psfT), dmask,
+					                    b->d<float>(b->o_conv), tj_strip, b->d<float>(b->o_out), nc};
+					if (!launch_strips<1, 1>(s, sc.nj, sc.c, a))
+						return set_err(PK_EINVAL, "no strip convolution for %u x %u PSF steps", sc.nj, sc.c);
+				}
 			for (const auto &cc : b->conv_classes)
-				PK_TRY(launch_conv<float>(s, djobs, nj, cc.count, c, b->d<float>(b->o_psfT), dmask,
-				                          b->d<float>(b->o_conv), cc.max_kw, cc.max_kh, tj_conv, b->tune.conv_tiles,
-				                          b->d<float>(b->o_out), nc, cc.start));
+				if (!(strips && cc.strips))
+					PK_TRY(launch_conv<float>(s, djobs, nj, cc.count, c, b->d<float>(b->o_psfT), dmask,
+					                          b->d<float>(b->o_conv), cc.max_kw, cc.max_kh, tj_conv,
+					                          b->tune.conv_tiles, b->d<float>(b->o_out), nc, cc.start));
 			PK_HIP(hipGetLastError());
 			PK_TRY(launch_fft_jobs<float>(b, s, c));
 		}
@@ -1986,8 +2030,16 @@ int autotune(pk_batch *b)
 	// act on both), any other on the stage a setting acts on
 	const bool whole = (b->flags & PK_BATCH_GRAD) && b->have_data;
 	const int s_conv = whole ? 5 : 3, s_ref = whole ? 5 : 2;
-	if (fp32 && b->any_conv && conv_mfma_lds_bytes((int)b->max_kw, (int)b->max_kh) <= LDS_LIMIT)
-		PK_TRY(search(&pk_tuning::conv_tiles, {PK_CONV_TILES_VALU, PK_CONV_TILES_MFMA}, s_conv));
+	if (fp32 && b->any_conv) {
+		const bool band = conv_mfma_lds_bytes((int)b->max_kw, (int)b->max_kh) <= LDS_LIMIT;
+		if (band && b->nstiles)
+			PK_TRY(search(&pk_tuning::conv_tiles, {PK_CONV_TILES_VALU, PK_CONV_TILES_MFMA, PK_CONV_TILES_STRIPS},
+			              s_conv));
+		else if (band)
+			PK_TRY(search(&pk_tuning::conv_tiles, {PK_CONV_TILES_VALU, PK_CONV_TILES_MFMA}, s_conv));
+		else if (b->nstiles)
+			PK_TRY(search(&pk_tuning::conv_tiles, {PK_CONV_TILES_VALU, PK_CONV_TILES_STRIPS}, s_conv));
+	}
 	// the refinement settings interact (claim sizes and spreading): two sweeps
 	for (int sweep = 0; fp32 && b->n_comps && sweep < 2; sweep++) {
 		PK_TRY(search(&pk_tuning::refine_occupancy, {50, 75, 100}, s_ref));
@@ -2021,6 +2073,13 @@ int ensure_tuned(pk_batch *b)
 		}
 	}
 	return autotune(b);
+}
+
+// A job the strip form convolves: direct (not FFT) convolution, PSF sides <= ST_KMAX.
+bool strip_job(const pk_batch *b, const PkJob &jb)
+{
+	return jb.has_conv && !b->is_fft[&jb - b->jobs.data()] && jb.kw <= (uint32_t)ST_KMAX &&
+	       jb.kh <= (uint32_t)ST_KMAX;
 }
 
 }  // namespace
@@ -2348,11 +2407,12 @@ int pk_batch_create(pk_ctx *ctx, const pk_model *models, uint32_t n_models, uint
 			jb.otiles_x = jb.fuse_out ? 0 : cdiv(jb.ow, 64);
 			b->notiles += jb.otiles_x * cdiv(jb.oh, PX_ROWS);
 		}
-		// convolution tiles grouped by PSF size (largest side <= 17, 21, ..., 41, more)
-		const uint32_t class_lim[] = {17, 21, 25, 29, 33, 37, 41, UINT32_MAX};
+		// convolution tiles grouped by PSF size (largest side <= 17, 21, ..., 41,
+		// ST_KMAX, more)
+		const uint32_t class_lim[] = {17, 21, 25, 29, 33, 37, 41, (uint32_t)ST_KMAX, UINT32_MAX};
 		const uint32_t n_class = sizeof(class_lim) / sizeof(class_lim[0]);
 		for (uint32_t lim_lo = 0, ci = 0; ci < n_class; lim_lo = class_lim[ci], ci++) {
-			pk_batch::ConvClass cc{b->nctiles, 0, 0, 0};
+			pk_batch::ConvClass cc{b->nctiles, 0, 0, 0, class_lim[ci] <= (uint32_t)ST_KMAX};
 			for (auto &jb : b->jobs) {
 				const uint32_t side = std::max(jb.kw, jb.kh);
 				if (!jb.has_conv || b->is_fft[&jb - b->jobs.data()] || side <= lim_lo || side > class_lim[ci]) continue;
@@ -2366,11 +2426,31 @@ int pk_batch_create(pk_ctx *ctx, const pk_model *models, uint32_t n_models, uint
 		}
 		for (auto &jb : b->jobs)
 			if (!jb.has_conv || b->is_fft[&jb - b->jobs.data()]) jb.ctile_off = b->nctiles;
+		// strip tiles of the tallest images first, so that a launch ends on
+		// short tiles (workgroups are started in tile order)
+		std::vector<uint32_t> by_height(b->jobs.size());
+		for (uint32_t k = 0; k < by_height.size(); k++) by_height[k] = k;
+		std::stable_sort(by_height.begin(), by_height.end(),
+		                 [&](uint32_t x, uint32_t y) { return b->jobs[x].ch > b->jobs[y].ch; });
+		for (uint32_t c = 1; c <= (ST_KMAX + 15) / 16; c++)
+			for (uint32_t nj = 1; nj <= (ST_KMAX + 3) / 4; nj++) {
+				pk_batch::StripClass sc{b->nstiles, 0, nj, c};
+				for (uint32_t k : by_height) {
+					PkJob &jb = b->jobs[k];
+					if (!strip_job(b, jb) || cdiv(jb.kw, 4) != nj || cdiv(jb.kh, 16) != c) continue;
+					jb.stile_off = b->nstiles;
+					b->nstiles += jb.ctiles_x * cdiv(jb.ch, ST_H);
+				}
+				sc.count = b->nstiles - sc.start;
+				if (sc.count) b->strip_classes.push_back(sc);
+			}
+		for (auto &jb : b->jobs)
+			if (!strip_job(b, jb)) jb.stile_off = b->nstiles;
 	}
 	if (img_off >= (1ull << 32)) return fail(set_err(PK_EINVAL, "batch too large (%llu pixels)", (unsigned long long)img_off));
 
 	// job of every tile of the three tilings (one scalar load per workgroup)
-	std::vector<uint32_t> tj((size_t)b->ntiles + b->nctiles + b->notiles);
+	std::vector<uint32_t> tj((size_t)b->ntiles + b->nctiles + b->notiles + b->nstiles);
 	for (uint32_t jn = 0; jn < n_models; jn++) {
 		const PkJob &jb = b->jobs[jn];
 		const uint32_t nt = (jn + 1 < n_models ? b->jobs[jn + 1].tile_off : b->ntiles) - jb.tile_off;
@@ -2379,6 +2459,8 @@ int pk_batch_create(pk_ctx *ctx, const pk_model *models, uint32_t n_models, uint
 		std::fill_n(tj.begin() + jb.tile_off, nt, jn);
 		std::fill_n(tj.begin() + b->ntiles + jb.ctile_off, nc, jn);
 		std::fill_n(tj.begin() + b->ntiles + b->nctiles + jb.otile_off, no, jn);
+		const uint32_t ns = strip_job(b, jb) ? jb.ctiles_x * cdiv(jb.ch, ST_H) : 0u;
+		std::fill_n(tj.begin() + b->ntiles + b->nctiles + b->notiles + jb.stile_off, ns, jn);
 	}
 	// adjoint of the convolution (PK_BATCH_GRAD): the correlation of the
 	// adjoint image with the PSF covers the whole drawing frame
@@ -2757,8 +2839,12 @@ int pk_batch_sync(pk_batch *b, pk_batch_info *info)
 			PK_HIP(hipMemcpy(&g, b->d<void>(b->o_gcnt), sizeof(g), hipMemcpyDeviceToHost));
 			info->n_grad_subcells = g.n_subcells;
 		}
-		info->n_conv_launches = (uint32_t)b->conv_classes.size();
 		info->conv_tiles = b->elem == 4 ? b->tune.conv_tiles : (uint32_t)PK_CONV_TILES_VALU;
+		info->n_conv_launches = 0;
+		for (const auto &cc : b->conv_classes)
+			if (!(info->conv_tiles == PK_CONV_TILES_STRIPS && cc.strips)) info->n_conv_launches++;
+		if (info->conv_tiles == PK_CONV_TILES_STRIPS)
+			info->n_conv_launches += (uint32_t)b->strip_classes.size();
 		if (b->ran && !(b->flags & PK_BATCH_GRAPH)) {
 			float t[6] = {};
 			(void)hipEventElapsedTime(&t[0], b->ev[0], b->ev[1]);

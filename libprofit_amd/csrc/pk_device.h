@@ -21,6 +21,8 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <utility>
+
 #include "pk_kernels.h"
 
 namespace pk {
@@ -38,8 +40,8 @@ __device__ inline void to_prof(const PkComp &c, double x, double y, double &xp, 
 }
 
 // Job of workgroup `block` in the tiling `which` (0 coarse, 1 convolution,
-// 2 composition): from the batch's per-tile table when given (one scalar
-// load), else by binary search over the jobs' tile offsets.
+// 2 composition, 3 convolution strips): from the batch's per-tile table when
+// given (one scalar load), else by binary search over the jobs' tile offsets.
 __device__ inline uint32_t find_job(const PkJob *jobs, uint32_t njobs, uint32_t block, int which,
                                     const uint32_t *__restrict__ tile_job = nullptr)
 {
@@ -47,7 +49,10 @@ __device__ inline uint32_t find_job(const PkJob *jobs, uint32_t njobs, uint32_t 
 	int lo = 0, hi = (int)njobs - 1;
 	while (lo < hi) {
 		int mid = (lo + hi + 1) >> 1;
-		uint32_t start = which == 0 ? jobs[mid].tile_off : which == 1 ? jobs[mid].ctile_off : jobs[mid].otile_off;
+		uint32_t start = which == 0   ? jobs[mid].tile_off
+		                 : which == 1 ? jobs[mid].ctile_off
+		                 : which == 2 ? jobs[mid].otile_off
+		                              : jobs[mid].stile_off;
 		if (start <= block) lo = mid;
 		else hi = mid - 1;
 	}
@@ -60,6 +65,20 @@ template <int N>
 struct IntTag {
 	static constexpr int value = N;
 };
+
+// f(IntTag<0>{}), ..., f(IntTag<N - 1>{}) in turn, each index a compile-time
+// constant (register arrays indexed by it stay in registers whatever the
+// loop unroller decides)
+template <class F, int... S>
+__device__ __forceinline__ void static_for_seq(F &&f, std::integer_sequence<int, S...>)
+{
+	(f(IntTag<S>{}), ...);
+}
+template <int N, class F>
+__device__ __forceinline__ void static_for(F &&f)
+{
+	static_for_seq(f, std::make_integer_sequence<int, N>{});
+}
 
 // ---------------------------------------------------------------------------
 // pixel passes (coarse, residual, coarse adjoint, composition): a 256-lane
@@ -1025,6 +1044,236 @@ __global__ __launch_bounds__(256, 1) void conv_mfma_kernel(const PkJob *__restri
 				conv_store(jb, o, X, Y, v, conv_unconvolved(jb, o, X, Y));
 			}
 		}
+}
+
+// ---------------------------------------------------------------------------
+// Matrix-core convolution by strips.  A wavefront owns a strip of 16 output
+// columns and up to ST_H rows and streams the source rows through it.  For
+// source row t of the strip (row 0: the strip's first output row shifted up by
+// the PSF's half height),
+//   D_t[l][x] = sum_d K'(d, l) S(t, x + d),   l < 16 C, d < 4 NJ,
+// is one chain of NJ v_mfma_f32_16x16x4f32 per 16 PSF rows (A: the PSF, in
+// registers for the whole strip; B: the source row, from LDS), and D_t[l][x]
+// is the part of output row t - l that source row t gives.  Of the products,
+// only the padding of kw to 4 NJ and of kh to 16 C is not a PSF tap, against
+// 16 + kw - 1 (rounded up to 4) contracted columns for kw taps in the banded
+// form; the cost is the source rows above and below the strip's rows, which
+// a block of 16 PSF rows skips where it adds to none of them.
+//
+// Diagonal sums.  Lane (x, q) of D holds the rows l = 16 c + 4 q + r, r < 4.
+// It adds them into a ring of R = 16 C running sums, the sum of output row y
+// in slot (y + 4 q) mod R, i.e. slot (t - 16 c - r) mod R at step t: a
+// compile-time index in a step loop unrolled R times.  The lane's last term of
+// row y comes at step y + 4 q + 16 (C - 1) + 3 and the slot's next row starts
+// at step y + R + 4 q, so at step t the lane takes slot (t + 1) mod R out:
+// lane group q hands over row t + 1 - R - 4 q to a 32-row LDS stage.  Row y
+// has all four hand-overs after step y + R + 11; rows are summed over the
+// groups and stored 16 at a time (at fixed steps of the unrolled loop, as R
+// is a multiple of 16), with the unconvolved parts of the returned pixels
+// loaded one batch ahead.
+// Rounding: per output a chain of NJ matrix-core steps (4 NJ additions), at
+// most 4 C ring additions and 3 group additions, so for terms of one sign the
+// FP32 error stays below (4 NJ + 4 C + 3) 2^-24 = 3.8e-6 of the sum (kw, kh <= 48).
+// ---------------------------------------------------------------------------
+constexpr int ST_H = 256;    // output rows of a strip tile (4 strips of 16 columns)
+constexpr int ST_KMAX = 48;  // largest PSF side of the strip form (NJ <= 12, C <= 3)
+constexpr int ST_PF = 4;     // source rows loaded ahead
+
+struct StripLds {
+	float row[2][64];         // source rows t and t + 1, columns x0 - kw/2 + 0..63
+	float stage[32][4][16];   // hand-overs of the lane groups, by output row mod 32
+};
+
+template <int NJ, int C>
+__device__ inline void conv_strip(const PkJob &jb, const float *__restrict__ src, const float *__restrict__ krn,
+                                  const uint8_t *__restrict__ masks, const ConvOut<float> &o, int x0, int y0,
+                                  int nrows, StripLds &L, int lane)
+{
+	constexpr int R = 16 * C;
+	static_assert(R % ST_PF == 0 && R % 16 == 0, "ring length");
+	const int W = (int)jb.dw, H = (int)jb.dh, kw = (int)jb.kw, kh = (int)jb.kh;
+	const int lo = lane & 15, kq = lane >> 4;
+	// A operand: lane (l, q) of step j of row block c holds K'(4 j + q, 16 c + l)
+	float kr[C][NJ];
+#pragma unroll
+	for (int c = 0; c < C; c++)
+#pragma unroll
+		for (int j = 0; j < NJ; j++) {
+			const int d = 4 * j + kq, l = 16 * c + lo;
+			kr[c][j] = (d < kw && l < kh) ? krn[(size_t)d * kh + l] : 0.f;
+		}
+	// source row t: image row sy0 + t, columns sx0 + lane, zero outside the
+	// image; loads from a clamped address (no branch), zeroed when written to LDS
+	const int sx0 = x0 + jb.csx - kw / 2, sy0 = y0 + jb.csy - kh / 2;
+	const int sc = sx0 + lane;
+	const bool col_in = sc >= 0 && sc < W;
+	const float *colp = src + (col_in ? sc : 0);
+	const int nsrc = nrows + kh - 1;    // steps with products
+	const int nsteps = nrows + R + 11;  // and the hand-over of the last row
+	auto load_row = [&](int t) -> float {
+		const int sr = min(max(sy0 + t, 0), H - 1);
+		return colp[(size_t)sr * W];
+	};
+	auto row_in = [&](int t) -> bool { return t < nsrc && sy0 + t >= 0 && sy0 + t < H; };
+	// row r is kept in pf[(r - 1) % ST_PF] until it is written to LDS at step r - 1
+	float pf[ST_PF];
+#pragma unroll
+	for (int u = 0; u < ST_PF; u++) pf[u] = load_row(u + 1);
+	{
+		const float v0 = load_row(0);
+		L.row[0][lane] = (col_in && row_in(0)) ? v0 : 0.f;
+	}
+	// B operands of the current row, read one step ahead
+	const float *rb0 = &L.row[0][lo + kq];
+	float b[NJ];
+#pragma unroll
+	for (int j = 0; j < NJ; j++) b[j] = rb0[4 * j];
+	// output: rows are stored 16 at a time, lane (x, q) rows yb + q + 4 i,
+	// each the sum of the four groups' hand-overs, as conv_store and
+	// conv_unconvolved do it, from values taken before the first store (a
+	// store could alias the job record and reload its fields); the
+	// unconvolved parts of the next 16 rows are loaded one batch ahead
+	const int X = x0 + lo;
+	const bool col_out = X < (int)jb.cw;
+	const bool fuse = o.out && jb.fuse_out;
+	float *const outp = fuse ? o.out + jb.out_off + X : o.dst + jb.conv_off + X;
+	const size_t ostride = fuse ? jb.ow : jb.cw;
+	const bool with_nc = fuse && jb.has_nc && X - jb.mox >= 0 && X - jb.mox < (int)jb.dw;
+	const float *const ncb = with_nc ? o.img_nc + jb.img_off + (X - jb.mox) : o.img_nc;
+	const int nc_y0 = y0 - jb.moy, nc_h = (int)jb.dh;
+	const size_t nc_stride = jb.dw;
+	const bool masked = jb.mask_off != NONE;
+	float ncp[4];
+	auto load_nc = [&](int yb) {
+#pragma unroll
+		for (int i = 0; i < 4; i++) {
+			const int yy = yb + kq + 4 * i, my = nc_y0 + yy;
+			ncp[i] = (with_nc && yy < nrows && my >= 0 && my < nc_h) ? ncb[(size_t)my * nc_stride] : 0.f;
+		}
+	};
+	auto flush = [&](int yb) {
+		float v[4];
+#pragma unroll
+		for (int i = 0; i < 4; i++) {
+			const float *g = &L.stage[(yb + kq + 4 * i) & 31][0][lo];
+			v[i] = ((g[0] + g[16]) + g[32]) + g[48];
+		}
+		float nc[4];
+#pragma unroll
+		for (int i = 0; i < 4; i++) nc[i] = ncp[i];
+		load_nc(yb + 16);
+#pragma unroll
+		for (int i = 0; i < 4; i++) {
+			const int yy = yb + kq + 4 * i, Y = y0 + yy;
+			if (yy < nrows && col_out) {
+				float w = v[i];
+				if (masked) {
+					const int mi = X - jb.cmx, mj = Y - jb.cmy;
+					if (mi < 0 || mj < 0 || mi >= W || mj >= H || !masks[jb.mask_off + mi + (size_t)mj * W]) w = 0.f;
+				}
+				outp[(size_t)Y * ostride] = fuse ? (float)((double)w + (double)nc[i]) : w;
+			}
+		}
+	};
+	load_nc(0);
+	float ring[R];
+#pragma unroll
+	for (int u = 0; u < R; u++) ring[u] = 0.f;
+	// one step: row t + 1 to LDS, its operands read, the products of row t,
+	// hand-over.  EDGE (the steps near the strip's ends): row block c only
+	// for t in [16 c, nrows + 16 c + 15), where it adds to the strip's rows,
+	// and t < nsrc; else all row blocks, their products interleaved.  Each
+	// interior step is a block of its own (the test of t against `guard`, a
+	// copy of nsrc the compiler cannot see through, so it cannot drop the
+	// test): without that the compiler interleaves the products of many
+	// steps and runs out of registers.
+	int guard = nsrc;
+	asm volatile("" : "+s"(guard));
+	auto step = [&](auto tag, auto edge, int tb) __attribute__((always_inline)) {
+		constexpr int s = decltype(tag)::value;
+		constexpr bool EDGE = decltype(edge)::value;
+		const int t = tb + s;
+		L.row[(s + 1) & 1][lane] = (col_in && row_in(t + 1)) ? pf[s % ST_PF] : 0.f;
+		pf[s % ST_PF] = load_row(t + 1 + ST_PF);
+		float bn[NJ];
+		const float *rb = &L.row[(s + 1) & 1][lo + kq];
+#pragma unroll
+		for (int j = 0; j < NJ; j++) bn[j] = rb[4 * j];
+		if constexpr (EDGE) {
+#pragma unroll
+			for (int c = 0; c < C; c++) {
+				if (t < nsrc && t >= 16 * c && t < nrows + 16 * c + 15) {
+					floatx4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+					for (int j = 0; j < NJ; j++)
+						acc = __builtin_amdgcn_mfma_f32_16x16x4f32(kr[c][j], b[j], acc, 0, 0, 0);
+#pragma unroll
+					for (int r = 0; r < 4; r++) ring[((s - 16 * c - r) % R + R) % R] += acc[r];
+				}
+			}
+		} else if (t < guard) {
+			floatx4 acc[C];
+#pragma unroll
+			for (int c = 0; c < C; c++) acc[c] = floatx4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+			for (int j = 0; j < NJ; j++)
+#pragma unroll
+				for (int c = 0; c < C; c++) acc[c] = __builtin_amdgcn_mfma_f32_16x16x4f32(kr[c][j], b[j], acc[c], 0, 0, 0);
+#pragma unroll
+			for (int c = 0; c < C; c++)
+#pragma unroll
+				for (int r = 0; r < 4; r++) ring[((s - 16 * c - r) % R + R) % R] += acc[c][r];
+		}
+#pragma unroll
+		for (int j = 0; j < NJ; j++) b[j] = bn[j];
+		// hand-over: group q gives row t + 1 - R - 4 q
+		constexpr int sl = (s + 1) % R;
+		L.stage[(t + 1 - R - 4 * kq) & 31][kq][lo] = ring[sl];
+		ring[sl] = 0.f;
+		// rows up to t + 1 - R - 12 have all four hand-overs; the last of a
+		// batch of 16 at s = 10 (mod 16), as tb and R are multiples of 16
+		if constexpr (s % 16 == 10) {
+			const int y3 = t + 1 - R - 12;
+			if (y3 >= 0) flush(y3 - 15);
+		}
+	};
+	// interior steps: all row blocks add to the strip's rows, t in
+	// [16 (C - 1), nrows + 15) and < nsrc; edge steps before and after (three
+	// loops: one loop holding both forms of the step runs out of registers)
+	const int t_full = min(nsrc, nrows + 15);
+	const int tb_full = C == 1 ? 0 : R;  // first multiple of R >= 16 (C - 1)
+	int tb = 0;
+	for (; tb < tb_full && tb < nsteps; tb += R)
+		static_for<R>([&](auto tag) __attribute__((always_inline)) { step(tag, IntTag<1>{}, tb); });
+	for (; tb + R <= t_full; tb += R)
+		static_for<R>([&](auto tag) __attribute__((always_inline)) { step(tag, IntTag<0>{}, tb); });
+	for (; tb < nsteps; tb += R)
+		static_for<R>([&](auto tag) __attribute__((always_inline)) { step(tag, IntTag<1>{}, tb); });
+	// the last, partial batch of rows
+	const int y_last = tb - R - 12;  // last row with all hand-overs
+	if ((y_last & 15) != 15) flush(y_last & ~15);
+}
+
+// One workgroup: 4 strips (64 columns) of one strip tile; the wavefronts work
+// independently (their own LDS, no barrier).
+template <int NJ, int C>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(C < 3 ? 4 : NJ < 12 ? 3 : 2))) void
+conv_strip_kernel(
+    const PkJob *__restrict__ jobs, uint32_t njobs, const float *__restrict__ src_all,
+    const float *__restrict__ krn_all, const uint8_t *__restrict__ masks, float *__restrict__ dst_all,
+    const uint32_t *__restrict__ tile_job, float *__restrict__ out_all, const float *__restrict__ img_nc,
+    uint32_t tile_base)
+{
+	__shared__ StripLds lds[4];
+	const uint32_t gt = tile_base + blockIdx.x;
+	const PkJob &jb = jobs[find_job(jobs, njobs, gt, 3, tile_job)];
+	const uint32_t t = gt - jb.stile_off;
+	const int w = (int)(threadIdx.x >> 6), lane = (int)(threadIdx.x & 63);
+	const int x0 = (int)(t % jb.ctiles_x) * CT_W + 16 * w, y0 = (int)(t / jb.ctiles_x) * ST_H;
+	if (x0 >= (int)jb.cw || y0 >= (int)jb.ch) return;
+	const int nrows = min(ST_H, (int)jb.ch - y0);
+	const ConvOut<float> o{dst_all, out_all, img_nc};
+	conv_strip<NJ, C>(jb, src_all + jb.img_off, krn_all + jb.psf_off, masks, o, x0, y0, nrows, lds[w], lane);
 }
 
 // ---------------------------------------------------------------------------

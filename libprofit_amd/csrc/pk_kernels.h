@@ -23,6 +23,7 @@ struct PkJob {
 	uint64_t psf_off;            // normalised PSF in the PSF buffer
 	uint32_t kw, kh;
 	uint32_t ctile_off, ctiles_x;  // convolution tiles (prefix over jobs)
+	uint32_t stile_off;            // strip tiles of the matrix-core strip form (ctiles_x per tile row)
 	uint64_t conv_off;           // convolved image (cw x ch) in the conv buffer
 	uint32_t cw, ch;             // dims of the convolved image
 	int32_t csx, csy;            // convolved(X,Y) = same-size convolution at (X+csx, Y+csy)

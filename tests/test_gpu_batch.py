@@ -87,7 +87,7 @@ def images(b, n):
 
 
 def assert_same_to_fp32_rounding(got, ref):
-    # the MFMA convolution tiles sum the PSF products in another order
+    # the matrix-core convolution forms sum the PSF products in another order
     for i, (g, r) in enumerate(zip(got, ref)):
         peak = np.max(np.abs(r))
         assert np.max(np.abs(g - r)) <= 2e-6 * peak, i
@@ -95,8 +95,8 @@ def assert_same_to_fp32_rounding(got, ref):
 
 def test_tuning_settings_keep_the_images(ctx32):
     """The refinement settings only move work between wavefronts: the
-    fixed-point sums make the images bit-identical.  The MFMA convolution
-    tiles change the FP32 rounding only."""
+    fixed-point sums make the images bit-identical.  The matrix-core
+    convolution forms (banded tiles, strips) change the FP32 rounding only."""
     print("CASE tuning settings", flush=True)
     from libprofit_amd._lib import InvalidParameter
     specs = small_batch(16, 11)
@@ -118,11 +118,12 @@ def test_tuning_settings_keep_the_images(ctx32):
             b.sync()
             for i, r in enumerate(ref):
                 assert np.array_equal(b.image(i), r), (setting, i)
-    b.tuning = {**DEFAULT_TUNING, "conv_tiles": 1}
-    b.evaluate()
-    b.sync()
-    assert_same_to_fp32_rounding(images(b, len(specs)), ref)
-    for bad in ({"handover_div": 0}, {"refine_occupancy": 101}, {"conv_tiles": 2}, {"claim_pixels": 0},
+    for tiles in (1, 2):
+        b.tuning = {**DEFAULT_TUNING, "conv_tiles": tiles}
+        b.evaluate()
+        b.sync()
+        assert_same_to_fp32_rounding(images(b, len(specs)), ref)
+    for bad in ({"handover_div": 0}, {"refine_occupancy": 101}, {"conv_tiles": 3}, {"claim_pixels": 0},
                 {"claim_spread": 3}, {"claim_spread": 8192}):
         with pytest.raises(InvalidParameter):
             b.tuning = {**DEFAULT_TUNING, **bad}
@@ -148,7 +149,7 @@ def test_autotune_and_its_cache(ctx32, tmp_path, monkeypatch):
     t = b.tuning.as_dict()
     print("CASE tuned", t, flush=True)
     assert t["ms_evaluate"] > 0
-    assert t["conv_tiles"] in (0, 1) and 1 <= t["refine_occupancy"] <= 100
+    assert t["conv_tiles"] in (0, 1, 2) and 1 <= t["refine_occupancy"] <= 100
     for _rep in range(2):
         got = images(b, len(specs))
         if t["conv_tiles"] == 0:
@@ -204,31 +205,34 @@ def test_batch_mixed_composition(ctx64, ctx32, oracle, prec):
                                    return_finesampled=rf, magzero=30.0))
     ctx = ctx64 if prec == "fp64" else ctx32
     tol = TOL_FP64 if prec == "fp64" else TOL_FP32
-    b = Batch(ctx, specs, graph=True)
-    for _rep in range(2):
-        b.evaluate()
-        b.sync()
-        for i, s in enumerate(specs):
-            ref, _ = oracle.evaluate(s)
-            got = b.image(i)
-            assert got.shape == ref.shape, (i, got.shape, ref.shape)
-            check_image(got, s, ref, oracle, tol, i)
-    b.close()
+    refs = [oracle.evaluate(s)[0] for s in specs]
+    for tiles in ((0,) if prec == "fp64" else (0, 1, 2)):
+        b = Batch(ctx, specs, graph=True)
+        b.tuning = {**DEFAULT_TUNING, "conv_tiles": tiles}
+        for _rep in range(2):
+            b.evaluate()
+            b.sync()
+            for i, (s, ref) in enumerate(zip(specs, refs)):
+                got = b.image(i)
+                assert got.shape == ref.shape, (tiles, i, got.shape, ref.shape)
+                check_image(got, s, ref, oracle, tol, (tiles, i))
+        b.close()
 
 
 # every PSF-size class of the batch convolution (largest side <= 17, 21, ...,
-# 41 and above), odd sides from 15 to 41, and even, non-square, tiny and
-# wide PSFs (the run-time column paths)
+# 41, 48 and above), odd sides from 15 to 41, and even, non-square, tiny and
+# wide PSFs (the run-time column paths; for the strip form: every count of
+# PSF column steps 1..12 and row blocks 1..3, and the banded tiles above 48)
 PSF_SHAPES = [(15, 15), (17, 17), (19, 19), (21, 21), (23, 23), (25, 25), (27, 27), (29, 29), (31, 31), (33, 33),
               (35, 35), (37, 37), (39, 39), (41, 41), (51, 51), (3, 3), (8, 8), (5, 9), (12, 7), (1, 5), (30, 30),
-              (16, 24), (21, 31), (41, 15)]
+              (16, 24), (21, 31), (41, 15), (45, 47), (48, 48), (47, 33)]
 
 
-@pytest.mark.parametrize("tiles", [0, 1])
+@pytest.mark.parametrize("tiles", [0, 1, 2])
 def test_batch_psf_shapes(ctx32, oracle, tiles):
-    """Both convolution forms (VALU tiles, MFMA tiles) against the oracle in
-    FP32 on stamps whose galaxy dominates its pixels, over every PSF-size
-    class and the odd, even and non-square PSF shapes."""
+    """The convolution forms (VALU tiles, banded MFMA tiles, MFMA strips)
+    against the oracle in FP32 on stamps whose galaxy dominates its pixels,
+    over every PSF-size class and the odd, even and non-square PSF shapes."""
     print("CASE psf shapes, conv_tiles", tiles, flush=True)
     specs = []
     for k, (pw, ph) in enumerate(PSF_SHAPES):
@@ -261,7 +265,7 @@ def test_batch_each_profile(ctx64, ctx32, oracle, prec, profile):
     specs = workloads.galaxy_batch(10, 7, smin=24, smax=160, profiles=(profile,))
     ctx = ctx64 if prec == "fp64" else ctx32
     tol = TOL_FP64 if prec == "fp64" else TOL_FP32
-    for tiles in ((0,) if prec == "fp64" else (0, 1)):
+    for tiles in ((0,) if prec == "fp64" else (0, 1, 2)):
         b = Batch(ctx, specs, graph=False)
         b.tuning = {**DEFAULT_TUNING, "conv_tiles": tiles}
         b.evaluate()
@@ -301,17 +305,21 @@ def test_batch_masks(ctx64, ctx32, oracle, prec):
     specs.append(model(s, None))                                                       # no mask
     ctx = ctx64 if prec == "fp64" else ctx32
     tol = TOL_FP64 if prec == "fp64" else TOL_FP32
-    b = Batch(ctx, specs, graph=True)
-    for _rep in range(2):
-        b.evaluate()
-        b.sync()
-        for i, sp in enumerate(specs):
-            ref, _ = oracle.evaluate(sp)
-            got = b.image(i)
-            assert got.shape == ref.shape, (i, got.shape, ref.shape)
-            assert np.array_equal(got == 0, ref == 0), i  # masked pixels are exactly 0 in both
-            check_image(got, sp, ref, oracle, tol, i)
-    b.close()
+    refs = [oracle.evaluate(sp)[0] for sp in specs]
+    # every convolution form of the precision (FP32: VALU tiles, banded MFMA
+    # tiles, MFMA strips), each masking the pixels it stores
+    for tiles in ((0,) if prec == "fp64" else (0, 1, 2)):
+        b = Batch(ctx, specs, graph=True)
+        b.tuning = {**DEFAULT_TUNING, "conv_tiles": tiles}
+        for _rep in range(2):
+            b.evaluate()
+            b.sync()
+            for i, (sp, ref) in enumerate(zip(specs, refs)):
+                got = b.image(i)
+                assert got.shape == ref.shape, (tiles, i, got.shape, ref.shape)
+                assert np.array_equal(got == 0, ref == 0), (tiles, i)  # masked pixels are exactly 0 in both
+                check_image(got, sp, ref, oracle, tol, (tiles, i))
+        b.close()
 
 
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])

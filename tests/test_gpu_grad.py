@@ -217,8 +217,8 @@ def test_graph_replays_match_eager(ctx32, oracle):
 
 def test_tuning_keeps_the_gradient(ctx32, oracle):
     """Launch settings of the adjoint: the refinement settings reorder only the
-    FP64 atomic sums of the gradient; the MFMA convolution tiles change the
-    FP32 rounding of the PSF correlation."""
+    FP64 atomic sums of the gradient; the matrix-core convolution forms change
+    the FP32 rounding of the forward convolution and the PSF correlation."""
     announce("grad tuning")
     fm = small_fit()
     data, sigma = make_data(oracle, fm)
@@ -231,7 +231,7 @@ def test_tuning_keeps_the_gradient(ctx32, oracle):
             "poll_rounds": 16, "claim_spread": 0}
     for setting, rtol in (({"handover_div": 2, "refine_occupancy": 50, "claim_pixels": 16, "poll_rounds": 4,
                             "claim_spread": 16}, 1e-11),
-                          ({"conv_tiles": 1}, 1e-5)):
+                          ({"conv_tiles": 1}, 1e-5), ({"conv_tiles": 2}, 1e-5)):
         b.tuning = {**base, **setting}
         b.evaluate_grad()
         c1, g1 = b.chi2_grad()
