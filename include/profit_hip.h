@@ -279,7 +279,8 @@ enum {
 	PK_CONV_TILES_VALU = 0,   /* FP32 FMA tiles */
 	PK_CONV_TILES_MFMA = 1,   /* matrix-core tiles (banded PSF rows) */
 	PK_CONV_TILES_STRIPS = 2  /* matrix-core strips (PSF rows as the matrix rows; PSF sides <= 48,
-	                             larger ones and the adjoint's correlation use the banded tiles) */
+	                             larger ones use the banded tiles; the adjoint's correlation too when
+	                             a gradient batch holds one) */
 };
 typedef struct pk_tuning {
 	uint32_t conv_tiles;       /* FP32 direct convolution: PK_CONV_TILES_* (FP64 always uses VALU tiles) */

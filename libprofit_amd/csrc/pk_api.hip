@@ -1712,6 +1712,11 @@ struct pk_batch {
 	// convolved frame, and the job of each of its tiles
 	size_t o_jobs_adj = 0, o_tja = 0;
 	uint32_t natiles = 0;
+	// the same correlations in strip tiles (when every convolved job's PSF
+	// sides are <= ST_KMAX), their job table after the adjoint tiles' in o_tja
+	bool adj_strips = false;
+	uint32_t nastiles = 0;
+	std::vector<StripClass> adj_strip_classes;
 	uint32_t qcap = 0;
 	std::vector<double> fx_scales;
 	uint64_t total_out = 0;
@@ -1875,10 +1880,22 @@ int grad_launch_t(pk_batch *b)
 	                   (const T *)b->d<T>(b->o_data), (const T *)b->d<T>(b->o_ivar), Gd, chi2, tj_coarse,
 	                   (const uint8_t *)b->d<uint8_t>(b->o_mask));
 	PK_HIP(hipGetLastError());
-	if (b->any_conv)
-		PK_TRY(launch_conv<T>(s, b->d<PkJob>(b->o_jobs_adj), nj, b->natiles, Gd, b->d<T>(b->o_psfF),
-		                      b->d<uint8_t>(b->o_mask), Gc, b->adj_kw, b->adj_kh, b->d<uint32_t>(b->o_tja),
-		                      b->tune.conv_tiles));
+	if (b->any_conv) {
+		if (sizeof(T) == 4 && b->adj_strips && b->tune.conv_tiles == PK_CONV_TILES_STRIPS) {
+			for (const auto &sc : b->adj_strip_classes) {
+				const StripLaunch a{b->d<PkJob>(b->o_jobs_adj), nj, sc.count, sc.start,
+				                    reinterpret_cast<const float *>(Gd), b->d<float>(b->o_psfF),
+				                    b->d<uint8_t>(b->o_mask), reinterpret_cast<float *>(Gc),
+				                    b->d<uint32_t>(b->o_tja) + b->natiles, nullptr, nullptr};
+				if (!launch_strips<1, 1>(s, sc.nj, sc.c, a))
+					return set_err(PK_EINVAL, "no strip correlation for %u x %u PSF steps", sc.nj, sc.c);
+			}
+		} else {
+			PK_TRY(launch_conv<T>(s, b->d<PkJob>(b->o_jobs_adj), nj, b->natiles, Gd, b->d<T>(b->o_psfF),
+			                      b->d<uint8_t>(b->o_mask), Gc, b->adj_kw, b->adj_kh, b->d<uint32_t>(b->o_tja),
+			                      b->tune.conv_tiles));
+		}
+	}
 	const bool fp32 = sizeof(T) == 4;
 	hipLaunchKernelGGL((coarse_grad_kernel<T, fp32>), dim3(b->ntiles), dim3(256), 0, s, djobs, nj,
 	                   (const PkComp *)dcomp, (const uint8_t *)b->d<uint8_t>(b->o_mask), (const T *)Gd,
@@ -2487,6 +2504,30 @@ int pk_batch_create(pk_ctx *ctx, const pk_model *models, uint32_t n_models, uint
 			const uint32_t n = jb.ctiles_x * cdiv(jb.dh, CT_H);
 			tja.insert(tja.end(), n, jn);
 			b->natiles += n;
+		}
+		b->adj_strips = b->any_conv;
+		for (const auto &jb : adj)
+			if (jb.has_conv && (jb.kw > (uint32_t)ST_KMAX || jb.kh > (uint32_t)ST_KMAX)) b->adj_strips = false;
+		if (b->adj_strips) {
+			// strip tiles by class, the tallest images first (as the forward ones)
+			std::vector<uint32_t> by_height(adj.size());
+			for (uint32_t k = 0; k < by_height.size(); k++) by_height[k] = k;
+			std::stable_sort(by_height.begin(), by_height.end(),
+			                 [&](uint32_t x, uint32_t y) { return adj[x].ch > adj[y].ch; });
+			for (uint32_t c = 1; c <= (ST_KMAX + 15) / 16; c++)
+				for (uint32_t nj = 1; nj <= (ST_KMAX + 3) / 4; nj++) {
+					pk_batch::StripClass sc{b->nastiles, 0, nj, c};
+					for (uint32_t k : by_height) {
+						PkJob &jb = adj[k];
+						if (!jb.has_conv || cdiv(jb.kw, 4) != nj || cdiv(jb.kh, 16) != c) continue;
+						jb.stile_off = b->nastiles;
+						const uint32_t n = jb.ctiles_x * cdiv(jb.ch, ST_H);
+						tja.insert(tja.end(), n, k);
+						b->nastiles += n;
+					}
+					sc.count = b->nastiles - sc.start;
+					if (sc.count) b->adj_strip_classes.push_back(sc);
+				}
 		}
 	}
 	// psf profiles in layers (k-th psf profile of each model), items in

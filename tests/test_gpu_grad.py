@@ -42,8 +42,16 @@ def make_data(oracle, fm_true, seed=0):
     return workloads.noisy(img, rng)
 
 
-def device_chi2_grad(ctx, fms, datas, sigmas, graph=False):
+# launch settings with each FP32 convolution form (VALU tiles, banded MFMA
+# tiles, MFMA strips) for the forward convolution and the adjoint correlation
+FORMS = [{"conv_tiles": t, "refine_occupancy": 100, "handover_div": 4, "claim_pixels": 64, "claim_cells": 16,
+          "poll_rounds": 16, "claim_spread": 0} for t in (0, 1, 2)]
+
+
+def device_chi2_grad(ctx, fms, datas, sigmas, graph=False, tuning=None):
     b = Batch(ctx, [fm.spec() for fm in fms], graph=graph, grad=True)
+    if tuning is not None:
+        b.tuning = tuning
     b.set_data(datas, sigmas)
     b.evaluate_grad()
     chi2, grad = b.chi2_grad()
@@ -83,9 +91,10 @@ def test_small_fit_gradient_matches_frozen_fd(ctx64, ctx32, oracle, prec):
     data, sigma = make_data(oracle, truth)
     g_fd, fd_err, g_rel, c0 = oracle_fd_grad(oracle, fm, fm.theta(), data, sigma, with_error=True)
     ctx = ctx64 if prec == "fp64" else ctx32
-    chi2, (g,) = device_chi2_grad(ctx, [fm], [data], [sigma])
-    assert chi2[0] == pytest.approx(c0, rel=1e-12 if prec == "fp64" else 1e-5)
-    check_close(g, g_fd, prec, fm.names(), fd_err, g_rel)
+    for tuning in ([None] if prec == "fp64" else FORMS):
+        chi2, (g,) = device_chi2_grad(ctx, [fm], [data], [sigma], tuning=tuning)
+        assert chi2[0] == pytest.approx(c0, rel=1e-12 if prec == "fp64" else 1e-5)
+        check_close(g, g_fd, prec, fm.names(), fd_err, g_rel)
 
 
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
@@ -97,9 +106,10 @@ def test_bulge_disk256_gradient_matches_frozen_fd(ctx64, ctx32, oracle, prec):
     data, sigma = make_data(oracle, truth)
     g_fd, fd_err, g_rel, c0 = oracle_fd_grad(oracle, fm, fm.theta(), data, sigma, with_error=True)
     ctx = ctx64 if prec == "fp64" else ctx32
-    chi2, (g,) = device_chi2_grad(ctx, [fm], [data], [sigma])
-    assert chi2[0] == pytest.approx(c0, rel=1e-12 if prec == "fp64" else 1e-5)
-    check_close(g, g_fd, prec, fm.names(), fd_err, g_rel)
+    for tuning in ([None] if prec == "fp64" else FORMS):
+        chi2, (g,) = device_chi2_grad(ctx, [fm], [data], [sigma], tuning=tuning)
+        assert chi2[0] == pytest.approx(c0, rel=1e-12 if prec == "fp64" else 1e-5)
+        check_close(g, g_fd, prec, fm.names(), fd_err, g_rel)
 
 
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
@@ -119,9 +129,10 @@ def test_masked_fit_gradient_matches_frozen_fd(ctx64, ctx32, oracle, prec):
     data, sigma = make_data(oracle, truth)
     g_fd, fd_err, g_rel, c0 = oracle_fd_grad(oracle, fm, fm.theta(), data, sigma, with_error=True)
     ctx = ctx64 if prec == "fp64" else ctx32
-    chi2, (g,) = device_chi2_grad(ctx, [fm], [data], [sigma])
-    assert chi2[0] == pytest.approx(c0, rel=1e-12 if prec == "fp64" else 1e-5)
-    check_close(g, g_fd, prec, fm.names(), fd_err, g_rel)
+    for tuning in ([None] if prec == "fp64" else FORMS):
+        chi2, (g,) = device_chi2_grad(ctx, [fm], [data], [sigma], tuning=tuning)
+        assert chi2[0] == pytest.approx(c0, rel=1e-12 if prec == "fp64" else 1e-5)
+        check_close(g, g_fd, prec, fm.names(), fd_err, g_rel)
 
 
 @pytest.mark.parametrize("shape", [(8, 8), (10, 7)])
@@ -142,9 +153,10 @@ def test_even_psf_gradient_matches_frozen_fd(ctx64, ctx32, oracle, prec, shape):
     data, sigma = make_data(oracle, truth)
     g_fd, fd_err, g_rel, c0 = oracle_fd_grad(oracle, fm, fm.theta(), data, sigma, with_error=True)
     ctx = ctx64 if prec == "fp64" else ctx32
-    chi2, (g,) = device_chi2_grad(ctx, [fm], [data], [sigma])
-    assert chi2[0] == pytest.approx(c0, rel=1e-12 if prec == "fp64" else 1e-5)
-    check_close(g, g_fd, prec, fm.names(), fd_err, g_rel)
+    for tuning in ([None] if prec == "fp64" else FORMS):
+        chi2, (g,) = device_chi2_grad(ctx, [fm], [data], [sigma], tuning=tuning)
+        assert chi2[0] == pytest.approx(c0, rel=1e-12 if prec == "fp64" else 1e-5)
+        check_close(g, g_fd, prec, fm.names(), fd_err, g_rel)
 
 
 @pytest.mark.parametrize("prec", ["fp64", "fp32"])
@@ -162,9 +174,10 @@ def test_fft_convolved_fit_gradient(ctx64, ctx32, oracle, prec):
     data, sigma = make_data(oracle, truth)
     g_fd, fd_err, g_rel, c0 = oracle_fd_grad(oracle, fm, fm.theta(), data, sigma, with_error=True)
     ctx = ctx64 if prec == "fp64" else ctx32
-    chi2, (g,) = device_chi2_grad(ctx, [fm], [data], [sigma])
-    assert chi2[0] == pytest.approx(c0, rel=1e-9 if prec == "fp64" else 1e-5)
-    check_close(g, g_fd, prec, fm.names(), fd_err, g_rel)
+    for tuning in ([None] if prec == "fp64" else FORMS):
+        chi2, (g,) = device_chi2_grad(ctx, [fm], [data], [sigma], tuning=tuning)
+        assert chi2[0] == pytest.approx(c0, rel=1e-9 if prec == "fp64" else 1e-5)
+        check_close(g, g_fd, prec, fm.names(), fd_err, g_rel)
 
 
 def test_batch_of_fits(ctx32, oracle):
