@@ -781,7 +781,7 @@ template <typename T>
 int launch_conv(hipStream_t s, const PkJob *djobs, uint32_t njobs, uint32_t ntiles, const T *src, const T *kt,
                 const uint8_t *masks, T *dst, uint32_t max_kw, uint32_t max_kh, const uint32_t *tile_job = nullptr,
                 uint32_t tiles = PK_CONV_TILES_VALU, T *out_all = nullptr, const T *img_nc = nullptr,
-                uint32_t tile_base = 0)
+                uint32_t tile_base = 0, const PkComp *comps = nullptr)
 {
 	if (tiles != PK_CONV_TILES_VALU && sizeof(T) == 4 && conv_mfma_lds_bytes((int)max_kw, (int)max_kh) <= LDS_LIMIT) {
 		// FP32 on request: the banded matrix-core form, whole PSF in one launch
@@ -790,7 +790,7 @@ int launch_conv(hipStream_t s, const PkJob *djobs, uint32_t njobs, uint32_t ntil
 		                   conv_mfma_lds_bytes((int)max_kw, (int)max_kh), s, djobs, njobs,
 		                   reinterpret_cast<const float *>(src), reinterpret_cast<const float *>(kt), masks,
 		                   reinterpret_cast<float *>(dst), tile_job, reinterpret_cast<float *>(out_all),
-		                   reinterpret_cast<const float *>(img_nc), tile_base);
+		                   reinterpret_cast<const float *>(img_nc), tile_base, comps);
 		PK_HIP(hipGetLastError());
 		return PK_OK;
 	}
@@ -802,7 +802,8 @@ int launch_conv(hipStream_t s, const PkJob *djobs, uint32_t njobs, uint32_t ntil
 		// the returned image is written directly only by a single launch
 		const bool last = l0 + rows >= max_kh;
 		hipLaunchKernelGGL(conv_kernel<T>, dim3(ntiles), dim3(256), lds, s, djobs, njobs, src, kt, masks, dst,
-		                   (int)l0, (int)rows, tile_job, (l0 == 0 && last) ? out_all : (T *)nullptr, img_nc, tile_base);
+		                   (int)l0, (int)rows, tile_job, (l0 == 0 && last) ? out_all : (T *)nullptr, img_nc, tile_base,
+		                   comps);
 		PK_HIP(hipGetLastError());
 	}
 	return PK_OK;
@@ -820,6 +821,7 @@ struct StripLaunch {
 	const uint32_t *tile_job;
 	float *out_all;
 	const float *img_nc;
+	const PkComp *comps;
 };
 
 template <int NJ, int C>
@@ -827,7 +829,7 @@ bool launch_strips(hipStream_t s, uint32_t nj, uint32_t c, const StripLaunch &a)
 {
 	if (nj == (uint32_t)NJ && c == (uint32_t)C) {
 		hipLaunchKernelGGL((conv_strip_kernel<NJ, C>), dim3(a.count), dim3(256), 0, s, a.jobs, a.njobs, a.src, a.kt,
-		                   a.masks, a.dst, a.tile_job, a.out_all, a.img_nc, a.start);
+		                   a.masks, a.dst, a.tile_job, a.out_all, a.img_nc, a.start, a.comps);
 		return true;
 	}
 	if constexpr (C < (ST_KMAX + 15) / 16) return launch_strips<NJ, C + 1>(s, nj, c, a);
@@ -1815,7 +1817,7 @@ int batch_launch(pk_batch *b)
 			if (strips)
 				for (const auto &sc : b->strip_classes) {
 					const StripLaunch a{djobs, nj, sc.count, sc.start, c, b->d<float>(b->o_psfT), dmask,
-					                    b->d<float>(b->o_conv), tj_strip, b->d<float>(b->o_out), nc};
+					                    b->d<float>(b->o_conv), tj_strip, b->d<float>(b->o_out), nc, dcomp};
 					if (!launch_strips<1, 1>(s, sc.nj, sc.c, a))
 						return set_err(PK_EINVAL, "no strip convolution for %u x %u PSF steps", sc.nj, sc.c);
 				}
@@ -1823,7 +1825,7 @@ int batch_launch(pk_batch *b)
 				if (!(strips && cc.strips))
 					PK_TRY(launch_conv<float>(s, djobs, nj, cc.count, c, b->d<float>(b->o_psfT), dmask,
 					                          b->d<float>(b->o_conv), cc.max_kw, cc.max_kh, tj_conv,
-					                          b->tune.conv_tiles, b->d<float>(b->o_out), nc, cc.start));
+					                          b->tune.conv_tiles, b->d<float>(b->o_out), nc, cc.start, dcomp));
 			PK_HIP(hipGetLastError());
 			PK_TRY(launch_fft_jobs<float>(b, s, c));
 		}
@@ -1842,7 +1844,7 @@ int batch_launch(pk_batch *b)
 			for (const auto &cc : b->conv_classes)
 				PK_TRY(launch_conv<double>(s, djobs, nj, cc.count, c, b->d<double>(b->o_psfT), dmask,
 				                           b->d<double>(b->o_conv), cc.max_kw, cc.max_kh, tj_conv, PK_CONV_TILES_VALU,
-				                           b->d<double>(b->o_out), nc, cc.start));
+				                           b->d<double>(b->o_out), nc, cc.start, dcomp));
 			PK_HIP(hipGetLastError());
 			PK_TRY(launch_fft_jobs<double>(b, s, c));
 		}
@@ -1886,7 +1888,7 @@ int grad_launch_t(pk_batch *b)
 				const StripLaunch a{b->d<PkJob>(b->o_jobs_adj), nj, sc.count, sc.start,
 				                    reinterpret_cast<const float *>(Gd), b->d<float>(b->o_psfF),
 				                    b->d<uint8_t>(b->o_mask), reinterpret_cast<float *>(Gc),
-				                    b->d<uint32_t>(b->o_tja) + b->natiles, nullptr, nullptr};
+				                    b->d<uint32_t>(b->o_tja) + b->natiles, nullptr, nullptr, nullptr};
 				if (!launch_strips<1, 1>(s, sc.nj, sc.c, a))
 					return set_err(PK_EINVAL, "no strip correlation for %u x %u PSF steps", sc.nj, sc.c);
 			}
@@ -2420,6 +2422,15 @@ int pk_batch_create(pk_ctx *ctx, const pk_model *models, uint32_t n_models, uint
 		const bool one_launch = !b->any_conv || rows >= b->max_kh;
 		for (auto &jb : b->jobs) {
 			if (!one_launch) jb.fuse_out = 0;
+			// FP32: a job whose unconvolved part is only sky, unmasked, whose
+			// convolution writes the returned image, takes that part as one
+			// value (the unconvolved image is never stored)
+			bool sky_only = elem == 4 && jb.has_nc && jb.fuse_out && jb.mask_off == NONE;
+			for (uint32_t k = jb.comp_off; sky_only && k < jb.comp_off + jb.ncomp; k++)
+				if (!b->setup[k].c.convolve && b->setup[k].c.kind != PK_PROFILE_SKY) sky_only = false;
+			for (const auto &it : b->psf_items)
+				if (&b->jobs[it.job] == &jb && !it.to_conv) sky_only = false;
+			jb.nc_sky = sky_only ? 1 : 0;
 			jb.otile_off = b->notiles;
 			jb.otiles_x = jb.fuse_out ? 0 : cdiv(jb.ow, 64);
 			b->notiles += jb.otiles_x * cdiv(jb.oh, PX_ROWS);
@@ -2492,6 +2503,7 @@ int pk_batch_create(pk_ctx *ctx, const pk_model *models, uint32_t n_models, uint
 			jb.mask_off = NONE;  // the adjoint image is zero at masked pixels already
 			jb.conv_off = jb.img_off;  // the correlation lands in the drawing-frame layout
 			jb.fuse_out = 0;
+			jb.nc_sky = 0;  // no unconvolved part is added to the correlation
 			if (!jb.has_conv) continue;
 			jb.cw = jb.dw;
 			jb.ch = jb.dh;

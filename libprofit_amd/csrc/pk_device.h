@@ -131,7 +131,7 @@ __device__ inline void coarse_store(const PkJob &jb, uint32_t jn, uint32_t i, ui
 	for (int r = 0; r < PX_RPT; r++) {
 		const uint32_t pix = (j0 + r) * jb.dw + i;
 		if (inside[r]) {
-			if (jb.has_nc) img_nc[jb.img_off + pix] = (TI)acc_nc[r];
+			if (jb.has_nc && !jb.nc_sky) img_nc[jb.img_off + pix] = (TI)acc_nc[r];
 			if (jb.has_conv) img_c[jb.img_off + pix] = (TI)acc_c[r];
 		}
 		ballot[r] = __ballot(cmask[r] != 0);
@@ -693,7 +693,22 @@ struct ConvOut {
 	T *__restrict__ dst;           // convolved image buffer
 	T *__restrict__ out;           // returned images (jobs with fuse_out), or nullptr
 	const T *__restrict__ img_nc;  // unconvolved drawing images added to them
+	T nc_const;                    // the job's unconvolved value when it has nc_sky
 };
+
+// Unconvolved value of a job with nc_sky: its unconvolved skies summed in
+// component order in FP64 and rounded, as the pixel pass would store it.
+template <typename T>
+__device__ inline T nc_sky_value(const PkJob &jb, const PkComp *__restrict__ comps)
+{
+	double v = 0;
+	if (jb.nc_sky && comps)
+		for (uint32_t k = jb.comp_off; k < jb.comp_off + jb.ncomp; k++) {
+			const PixHot &h = comps[k].hot;
+			if (h.kind == PK_PROFILE_SKY && !(h.flags & PIX_CONVOLVE)) v += h.vscale;
+		}
+	return (T)v;
+}
 
 // Unconvolved part of returned pixel (X, Y) of a job with fuse_out (0 when
 // the job has none there, or the pixel lies outside the convolved frame).
@@ -703,6 +718,7 @@ __device__ inline T conv_unconvolved(const PkJob &jb, const ConvOut<T> &o, int X
 	if (!(o.out && jb.fuse_out && jb.has_nc) || X >= (int)jb.cw || Y >= (int)jb.ch) return (T)0;
 	const int mx = X - jb.mox, my = Y - jb.moy;
 	if (mx < 0 || my < 0 || mx >= (int)jb.dw || my >= (int)jb.dh) return (T)0;
+	if (jb.nc_sky) return o.nc_const;
 	return o.img_nc[jb.img_off + mx + (size_t)my * jb.dw];
 }
 
@@ -835,7 +851,7 @@ __global__ __launch_bounds__(256, 1) void conv_kernel(const PkJob *__restrict__ 
                                                    const uint8_t *__restrict__ masks, T *__restrict__ dst_all,
                                                    int l0, int nl, const uint32_t *__restrict__ tile_job,
                                                    T *__restrict__ out_all, const T *__restrict__ img_nc,
-                                                   uint32_t tile_base)
+                                                   uint32_t tile_base, const PkComp *__restrict__ comps)
 {
 	extern __shared__ unsigned char smem_raw[];
 	T *tile = reinterpret_cast<T *>(smem_raw);
@@ -846,7 +862,7 @@ __global__ __launch_bounds__(256, 1) void conv_kernel(const PkJob *__restrict__ 
 	const int bx = (int)(t % jb.ctiles_x) * CT_W, by = (int)(t / jb.ctiles_x) * CT_H;
 	const T *src = src_all + jb.img_off;
 	const T *kt = krn_all + jb.psf_off;
-	const ConvOut<T> o{dst_all, out_all, img_nc};
+	const ConvOut<T> o{dst_all, out_all, img_nc, nc_sky_value<T>(jb, comps)};
 	const int kw = (int)jb.kw, kh = (int)jb.kh;
 	if (l0 == 0 && nl >= kh && kw == kh && (kw & 1) && kw >= CT_KMIN && kw <= CT_KMAX &&
 	    conv_dispatch<T, CT_KMIN>(kw, jb, src, kt, tile, bx, by, masks, o))
@@ -960,7 +976,8 @@ __global__ __launch_bounds__(256, 1) void conv_mfma_kernel(const PkJob *__restri
                                                            float *__restrict__ dst_all,
                                                            const uint32_t *__restrict__ tile_job,
                                                            float *__restrict__ out_all,
-                                                           const float *__restrict__ img_nc, uint32_t tile_base)
+                                                           const float *__restrict__ img_nc, uint32_t tile_base,
+                                                           const PkComp *__restrict__ comps)
 {
 	extern __shared__ unsigned char smem_raw[];
 	float *tile = reinterpret_cast<float *>(smem_raw);
@@ -971,7 +988,7 @@ __global__ __launch_bounds__(256, 1) void conv_mfma_kernel(const PkJob *__restri
 	const int bx = (int)(t % jb.ctiles_x) * CT_W, by = (int)(t / jb.ctiles_x) * CT_H;
 	const float *src = src_all + jb.img_off;
 	const float *krn = krn_all + jb.psf_off;
-	const ConvOut<float> o{dst_all, out_all, img_nc};
+	const ConvOut<float> o{dst_all, out_all, img_nc, nc_sky_value<float>(jb, comps)};
 	const int W = (int)jb.dw, H = (int)jb.dh, kw = (int)jb.kw, kh = (int)jb.kh;
 	const int hw = kw / 2, hh = kh / 2;
 	const int ox = bx + jb.csx, oy = by + jb.csy;
@@ -1142,13 +1159,15 @@ __device__ inline void conv_strip(const PkJob &jb, const float *__restrict__ src
 	const float *const ncb = with_nc ? o.img_nc + jb.img_off + (X - jb.mox) : o.img_nc;
 	const int nc_y0 = y0 - jb.moy, nc_h = (int)jb.dh;
 	const size_t nc_stride = jb.dw;
+	const bool nc_const = jb.nc_sky != 0;
 	const bool masked = jb.mask_off != NONE;
 	float ncp[4];
 	auto load_nc = [&](int yb) {
 #pragma unroll
 		for (int i = 0; i < 4; i++) {
 			const int yy = yb + kq + 4 * i, my = nc_y0 + yy;
-			ncp[i] = (with_nc && yy < nrows && my >= 0 && my < nc_h) ? ncb[(size_t)my * nc_stride] : 0.f;
+			const bool in = with_nc && yy < nrows && my >= 0 && my < nc_h;
+			ncp[i] = !in ? 0.f : nc_const ? o.nc_const : ncb[(size_t)my * nc_stride];
 		}
 	};
 	auto flush = [&](int yb) {
@@ -1262,7 +1281,7 @@ conv_strip_kernel(
     const PkJob *__restrict__ jobs, uint32_t njobs, const float *__restrict__ src_all,
     const float *__restrict__ krn_all, const uint8_t *__restrict__ masks, float *__restrict__ dst_all,
     const uint32_t *__restrict__ tile_job, float *__restrict__ out_all, const float *__restrict__ img_nc,
-    uint32_t tile_base)
+    uint32_t tile_base, const PkComp *__restrict__ comps)
 {
 	__shared__ StripLds lds[4];
 	const uint32_t gt = tile_base + blockIdx.x;
@@ -1272,7 +1291,7 @@ conv_strip_kernel(
 	const int x0 = (int)(t % jb.ctiles_x) * CT_W + 16 * w, y0 = (int)(t / jb.ctiles_x) * ST_H;
 	if (x0 >= (int)jb.cw || y0 >= (int)jb.ch) return;
 	const int nrows = min(ST_H, (int)jb.ch - y0);
-	const ConvOut<float> o{dst_all, out_all, img_nc};
+	const ConvOut<float> o{dst_all, out_all, img_nc, nc_sky_value<float>(jb, comps)};
 	conv_strip<NJ, C>(jb, src_all + jb.img_off, krn_all + jb.psf_off, masks, o, x0, y0, nrows, lds[w], lane);
 }
 

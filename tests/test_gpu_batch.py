@@ -50,10 +50,14 @@ def test_graph_replays_match_eager(ctx32):
     eager.close()
 
 
-def test_set_components(ctx64, oracle):
-    print("CASE set_components", flush=True)
+@pytest.mark.parametrize("prec", ["fp64", "fp32"])
+def test_set_components(ctx64, ctx32, oracle, prec):
+    """New parameters for every component, the sky levels included (an FP32
+    job whose unconvolved part is only sky takes it as one value computed on
+    the device at each evaluation): the graph replays the new images."""
+    print("CASE set_components", prec, flush=True)
     specs = small_batch(6, 9)
-    b = Batch(ctx64, specs, graph=True)
+    b = Batch(ctx64 if prec == "fp64" else ctx32, specs, graph=True)
     b.evaluate()
     b.sync()
     # move every component and brighten it, as one step of a fit would
@@ -66,6 +70,8 @@ def test_set_components(ctx64, oracle):
                 c.xcen += 0.37
                 c.ycen -= 0.21
                 c.mag -= 0.5
+            else:
+                c.p[0] = 1.5 * c.p[0] + 0.25  # bg
             cs.append(c)
         comps += cs
         new_specs.append(abi.ModelSpec(s.m.width, s.m.height, cs, psf=s._psf, magzero=s.m.magzero))
@@ -74,7 +80,7 @@ def test_set_components(ctx64, oracle):
     b.sync()
     for i, s in enumerate(new_specs):
         ref, _ = oracle.evaluate(s)
-        check_image(b.image(i), s, ref, oracle, TOL_FP64, i)
+        check_image(b.image(i), s, ref, oracle, TOL_FP64 if prec == "fp64" else TOL_FP32, i)
     b.close()
 
 
