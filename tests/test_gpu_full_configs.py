@@ -4,9 +4,10 @@ galaxy4096: 4096 galaxies, square stamps with side U{32..512}, 1-3 components
 of the six radial profiles (magzero 30) + sky, per-galaxy Moffat PSF of odd
 side U{15..41}.  The FP64 batch is compared with the oracle (1e-12) and the
 FP32 batch with the oracle (1e-5) on a sample of 72 galaxies covering every
-PSF side's smallest and largest stamp; the FP32 batch is then compared with
-the FP64 batch on all 4096 images (1e-5 per pixel above 1e-6 of the peak and
-in total flux), and every image is checked for finite, non-negative pixels.
+PSF side's smallest and largest stamp; the FP32 batch, with each of its
+convolution forms, is then compared with the FP64 batch on all 4096 images
+(1e-5 per pixel above 1e-6 of the peak and in total flux), and every image is
+checked for finite, non-negative pixels.
 The total flux of an image is not a property of the model by itself: outside
 the refinement radius libprofit samples profiles at pixel centres, which for
 compact profiles does not conserve flux (a ferrer of rout 7.9 px and axrat
@@ -41,30 +42,39 @@ def test_galaxy4096_full_size(ctx32, ctx64, oracle):
     print("CASE galaxy4096 full size", flush=True)
     oracle.set_threads(16)
     specs = workloads.galaxy_batch(4096, 0)
-    b32 = Batch(ctx32, specs, graph=False)
-    b32.evaluate()
-    b32.sync()  # raises on a work-queue overflow or non-finite refinement values
     b64 = Batch(ctx64, specs, graph=False)
     b64.evaluate()
     b64.sync()
     sample = stratified_sample(specs, 72)
     print("  sample of %d galaxies, PSF sides %s" % (len(sample), sorted({specs[i].m.psf_width for i in sample})),
           flush=True)
+    refs = {}
     for i in sample:
-        ref, _ = oracle.evaluate(specs[i])
-        check_image(b64.image(i), specs[i], ref, oracle, TOL_FP64, ("fp64", i))
-        check_image(b32.image(i), specs[i], ref, oracle, TOL_FP32, ("fp32", i))
-    worst_pix = worst_flux = 0.0
-    for i, s in enumerate(specs):
-        g32, g64 = b32.image(i), b64.image(i)
-        assert g32.shape == (s.m.height, s.m.width)
-        assert np.all(np.isfinite(g32)) and np.all(g32 >= 0), i
-        pix, flux = rel_errors(g32, g64)
-        worst_pix, worst_flux = max(worst_pix, pix), max(worst_flux, flux)
-    print("  all 4096: FP32 vs FP64 max pixel %.3g, max flux %.3g" % (worst_pix, worst_flux), flush=True)
-    assert worst_pix <= TOL_FP32 and worst_flux <= TOL_FP32
-    b32.close()
+        refs[i], _ = oracle.evaluate(specs[i])
+        check_image(b64.image(i), specs[i], refs[i], oracle, TOL_FP64, ("fp64", i))
+    g64 = [b64.image(i) for i in range(len(specs))]
     b64.close()
+    # every FP32 convolution form (VALU tiles, banded MFMA tiles, MFMA strips
+    # over images up to two strip tiles tall)
+    b32 = Batch(ctx32, specs, graph=False)
+    for tiles in (0, 1, 2):
+        b32.tuning = {"conv_tiles": tiles, "refine_occupancy": 100, "handover_div": 4, "claim_pixels": 256,
+                      "claim_cells": 256, "poll_rounds": 16, "claim_spread": 64}
+        b32.evaluate()
+        b32.sync()  # raises on a work-queue overflow or non-finite refinement values
+        for i in sample:
+            check_image(b32.image(i), specs[i], refs[i], oracle, TOL_FP32, ("fp32", tiles, i))
+        worst_pix = worst_flux = 0.0
+        for i, s in enumerate(specs):
+            g32 = b32.image(i)
+            assert g32.shape == (s.m.height, s.m.width)
+            assert np.all(np.isfinite(g32)) and np.all(g32 >= 0), (tiles, i)
+            pix, flux = rel_errors(g32, g64[i])
+            worst_pix, worst_flux = max(worst_pix, pix), max(worst_flux, flux)
+        print("  all 4096, conv_tiles %d: FP32 vs FP64 max pixel %.3g, max flux %.3g" % (tiles, worst_pix, worst_flux),
+              flush=True)
+        assert worst_pix <= TOL_FP32 and worst_flux <= TOL_FP32, tiles
+    b32.close()
 
 
 def _fit_model(spec):
