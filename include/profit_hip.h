@@ -274,7 +274,7 @@ enum {
  * or how; the refinement settings do not change the result at all (its sums
  * are exact fixed-point), the matrix-core convolution forms change the FP32
  * rounding of convolved pixels only.
- * Defaults: VALU tiles, 100, 4, 64, 16, 16, 0. */
+ * Defaults: VALU tiles, 100, 4, 256, 256, 5, 0. */
 enum {
 	PK_CONV_TILES_VALU = 0,   /* FP32 FMA tiles */
 	PK_CONV_TILES_MFMA = 1,   /* matrix-core tiles (banded PSF rows) */
@@ -289,7 +289,10 @@ typedef struct pk_tuning {
 	                              level and hands all failing sub-cells to the next (>= 1) */
 	uint32_t claim_pixels;     /* refinement pixels one wavefront claims at once, at most (>= 1) */
 	uint32_t claim_cells;      /* cells of the previous level one wavefront claims at once, at most (>= 1) */
-	uint32_t poll_rounds;      /* no effect (kept for the layout) (>= 1) */
+	uint32_t refine_passes;    /* FP32 refinement launches, 4..7: one per level for levels 0 .. n - 2 (a level's
+	                              failing sub-cells are the next launch's work), the last one every deeper
+	                              level depth-first; a batch raises it where its components' max_recursions
+	                              and resolution need more level launches for the last one's cell stack */
 	uint32_t claim_spread;     /* 0: refinement pixels are claimed in the order the pixel pass listed
 	                              them; a power of two n <= 4096: in runs of n taken from 64 places of
 	                              the list in turn (spreads deep galaxy centres over wavefronts) */

@@ -157,7 +157,7 @@ class Tuning(C.Structure):
         ("handover_div", C.c_uint32),
         ("claim_pixels", C.c_uint32),
         ("claim_cells", C.c_uint32),
-        ("poll_rounds", C.c_uint32),
+        ("refine_passes", C.c_uint32),
         ("claim_spread", C.c_uint32),
         ("ms_evaluate", C.c_double),
     ]

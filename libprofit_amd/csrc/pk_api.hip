@@ -133,9 +133,24 @@ struct Layout {
 
 namespace {
 
+// FP32 refinement launches a component needs: the last launch refines levels
+// passes - 1 .. max_recursions depth-first on an LDS stack of R32_CAP_LAST
+// cells, which holds up to 128 claimed cells plus res^2 children per level
+// (the Sersic centre override, resolution 8 and 10 levels, always fits).
+uint32_t refine_passes_needed(const pk_component &c)
+{
+	if (c.kind == PK_PROFILE_SKY || c.kind == PK_PROFILE_NULL || c.kind == PK_PROFILE_PSF || c.rough)
+		return R32_PASSES_MIN;
+	const uint64_t res = c.adjust ? 16u : std::max(1u, c.resolution);
+	const int levels = (int)((R32_CAP_LAST - 128) / (res * res));
+	const int need = (int)std::min(c.max_recursions, 64u) + 2 - levels;
+	return (uint32_t)std::min(std::max(need, (int)R32_PASSES_MIN), R32_PASSES);
+}
+
 // ---- autotuner cache -----------------------------------------------------------
 // One text line per tuned workload: "<key>\t<conv> <occupancy> <handover> <claim
-// pixels> <claim cells> <poll> <spread> <ms>", in $PROFIT_HOME/hip_tuning.
+// pixels> <claim cells> <refinement launches> <spread> <ms>", in
+// $PROFIT_HOME/hip_tuning (a line whose settings are not valid is skipped).
 std::mutex g_tune_mu;
 std::map<std::string, pk_tuning> g_tune_cache;
 bool g_tune_dirty = false;
@@ -157,8 +172,8 @@ std::string tune_cache_file()
 bool tuning_valid(const pk_tuning &t)
 {
 	return t.conv_tiles <= PK_CONV_TILES_STRIPS && t.refine_occupancy >= 1 && t.refine_occupancy <= 100 &&
-	       t.handover_div >= 1 && t.claim_pixels >= 1 && t.claim_cells >= 1 && t.poll_rounds >= 1 &&
-	       t.handover_div <= 1024 && t.claim_pixels <= 4096 && t.claim_cells <= 4096 && t.poll_rounds <= 4096 &&
+	       t.handover_div >= 1 && t.claim_pixels >= 1 && t.claim_cells >= 1 && t.refine_passes >= R32_PASSES_MIN &&
+	       t.handover_div <= 1024 && t.claim_pixels <= 4096 && t.claim_cells <= 4096 && t.refine_passes <= R32_PASSES &&
 	       t.claim_spread <= 4096 && (t.claim_spread & (t.claim_spread - 1)) == 0;
 }
 
@@ -175,7 +190,7 @@ void tune_cache_load()
 		*tab = 0;
 		pk_tuning t{};
 		if (std::sscanf(tab + 1, "%u %u %u %u %u %u %u %lf", &t.conv_tiles, &t.refine_occupancy, &t.handover_div,
-		                &t.claim_pixels, &t.claim_cells, &t.poll_rounds, &t.claim_spread, &t.ms_evaluate) == 8 &&
+		                &t.claim_pixels, &t.claim_cells, &t.refine_passes, &t.claim_spread, &t.ms_evaluate) == 8 &&
 		    tuning_valid(t))
 			g_tune_cache[line] = t;
 	}
@@ -193,7 +208,7 @@ int tune_cache_save()
 	for (const auto &kv : g_tune_cache) {
 		const pk_tuning &t = kv.second;
 		std::fprintf(fp, "%s\t%u %u %u %u %u %u %u %.6f\n", kv.first.c_str(), t.conv_tiles, t.refine_occupancy,
-		             t.handover_div, t.claim_pixels, t.claim_cells, t.poll_rounds, t.claim_spread, t.ms_evaluate);
+		             t.handover_div, t.claim_pixels, t.claim_cells, t.refine_passes, t.claim_spread, t.ms_evaluate);
 	}
 	const bool ok = std::fclose(fp) == 0;
 	if (!ok || std::rename(tmp.c_str(), f.c_str()) != 0) {
@@ -983,7 +998,8 @@ int launch_profiles(hipStream_t s, bool fp32, const PkJob *djobs, uint32_t njobs
 		// holds the guard-band sub-cells of the pass that just ran
 		RCell *q0 = dq, *q1 = dq + qcap, *pq = dq + 2 * (size_t)qcap;
 		const uint32_t gm = std::min(grid_main, ref_blocks), gl = std::min(grid_last, ref_blocks);
-		for (int p = 0; p < R32_PASSES - 1; p++) {
+		const int np = (int)std::min<uint32_t>(std::max(tune.passes, R32_PASSES_MIN), R32_PASSES);
+		for (int p = 0; p < np - 1; p++) {
 			RCell *qin = (p & 1) ? q0 : q1, *qout = (p & 1) ? q1 : q0;
 			if (p == 0)
 				hipLaunchKernelGGL((refine32_kernel<0, false>), dim3(gm), dim3(64 * r32_waves_per_group(false, false)), 0,
@@ -999,7 +1015,7 @@ int launch_profiles(hipStream_t s, bool fp32, const PkJob *djobs, uint32_t njobs
 			                   (const RCell *)pq, qout, qcap, p, dacc, (const double *)nullptr, (double *)nullptr);
 		}
 		hipLaunchKernelGGL((refine32_kernel<1, true>), dim3(gl), dim3(64), 0, s, djobs, dcomp, dlite, dent, dcnt,
-		                   (const RCell *)((R32_PASSES & 1) ? q1 : q0), (RCell *)nullptr, qcap, R32_PASSES - 1, dacc,
+		                   (const RCell *)((np & 1) ? q1 : q0), (RCell *)nullptr, qcap, np - 1, dacc,
 		                   tune, (RCell *)nullptr, (const double *)nullptr, (double *)nullptr);
 		hipLaunchKernelGGL(finalize32_kernel<TI>, dim3(1024), dim3(256), 0, s, djobs, dent, dcnt, dacc, img_nc, img_c);
 	} else {
@@ -1183,10 +1199,13 @@ int evaluate_impl(pk_ctx *ctx, const pk_model *m, const Analysis &a, double *out
 
 	const uint32_t ntiles = jb.tiles_x * cdiv(a.dh, PX_ROWS);
 	const uint32_t nref_blocks = (uint32_t)std::min<size_t>(npix, 2048);
+	R32Tune r32 = R32_TUNE_DEFAULT;
+	for (uint32_t c = 0; c < m->n_components; c++)
+		r32.passes = std::max(r32.passes, refine_passes_needed(m->components[c]));
 	PK_TRY(launch_profiles<TI>(s, fp32, djob, 1u, ntiles, dcomp, dptr<CompLite>(ctx, o_lite), dm, img_nc, img_c, dent, dcnt,
 	                           dptr<unsigned long long>(ctx, o_acc), dptr<RCell>(ctx, o_q), qcap, nref_blocks,
 	                           ctx->r32_grid_main, ctx->r32_grid_last,
-	                           !comps.empty(), nullptr));
+	                           !comps.empty(), nullptr, nullptr, r32));
 	if (!comps.empty()) {
 		hipLaunchKernelGGL(integrations_kernel, dim3(64), dim3(256), 0, s, (const RefineEntry *)dent,
 		                   (const PkCounters *)dcnt, dptr<unsigned long long>(ctx, o_hist));
@@ -1632,7 +1651,7 @@ static pk_tuning default_tuning()
 	t.handover_div = R32_TUNE_DEFAULT.handover_div;
 	t.claim_pixels = R32_TUNE_DEFAULT.chunk;
 	t.claim_cells = R32_TUNE_DEFAULT.chunk_cells;
-	t.poll_rounds = R32_TUNE_DEFAULT.poll;
+	t.refine_passes = R32_TUNE_DEFAULT.passes;
 	t.claim_spread = 0;
 	t.ms_evaluate = 0;
 	return t;
@@ -1648,6 +1667,7 @@ struct pk_batch {
 	std::vector<uint32_t> comp_index;  // user component index -> setup slot (or ~0 for null)
 	std::vector<int32_t> comp_kind;
 	uint32_t ntiles = 0, nctiles = 0, notiles = 0, nstiles = 0;
+	uint32_t min_passes = R32_PASSES_MIN;  // FP32 refinement launches its components need at least
 	uint32_t max_kw = 0, max_kh = 0;
 	uint32_t adj_kw = 0, adj_kh = 0;  // largest PSF of the adjoint correlations (FFT jobs included)
 	bool any_conv = false;
@@ -1739,7 +1759,8 @@ struct pk_batch {
 	R32Tune r32() const
 	{
 		const uint32_t spread_log2 = tune.claim_spread ? (uint32_t)__builtin_ctz(tune.claim_spread) : 0u;
-		return R32Tune{tune.handover_div, tune.claim_pixels, tune.claim_cells, tune.poll_rounds, spread_log2};
+		return R32Tune{tune.handover_div, tune.claim_pixels, tune.claim_cells,
+		               std::max(tune.refine_passes, min_passes), spread_log2};
 	}
 	uint32_t grid(uint32_t full) const { return std::max(1u, (uint32_t)((uint64_t)full * tune.refine_occupancy / 100)); }
 	template <typename T> T *d(size_t off) const
@@ -1917,7 +1938,8 @@ int grad_launch_t(pk_batch *b)
 		PK_HIP(hipGetLastError());
 		if (timed) PK_HIP(hipEventRecord(b->ev_grad[2], s));
 		RCell *pq = q0 + 2 * (size_t)b->qcap;
-		for (int p = 0; p < R32_PASSES - 1; p++) {
+		const int np = (int)std::min<uint32_t>(std::max(hd.passes, R32_PASSES_MIN), R32_PASSES);
+		for (int p = 0; p < np - 1; p++) {
 			RCell *qin = (p & 1) ? q0 : q1, *qout = (p & 1) ? q1 : q0;
 			if (p == 0)
 				hipLaunchKernelGGL((refine32_kernel<0, false, true>), dim3(gm), dim3(64), 0, s, djobs, dcomp, dlite,
@@ -1932,7 +1954,7 @@ int grad_launch_t(pk_batch *b)
 			                   gacc);
 		}
 		hipLaunchKernelGGL((refine32_kernel<1, true, true>), dim3(gl), dim3(64), 0, s, djobs, dcomp, dlite, dent, gcnt,
-		                   (const RCell *)((R32_PASSES & 1) ? q1 : q0), (RCell *)nullptr, b->qcap, R32_PASSES - 1,
+		                   (const RCell *)((np & 1) ? q1 : q0), (RCell *)nullptr, b->qcap, np - 1,
 		                   (unsigned long long *)nullptr, hd, (RCell *)nullptr, (const double *)gent, gacc);
 	} else {
 		if (timed) PK_HIP(hipEventRecord(b->ev_grad[2], s));
@@ -2065,6 +2087,7 @@ int autotune(pk_batch *b)
 		PK_TRY(search(&pk_tuning::claim_pixels, {32, 64, 128, 256, 512}, s_ref));
 		PK_TRY(search(&pk_tuning::claim_cells, {16, 64, 256, 512}, s_ref));
 		PK_TRY(search(&pk_tuning::claim_spread, {0, 4, 16, 64}, s_ref));
+		PK_TRY(search(&pk_tuning::refine_passes, {4, 5, 6, 7}, s_ref));
 	}
 	PK_TRY(tune_run(b, best, st));
 	best.ms_evaluate = st[5];
@@ -2254,6 +2277,7 @@ int pk_batch_create(pk_ctx *ctx, const pk_model *models, uint32_t n_models, uint
 		for (uint32_t c = 0; c < m->n_components; c++) {
 			const pk_component &pc = m->components[c];
 			b->comp_kind.push_back(pc.kind);
+			b->min_passes = std::max(b->min_passes, refine_passes_needed(pc));
 			if (pc.kind == PK_PROFILE_NULL) {
 				b->comp_index.push_back(~0u);
 				continue;
@@ -2685,9 +2709,9 @@ int pk_batch_set_tuning(pk_batch *b, const pk_tuning *t)
 	if (!b || !t) return set_err(PK_EINVAL, "NULL argument");
 	if (!tuning_valid(*t))
 		return set_err(PK_EINVAL,
-		               "invalid tuning (conv_tiles %u, occupancy %u, handover %u, claims %u/%u, poll %u, spread %u)",
+		               "invalid tuning (conv_tiles %u, occupancy %u, handover %u, claims %u/%u, passes %u, spread %u)",
 		               t->conv_tiles, t->refine_occupancy, t->handover_div, t->claim_pixels, t->claim_cells,
-		               t->poll_rounds, t->claim_spread);
+		               t->refine_passes, t->claim_spread);
 	std::lock_guard<std::mutex> lock(b->ctx->mu);
 	PK_HIP(hipSetDevice(b->ctx->device));
 	PK_HIP(hipStreamSynchronize(b->ctx->stream));
@@ -2807,6 +2831,8 @@ int pk_batch_set_components(pk_batch *b, const pk_component *comps, uint32_t n)
 	}
 	for (uint32_t i = 0; i < n; i++)
 		if (b->comp_index[i] != ~0u) b->setup[b->comp_index[i]].c = comps[i];
+	uint32_t passes = R32_PASSES_MIN;
+	for (uint32_t i = 0; i < n; i++) passes = std::max(passes, refine_passes_needed(comps[i]));
 	// fixed-point scales follow the new magnitudes
 	for (auto &jb : b->jobs) {
 		double f = 0;
@@ -2820,6 +2846,13 @@ int pk_batch_set_components(pk_batch *b, const pk_component *comps, uint32_t n)
 	}
 	std::lock_guard<std::mutex> lock(b->ctx->mu);
 	PK_HIP(hipSetDevice(b->ctx->device));
+	// deeper recursion than before may need more refinement launches: the
+	// captured graphs are dropped (recaptured at the next evaluation)
+	if (passes > b->min_passes) {
+		PK_HIP(hipStreamSynchronize(b->ctx->stream));
+		drop_graphs(b);
+		b->min_passes = passes;
+	}
 	PK_HIP(hipMemcpyAsync(b->d<void>(b->o_jobs), b->jobs.data(), sizeof(PkJob) * b->jobs.size(),
 	                      hipMemcpyHostToDevice, b->ctx->stream));
 	PK_HIP(hipMemcpyAsync(b->d<void>(b->o_setup), b->setup.data(), sizeof(SetupIn) * b->setup.size(),

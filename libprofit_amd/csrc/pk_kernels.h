@@ -55,10 +55,10 @@ struct RefineEntry {
 	uint32_t job, pix, cmask;
 };
 
-// The FP32-mode refinement runs in R32_PASSES launches: one per level of the
-// sub-pixel recursion (the failing sub-cells of a level are the next launch's
-// work, spread evenly over its wavefronts), the last one refining all deeper
-// levels.
+// The FP32-mode refinement runs in at most R32_PASSES launches (a launch
+// setting, R32Tune::passes): one per level of the sub-pixel recursion (the
+// failing sub-cells of a level are the next launch's work, spread evenly over
+// its wavefronts), the last one refining all deeper levels.
 constexpr int R32_PASSES = 7;
 
 // Run-time settings of the FP32 refinement launches (chosen per batch by the
@@ -67,7 +67,7 @@ struct R32Tune {
 	uint32_t handover_div;  // hand over once 1/handover_div of a pass's wavefronts stopped
 	uint32_t chunk;         // pixels per claim at most (pass 0)
 	uint32_t chunk_cells;   // handed-over cells per claim at most (later passes)
-	uint32_t poll;          // rounds between two checks of the hand-over condition
+	uint32_t passes;        // launches of the refinement (R32_PASSES_MIN .. R32_PASSES)
 	uint32_t spread_log2;   // pass 0 claims runs of 2^spread_log2 entries from 64 places of the entry
 	                        // list in turn (0: in entry order)
 };

@@ -11,8 +11,9 @@
 // neither the packing of cells into rounds nor the scheduling of wavefronts
 // can change the result.
 //
-// Load balance: the refinement runs as R32_PASSES launches, one level of the
-// sub-pixel recursion per launch but the last.  Pass 0 takes pixels from the
+// Load balance: the refinement runs as n launches (4 <= n <= R32_PASSES, a
+// tuned launch setting), one level of the sub-pixel recursion per launch but
+// the last.  Pass 0 takes pixels from the
 // coarse pass's entry list; pass p > 0 takes the cells of level p that pass
 // p - 1 produced.  The failing sub-cells of a level-pass go to the next
 // pass's queue (staged in LDS and written 64 at a time, one atomic per 64),
@@ -20,8 +21,8 @@
 // items cost at most res^2 sub-cells each: no wavefront is left with a deep
 // pixel's whole subtree while the others idle.  Work is claimed in chunks
 // that shrink as the source runs dry.  The last pass refines what remains
-// (levels >= R32_PASSES - 1: the Sersic centre pixels' deep recursion)
-// depth-first on a LIFO.  A queue that fills up keeps the cells on the
+// (levels >= n - 1: the Sersic centre pixels' deep recursion) depth-first on
+// a LIFO.  A queue that fills up keeps the cells on the
 // wavefront's LIFO instead.
 //
 // Guard band: sub-cells whose FP32 accuracy test falls within the guard band
@@ -66,8 +67,9 @@ __device__ inline void wave_sync()
 	__builtin_amdgcn_wave_barrier();
 }
 // untuned settings: claims of up to 256 pixels / 256 cells, in entry order
-// (smaller claims contend on the queue counters)
-constexpr R32Tune R32_TUNE_DEFAULT = {4u, 256u, 256u, 16u, 0u};
+// (smaller claims contend on the queue counters), five launches
+constexpr R32Tune R32_TUNE_DEFAULT = {4u, 256u, 256u, 5u, 0u};
+constexpr uint32_t R32_PASSES_MIN = 4;  // the last launch refines levels >= passes - 1
 
 template <int CAP, int OUTB>
 struct R32Stack {

@@ -85,7 +85,7 @@ def test_set_components(ctx64, ctx32, oracle, prec):
 
 
 DEFAULT_TUNING = {"conv_tiles": 0, "refine_occupancy": 100, "handover_div": 4, "claim_pixels": 256,
-                  "claim_cells": 256, "poll_rounds": 16, "claim_spread": 0}
+                  "claim_cells": 256, "refine_passes": 5, "claim_spread": 0}
 
 
 def images(b, n):
@@ -113,10 +113,10 @@ def test_tuning_settings_keep_the_images(ctx32):
     b.sync()
     ref = images(b, len(specs))
     for setting in ({"handover_div": 2, "refine_occupancy": 50, "claim_pixels": 16, "claim_cells": 4,
-                     "poll_rounds": 4, "claim_spread": 4},
+                     "refine_passes": 4, "claim_spread": 4},
                     {"handover_div": 16, "refine_occupancy": 75, "claim_pixels": 128, "claim_cells": 64,
-                     "poll_rounds": 64, "claim_spread": 64},
-                    {"handover_div": 1, "refine_occupancy": 1, "claim_spread": 4096}):
+                     "refine_passes": 7, "claim_spread": 64},
+                    {"handover_div": 1, "refine_occupancy": 1, "claim_spread": 4096, "refine_passes": 6}):
         print("CASE tuning", setting, flush=True)
         b.tuning = {**DEFAULT_TUNING, **setting}
         for _rep in range(2):
@@ -130,6 +130,7 @@ def test_tuning_settings_keep_the_images(ctx32):
         b.sync()
         assert_same_to_fp32_rounding(images(b, len(specs)), ref)
     for bad in ({"handover_div": 0}, {"refine_occupancy": 101}, {"conv_tiles": 3}, {"claim_pixels": 0},
+                {"refine_passes": 3}, {"refine_passes": 8},
                 {"claim_spread": 3}, {"claim_spread": 8192}):
         with pytest.raises(InvalidParameter):
             b.tuning = {**DEFAULT_TUNING, **bad}

@@ -59,7 +59,7 @@ def test_galaxy4096_full_size(ctx32, ctx64, oracle):
     b32 = Batch(ctx32, specs, graph=False)
     for tiles in (0, 1, 2):
         b32.tuning = {"conv_tiles": tiles, "refine_occupancy": 100, "handover_div": 4, "claim_pixels": 256,
-                      "claim_cells": 256, "poll_rounds": 16, "claim_spread": 64}
+                      "claim_cells": 256, "refine_passes": 5, "claim_spread": 64}
         b32.evaluate()
         b32.sync()  # raises on a work-queue overflow or non-finite refinement values
         for i in sample:

@@ -45,7 +45,7 @@ def make_data(oracle, fm_true, seed=0):
 # launch settings with each FP32 convolution form (VALU tiles, banded MFMA
 # tiles, MFMA strips) for the forward convolution and the adjoint correlation
 FORMS = [{"conv_tiles": t, "refine_occupancy": 100, "handover_div": 4, "claim_pixels": 64, "claim_cells": 16,
-          "poll_rounds": 16, "claim_spread": 0} for t in (0, 1, 2)]
+          "refine_passes": 5, "claim_spread": 0} for t in (0, 1, 2)]
 
 
 def device_chi2_grad(ctx, fms, datas, sigmas, graph=False, tuning=None):
@@ -241,8 +241,8 @@ def test_tuning_keeps_the_gradient(ctx32, oracle):
     c0, g0 = b.chi2_grad()
     scale = np.max(np.abs(g0), axis=1, keepdims=True)
     base = {"conv_tiles": 0, "refine_occupancy": 100, "handover_div": 4, "claim_pixels": 64, "claim_cells": 16,
-            "poll_rounds": 16, "claim_spread": 0}
-    for setting, rtol in (({"handover_div": 2, "refine_occupancy": 50, "claim_pixels": 16, "poll_rounds": 4,
+            "refine_passes": 5, "claim_spread": 0}
+    for setting, rtol in (({"handover_div": 2, "refine_occupancy": 50, "claim_pixels": 16, "refine_passes": 4,
                             "claim_spread": 16}, 1e-11),
                           ({"conv_tiles": 1}, 1e-5), ({"conv_tiles": 2}, 1e-5)):
         b.tuning = {**base, **setting}
