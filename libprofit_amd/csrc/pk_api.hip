@@ -2444,16 +2444,18 @@ int pk_batch_create(pk_ctx *ctx, const pk_model *models, uint32_t n_models, uint
 		// the convolution writes returned images only when it runs in one
 		// launch; the composition covers every other job
 		const bool one_launch = !b->any_conv || rows >= b->max_kh;
+		std::vector<uint8_t> nc_psf(b->jobs.size(), 0);  // job has an unconvolved psf profile
+		for (const auto &it : b->psf_items)
+			if (!it.to_conv) nc_psf[it.job] = 1;
 		for (auto &jb : b->jobs) {
 			if (!one_launch) jb.fuse_out = 0;
 			// FP32: a job whose unconvolved part is only sky, unmasked, whose
 			// convolution writes the returned image, takes that part as one
 			// value (the unconvolved image is never stored)
-			bool sky_only = elem == 4 && jb.has_nc && jb.fuse_out && jb.mask_off == NONE;
+			bool sky_only = elem == 4 && jb.has_nc && jb.fuse_out && jb.mask_off == NONE &&
+			                !nc_psf[&jb - b->jobs.data()];
 			for (uint32_t k = jb.comp_off; sky_only && k < jb.comp_off + jb.ncomp; k++)
 				if (!b->setup[k].c.convolve && b->setup[k].c.kind != PK_PROFILE_SKY) sky_only = false;
-			for (const auto &it : b->psf_items)
-				if (&b->jobs[it.job] == &jb && !it.to_conv) sky_only = false;
 			jb.nc_sky = sky_only ? 1 : 0;
 			jb.otile_off = b->notiles;
 			jb.otiles_x = jb.fuse_out ? 0 : cdiv(jb.ow, 64);
