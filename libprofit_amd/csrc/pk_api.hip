@@ -2085,7 +2085,7 @@ int autotune(pk_batch *b)
 	for (int sweep = 0; fp32 && b->n_comps && sweep < 2; sweep++) {
 		PK_TRY(search(&pk_tuning::refine_occupancy, {50, 75, 100}, s_ref));
 		PK_TRY(search(&pk_tuning::claim_pixels, {32, 64, 128, 256, 512}, s_ref));
-		PK_TRY(search(&pk_tuning::claim_cells, {16, 64, 256, 512}, s_ref));
+		PK_TRY(search(&pk_tuning::claim_cells, {16, 64, 128, 256, 512}, s_ref));
 		PK_TRY(search(&pk_tuning::claim_spread, {0, 4, 16, 64}, s_ref));
 		PK_TRY(search(&pk_tuning::refine_passes, {4, 5, 6, 7}, s_ref));
 	}
