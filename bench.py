@@ -280,8 +280,8 @@ def grad_bench(args, Batch, Context, evaluate_spec, ctx32, ctx64):
     slots = SLOTS_EVAL_ADJ * inf.n_grad_subcells
     ach = slots / (inf.ms_grad_refine * 1e-3) / 1e12 if inf.ms_grad_refine > 0 else 0.0
     out["fits_adjoint_refine_roofline"] = {
-        "kernel": "refine32_kernel<..., GRAD> (7 level passes) + r32_resolve_kernel<GRAD> + grad_finalize_kernel, "
-                  "per step",
+        "kernel": "refine32_kernel<..., GRAD> (%d launches) + r32_resolve_kernel<GRAD> + grad_finalize_kernel, "
+                  "per step" % out["fits_tuning"]["refine_passes"],
         "bound": "valu", "achieved": round(ach, 3), "peak": round(VALU_PEAK_TSLOTS, 2), "unit": "Tslot/s",
         "frac": round(ach / VALU_PEAK_TSLOTS, 4), "evaluations": int(inf.n_grad_subcells),
         "slots_per_evaluation": SLOTS_EVAL_ADJ, "ms": round(inf.ms_grad_refine, 4)}
@@ -498,7 +498,8 @@ def main():
                              SLOTS_CLASSIFY * pairs + SLOTS_EVAL * wc.coarse_evaluations, stages["coarse"],
                              pixel_component_pairs=pairs, evaluations=int(wc.coarse_evaluations),
                              slots_per_pair=SLOTS_CLASSIFY, slots_per_evaluation=SLOTS_EVAL),
-        "refine": valu_entry("refine32_kernel (7 level passes) + r32_resolve_kernel + finalize32_kernel, per step",
+        "refine": valu_entry("refine32_kernel (%d launches) + r32_resolve_kernel + finalize32_kernel, per step"
+                             % tuning["refine_passes"],
                              SLOTS_EVAL * inf.n_subcells, stages["refine"], evaluations=int(inf.n_subcells),
                              slots_per_evaluation=SLOTS_EVAL),
     }
