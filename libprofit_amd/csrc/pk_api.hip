@@ -2117,6 +2117,14 @@ int ensure_tuned(pk_batch *b)
 	return autotune(b);
 }
 
+// Rows of the strip tiles of an image of ch rows: the fewest tiles of at most
+// ST_H rows, of equal height (a short last tile would end a launch on it).
+uint32_t strip_rows(uint32_t ch)
+{
+	const uint32_t n = std::max(1u, cdiv(ch, (uint32_t)ST_H));
+	return std::max(1u, cdiv(ch, n));
+}
+
 // A job the strip form convolves: direct (not FFT) convolution, PSF sides <= ST_KMAX.
 bool strip_job(const pk_batch *b, const PkJob &jb)
 {
@@ -2493,7 +2501,8 @@ int pk_batch_create(pk_ctx *ctx, const pk_model *models, uint32_t n_models, uint
 					PkJob &jb = b->jobs[k];
 					if (!strip_job(b, jb) || cdiv(jb.kw, 4) != nj || cdiv(jb.kh, 16) != c) continue;
 					jb.stile_off = b->nstiles;
-					b->nstiles += jb.ctiles_x * cdiv(jb.ch, ST_H);
+					jb.st_rows = strip_rows(jb.ch);
+					b->nstiles += jb.ctiles_x * cdiv(jb.ch, jb.st_rows);
 				}
 				sc.count = b->nstiles - sc.start;
 				if (sc.count) b->strip_classes.push_back(sc);
@@ -2513,7 +2522,7 @@ int pk_batch_create(pk_ctx *ctx, const pk_model *models, uint32_t n_models, uint
 		std::fill_n(tj.begin() + jb.tile_off, nt, jn);
 		std::fill_n(tj.begin() + b->ntiles + jb.ctile_off, nc, jn);
 		std::fill_n(tj.begin() + b->ntiles + b->nctiles + jb.otile_off, no, jn);
-		const uint32_t ns = strip_job(b, jb) ? jb.ctiles_x * cdiv(jb.ch, ST_H) : 0u;
+		const uint32_t ns = strip_job(b, jb) ? jb.ctiles_x * cdiv(jb.ch, jb.st_rows) : 0u;
 		std::fill_n(tj.begin() + b->ntiles + b->nctiles + b->notiles + jb.stile_off, ns, jn);
 	}
 	// adjoint of the convolution (PK_BATCH_GRAD): the correlation of the
@@ -2559,7 +2568,8 @@ int pk_batch_create(pk_ctx *ctx, const pk_model *models, uint32_t n_models, uint
 						PkJob &jb = adj[k];
 						if (!jb.has_conv || cdiv(jb.kw, 4) != nj || cdiv(jb.kh, 16) != c) continue;
 						jb.stile_off = b->nastiles;
-						const uint32_t n = jb.ctiles_x * cdiv(jb.ch, ST_H);
+						jb.st_rows = strip_rows(jb.ch);
+						const uint32_t n = jb.ctiles_x * cdiv(jb.ch, jb.st_rows);
 						tja.insert(tja.end(), n, k);
 						b->nastiles += n;
 					}

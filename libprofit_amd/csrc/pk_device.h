@@ -1092,7 +1092,8 @@ __global__ __launch_bounds__(256, 1) void conv_mfma_kernel(const PkJob *__restri
 // most 4 C ring additions and 3 group additions, so for terms of one sign the
 // FP32 error stays below (4 NJ + 4 C + 3) 2^-24 = 3.8e-6 of the sum (kw, kh <= 48).
 // ---------------------------------------------------------------------------
-constexpr int ST_H = 256;    // output rows of a strip tile (4 strips of 16 columns)
+constexpr int ST_H = 512;    // most output rows of a strip tile (4 strips of 16 columns); an image is split
+                             // into equal tiles of at most ST_H rows (PkJob::st_rows)
 constexpr int ST_KMAX = 48;  // largest PSF side of the strip form (NJ <= 12, C <= 3)
 constexpr int ST_PF = 4;     // source rows loaded ahead
 
@@ -1288,9 +1289,9 @@ conv_strip_kernel(
 	const PkJob &jb = jobs[find_job(jobs, njobs, gt, 3, tile_job)];
 	const uint32_t t = gt - jb.stile_off;
 	const int w = (int)(threadIdx.x >> 6), lane = (int)(threadIdx.x & 63);
-	const int x0 = (int)(t % jb.ctiles_x) * CT_W + 16 * w, y0 = (int)(t / jb.ctiles_x) * ST_H;
+	const int x0 = (int)(t % jb.ctiles_x) * CT_W + 16 * w, y0 = (int)(t / jb.ctiles_x) * (int)jb.st_rows;
 	if (x0 >= (int)jb.cw || y0 >= (int)jb.ch) return;
-	const int nrows = min(ST_H, (int)jb.ch - y0);
+	const int nrows = min((int)jb.st_rows, (int)jb.ch - y0);
 	const ConvOut<float> o{dst_all, out_all, img_nc, nc_sky_value<float>(jb, comps)};
 	conv_strip<NJ, C>(jb, src_all + jb.img_off, krn_all + jb.psf_off, masks, o, x0, y0, nrows, lds[w], lane);
 }

@@ -24,6 +24,7 @@ struct PkJob {
 	uint32_t kw, kh;
 	uint32_t ctile_off, ctiles_x;  // convolution tiles (prefix over jobs)
 	uint32_t stile_off;            // strip tiles of the matrix-core strip form (ctiles_x per tile row)
+	uint32_t st_rows;              // output rows of its strip tiles (the image split evenly, <= ST_H)
 	uint32_t nc_sky;               // FP32 batch job whose unconvolved components are all skies, without
 	                               // mask, whose convolution writes the returned image: the unconvolved
 	                               // image is one value, taken from the components, and never stored
