@@ -5,6 +5,7 @@ Usage: python -m libprofit_amd.build  (or libprofit_amd.build.build()).
 import os
 import subprocess
 import sys
+from concurrent.futures import ThreadPoolExecutor
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(HERE)
@@ -14,14 +15,17 @@ LIB = os.path.join(OUT_DIR, "libprofit_hip.so")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 SOURCES = ["pk_api.hip"]
-HEADERS = ["pk_math.h", "pk_profiles.h", "pk_kernels.h", "pk_device.h"]
+# the strip-form convolution kernels, compiled once per part of the PSF
+# column steps (pk_strips.h: ST_PARTS) so that the parts build in parallel
+STRIP_SOURCE = "pk_strips.hip"
+STRIP_PARTS = 6
+HEADERS = ["pk_math.h", "pk_profiles.h", "pk_kernels.h", "pk_device.h", "pk_strips.h"]
 
 FLAGS = [
     "--offload-arch=gfx950",
     "-O3",
     "-std=c++17",
     "-fPIC",
-    "-shared",
     # products and sums are not fused unless written as fma(): FP64 results
     # then round like libprofit's CPU build
     "-ffp-contract=off",
@@ -32,14 +36,30 @@ FLAGS = [
 
 
 def build(verbose=False, csrc=CSRC, out=LIB, defines=()):
-    """Compiles `csrc` (default: the package sources) into `out`."""
+    """Compiles `csrc` (default: the package sources) into `out`: one object
+    per source and strip part, compiled in parallel, then linked."""
     os.makedirs(os.path.dirname(out), exist_ok=True)
-    tmp = out + ".tmp"
-    cmd = [HIPCC] + FLAGS + ["-D" + d for d in defines] + ["-I", os.path.join(ROOT, "include"), "-I", csrc]
-    cmd += [os.path.join(csrc, s) for s in SOURCES] + ["-o", tmp]
+    obj_dir = out + ".obj"
+    os.makedirs(obj_dir, exist_ok=True)
+    base = [HIPCC] + FLAGS + ["-D" + d for d in defines] + ["-I", os.path.join(ROOT, "include"), "-I", csrc]
+    jobs = [(os.path.join(csrc, s), os.path.join(obj_dir, s + ".o"), []) for s in SOURCES]
+    jobs += [(os.path.join(csrc, STRIP_SOURCE), os.path.join(obj_dir, "strips%d.o" % p), ["-DPK_STRIP_PART=%d" % p])
+             for p in range(STRIP_PARTS)]
+    cmds = [base + extra + ["-c", src, "-o", obj] for src, obj, extra in jobs]
     if verbose:
-        print(" ".join(cmd))
-    subprocess.check_call(cmd)
+        for cmd in cmds:
+            print(" ".join(cmd))
+    workers = max(1, min(len(cmds), os.cpu_count() or 1, 16))
+    with ThreadPoolExecutor(workers) as pool:
+        results = list(pool.map(lambda cmd: subprocess.run(cmd).returncode, cmds))
+    for cmd, rc in zip(cmds, results):
+        if rc:
+            raise subprocess.CalledProcessError(rc, cmd)
+    tmp = out + ".tmp"
+    link = [HIPCC, "--offload-arch=gfx950", "-shared", "-fPIC"] + [obj for _, obj, _ in jobs] + ["-o", tmp]
+    if verbose:
+        print(" ".join(link))
+    subprocess.check_call(link)
     os.replace(tmp, out)
     return out
 

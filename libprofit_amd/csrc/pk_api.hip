@@ -24,6 +24,7 @@
 
 #include "pk_device.h"
 #include "pk_fft.h"
+#include "pk_strips.h"
 #include "profit_hip.h"
 
 using namespace pk;
@@ -824,32 +825,14 @@ int launch_conv(hipStream_t s, const PkJob *djobs, uint32_t njobs, uint32_t ntil
 	return PK_OK;
 }
 
-// ---- matrix-core strips ---------------------------------------------------------
-// Arguments of one strip-form launch: the jobs of one class share the PSF
-// column steps NJ = ceil(kw / 4) and row blocks C = ceil(kh / 16).
-struct StripLaunch {
-	const PkJob *jobs;
-	uint32_t njobs, count, start;
-	const float *src, *kt;
-	const uint8_t *masks;
-	float *dst;
-	const uint32_t *tile_job;
-	float *out_all;
-	const float *img_nc;
-	const PkComp *comps;
-};
-
-template <int NJ, int C>
-bool launch_strips(hipStream_t s, uint32_t nj, uint32_t c, const StripLaunch &a)
+// ---- matrix-core strips (launches in pk_strips.hip, pk_strips.h) ----------------
+// Rows of a PSF of kh rows in the strip form: c blocks of 16 on the matrix
+// cores and tm tail rows on the ALU (where kh exceeds a multiple of 16 by at
+// most ST_TAIL rows)
+void strip_row_form(uint32_t kh, uint32_t &c, uint32_t &tm)
 {
-	if (nj == (uint32_t)NJ && c == (uint32_t)C) {
-		hipLaunchKernelGGL((conv_strip_kernel<NJ, C>), dim3(a.count), dim3(256), 0, s, a.jobs, a.njobs, a.src, a.kt,
-		                   a.masks, a.dst, a.tile_job, a.out_all, a.img_nc, a.start, a.comps);
-		return true;
-	}
-	if constexpr (C < (ST_KMAX + 15) / 16) return launch_strips<NJ, C + 1>(s, nj, c, a);
-	else if constexpr (NJ < (ST_KMAX + 3) / 4) return launch_strips<NJ + 1, 1>(s, nj, c, a);
-	else return false;
+	tm = kh > 16 && kh % 16 <= (uint32_t)ST_TAIL ? kh % 16 : 0u;
+	c = tm ? kh / 16 : (kh + 15) / 16;
 }
 
 // ---- FFT convolution ---------------------------------------------------------
@@ -1688,7 +1671,7 @@ struct pk_batch {
 	// the jobs with PSF sides <= ST_KMAX again, numbered in strip tiles by
 	// strip-form class (PSF column steps nj, row blocks c): one launch each
 	struct StripClass {
-		uint32_t start, count, nj, c;
+		uint32_t start, count, nj, c, tm;
 	};
 	std::vector<StripClass> strip_classes;
 	// psf profiles: rebinning items (PsfProfile::evaluate), the model data their
@@ -1839,8 +1822,9 @@ int batch_launch(pk_batch *b)
 				for (const auto &sc : b->strip_classes) {
 					const StripLaunch a{djobs, nj, sc.count, sc.start, c, b->d<float>(b->o_psfT), dmask,
 					                    b->d<float>(b->o_conv), tj_strip, b->d<float>(b->o_out), nc, dcomp};
-					if (!launch_strips<1, 1>(s, sc.nj, sc.c, a))
-						return set_err(PK_EINVAL, "no strip convolution for %u x %u PSF steps", sc.nj, sc.c);
+					if (!launch_strips(s, sc.nj, sc.c, sc.tm, a))
+						return set_err(PK_EINVAL, "no strip convolution for %u x %u + %u PSF steps", sc.nj, sc.c,
+						               sc.tm);
 				}
 			for (const auto &cc : b->conv_classes)
 				if (!(strips && cc.strips))
@@ -1910,8 +1894,8 @@ int grad_launch_t(pk_batch *b)
 				                    reinterpret_cast<const float *>(Gd), b->d<float>(b->o_psfF),
 				                    b->d<uint8_t>(b->o_mask), reinterpret_cast<float *>(Gc),
 				                    b->d<uint32_t>(b->o_tja) + b->natiles, nullptr, nullptr, nullptr};
-				if (!launch_strips<1, 1>(s, sc.nj, sc.c, a))
-					return set_err(PK_EINVAL, "no strip correlation for %u x %u PSF steps", sc.nj, sc.c);
+				if (!launch_strips(s, sc.nj, sc.c, sc.tm, a))
+					return set_err(PK_EINVAL, "no strip correlation for %u x %u + %u PSF steps", sc.nj, sc.c, sc.tm);
 			}
 		} else {
 			PK_TRY(launch_conv<T>(s, b->d<PkJob>(b->o_jobs_adj), nj, b->natiles, Gd, b->d<T>(b->o_psfF),
@@ -2495,11 +2479,14 @@ int pk_batch_create(pk_ctx *ctx, const pk_model *models, uint32_t n_models, uint
 		std::stable_sort(by_height.begin(), by_height.end(),
 		                 [&](uint32_t x, uint32_t y) { return b->jobs[x].ch > b->jobs[y].ch; });
 		for (uint32_t c = 1; c <= (ST_KMAX + 15) / 16; c++)
+			for (uint32_t tm = 0; tm <= (uint32_t)ST_TAIL; tm++)
 			for (uint32_t nj = 1; nj <= (ST_KMAX + 3) / 4; nj++) {
-				pk_batch::StripClass sc{b->nstiles, 0, nj, c};
+				pk_batch::StripClass sc{b->nstiles, 0, nj, c, tm};
 				for (uint32_t k : by_height) {
 					PkJob &jb = b->jobs[k];
-					if (!strip_job(b, jb) || cdiv(jb.kw, 4) != nj || cdiv(jb.kh, 16) != c) continue;
+					uint32_t jc, jtm;
+					strip_row_form(jb.kh, jc, jtm);
+					if (!strip_job(b, jb) || cdiv(jb.kw, 4) != nj || jc != c || jtm != tm) continue;
 					jb.stile_off = b->nstiles;
 					jb.st_rows = strip_rows(jb.ch);
 					b->nstiles += jb.ctiles_x * cdiv(jb.ch, jb.st_rows);
@@ -2562,11 +2549,14 @@ int pk_batch_create(pk_ctx *ctx, const pk_model *models, uint32_t n_models, uint
 			std::stable_sort(by_height.begin(), by_height.end(),
 			                 [&](uint32_t x, uint32_t y) { return adj[x].ch > adj[y].ch; });
 			for (uint32_t c = 1; c <= (ST_KMAX + 15) / 16; c++)
+				for (uint32_t tm = 0; tm <= (uint32_t)ST_TAIL; tm++)
 				for (uint32_t nj = 1; nj <= (ST_KMAX + 3) / 4; nj++) {
-					pk_batch::StripClass sc{b->nastiles, 0, nj, c};
+					pk_batch::StripClass sc{b->nastiles, 0, nj, c, tm};
 					for (uint32_t k : by_height) {
 						PkJob &jb = adj[k];
-						if (!jb.has_conv || cdiv(jb.kw, 4) != nj || cdiv(jb.kh, 16) != c) continue;
+						uint32_t jc, jtm;
+						strip_row_form(jb.kh, jc, jtm);
+						if (!jb.has_conv || cdiv(jb.kw, 4) != nj || jc != c || jtm != tm) continue;
 						jb.stile_off = b->nastiles;
 						jb.st_rows = strip_rows(jb.ch);
 						const uint32_t n = jb.ctiles_x * cdiv(jb.ch, jb.st_rows);

@@ -969,6 +969,7 @@ __device__ inline bool conv_mfma_dispatch(int kw, const float *tile, const float
 	return false;
 }
 
+#ifndef PK_STRIP_KERNELS_ONLY  // (pk_strips.hip: the strip-form kernels alone)
 __global__ __launch_bounds__(256, 1) void conv_mfma_kernel(const PkJob *__restrict__ jobs, uint32_t njobs,
                                                            const float *__restrict__ src_all,
                                                            const float *__restrict__ krn_all,
@@ -1062,6 +1063,7 @@ __global__ __launch_bounds__(256, 1) void conv_mfma_kernel(const PkJob *__restri
 			}
 		}
 }
+#endif
 
 // ---------------------------------------------------------------------------
 // Matrix-core convolution by strips.  A wavefront owns a strip of 16 output
@@ -1088,10 +1090,19 @@ __global__ __launch_bounds__(256, 1) void conv_mfma_kernel(const PkJob *__restri
 // groups and stored 16 at a time (at fixed steps of the unrolled loop, as R
 // is a multiple of 16), with the unconvolved parts of the returned pixels
 // loaded one batch ahead.
+// Tail rows.  A PSF of kh = 16 C + TM rows, 0 < TM <= ST_TAIL, takes C
+// blocks of matrix-core steps and its last TM rows on the vector ALU: lane
+// (x, q) sums K'(4 j + q, 16 C + i) S(t, x + 4 j + q) over j (the B operands it
+// holds) and adds the sum to its slot of output row t - 16 C - i, one of four
+// compile-time slots by lane group, so the ring is that of C + 1 blocks
+// (R = 16 (C + 1)) and the hand-over is unchanged.  Without it the last block
+// would be 16 rows of products for TM PSF rows.
 // Rounding: per output a chain of NJ matrix-core steps (4 NJ additions), at
 // most 4 C ring additions and 3 group additions, so for terms of one sign the
-// FP32 error stays below (4 NJ + 4 C + 3) 2^-24 = 3.8e-6 of the sum (kw, kh <= 48).
+// FP32 error stays below (4 NJ + 4 C + 3) 2^-24 = 3.8e-6 of the sum (kw, kh <= 48);
+// a tail row's NJ - 1 additions and one ring addition are fewer.
 // ---------------------------------------------------------------------------
+constexpr int ST_TAIL = 3;   // most tail rows (PSF rows past the last full block of 16 taken on the ALU)
 constexpr int ST_H = 512;    // most output rows of a strip tile (4 strips of 16 columns); an image is split
                              // into equal tiles of at most ST_H rows (PkJob::st_rows)
 constexpr int ST_KMAX = 48;  // largest PSF side of the strip form (NJ <= 12, C <= 3)
@@ -1102,13 +1113,14 @@ struct StripLds {
 	float stage[32][4][16];   // hand-overs of the lane groups, by output row mod 32
 };
 
-template <int NJ, int C>
+template <int NJ, int C, int TM>
 __device__ inline void conv_strip(const PkJob &jb, const float *__restrict__ src, const float *__restrict__ krn,
                                   const uint8_t *__restrict__ masks, const ConvOut<float> &o, int x0, int y0,
                                   int nrows, StripLds &L, int lane)
 {
-	constexpr int R = 16 * C;
+	constexpr int R = 16 * (C + (TM > 0));
 	static_assert(R % ST_PF == 0 && R % 16 == 0, "ring length");
+	static_assert(TM >= 0 && TM <= ST_TAIL && 16 * C + TM <= ST_KMAX, "tail rows");
 	const int W = (int)jb.dw, H = (int)jb.dh, kw = (int)jb.kw, kh = (int)jb.kh;
 	const int lo = lane & 15, kq = lane >> 4;
 	// A operand: lane (l, q) of step j of row block c holds K'(4 j + q, 16 c + l)
@@ -1119,6 +1131,15 @@ __device__ inline void conv_strip(const PkJob &jb, const float *__restrict__ src
 		for (int j = 0; j < NJ; j++) {
 			const int d = 4 * j + kq, l = 16 * c + lo;
 			kr[c][j] = (d < kw && l < kh) ? krn[(size_t)d * kh + l] : 0.f;
+		}
+	// tail rows: lane (x, q) holds K'(4 j + q, 16 C + i)
+	float ktl[TM > 0 ? TM : 1][NJ];
+#pragma unroll
+	for (int i = 0; i < TM; i++)
+#pragma unroll
+		for (int j = 0; j < NJ; j++) {
+			const int d = 4 * j + kq, l = 16 * C + i;
+			ktl[i][j] = (d < kw && l < kh) ? krn[(size_t)d * kh + l] : 0.f;
 		}
 	// source row t: image row sy0 + t, columns sx0 + lane, zero outside the
 	// image; loads from a clamped address (no branch), zeroed when written to LDS
@@ -1209,6 +1230,20 @@ __device__ inline void conv_strip(const PkJob &jb, const float *__restrict__ src
 	// steps and runs out of registers.
 	int guard = nsrc;
 	asm volatile("" : "+s"(guard));
+	// the tail rows' part of source row t: row t - 16 C - i is in slot
+	// (t - 16 C - i + 4 q) mod R of lane group q (in the interior steps also
+	// where it adds to no row of the strip: such a row is never stored)
+	auto tail = [&](auto tag) __attribute__((always_inline)) {
+		constexpr int s = decltype(tag)::value;
+#pragma unroll
+		for (int i = 0; i < TM; i++) {
+			float p = ktl[i][0] * b[0];
+#pragma unroll
+			for (int j = 1; j < NJ; j++) p = fmaf(ktl[i][j], b[j], p);
+#pragma unroll
+			for (int q = 0; q < 4; q++) ring[((s - 16 * C - i + 4 * q) % R + R) % R] += kq == q ? p : 0.f;
+		}
+	};
 	auto step = [&](auto tag, auto edge, int tb) __attribute__((always_inline)) {
 		constexpr int s = decltype(tag)::value;
 		constexpr bool EDGE = decltype(edge)::value;
@@ -1231,6 +1266,7 @@ __device__ inline void conv_strip(const PkJob &jb, const float *__restrict__ src
 					for (int r = 0; r < 4; r++) ring[((s - 16 * c - r) % R + R) % R] += acc[r];
 				}
 			}
+			if (TM > 0 && t < nsrc && t >= 16 * C && t < nrows + 16 * C + TM - 1) tail(tag);
 		} else if (t < guard) {
 			floatx4 acc[C];
 #pragma unroll
@@ -1243,6 +1279,7 @@ __device__ inline void conv_strip(const PkJob &jb, const float *__restrict__ src
 			for (int c = 0; c < C; c++)
 #pragma unroll
 				for (int r = 0; r < 4; r++) ring[((s - 16 * c - r) % R + R) % R] += acc[c][r];
+			tail(tag);
 		}
 #pragma unroll
 		for (int j = 0; j < NJ; j++) b[j] = bn[j];
@@ -1276,8 +1313,8 @@ __device__ inline void conv_strip(const PkJob &jb, const float *__restrict__ src
 
 // One workgroup: 4 strips (64 columns) of one strip tile; the wavefronts work
 // independently (their own LDS, no barrier).
-template <int NJ, int C>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(C < 3 ? 4 : NJ < 12 ? 3 : 2))) void
+template <int NJ, int C, int TM>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(C + (TM > 0) < 3 ? 4 : NJ < 12 ? 3 : 2))) void
 conv_strip_kernel(
     const PkJob *__restrict__ jobs, uint32_t njobs, const float *__restrict__ src_all,
     const float *__restrict__ krn_all, const uint8_t *__restrict__ masks, float *__restrict__ dst_all,
@@ -1293,9 +1330,10 @@ conv_strip_kernel(
 	if (x0 >= (int)jb.cw || y0 >= (int)jb.ch) return;
 	const int nrows = min((int)jb.st_rows, (int)jb.ch - y0);
 	const ConvOut<float> o{dst_all, out_all, img_nc, nc_sky_value<float>(jb, comps)};
-	conv_strip<NJ, C>(jb, src_all + jb.img_off, krn_all + jb.psf_off, masks, o, x0, y0, nrows, lds[w], lane);
+	conv_strip<NJ, C, TM>(jb, src_all + jb.img_off, krn_all + jb.psf_off, masks, o, x0, y0, nrows, lds[w], lane);
 }
 
+#ifndef PK_STRIP_KERNELS_ONLY
 // ---------------------------------------------------------------------------
 // composition: returned(i,j) = sum over the fs x fs block of
 //   conv(X,Y) + model(X - mox, Y - moy), (X,Y) = crop origin + fine position
@@ -1674,5 +1712,7 @@ __global__ __launch_bounds__(64) void setup_kernel(const SetupIn *__restrict__ i
 		if (lite) lite[t] = make_lite(c, in[t]);
 	}
 }
+
+#endif  // PK_STRIP_KERNELS_ONLY
 
 }  // namespace pk

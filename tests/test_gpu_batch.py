@@ -229,10 +229,11 @@ def test_batch_mixed_composition(ctx64, ctx32, oracle, prec):
 # every PSF-size class of the batch convolution (largest side <= 17, 21, ...,
 # 41, 48 and above), odd sides from 15 to 41, and even, non-square, tiny and
 # wide PSFs (the run-time column paths; for the strip form: every count of
-# PSF column steps 1..12 and row blocks 1..3, and the banded tiles above 48)
+# PSF column steps 1..12 and row blocks 1..3, tail rows 1..3 after one or two
+# blocks, and the banded tiles above 48)
 PSF_SHAPES = [(15, 15), (17, 17), (19, 19), (21, 21), (23, 23), (25, 25), (27, 27), (29, 29), (31, 31), (33, 33),
               (35, 35), (37, 37), (39, 39), (41, 41), (51, 51), (3, 3), (8, 8), (5, 9), (12, 7), (1, 5), (30, 30),
-              (16, 24), (21, 31), (41, 15), (45, 47), (48, 48), (47, 33)]
+              (16, 24), (21, 31), (41, 15), (45, 47), (48, 48), (47, 33), (20, 18), (9, 34), (48, 35), (2, 17)]
 
 
 @pytest.mark.parametrize("tiles", [0, 1, 2])

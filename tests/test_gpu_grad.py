@@ -181,10 +181,11 @@ def test_fft_convolved_fit_gradient(ctx64, ctx32, oracle, prec):
 
 
 def test_batch_of_fits(ctx32, oracle):
-    """Several independent fits of different sizes and PSFs in one batch."""
+    """Several independent fits of different sizes and PSFs in one batch, in
+    every convolution form (PSF heights 19 and 33: the strips' tail rows)."""
     announce("batch of fits")
     fms, datas, sigmas = [], [], []
-    for i, (size, k) in enumerate([(48, 15), (40, 9), (56, 11)]):
+    for i, (size, k) in enumerate([(48, 15), (40, 9), (56, 11), (52, 19), (66, 33)]):
         start, true = workloads.fit_batch(1, seed=100 + i, size=size, psf_size=k)
         img, _ = oracle.evaluate(true[0])
         d, s = workloads.noisy(img, np.random.Generator(np.random.PCG64(i)))
@@ -199,11 +200,12 @@ def test_batch_of_fits(ctx32, oracle):
         fms.append(FitModel(size, comps, start[0]._psf, magzero=start[0].m.magzero))
         datas.append(d)
         sigmas.append(s)
-    chi2, gs = device_chi2_grad(ctx32, fms, datas, sigmas)
-    for fm, d, s, c2, g in zip(fms, datas, sigmas, chi2, gs):
-        g_fd, fd_err, g_rel, c0 = oracle_fd_grad(oracle, fm, fm.theta(), d, s, with_error=True)
-        assert c2 == pytest.approx(c0, rel=1e-5)
-        check_close(g, g_fd, "fp32", fm.names(), fd_err, g_rel)
+    ref = [oracle_fd_grad(oracle, fm, fm.theta(), d, s, with_error=True) for fm, d, s in zip(fms, datas, sigmas)]
+    for tuning in FORMS:
+        chi2, gs = device_chi2_grad(ctx32, fms, datas, sigmas, tuning=tuning)
+        for fm, c2, g, (g_fd, fd_err, g_rel, c0) in zip(fms, chi2, gs, ref):
+            assert c2 == pytest.approx(c0, rel=1e-5), tuning["conv_tiles"]
+            check_close(g, g_fd, "fp32", fm.names(), fd_err, g_rel)
 
 
 def test_graph_replays_match_eager(ctx32, oracle):
