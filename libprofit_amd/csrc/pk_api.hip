@@ -66,6 +66,11 @@ struct pk_ctx {
 	int device = 0;
 	int precision = PK_FP64;
 	hipStream_t stream = nullptr;
+	// a second stream (default priority, as `stream`) for the strip-form
+	// convolution classes that run beside the others, forked from and joined
+	// to `stream` by events (in graph capture: parallel branches)
+	hipStream_t side = nullptr;
+	hipEvent_t fork = nullptr, join = nullptr;
 	std::mutex mu;
 	void *dmem = nullptr;
 	size_t dcap = 0;
@@ -283,7 +288,14 @@ int pk_ctx_create(int device, int precision, pk_ctx **ctx_out)
 		delete ctx;
 		return set_err(PK_EDEVICE, "hipStreamCreate failed");
 	}
+	if (hipStreamCreateWithFlags(&ctx->side, hipStreamNonBlocking) != hipSuccess) {
+		(void)hipStreamDestroy(ctx->stream);
+		delete ctx;
+		return set_err(PK_EDEVICE, "hipStreamCreate failed");
+	}
 	for (auto &e : ctx->ev) PK_HIP(hipEventCreate(&e));
+	PK_HIP(hipEventCreateWithFlags(&ctx->fork, hipEventDisableTiming));
+	PK_HIP(hipEventCreateWithFlags(&ctx->join, hipEventDisableTiming));
 	hipDeviceProp_t prop;
 	PK_HIP(hipGetDeviceProperties(&prop, device));
 	int nb_main = 0, nb_last = 0;
@@ -311,11 +323,15 @@ int pk_ctx_destroy(pk_ctx *ctx)
 	if (!ctx) return PK_OK;
 	(void)hipSetDevice(ctx->device);
 	(void)hipStreamSynchronize(ctx->stream);
+	(void)hipStreamSynchronize(ctx->side);
 	for (auto &e : ctx->ev)
 		if (e) (void)hipEventDestroy(e);
+	if (ctx->fork) (void)hipEventDestroy(ctx->fork);
+	if (ctx->join) (void)hipEventDestroy(ctx->join);
 	if (ctx->dmem) (void)hipFree(ctx->dmem);
 	if (ctx->hmem) (void)hipHostFree(ctx->hmem);
 	(void)hipStreamDestroy(ctx->stream);
+	if (ctx->side) (void)hipStreamDestroy(ctx->side);
 	delete ctx;
 	return PK_OK;
 }
@@ -1818,14 +1834,27 @@ int batch_launch(pk_batch *b)
 		if (timed) PK_HIP(hipEventRecord(b->ev[3], s));
 		if (b->any_conv) {
 			const bool strips = b->tune.conv_tiles == PK_CONV_TILES_STRIPS;
+			// the strip classes (disjoint jobs) on two streams, every other
+			// class beside the first stream's, so that one launch's last
+			// workgroups overlap the next launch's first
+			const bool two = strips && b->strip_classes.size() > 1;
+			if (two) {
+				PK_HIP(hipEventRecord(b->ctx->fork, s));
+				PK_HIP(hipStreamWaitEvent(b->ctx->side, b->ctx->fork, 0));
+			}
 			if (strips)
-				for (const auto &sc : b->strip_classes) {
+				for (size_t k = 0; k < b->strip_classes.size(); k++) {
+					const auto &sc = b->strip_classes[k];
 					const StripLaunch a{djobs, nj, sc.count, sc.start, c, b->d<float>(b->o_psfT), dmask,
 					                    b->d<float>(b->o_conv), tj_strip, b->d<float>(b->o_out), nc, dcomp};
-					if (!launch_strips(s, sc.nj, sc.c, sc.tm, a))
+					if (!launch_strips(two && (k & 1) ? b->ctx->side : s, sc.nj, sc.c, sc.tm, a))
 						return set_err(PK_EINVAL, "no strip convolution for %u x %u + %u PSF steps", sc.nj, sc.c,
 						               sc.tm);
 				}
+			if (two) {
+				PK_HIP(hipEventRecord(b->ctx->join, b->ctx->side));
+				PK_HIP(hipStreamWaitEvent(s, b->ctx->join, 0));
+			}
 			for (const auto &cc : b->conv_classes)
 				if (!(strips && cc.strips))
 					PK_TRY(launch_conv<float>(s, djobs, nj, cc.count, c, b->d<float>(b->o_psfT), dmask,
