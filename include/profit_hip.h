@@ -16,8 +16,10 @@
  *    (include/profit/image.h:522-530).  Host buffers are caller owned.
  *  - Masks are one byte per pixel, non-zero meaning "evaluate this pixel"
  *    (include/profit/image.h Mask = surface<bool>).
- *  - A context binds one device and one HIP stream; calls on one context are
- *    serialised internally, separate contexts may run concurrently.
+ *  - A context binds one device and one HIP stream (a batch's convolution
+ *    also forks onto a second stream of the context and joins back within the
+ *    call); calls on one context are serialised internally, separate contexts
+ *    may run concurrently.
  */
 #ifndef PROFIT_HIP_H
 #define PROFIT_HIP_H
